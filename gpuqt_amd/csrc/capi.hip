@@ -1,0 +1,803 @@
+// capi.hip -- the C ABI of liblsqt_hip.so (include/lsqt_hip.h): contexts,
+// Hamiltonian upload and layout selection, vectors, and the fused moment /
+// evolution drivers that keep the whole recursion on the device.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bessel.h"
+#include "lsqt_capi_internal.h"
+#include "lsqt_hip.h"
+#include "lsqt_launch.h"
+
+using namespace lsqt;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local char g_err[1024];
+
+int lsqt::set_error(int code, const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+extern "C" const char* lsqt_last_error(void)
+{
+  return g_err;
+}
+
+extern "C" const char* lsqt_strerror(int status)
+{
+  switch (status) {
+    case LSQT_OK:
+      return "ok";
+    case LSQT_ERR_INVALID:
+      return "invalid argument";
+    case LSQT_ERR_HIP:
+      return "HIP runtime error";
+    case LSQT_ERR_NOMEM:
+      return "out of memory";
+    case LSQT_ERR_NODEVICE:
+      return "no usable GPU";
+    case LSQT_ERR_UNSUPPORTED:
+      return "unsupported";
+    default:
+      return "unknown status";
+  }
+}
+
+extern "C" const char* lsqt_version(void)
+{
+  return "lsqt_hip 0.1 gfx950";
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+int lsqt::ctx_activate(lsqt_ctx* ctx)
+{
+  LSQT_HIP(hipSetDevice(ctx->device));
+  return LSQT_OK;
+}
+
+int lsqt::ensure_device(void** p, size_t* cap, size_t bytes)
+{
+  if (*cap >= bytes)
+    return LSQT_OK;
+  if (*p)
+    LSQT_HIP(hipFree(*p));
+  *p = nullptr;
+  *cap = 0;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(LSQT_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+  }
+  *cap = bytes;
+  return LSQT_OK;
+}
+
+int lsqt::ensure_pinned(void** p, size_t* cap, size_t bytes)
+{
+  if (*cap >= bytes)
+    return LSQT_OK;
+  if (*p)
+    LSQT_HIP(hipHostFree(*p));
+  *p = nullptr;
+  *cap = 0;
+  hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(LSQT_ERR_NOMEM, "hipHostMalloc(%zu bytes): %s", bytes,
+                     hipGetErrorString(e));
+  }
+  *cap = bytes;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_device_count(int* count)
+{
+  if (!count)
+    return set_error(LSQT_ERR_INVALID, "null count");
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return set_error(LSQT_ERR_NODEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  }
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_ctx_create(int device, lsqt_ctx** out)
+{
+  if (!out)
+    return set_error(LSQT_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return set_error(LSQT_ERR_NODEVICE, "no HIP device visible");
+  if (device < 0 || device >= count)
+    return set_error(LSQT_ERR_INVALID, "device %d out of range (%d devices)", device, count);
+  hipDeviceProp_t prop;
+  LSQT_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_error(LSQT_ERR_NODEVICE, "device %d is %s; this library is built for gfx950",
+                     device, prop.gcnArchName);
+  LSQT_HIP(hipSetDevice(device));
+  lsqt_ctx* c = new lsqt_ctx();
+  c->device = device;
+  c->cu_count = prop.multiProcessorCount;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return set_error(LSQT_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_ctx_destroy(lsqt_ctx* ctx)
+{
+  if (!ctx)
+    return LSQT_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->partial)
+    (void)hipFree(ctx->partial);
+  if (ctx->mu_dev)
+    (void)hipFree(ctx->mu_dev);
+  if (ctx->mu_host)
+    (void)hipHostFree(ctx->mu_host);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_ctx_sync(lsqt_ctx* ctx)
+{
+  if (!ctx)
+    return set_error(LSQT_ERR_INVALID, "null context");
+  LSQT_TRY(ctx_activate(ctx));
+  LSQT_HIP(hipStreamSynchronize(ctx->stream));
+  return LSQT_OK;
+}
+
+extern "C" void* lsqt_ctx_stream(lsqt_ctx* ctx)
+{
+  return ctx ? (void*)ctx->stream : nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Hamiltonian
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TripleKey {
+  uint64_t a, b, c;
+  bool operator==(const TripleKey& o) const { return a == o.a && b == o.b && c == o.c; }
+};
+
+struct TripleHash {
+  size_t operator()(const TripleKey& k) const
+  {
+    uint64_t h = k.a * 0x9E3779B97F4A7C15ull;
+    h ^= (k.b + 0x632BE59BD9B4E019ull) + (h << 6) + (h >> 2);
+    h ^= (k.c + 0x85EBCA77C2B2AE63ull) + (h << 6) + (h >> 2);
+    return (size_t)h;
+  }
+};
+
+uint64_t bits_of(double x)
+{
+  uint64_t u;
+  std::memcpy(&u, &x, 8);
+  return u;
+}
+
+int index_bits_for(int n)
+{
+  int bits = 1;
+  while (bits < 31 && (1u << bits) < (uint32_t)n)
+    ++bits;
+  return bits;
+}
+
+template <typename T>
+int upload(lsqt_ham* h, const T** dst, const T* src, size_t count)
+{
+  void* p = nullptr;
+  size_t bytes = sizeof(T) * (count ? count : 1);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess)
+    return set_error(LSQT_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+  h->allocations.push_back(p);
+  if (count) {
+    e = hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+      return set_error(LSQT_ERR_HIP, "hipMemcpy H2D: %s", hipGetErrorString(e));
+  }
+  *dst = (const T*)p;
+  return LSQT_OK;
+}
+
+}  // namespace
+
+int lsqt::ham_scratch(lsqt_ham* h, int count)
+{
+  for (int i = 0; i < count; ++i) {
+    if (!h->scratch[i]) {
+      void* p = nullptr;
+      size_t bytes = sizeof(double2) * (size_t)(h->d.n > 0 ? h->d.n : 1);
+      hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess)
+        return set_error(LSQT_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes,
+                         hipGetErrorString(e));
+      h->scratch[i] = (double2*)p;
+    }
+  }
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const int* nl,
+                               const double* pot, const double* hr, const double* hi,
+                               const double* xx, double energy_max, int layout_hint,
+                               lsqt_ham** out)
+{
+  if (!ctx || !out)
+    return set_error(LSQT_ERR_INVALID, "null context or output handle");
+  *out = nullptr;
+  if (n <= 0 || K <= 0)
+    return set_error(LSQT_ERR_INVALID, "n=%d and max_neighbor=%d must be positive", n, K);
+  if (!nn || !nl || !pot || !hr || !hi || !xx)
+    return set_error(LSQT_ERR_INVALID, "null Hamiltonian array");
+  if (!(energy_max > 0.0))
+    return set_error(LSQT_ERR_INVALID, "energy_max must be > 0");
+  if (layout_hint < LSQT_LAYOUT_AUTO || layout_hint > LSQT_LAYOUT_PACKED)
+    return set_error(LSQT_ERR_INVALID, "unknown layout %d", layout_hint);
+  // validate the structure before anything touches the device
+  for (int i = 0; i < n; ++i) {
+    if (nn[i] < 0 || nn[i] > K)
+      return set_error(LSQT_ERR_INVALID, "site %d has %d neighbours (max %d)", i, nn[i], K);
+    for (int m = 0; m < nn[i]; ++m) {
+      int j = nl[(size_t)i * K + m];
+      if (j < 0 || j >= n)
+        return set_error(LSQT_ERR_INVALID, "site %d slot %d: neighbour %d out of range", i, m,
+                         j);
+    }
+  }
+  LSQT_TRY(ctx_activate(ctx));
+
+  const size_t P = (size_t)n * K;
+  lsqt_ham* h = new lsqt_ham();
+  h->ctx = ctx;
+  HamDev& d = h->d;
+  std::memset(&d, 0, sizeof(d));
+  d.n = n;
+  d.K = K;
+  d.emax = energy_max;
+
+  // distinct (re, im, xx) triples, numbered by first appearance
+  const int ib = index_bits_for(n);
+  const int cb = 32 - ib;
+  const long long max_classes =
+    std::min<long long>((1ll << cb) - 1, (long long)kPackedMaxClasses);
+  std::vector<uint32_t> words;
+  std::vector<double2> tab;
+  std::vector<double> tab_xx;
+  bool packed = layout_hint != LSQT_LAYOUT_FULL;
+  if (packed) {
+    std::unordered_map<TripleKey, int, TripleHash> ids;
+    words.assign(P, kEmptySlot);
+    TripleKey last{~0ull, ~0ull, ~0ull};
+    int last_id = -1;
+    for (int i = 0; i < n && packed; ++i) {
+      for (int m = 0; m < nn[i]; ++m) {
+        size_t q = (size_t)i * K + m;
+        TripleKey key{bits_of(hr[q]), bits_of(hi[q]), bits_of(xx[q])};
+        int id;
+        if (key == last) {
+          id = last_id;
+        } else {
+          auto it = ids.find(key);
+          if (it == ids.end()) {
+            if ((long long)tab.size() >= max_classes) {
+              packed = false;
+              break;
+            }
+            id = (int)tab.size();
+            ids.emplace(key, id);
+            tab.push_back(make_double2(hr[q], hi[q]));
+            tab_xx.push_back(xx[q]);
+          } else {
+            id = it->second;
+          }
+          last = key;
+          last_id = id;
+        }
+        words[(size_t)m * n + i] = ((uint32_t)id << ib) | (uint32_t)nl[q];
+      }
+    }
+    if (!packed && layout_hint == LSQT_LAYOUT_PACKED) {
+      delete h;
+      return set_error(LSQT_ERR_UNSUPPORTED,
+                       "packed layout needs at most %lld distinct hopping triples", max_classes);
+    }
+  }
+
+  int rc = upload(h, &d.pot, pot, (size_t)n);
+  if (rc == LSQT_OK && packed) {
+    d.layout = tab.size() <= (size_t)kLdsTableMax ? kPackedLds : kPackedGlobal;
+    d.idx_bits = ib;
+    d.idx_mask = (ib >= 32) ? 0xFFFFFFFFu : ((1u << ib) - 1u);
+    d.ncls = (int)tab.size();
+    rc = upload(h, &d.words, words.data(), P);
+    if (rc == LSQT_OK)
+      rc = upload(h, &d.tab, tab.data(), tab.size());
+    if (rc == LSQT_OK)
+      rc = upload(h, &d.tab_xx, tab_xx.data(), tab_xx.size());
+    words.clear();
+    words.shrink_to_fit();
+  } else if (rc == LSQT_OK) {
+    d.layout = kFull;
+    std::vector<int> nl_cm(P, -1);
+    std::vector<double> hr_cm(P, 0.0), hi_cm(P, 0.0), xx_cm(P, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int m = 0; m < nn[i]; ++m) {
+        size_t q = (size_t)i * K + m, c = (size_t)m * n + i;
+        nl_cm[c] = nl[q];
+        hr_cm[c] = hr[q];
+        hi_cm[c] = hi[q];
+        xx_cm[c] = xx[q];
+      }
+    rc = upload(h, &d.nl, nl_cm.data(), P);
+    if (rc == LSQT_OK)
+      rc = upload(h, &d.hr, hr_cm.data(), P);
+    if (rc == LSQT_OK)
+      rc = upload(h, &d.hi, hi_cm.data(), P);
+    if (rc == LSQT_OK)
+      rc = upload(h, &d.xx, xx_cm.data(), P);
+  }
+  if (rc != LSQT_OK) {
+    lsqt_ham_destroy(h);
+    return rc;
+  }
+  // host copies for the FP32 / block paths, built on demand
+  h->host_pot.assign(pot, pot + n);
+  h->host_tab = tab;
+  h->host_tab_xx = tab_xx;
+  *out = h;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_ham_destroy(lsqt_ham* h)
+{
+  if (!h)
+    return LSQT_OK;
+  (void)hipSetDevice(h->ctx->device);
+  (void)hipStreamSynchronize(h->ctx->stream);
+  for (void* p : h->allocations)
+    (void)hipFree(p);
+  for (double2* p : h->scratch)
+    if (p)
+      (void)hipFree(p);
+  ham_release_extra(h);
+  delete h;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_ham_get_info(const lsqt_ham* h, lsqt_ham_info* info)
+{
+  if (!h || !info)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  info->n = h->d.n;
+  info->max_neighbor = h->d.K;
+  info->layout = h->d.layout == kFull ? LSQT_LAYOUT_FULL : LSQT_LAYOUT_PACKED;
+  info->classes = h->d.ncls;
+  info->index_bits = h->d.idx_bits;
+  info->energy_max = h->d.emax;
+  // potential 8 + per slot (word 4 | index 4 + re 8 + im 8) + psi_{m-1} 16 +
+  // psi_{m-2} 16 + psi_m 16 + left vector 16
+  double per_slot = h->d.layout == kFull ? 20.0 : 4.0;
+  info->bytes_per_site_f64 = 8.0 + per_slot * h->d.K + 64.0;
+  return LSQT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// vectors
+// ---------------------------------------------------------------------------
+extern "C" int lsqt_vec_create(lsqt_ctx* ctx, int n, lsqt_vec** out)
+{
+  if (!ctx || !out)
+    return set_error(LSQT_ERR_INVALID, "null context or output handle");
+  *out = nullptr;
+  if (n <= 0)
+    return set_error(LSQT_ERR_INVALID, "vector length %d must be positive", n);
+  LSQT_TRY(ctx_activate(ctx));
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, sizeof(double2) * (size_t)n);
+  if (e != hipSuccess)
+    return set_error(LSQT_ERR_NOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  lsqt_vec* v = new lsqt_vec();
+  v->ctx = ctx;
+  v->n = n;
+  v->d = (double2*)p;
+  launch_zero(v->d, n, ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  *out = v;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_clone(const lsqt_vec* src, lsqt_vec** out)
+{
+  if (!src || !out)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  LSQT_TRY(lsqt_vec_create(src->ctx, src->n, out));
+  launch_copy((*out)->d, src->d, src->n, src->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_destroy(lsqt_vec* v)
+{
+  if (!v)
+    return LSQT_OK;
+  (void)hipSetDevice(v->ctx->device);
+  (void)hipStreamSynchronize(v->ctx->stream);
+  (void)hipFree(v->d);
+  delete v;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_size(const lsqt_vec* v, int* n)
+{
+  if (!v || !n)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  *n = v->n;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_from_host(lsqt_vec* v, const double* re, const double* im)
+{
+  if (!v || !re || !im)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  LSQT_TRY(ctx_activate(v->ctx));
+  std::vector<double2> tmp((size_t)v->n);
+  for (int i = 0; i < v->n; ++i)
+    tmp[i] = make_double2(re[i], im[i]);
+  LSQT_HIP(hipStreamSynchronize(v->ctx->stream));
+  LSQT_HIP(hipMemcpy(v->d, tmp.data(), sizeof(double2) * (size_t)v->n, hipMemcpyHostToDevice));
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_to_host(const lsqt_vec* v, double* re, double* im)
+{
+  if (!v || !re || !im)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  LSQT_TRY(ctx_activate(v->ctx));
+  std::vector<double2> tmp((size_t)v->n);
+  LSQT_HIP(hipStreamSynchronize(v->ctx->stream));
+  LSQT_HIP(hipMemcpy(tmp.data(), v->d, sizeof(double2) * (size_t)v->n, hipMemcpyDeviceToHost));
+  for (int i = 0; i < v->n; ++i) {
+    re[i] = tmp[i].x;
+    im[i] = tmp[i].y;
+  }
+  return LSQT_OK;
+}
+
+static int same_shape(const lsqt_vec* a, const lsqt_vec* b)
+{
+  if (!a || !b)
+    return set_error(LSQT_ERR_INVALID, "null vector");
+  if (a->n != b->n || a->ctx != b->ctx)
+    return set_error(LSQT_ERR_INVALID, "vector sizes/contexts differ (%d vs %d)", a->n, b->n);
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_copy(lsqt_vec* dst, const lsqt_vec* src)
+{
+  LSQT_TRY(same_shape(dst, src));
+  LSQT_TRY(ctx_activate(dst->ctx));
+  if (dst->d != src->d)
+    launch_copy(dst->d, src->d, dst->n, dst->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_add(lsqt_vec* dst, const lsqt_vec* src)
+{
+  LSQT_TRY(same_shape(dst, src));
+  if (dst->d == src->d)
+    return set_error(LSQT_ERR_INVALID, "add: source and destination are the same vector");
+  LSQT_TRY(ctx_activate(dst->ctx));
+  launch_add(dst->d, src->d, dst->n, dst->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_apply_sz(lsqt_vec* dst, const lsqt_vec* src)
+{
+  LSQT_TRY(same_shape(dst, src));
+  if (dst->d == src->d)
+    return set_error(LSQT_ERR_INVALID, "apply_sz: source and destination are the same vector");
+  LSQT_TRY(ctx_activate(dst->ctx));
+  launch_sz(dst->d, src->d, dst->n, dst->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_swap(lsqt_vec* a, lsqt_vec* b)
+{
+  LSQT_TRY(same_shape(a, b));
+  double2* t = a->d;
+  a->d = b->d;
+  b->d = t;
+  return LSQT_OK;
+}
+
+int lsqt::ctx_reserve_moments(lsqt_ctx* ctx, int M, int tiles)
+{
+  LSQT_TRY(ensure_device((void**)&ctx->partial, &ctx->partial_cap,
+                         sizeof(double2) * (size_t)M * (size_t)tiles));
+  LSQT_TRY(ensure_device((void**)&ctx->mu_dev, &ctx->mu_cap, sizeof(double2) * (size_t)M));
+  LSQT_TRY(ensure_pinned((void**)&ctx->mu_host, &ctx->mu_host_cap,
+                         sizeof(double2) * (size_t)M));
+  return LSQT_OK;
+}
+
+int lsqt::ctx_download_moments(lsqt_ctx* ctx, int M, double* mu_re, double* mu_im)
+{
+  LSQT_HIP(hipMemcpyAsync(ctx->mu_host, ctx->mu_dev, sizeof(double2) * (size_t)M,
+                          hipMemcpyDeviceToHost, ctx->stream));
+  LSQT_HIP(hipStreamSynchronize(ctx->stream));
+  for (int m = 0; m < M; ++m) {
+    mu_re[m] = ctx->mu_host[m].x;
+    mu_im[m] = ctx->mu_host[m].y;
+  }
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_dot(const lsqt_vec* left, const lsqt_vec* psi, double* re, double* im)
+{
+  LSQT_TRY(same_shape(left, psi));
+  if (!re || !im)
+    return set_error(LSQT_ERR_INVALID, "null output");
+  lsqt_ctx* ctx = left->ctx;
+  LSQT_TRY(ctx_activate(ctx));
+  int tiles = dot_tiles(left->n);
+  LSQT_TRY(ctx_reserve_moments(ctx, 1, tiles));
+  launch_dot(psi->d, left->d, ctx->partial, left->n, ctx->stream);
+  launch_reduce_partials(ctx->partial, (size_t)tiles, tiles, 1, ctx->mu_dev, ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return ctx_download_moments(ctx, 1, re, im);
+}
+
+// ---------------------------------------------------------------------------
+// operators
+// ---------------------------------------------------------------------------
+static int check_ham_vecs(lsqt_ham* h, std::initializer_list<const lsqt_vec*> vs)
+{
+  if (!h)
+    return set_error(LSQT_ERR_INVALID, "null Hamiltonian");
+  for (const lsqt_vec* v : vs) {
+    if (!v)
+      return set_error(LSQT_ERR_INVALID, "null vector");
+    if (v->n != h->d.n || v->ctx != h->ctx)
+      return set_error(LSQT_ERR_INVALID, "vector of length %d does not match the %d-site "
+                       "Hamiltonian", v->n, h->d.n);
+  }
+  return ctx_activate(h->ctx);
+}
+
+extern "C" int lsqt_apply(lsqt_ham* h, const lsqt_vec* in, lsqt_vec* out)
+{
+  LSQT_TRY(check_ham_vecs(h, {in, out}));
+  if (in->d == out->d)
+    return set_error(LSQT_ERR_INVALID, "apply: input and output must differ");
+  launch_hpsi(h->d, in->d, nullptr, out->d, nullptr, nullptr, false, false, h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_apply_commutator(lsqt_ham* h, const lsqt_vec* in, lsqt_vec* out)
+{
+  LSQT_TRY(check_ham_vecs(h, {in, out}));
+  if (in->d == out->d)
+    return set_error(LSQT_ERR_INVALID, "apply_commutator: input and output must differ");
+  launch_xop(h->d, in->d, out->d, false, h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_apply_current(lsqt_ham* h, const lsqt_vec* in, lsqt_vec* out)
+{
+  LSQT_TRY(check_ham_vecs(h, {in, out}));
+  if (in->d == out->d)
+    return set_error(LSQT_ERR_INVALID, "apply_current: input and output must differ");
+  launch_xop(h->d, in->d, out->d, true, h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_kernel_polynomial(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1,
+                                      lsqt_vec* s2)
+{
+  LSQT_TRY(check_ham_vecs(h, {s0, s1, s2}));
+  if (s1->d == s2->d || s1->d == s0->d)
+    return set_error(LSQT_ERR_INVALID, "kernel_polynomial: s1 must differ from s0 and s2");
+  launch_hpsi(h->d, s1->d, s0->d, s2->d, nullptr, nullptr, true, false, h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_chebyshev_01(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1,
+                                 lsqt_vec* s, double b0, double b1, int direction)
+{
+  LSQT_TRY(check_ham_vecs(h, {s0, s1, s}));
+  launch_cheb01(s0->d, s1->d, s->d, b0, b1, direction, h->d.n, h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_chebyshev_2(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1,
+                                lsqt_vec* s2, lsqt_vec* s, double bm, int label)
+{
+  LSQT_TRY(check_ham_vecs(h, {s0, s1, s2, s}));
+  if (label < 1 || label > 4)
+    return set_error(LSQT_ERR_INVALID, "label %d must be 1..4", label);
+  if (s1->d == s2->d || s1->d == s->d || s2->d == s->d)
+    return set_error(LSQT_ERR_INVALID, "chebyshev_2: s1, s2 and s must be distinct");
+  launch_cheb2(h->d, s0->d, s1->d, s2->d, s->d, bm, label, h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_chebyshev_1x(lsqt_ham* h, const lsqt_vec* s1x, lsqt_vec* s, double b1)
+{
+  LSQT_TRY(check_ham_vecs(h, {s1x, s}));
+  launch_cheb1x(s1x->d, s->d, b1, h->d.n, h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_chebyshev_2x(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s0x,
+                                 const lsqt_vec* s1, const lsqt_vec* s1x, lsqt_vec* s2,
+                                 lsqt_vec* s2x, lsqt_vec* s, double bm, int label)
+{
+  LSQT_TRY(check_ham_vecs(h, {s0, s0x, s1, s1x, s2, s2x, s}));
+  if (label < 1 || label > 4)
+    return set_error(LSQT_ERR_INVALID, "label %d must be 1..4", label);
+  launch_cheb2x(h->d, s0->d, s0x->d, s1->d, s1x->d, s2->d, s2x->d, s->d, bm, label,
+                h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fused algorithms
+// ---------------------------------------------------------------------------
+int lsqt::moments_enqueue(lsqt_ham* h, const double2* left, const double2* right, int M)
+{
+  lsqt_ctx* ctx = h->ctx;
+  hipStream_t st = ctx->stream;
+  const int n = h->d.n;
+  const int tiles = hpsi_tiles(n);
+  LSQT_TRY(ctx_reserve_moments(ctx, M, tiles));
+  LSQT_TRY(ham_scratch(h, 2));
+  double2* A = h->scratch[0];
+  double2* B = h->scratch[1];
+  const size_t stride = (size_t)tiles;
+  // psi_0 = right; mu_0 = <left|psi_0>
+  launch_copy(A, right, n, st);
+  launch_dot(A, left, ctx->partial, n, st);
+  // psi_1 = H~ psi_0; mu_1
+  launch_hpsi(h->d, A, nullptr, B, left, ctx->partial + stride, false, true, st);
+  // psi_m = 2 H~ psi_{m-1} - psi_{m-2}, written over psi_{m-2}
+  for (int m = 2; m < M; ++m) {
+    launch_hpsi(h->d, B, A, A, left, ctx->partial + (size_t)m * stride, true, true, st);
+    double2* t = A;
+    A = B;
+    B = t;
+  }
+  launch_reduce_partials(ctx->partial, stride, tiles, M, ctx->mu_dev, st);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_moments(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right, int M,
+                            double* mu_re, double* mu_im)
+{
+  LSQT_TRY(check_ham_vecs(h, {left, right}));
+  if (M < 2)
+    return set_error(LSQT_ERR_INVALID, "number of moments %d must be >= 2", M);
+  if (!mu_re || !mu_im)
+    return set_error(LSQT_ERR_INVALID, "null moment output");
+  LSQT_TRY(moments_enqueue(h, left->d, right->d, M));
+  return ctx_download_moments(h->ctx, M, mu_re, mu_im);
+}
+
+int lsqt::evolve_enqueue(lsqt_ham* h, int direction, double t, double2* state)
+{
+  hipStream_t st = h->ctx->stream;
+  const int n = h->d.n;
+  LSQT_TRY(ham_scratch(h, 2));
+  double2* A = h->scratch[0];  // psi_{m-2}
+  double2* B = h->scratch[1];  // psi_{m-1}
+  BesselTerms bt = evolve_terms(direction, t);
+  launch_copy(A, state, n, st);
+  launch_hpsi(h->d, state, nullptr, B, nullptr, nullptr, false, false, st);
+  launch_cheb01(A, B, state, bt.b0, bt.b1, direction, n, st);
+  for (size_t k = 0; k < bt.bm.size(); ++k) {
+    launch_cheb2(h->d, A, B, A, state, bt.bm[k], bt.label[k], st);
+    double2* tmp = A;
+    A = B;
+    B = tmp;
+  }
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+int lsqt::evolvex_enqueue(lsqt_ham* h, int direction, double t, double2* state)
+{
+  hipStream_t st = h->ctx->stream;
+  const int n = h->d.n;
+  LSQT_TRY(ham_scratch(h, 4));
+  double2 *A = h->scratch[0], *Ax = h->scratch[1];  // psi_{m-2}, [X, .] part
+  double2 *B = h->scratch[2], *Bx = h->scratch[3];  // psi_{m-1}, [X, .] part
+  BesselTerms bt = evolvex_terms(direction, t);
+  launch_copy(A, state, n, st);
+  launch_zero(Ax, n, st);
+  launch_hpsi(h->d, state, nullptr, B, nullptr, nullptr, false, false, st);
+  launch_xop(h->d, state, Bx, false, st);
+  launch_cheb1x(Bx, state, bt.b1, n, st);
+  for (size_t k = 0; k < bt.bm.size(); ++k) {
+    launch_cheb2x(h->d, A, Ax, B, Bx, A, Ax, state, bt.bm[k], bt.label[k], st);
+    double2* tmp = A;
+    A = B;
+    B = tmp;
+    tmp = Ax;
+    Ax = Bx;
+    Bx = tmp;
+  }
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_evolve(lsqt_ham* h, int direction, double t, lsqt_vec* state)
+{
+  LSQT_TRY(check_ham_vecs(h, {state}));
+  if (direction != 1 && direction != -1)
+    return set_error(LSQT_ERR_INVALID, "direction %d must be +1 or -1", direction);
+  return evolve_enqueue(h, direction, t, state->d);
+}
+
+extern "C" int lsqt_evolvex(lsqt_ham* h, int direction, double t, lsqt_vec* state)
+{
+  LSQT_TRY(check_ham_vecs(h, {state}));
+  if (direction != 1 && direction != -1)
+    return set_error(LSQT_ERR_INVALID, "direction %d must be +1 or -1", direction);
+  return evolvex_enqueue(h, direction, t, state->d);
+}
+
+extern "C" int lsqt_bessel_terms(double t, int* last_m)
+{
+  if (!last_m)
+    return set_error(LSQT_ERR_INVALID, "null output");
+  BesselTerms bt = evolve_terms(1, t);
+  *last_m = 1 + (int)bt.bm.size();
+  return LSQT_OK;
+}
+
+void lsqt::ham_release_extra(lsqt_ham* h)
+{
+  (void)h;
+}

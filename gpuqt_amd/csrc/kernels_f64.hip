@@ -1,0 +1,601 @@
+// kernels_f64.hip -- FP64 kernels of the drop-in path for gfx950.
+//
+// Every kernel reproduces the per-site arithmetic of the reference CPU
+// build in the same order -- on-site term first, neighbours in slot order,
+// then the division by energy_max, then the recursion / accumulation
+// (src/hamiltonian.cu:188-221, 273-304, 354-383, 425-443, 531-595, 635-648,
+// 760-852, 927-966) -- and this file is compiled with -ffp-contract=off, so
+// the state vectors are bit-identical to the reference's.
+//
+// Data layout: complex vectors are interleaved double2 (one 16-byte load per
+// neighbour gather); ELL slots are column-major so that slot m of 256
+// consecutive sites is one coalesced load; a packed pair word carries the
+// neighbour index and the index of its (re, im, xx) triple.  One workgroup of
+// 256 threads owns a tile of 256*SPT consecutive sites, tiles are assigned so
+// that each XCD sweeps one contiguous range (neighbour gathers then hit that
+// XCD's L2), and the moment product of a Chebyshev step is fused into the
+// step: per-tile partial sums, reduced by a fixed wave64 shuffle tree, are
+// summed per moment in tile order at the end.  Nothing uses atomics, so
+// every result is run-to-run deterministic.
+#include "lsqt_device.h"
+#include "lsqt_launch.h"
+
+namespace lsqt {
+
+// --- pair access ---------------------------------------------------------
+template <int LAYOUT>
+__device__ __forceinline__ bool load_pair(const HamDev& h, const double2* s_tab, size_t q,
+                                          int& j, double& a, double& b)
+{
+  if constexpr (LAYOUT == kFull) {
+    j = h.nl[q];
+    if (j < 0)
+      return false;
+    a = h.hr[q];
+    b = h.hi[q];
+    return true;
+  } else {
+    uint32_t w = h.words[q];
+    if (w == kEmptySlot)
+      return false;
+    j = (int)(w & h.idx_mask);
+    int c = (int)(w >> h.idx_bits);
+    double2 t = (LAYOUT == kPackedLds) ? s_tab[c] : h.tab[c];
+    a = t.x;
+    b = t.y;
+    return true;
+  }
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ bool load_pair_x(const HamDev& h, const double2* s_tab,
+                                            const double* s_xx, size_t q, int& j, double& a,
+                                            double& b, double& x)
+{
+  if constexpr (LAYOUT == kFull) {
+    j = h.nl[q];
+    if (j < 0)
+      return false;
+    a = h.hr[q];
+    b = h.hi[q];
+    x = h.xx[q];
+    return true;
+  } else {
+    uint32_t w = h.words[q];
+    if (w == kEmptySlot)
+      return false;
+    j = (int)(w & h.idx_mask);
+    int c = (int)(w >> h.idx_bits);
+    if constexpr (LAYOUT == kPackedLds) {
+      double2 t = s_tab[c];
+      a = t.x;
+      b = t.y;
+      x = s_xx[c];
+    } else {
+      double2 t = h.tab[c];
+      a = t.x;
+      b = t.y;
+      x = h.tab_xx[c];
+    }
+    return true;
+  }
+}
+
+template <int LAYOUT, bool WITH_XX>
+__device__ __forceinline__ void stage_table(const HamDev& h, double2* s_tab, double* s_xx)
+{
+  if constexpr (LAYOUT == kPackedLds) {
+    for (int i = threadIdx.x; i < h.ncls; i += kBlock) {
+      s_tab[i] = h.tab[i];
+      if constexpr (WITH_XX)
+        s_xx[i] = h.tab_xx[i];
+    }
+    __syncthreads();
+  }
+}
+
+// --- deterministic workgroup reduction of a complex partial -------------
+__device__ __forceinline__ void block_sum_store(double re, double im, double2* out)
+{
+  __shared__ double s_re[kBlock / 64], s_im[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    re += __shfl_xor(re, off, 64);
+    im += __shfl_xor(im, off, 64);
+  }
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_re[wave] = re;
+    s_im[wave] = im;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = s_re[0], i = s_im[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) {
+      r += s_re[w];
+      i += s_im[w];
+    }
+    *out = make_double2(r, i);
+  }
+}
+
+// --- H~ psi1, optionally 2 H~ psi1 - psi0, optionally the moment product --
+// RECUR: out = 2 H~ psi1 - psi0 (kernel_polynomial) else out = H~ psi1 (apply)
+// IP:    partial[tile] = sum over the tile of conj(L) out
+// out may alias psi0 (each site reads its own psi0 before writing out).
+template <int LAYOUT, int KT, int SPT, bool RECUR, bool IP>
+__global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __restrict__ psi1,
+                                                 const double2* psi0, double2* out,
+                                                 const double2* __restrict__ L,
+                                                 double2* __restrict__ partial, int tiles,
+                                                 int blocks)
+{
+  __shared__ double2 s_tab[LAYOUT == kPackedLds ? kLdsTableMax : 1];
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  stage_table<LAYOUT, false>(h, s_tab, nullptr);
+  const int K = KT > 0 ? KT : h.K;
+  double acc_re = 0.0, acc_im = 0.0;
+#pragma unroll
+  for (int k = 0; k < SPT; ++k) {
+    int n = (tile * SPT + k) * kBlock + threadIdx.x;
+    if (n < h.n) {
+      double2 p1 = psi1[n];
+      double v = h.pot[n];
+      double tr = v * p1.x;
+      double ti = v * p1.y;
+#pragma unroll
+      for (int m = 0; m < K; ++m) {
+        int j;
+        double a, b;
+        if (load_pair<LAYOUT>(h, s_tab, (size_t)m * h.n + n, j, a, b)) {
+          double2 c = psi1[j];
+          tr += a * c.x - b * c.y;
+          ti += a * c.y + b * c.x;
+        }
+      }
+      tr /= h.emax;
+      ti /= h.emax;
+      if constexpr (RECUR) {
+        double2 p0 = psi0[n];
+        tr = 2.0 * tr - p0.x;
+        ti = 2.0 * ti - p0.y;
+      }
+      out[n] = make_double2(tr, ti);
+      if constexpr (IP) {
+        double2 l = L[n];
+        acc_re += tr * l.x + ti * l.y;
+        acc_im += ti * l.x - tr * l.y;
+      }
+    }
+  }
+  if constexpr (IP)
+    block_sum_store(acc_re, acc_im, partial + tile);
+}
+
+// --- [X, H~] psi and V psi ----------------------------------------------
+template <int LAYOUT, int KT, bool CURRENT>
+__global__ __launch_bounds__(kBlock) void k_xop(HamDev h, const double2* __restrict__ in,
+                                                double2* __restrict__ out, int tiles,
+                                                int blocks)
+{
+  __shared__ double2 s_tab[LAYOUT == kPackedLds ? kLdsTableMax : 1];
+  __shared__ double s_xx[LAYOUT == kPackedLds ? kLdsTableMax : 1];
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  stage_table<LAYOUT, true>(h, s_tab, s_xx);
+  const int K = KT > 0 ? KT : h.K;
+  int n = tile * kBlock + threadIdx.x;
+  if (n >= h.n)
+    return;
+  double tr = 0.0, ti = 0.0;
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    int j;
+    double a, b, x;
+    if (load_pair_x<LAYOUT>(h, s_tab, s_xx, (size_t)m * h.n + n, j, a, b, x)) {
+      double2 c = in[j];
+      if constexpr (CURRENT) {
+        tr += (a * c.x - b * c.y) * x;
+        ti += (a * c.y + b * c.x) * x;
+      } else {
+        tr -= (a * c.x - b * c.y) * x;
+        ti -= (a * c.y + b * c.x) * x;
+      }
+    }
+  }
+  if constexpr (CURRENT)
+    out[n] = make_double2(+ti, -tr);
+  else
+    out[n] = make_double2(tr / h.emax, ti / h.emax);
+}
+
+// the four (-i)^m phase cases of src/hamiltonian.cu:504-525
+__device__ __forceinline__ void accumulate_label(double2& s, double bm, double tr, double ti,
+                                                 int label)
+{
+  switch (label) {
+    case 1:
+      s.x += bm * tr;
+      s.y += bm * ti;
+      break;
+    case 2:
+      s.x -= bm * tr;
+      s.y -= bm * ti;
+      break;
+    case 3:
+      s.x += bm * ti;
+      s.y -= bm * tr;
+      break;
+    case 4:
+      s.x -= bm * ti;
+      s.y += bm * tr;
+      break;
+  }
+}
+
+// --- one term of U(t): s2 = 2 H~ s1 - s0, s += b (phase) s2 -------------
+template <int LAYOUT, int KT>
+__global__ __launch_bounds__(kBlock) void k_cheb2(HamDev h, const double2* s0,
+                                                  const double2* __restrict__ s1, double2* s2,
+                                                  double2* __restrict__ s, double bm, int label,
+                                                  int tiles, int blocks)
+{
+  __shared__ double2 s_tab[LAYOUT == kPackedLds ? kLdsTableMax : 1];
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  stage_table<LAYOUT, false>(h, s_tab, nullptr);
+  const int K = KT > 0 ? KT : h.K;
+  int n = tile * kBlock + threadIdx.x;
+  if (n >= h.n)
+    return;
+  double2 p1 = s1[n];
+  double v = h.pot[n];
+  double tr = v * p1.x;
+  double ti = v * p1.y;
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    int j;
+    double a, b;
+    if (load_pair<LAYOUT>(h, s_tab, (size_t)m * h.n + n, j, a, b)) {
+      double2 c = s1[j];
+      tr += a * c.x - b * c.y;
+      ti += a * c.y + b * c.x;
+    }
+  }
+  tr /= h.emax;
+  ti /= h.emax;
+  double2 p0 = s0[n];
+  tr = 2.0 * tr - p0.x;
+  ti = 2.0 * ti - p0.y;
+  double2 st = s[n];
+  accumulate_label(st, bm, tr, ti, label);
+  s[n] = st;
+  s2[n] = make_double2(tr, ti);
+}
+
+// --- one term of [X, U(t)]: coupled recursion of s and sx ----------------
+template <int LAYOUT, int KT>
+__global__ __launch_bounds__(kBlock) void k_cheb2x(HamDev h, const double2* s0,
+                                                   const double2* s0x,
+                                                   const double2* __restrict__ s1,
+                                                   const double2* __restrict__ s1x, double2* s2,
+                                                   double2* s2x, double2* __restrict__ s,
+                                                   double bm, int label, int tiles, int blocks)
+{
+  __shared__ double2 s_tab[LAYOUT == kPackedLds ? kLdsTableMax : 1];
+  __shared__ double s_xx[LAYOUT == kPackedLds ? kLdsTableMax : 1];
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  stage_table<LAYOUT, true>(h, s_tab, s_xx);
+  const int K = KT > 0 ? KT : h.K;
+  int n = tile * kBlock + threadIdx.x;
+  if (n >= h.n)
+    return;
+  double v = h.pot[n];
+  double2 p1 = s1[n], p1x = s1x[n];
+  double tr = v * p1.x, ti = v * p1.y;
+  double txr = v * p1x.x, txi = v * p1x.y;
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    int j;
+    double a, b, x;
+    if (load_pair_x<LAYOUT>(h, s_tab, s_xx, (size_t)m * h.n + n, j, a, b, x)) {
+      double2 c = s1[j], cx = s1x[j];
+      tr += a * c.x - b * c.y;
+      ti += a * c.y + b * c.x;
+      txr += a * cx.x - b * cx.y;
+      txi += a * cx.y + b * cx.x;
+      txr -= (a * c.x - b * c.y) * x;
+      txi -= (a * c.y + b * c.x) * x;
+    }
+  }
+  tr /= h.emax;
+  ti /= h.emax;
+  double2 p0 = s0[n];
+  tr = 2.0 * tr - p0.x;
+  ti = 2.0 * ti - p0.y;
+  s2[n] = make_double2(tr, ti);
+  txr /= h.emax;
+  txi /= h.emax;
+  double2 p0x = s0x[n];
+  txr = 2.0 * txr - p0x.x;
+  txi = 2.0 * txi - p0x.y;
+  s2x[n] = make_double2(txr, txi);
+  double2 st = s[n];
+  accumulate_label(st, bm, txr, txi, label);
+  s[n] = st;
+}
+
+// --- element-wise kernels -------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_zero(double2* v, int n)
+{
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n)
+    v[i] = make_double2(0.0, 0.0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(double2* __restrict__ dst,
+                                                 const double2* __restrict__ src, int n)
+{
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n)
+    dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_add(double2* __restrict__ dst,
+                                                const double2* __restrict__ src, int n)
+{
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    double2 d = dst[i], s = src[i];
+    d.x += s.x;
+    d.y += s.y;
+    dst[i] = d;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sz(double2* __restrict__ dst,
+                                               const double2* __restrict__ src, int n)
+{
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    double2 s = src[i];
+    dst[i] = (i % 2 == 0) ? s : make_double2(-s.x, -s.y);
+  }
+}
+
+// state = b0 s0 + b1 dir (Im s1, -Re s1)   (src/hamiltonian.cu:404-423)
+__global__ __launch_bounds__(kBlock) void k_cheb01(const double2* __restrict__ s0,
+                                                   const double2* __restrict__ s1,
+                                                   double2* __restrict__ s, double b0,
+                                                   double b1, int direction, int n)
+{
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    double bessel_1 = b1 * direction;
+    double2 a = s0[i], c = s1[i];
+    s[i] = make_double2(b0 * a.x + bessel_1 * c.y, b0 * a.y - bessel_1 * c.x);
+  }
+}
+
+// state = b1 (Im s1x, -Re s1x)   (src/hamiltonian.cu:619-633)
+__global__ __launch_bounds__(kBlock) void k_cheb1x(const double2* __restrict__ s1x,
+                                                   double2* __restrict__ s, double b1, int n)
+{
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    double2 c = s1x[i];
+    s[i] = make_double2(+b1 * c.y, -b1 * c.x);
+  }
+}
+
+// partial[tile] = sum over the tile of conj(L) psi
+__global__ __launch_bounds__(kBlock) void k_dot(const double2* __restrict__ psi,
+                                                const double2* __restrict__ L,
+                                                double2* __restrict__ partial, int n, int tiles,
+                                                int blocks)
+{
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  double re = 0.0, im = 0.0;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    int i = (tile * 2 + k) * kBlock + threadIdx.x;
+    if (i < n) {
+      double2 p = psi[i], l = L[i];
+      re += p.x * l.x + p.y * l.y;
+      im += p.y * l.x - p.x * l.y;
+    }
+  }
+  block_sum_store(re, im, partial + tile);
+}
+
+// mu[m] = sum_t partial[m * stride + t], t = 0..tiles-1, in a fixed order
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double2* __restrict__ partial,
+                                                            size_t stride, int tiles,
+                                                            double2* __restrict__ mu)
+{
+  const double2* p = partial + (size_t)blockIdx.x * stride;
+  double re = 0.0, im = 0.0;
+  for (int t = threadIdx.x; t < tiles; t += kBlock) {
+    double2 v = p[t];
+    re += v.x;
+    im += v.y;
+  }
+  block_sum_store(re, im, mu + blockIdx.x);
+}
+
+// ==========================================================================
+// launchers
+// ==========================================================================
+namespace {
+
+inline int blocks_1d(int n)
+{
+  return (n + kBlock - 1) / kBlock;
+}
+
+}  // namespace
+
+#define LSQT_LAYOUT_SWITCH(layout, L, ...)        \
+  switch (layout) {                               \
+    case kFull: {                                 \
+      constexpr int L = kFull;                    \
+      __VA_ARGS__;                                \
+    } break;                                      \
+    case kPackedLds: {                            \
+      constexpr int L = kPackedLds;               \
+      __VA_ARGS__;                                \
+    } break;                                      \
+    default: {                                    \
+      constexpr int L = kPackedGlobal;            \
+      __VA_ARGS__;                                \
+    } break;                                      \
+  }
+
+#define LSQT_K_SWITCH(kval, KT, ...) \
+  switch (kval) {                    \
+    case 2: {                        \
+      constexpr int KT = 2;          \
+      __VA_ARGS__;                   \
+    } break;                         \
+    case 3: {                        \
+      constexpr int KT = 3;          \
+      __VA_ARGS__;                   \
+    } break;                         \
+    case 4: {                        \
+      constexpr int KT = 4;          \
+      __VA_ARGS__;                   \
+    } break;                         \
+    case 6: {                        \
+      constexpr int KT = 6;          \
+      __VA_ARGS__;                   \
+    } break;                         \
+    default: {                       \
+      constexpr int KT = 0;          \
+      __VA_ARGS__;                   \
+    } break;                         \
+  }
+
+constexpr int kSptHpsi = 2;
+
+void launch_hpsi(const HamDev& h, const double2* psi1, const double2* psi0, double2* out,
+                 const double2* L, double2* partial, bool recur, bool ip, hipStream_t st)
+{
+  Grid g = make_grid(h.n, kSptHpsi);
+  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, {
+    if (recur && ip)
+      k_hpsi<LAY, KT, kSptHpsi, true, true>
+        <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+    else if (recur)
+      k_hpsi<LAY, KT, kSptHpsi, true, false>
+        <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+    else if (ip)
+      k_hpsi<LAY, KT, kSptHpsi, false, true>
+        <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+    else
+      k_hpsi<LAY, KT, kSptHpsi, false, false>
+        <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+  }))
+}
+
+int hpsi_tiles(int n)
+{
+  return make_grid(n, kSptHpsi).tiles;
+}
+
+void launch_xop(const HamDev& h, const double2* in, double2* out, bool current, hipStream_t st)
+{
+  Grid g = make_grid(h.n, 1);
+  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, {
+    if (current)
+      k_xop<LAY, KT, true><<<g.blocks, kBlock, 0, st>>>(h, in, out, g.tiles, g.blocks);
+    else
+      k_xop<LAY, KT, false><<<g.blocks, kBlock, 0, st>>>(h, in, out, g.tiles, g.blocks);
+  }))
+}
+
+void launch_cheb2(const HamDev& h, const double2* s0, const double2* s1, double2* s2,
+                  double2* s, double bm, int label, hipStream_t st)
+{
+  Grid g = make_grid(h.n, 1);
+  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, {
+    k_cheb2<LAY, KT><<<g.blocks, kBlock, 0, st>>>(h, s0, s1, s2, s, bm, label, g.tiles,
+                                                    g.blocks);
+  }))
+}
+
+void launch_cheb2x(const HamDev& h, const double2* s0, const double2* s0x, const double2* s1,
+                   const double2* s1x, double2* s2, double2* s2x, double2* s, double bm,
+                   int label, hipStream_t st)
+{
+  Grid g = make_grid(h.n, 1);
+  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, {
+    k_cheb2x<LAY, KT><<<g.blocks, kBlock, 0, st>>>(h, s0, s0x, s1, s1x, s2, s2x, s, bm, label,
+                                                     g.tiles, g.blocks);
+  }))
+}
+
+void launch_zero(double2* v, int n, hipStream_t st)
+{
+  if (n > 0)
+    k_zero<<<blocks_1d(n), kBlock, 0, st>>>(v, n);
+}
+
+void launch_copy(double2* dst, const double2* src, int n, hipStream_t st)
+{
+  if (n > 0)
+    k_copy<<<blocks_1d(n), kBlock, 0, st>>>(dst, src, n);
+}
+
+void launch_add(double2* dst, const double2* src, int n, hipStream_t st)
+{
+  if (n > 0)
+    k_add<<<blocks_1d(n), kBlock, 0, st>>>(dst, src, n);
+}
+
+void launch_sz(double2* dst, const double2* src, int n, hipStream_t st)
+{
+  if (n > 0)
+    k_sz<<<blocks_1d(n), kBlock, 0, st>>>(dst, src, n);
+}
+
+void launch_cheb01(const double2* s0, const double2* s1, double2* s, double b0, double b1,
+                   int direction, int n, hipStream_t st)
+{
+  if (n > 0)
+    k_cheb01<<<blocks_1d(n), kBlock, 0, st>>>(s0, s1, s, b0, b1, direction, n);
+}
+
+void launch_cheb1x(const double2* s1x, double2* s, double b1, int n, hipStream_t st)
+{
+  if (n > 0)
+    k_cheb1x<<<blocks_1d(n), kBlock, 0, st>>>(s1x, s, b1, n);
+}
+
+int dot_tiles(int n)
+{
+  return make_grid(n, 2).tiles;
+}
+
+void launch_dot(const double2* psi, const double2* L, double2* partial, int n, hipStream_t st)
+{
+  Grid g = make_grid(n, 2);
+  k_dot<<<g.blocks, kBlock, 0, st>>>(psi, L, partial, n, g.tiles, g.blocks);
+}
+
+void launch_reduce_partials(const double2* partial, size_t stride, int tiles, int count,
+                            double2* mu, hipStream_t st)
+{
+  if (count > 0)
+    k_reduce_partials<<<count, kBlock, 0, st>>>(partial, stride, tiles, mu);
+}
+
+}  // namespace lsqt

@@ -1,0 +1,76 @@
+// lsqt_capi_internal.h -- handle structs and helpers behind the C ABI
+// (internal to liblsqt_hip.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "lsqt_device.h"
+#include "lsqt_hip.h"
+
+struct lsqt_ctx {
+  int device = 0;
+  int cu_count = 0;
+  hipStream_t stream = nullptr;
+  // moment partial sums [M][tiles], moments [M] on device, pinned host copy
+  double2* partial = nullptr;
+  size_t partial_cap = 0;
+  double2* mu_dev = nullptr;
+  size_t mu_cap = 0;
+  double2* mu_host = nullptr;
+  size_t mu_host_cap = 0;
+};
+
+struct lsqt_f32_state;  // FP32 / block-path device data (kernels_f32.hip)
+
+struct lsqt_ham {
+  lsqt_ctx* ctx = nullptr;
+  lsqt::HamDev d{};
+  std::vector<void*> allocations;
+  double2* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
+  // host copies kept for building the FP32 data on demand
+  std::vector<double> host_pot;
+  std::vector<double2> host_tab;
+  std::vector<double> host_tab_xx;
+  lsqt_f32_state* f32 = nullptr;
+};
+
+struct lsqt_vec {
+  lsqt_ctx* ctx = nullptr;
+  int n = 0;
+  double2* d = nullptr;
+};
+
+namespace lsqt {
+
+constexpr long long kPackedMaxClasses = 4096;
+
+int set_error(int code, const char* fmt, ...);
+int ctx_activate(lsqt_ctx* ctx);
+int ensure_device(void** p, size_t* cap, size_t bytes);
+int ensure_pinned(void** p, size_t* cap, size_t bytes);
+int ctx_reserve_moments(lsqt_ctx* ctx, int M, int tiles);
+int ctx_download_moments(lsqt_ctx* ctx, int M, double* mu_re, double* mu_im);
+int ham_scratch(lsqt_ham* h, int count);
+void ham_release_extra(lsqt_ham* h);
+int moments_enqueue(lsqt_ham* h, const double2* left, const double2* right, int M);
+int evolve_enqueue(lsqt_ham* h, int direction, double t, double2* state);
+int evolvex_enqueue(lsqt_ham* h, int direction, double t, double2* state);
+
+}  // namespace lsqt
+
+#define LSQT_HIP(call)                                                                   \
+  do {                                                                                   \
+    hipError_t _e = (call);                                                              \
+    if (_e != hipSuccess)                                                                \
+      return lsqt::set_error(LSQT_ERR_HIP, "%s failed: %s (%s:%d)", #call,               \
+                             hipGetErrorString(_e), __FILE__, __LINE__);                 \
+  } while (0)
+
+#define LSQT_TRY(call)       \
+  do {                       \
+    int _rc = (call);        \
+    if (_rc != LSQT_OK)      \
+      return _rc;            \
+  } while (0)

@@ -1,0 +1,84 @@
+// lsqt_device.h -- internal types shared by the device sources of
+// liblsqt_hip.so.  Not part of the ABI (that is include/lsqt_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace lsqt {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kXcds = 8;             // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;  // packed word of an unused ELL slot
+constexpr int kLdsTableMax = 64;     // packed tables up to this size are staged in LDS
+
+enum Layout { kFull = 0, kPackedLds = 1, kPackedGlobal = 2 };
+
+// Device view of a Hamiltonian.  ELL slots are column-major: slot m of site
+// n is at m*n_sites + n.  Unused slots are marked (kEmptySlot / index -1)
+// and always come after the used ones, as in the reference's row-major
+// arrays (src/model_lattice.cu:356-383).
+struct HamDev {
+  int n;
+  int K;
+  double emax;
+  const double* pot;
+  // packed layout
+  const uint32_t* words;
+  uint32_t idx_mask;
+  int idx_bits;
+  const double2* tab;     // (re, im) per class
+  const double* tab_xx;   // xx per class
+  int ncls;
+  // full layout
+  const int* nl;
+  const double* hr;
+  const double* hi;
+  const double* xx;
+  int layout;
+};
+
+// FP32 copies of the per-site / per-class data (the precision path)
+struct HamDevF32 {
+  int n;
+  int K;
+  float emax;
+  float inv_emax;
+  const float* pot;
+  const uint32_t* words;
+  uint32_t idx_mask;
+  int idx_bits;
+  const float2* tab;
+  const float* tab_xx;
+  int ncls;
+  const int* nl;
+  const float2* hop;   // full layout: (re, im) per pair, column-major
+  const float* xx;
+  int layout;
+};
+
+// Grid shape of a one-tile-per-workgroup launch: `tiles` tiles of
+// kBlock*spt sites, padded to a multiple of the XCD count so that the
+// round-robin dispatch gives every XCD one contiguous range of tiles.
+struct Grid {
+  int tiles;
+  int blocks;
+};
+
+inline Grid make_grid(int n, int spt)
+{
+  int per_tile = kBlock * spt;
+  int tiles = (n + per_tile - 1) / per_tile;
+  int blocks = (tiles + kXcds - 1) / kXcds * kXcds;
+  return Grid{tiles, blocks};
+}
+
+// workgroup b runs on XCD b % 8; map it to tile (b % 8) * (blocks/8) + b / 8
+__device__ __forceinline__ int xcd_tile(int b, int blocks)
+{
+  return (b % kXcds) * (blocks / kXcds) + b / kXcds;
+}
+
+}  // namespace lsqt

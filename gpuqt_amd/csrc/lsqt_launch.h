@@ -1,0 +1,30 @@
+// lsqt_launch.h -- host-side launchers of the device kernels (internal).
+#pragma once
+
+#include "lsqt_device.h"
+
+namespace lsqt {
+
+// FP64 drop-in path (kernels_f64.hip)
+int hpsi_tiles(int n);
+void launch_hpsi(const HamDev& h, const double2* psi1, const double2* psi0, double2* out,
+                 const double2* L, double2* partial, bool recur, bool ip, hipStream_t st);
+void launch_xop(const HamDev& h, const double2* in, double2* out, bool current, hipStream_t st);
+void launch_cheb2(const HamDev& h, const double2* s0, const double2* s1, double2* s2,
+                  double2* s, double bm, int label, hipStream_t st);
+void launch_cheb2x(const HamDev& h, const double2* s0, const double2* s0x, const double2* s1,
+                   const double2* s1x, double2* s2, double2* s2x, double2* s, double bm,
+                   int label, hipStream_t st);
+void launch_zero(double2* v, int n, hipStream_t st);
+void launch_copy(double2* dst, const double2* src, int n, hipStream_t st);
+void launch_add(double2* dst, const double2* src, int n, hipStream_t st);
+void launch_sz(double2* dst, const double2* src, int n, hipStream_t st);
+void launch_cheb01(const double2* s0, const double2* s1, double2* s, double b0, double b1,
+                   int direction, int n, hipStream_t st);
+void launch_cheb1x(const double2* s1x, double2* s, double b1, int n, hipStream_t st);
+int dot_tiles(int n);
+void launch_dot(const double2* psi, const double2* L, double2* partial, int n, hipStream_t st);
+void launch_reduce_partials(const double2* partial, size_t stride, int tiles, int count,
+                            double2* mu, hipStream_t st);
+
+}  // namespace lsqt

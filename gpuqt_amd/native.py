@@ -1,0 +1,284 @@
+"""ctypes binding of liblsqt_hip.so (the C ABI of include/lsqt_hip.h).
+
+This is the Python face of the product path: every call goes through the
+HIP library.  There is no fallback -- if the library is missing or the GPU
+cannot be used, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblsqt_hip.so")
+
+LAYOUT_AUTO, LAYOUT_FULL, LAYOUT_PACKED = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class HamInfo(C.Structure):
+    _fields_ = [("n", C.c_int), ("max_neighbor", C.c_int), ("layout", C.c_int),
+                ("classes", C.c_int), ("index_bits", C.c_int), ("energy_max", C.c_double),
+                ("bytes_per_site_f64", C.c_double)]
+
+
+# every symbol declared in include/lsqt_hip.h, with (restype, argtypes)
+SIGNATURES = {
+    "lsqt_strerror": (C.c_char_p, [C.c_int]),
+    "lsqt_last_error": (C.c_char_p, []),
+    "lsqt_version": (C.c_char_p, []),
+    "lsqt_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "lsqt_ctx_destroy": (C.c_int, [_vp]),
+    "lsqt_ctx_sync": (C.c_int, [_vp]),
+    "lsqt_ctx_stream": (_vp, [_vp]),
+    "lsqt_device_count": (C.c_int, [_ip]),
+    "lsqt_ham_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp,
+                                  C.c_double, C.c_int, C.POINTER(_vp)]),
+    "lsqt_ham_destroy": (C.c_int, [_vp]),
+    "lsqt_ham_get_info": (C.c_int, [_vp, C.POINTER(HamInfo)]),
+    "lsqt_vec_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "lsqt_vec_clone": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "lsqt_vec_destroy": (C.c_int, [_vp]),
+    "lsqt_vec_size": (C.c_int, [_vp, _ip]),
+    "lsqt_vec_from_host": (C.c_int, [_vp, _dp, _dp]),
+    "lsqt_vec_to_host": (C.c_int, [_vp, _dp, _dp]),
+    "lsqt_vec_copy": (C.c_int, [_vp, _vp]),
+    "lsqt_vec_add": (C.c_int, [_vp, _vp]),
+    "lsqt_vec_apply_sz": (C.c_int, [_vp, _vp]),
+    "lsqt_vec_swap": (C.c_int, [_vp, _vp]),
+    "lsqt_vec_dot": (C.c_int, [_vp, _vp, _dp, _dp]),
+    "lsqt_apply": (C.c_int, [_vp, _vp, _vp]),
+    "lsqt_apply_commutator": (C.c_int, [_vp, _vp, _vp]),
+    "lsqt_apply_current": (C.c_int, [_vp, _vp, _vp]),
+    "lsqt_kernel_polynomial": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "lsqt_chebyshev_01": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int]),
+    "lsqt_chebyshev_2": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int]),
+    "lsqt_chebyshev_1x": (C.c_int, [_vp, _vp, _vp, C.c_double]),
+    "lsqt_chebyshev_2x": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
+                                    C.c_int]),
+    "lsqt_moments": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp, _dp]),
+    "lsqt_evolve": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
+    "lsqt_evolvex": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
+    "lsqt_bessel_terms": (C.c_int, [C.c_double, _ip]),
+}
+
+_lib = None
+
+
+class LsqtError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads liblsqt_hip.so; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LsqtError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        lib = load()
+        raise LsqtError(f"{lib.lsqt_strerror(rc).decode()}: {lib.lsqt_last_error().decode()}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = load().lsqt_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = _vp()
+        _check(load().lsqt_ctx_create(device, C.byref(self.h)))
+
+    @property
+    def stream(self):
+        return load().lsqt_ctx_stream(self.h)
+
+    def sync(self):
+        _check(load().lsqt_ctx_sync(self.h))
+
+    def close(self):
+        if self.h:
+            load().lsqt_ctx_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Vector:
+    """Vector (src/vector.h:23-47) on the device."""
+
+    def __init__(self, ctx: Context, n: int, _handle=None):
+        self.ctx = ctx
+        self.n = n
+        if _handle is not None:
+            self.h = _handle
+        else:
+            self.h = _vp()
+            _check(load().lsqt_vec_create(ctx.h, n, C.byref(self.h)))
+
+    @classmethod
+    def from_host(cls, ctx, re, im):
+        v = cls(ctx, len(re))
+        v.upload(re, im)
+        return v
+
+    def upload(self, re, im):
+        re = np.ascontiguousarray(re, dtype=np.float64)
+        im = np.ascontiguousarray(im, dtype=np.float64)
+        assert re.shape == (self.n,) and im.shape == (self.n,)
+        _check(load().lsqt_vec_from_host(self.h, _dptr(re), _dptr(im)))
+
+    def download(self):
+        re, im = np.empty(self.n), np.empty(self.n)
+        _check(load().lsqt_vec_to_host(self.h, _dptr(re), _dptr(im)))
+        return re, im
+
+    def clone(self):
+        h = _vp()
+        _check(load().lsqt_vec_clone(self.h, C.byref(h)))
+        return Vector(self.ctx, self.n, _handle=h)
+
+    def copy_from(self, other):
+        _check(load().lsqt_vec_copy(self.h, other.h))
+
+    def add(self, other):
+        _check(load().lsqt_vec_add(self.h, other.h))
+
+    def apply_sz(self, other):
+        _check(load().lsqt_vec_apply_sz(self.h, other.h))
+
+    def swap(self, other):
+        _check(load().lsqt_vec_swap(self.h, other.h))
+
+    def dot(self, psi):
+        re, im = C.c_double(), C.c_double()
+        _check(load().lsqt_vec_dot(self.h, psi.h, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+    def close(self):
+        if self.h:
+            load().lsqt_vec_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Hamiltonian:
+    """Hamiltonian (src/hamiltonian.h:25-54) on the device, from row-major ELL arrays."""
+
+    def __init__(self, ctx: Context, n, max_neighbor, neighbor_number, neighbor_list, potential,
+                 hopping_real, hopping_imag, xx, energy_max, layout=LAYOUT_AUTO):
+        self.ctx = ctx
+        self.n = int(n)
+        nn = np.ascontiguousarray(neighbor_number, dtype=np.int32)
+        nl = np.ascontiguousarray(neighbor_list, dtype=np.int32)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64)
+                for a in (potential, hopping_real, hopping_imag, xx)]
+        assert nn.shape == (n,) and nl.shape == (n * max_neighbor,)
+        self.h = _vp()
+        _check(load().lsqt_ham_create(ctx.h, int(n), int(max_neighbor), _iptr(nn), _iptr(nl),
+                                      *(_dptr(a) for a in arrs), float(energy_max), int(layout),
+                                      C.byref(self.h)))
+
+    @classmethod
+    def from_arrays(cls, ctx, arrays, n, max_neighbor, energy_max, layout=LAYOUT_AUTO):
+        return cls(ctx, n, max_neighbor, arrays["neighbor_number"], arrays["neighbor_list"],
+                   arrays["potential"], arrays["hopping_real"], arrays["hopping_imag"],
+                   arrays["xx"], energy_max, layout)
+
+    def info(self) -> HamInfo:
+        inf = HamInfo()
+        _check(load().lsqt_ham_get_info(self.h, C.byref(inf)))
+        return inf
+
+    def vector(self, re=None, im=None):
+        v = Vector(self.ctx, self.n)
+        if re is not None:
+            v.upload(re, im)
+        return v
+
+    def apply(self, vin, vout):
+        _check(load().lsqt_apply(self.h, vin.h, vout.h))
+
+    def apply_commutator(self, vin, vout):
+        _check(load().lsqt_apply_commutator(self.h, vin.h, vout.h))
+
+    def apply_current(self, vin, vout):
+        _check(load().lsqt_apply_current(self.h, vin.h, vout.h))
+
+    def kernel_polynomial(self, s0, s1, s2):
+        _check(load().lsqt_kernel_polynomial(self.h, s0.h, s1.h, s2.h))
+
+    def chebyshev_01(self, s0, s1, s, b0, b1, direction):
+        _check(load().lsqt_chebyshev_01(self.h, s0.h, s1.h, s.h, b0, b1, direction))
+
+    def chebyshev_2(self, s0, s1, s2, s, bm, label):
+        _check(load().lsqt_chebyshev_2(self.h, s0.h, s1.h, s2.h, s.h, bm, label))
+
+    def chebyshev_1x(self, s1x, s, b1):
+        _check(load().lsqt_chebyshev_1x(self.h, s1x.h, s.h, b1))
+
+    def chebyshev_2x(self, s0, s0x, s1, s1x, s2, s2x, s, bm, label):
+        _check(load().lsqt_chebyshev_2x(self.h, s0.h, s0x.h, s1.h, s1x.h, s2.h, s2x.h, s.h, bm,
+                                        label))
+
+    def moments(self, left, right, M):
+        re, im = np.empty(M), np.empty(M)
+        _check(load().lsqt_moments(self.h, left.h, right.h, int(M), _dptr(re), _dptr(im)))
+        return re + 1j * im
+
+    def evolve(self, direction, t_scaled, state):
+        _check(load().lsqt_evolve(self.h, int(direction), float(t_scaled), state.h))
+
+    def evolvex(self, direction, t_scaled, state):
+        _check(load().lsqt_evolvex(self.h, int(direction), float(t_scaled), state.h))
+
+    def close(self):
+        if self.h:
+            load().lsqt_ham_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bessel_last_term(t_scaled: float) -> int:
+    m = C.c_int()
+    _check(load().lsqt_bessel_terms(float(t_scaled), C.byref(m)))
+    return m.value

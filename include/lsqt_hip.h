@@ -1,0 +1,171 @@
+/*
+ * lsqt_hip.h -- C ABI of the MI355X (gfx950) LSQT device library,
+ * liblsqt_hip.so.
+ *
+ * The reference has no FFI: its boundary is the C++ classes Hamiltonian
+ * (src/hamiltonian.h:25-54), Vector (src/vector.h:23-47) and the algorithm
+ * functions of src/sigma.cu.  Each entry point below replaces one member or
+ * function of that interface (cited per entry point); the C++ host classes
+ * in gpuqt_amd/host/ keep the reference's names on top of it.
+ *
+ * Conventions
+ *   - Every call returns an int status (LSQT_OK == 0).  The library never
+ *     exits the process; lsqt_last_error() gives a message for the calling
+ *     thread.  (The reference prints and exit(1)s on any CUDA error,
+ *     src/common.h:31-42; the host wrapper keeps that behaviour.)
+ *   - Plain pointers and sizes only.  Host arrays are caller-owned and never
+ *     freed by the library; device buffers live in opaque handles released
+ *     by the matching *_destroy.
+ *   - Complex vectors cross the ABI as split real/imag host arrays, as the
+ *     reference's Vector::copy_from_host/copy_to_host do; on the device they
+ *     are stored interleaved (re, im) so a neighbour gather is one load.
+ *   - One HIP stream per context; calls on a context are ordered and not
+ *     thread-safe.  Calls that return host data (vector download, moments)
+ *     synchronise the stream; everything else is asynchronous.
+ *   - FP64 entry points reproduce the reference CPU build's per-site
+ *     arithmetic in the same operation order without contraction, so state
+ *     vectors are bit-identical to it; only moment sums differ, by their
+ *     reduction order.
+ */
+#ifndef LSQT_HIP_H
+#define LSQT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LSQT_OK = 0,
+  LSQT_ERR_INVALID = 1,  /* bad argument, size mismatch, null handle */
+  LSQT_ERR_HIP = 2,      /* a HIP runtime call failed */
+  LSQT_ERR_NOMEM = 3,    /* device or host allocation failed */
+  LSQT_ERR_NODEVICE = 4, /* no usable gfx950 device */
+  LSQT_ERR_UNSUPPORTED = 5
+};
+
+/* Hamiltonian storage layouts on the device (lsqt_ham_create layout_hint) */
+enum {
+  LSQT_LAYOUT_AUTO = 0,   /* packed when the pair values have few distinct
+                             triples, full otherwise */
+  LSQT_LAYOUT_FULL = 1,   /* column-major int32 index + FP64 re/im/xx per pair */
+  LSQT_LAYOUT_PACKED = 2  /* column-major 32-bit word per pair: neighbour
+                             index in the low bits, index of the pair's
+                             (re, im, xx) triple in a small table above it */
+};
+
+/* Operators for lsqt_apply_op */
+enum { LSQT_OP_H = 0, LSQT_OP_COMMUTATOR = 1, LSQT_OP_CURRENT = 2 };
+
+typedef struct lsqt_ctx lsqt_ctx;
+typedef struct lsqt_ham lsqt_ham;
+typedef struct lsqt_vec lsqt_vec;
+
+typedef struct {
+  int n;              /* sites */
+  int max_neighbor;   /* slots per site */
+  int layout;         /* LSQT_LAYOUT_FULL or LSQT_LAYOUT_PACKED */
+  int classes;        /* distinct (re, im, xx) triples (packed layout) */
+  int index_bits;     /* neighbour-index bits of a packed word */
+  double energy_max;
+  double bytes_per_site_f64; /* compulsory HBM bytes of one FP64 Chebyshev
+                                site-update with this layout (fused moment
+                                product included) */
+} lsqt_ham_info;
+
+const char* lsqt_strerror(int status);
+const char* lsqt_last_error(void);
+/* library version string (build identity) */
+const char* lsqt_version(void);
+
+/* ---- context: device + stream --------------------------------------------
+ * (no counterpart: the reference uses the CUDA default device and stream) */
+int lsqt_ctx_create(int device, lsqt_ctx** out);
+int lsqt_ctx_destroy(lsqt_ctx* ctx);
+int lsqt_ctx_sync(lsqt_ctx* ctx);
+/* the hipStream_t the context launches on, for events recorded by callers */
+void* lsqt_ctx_stream(lsqt_ctx* ctx);
+int lsqt_device_count(int* count);
+
+/* ---- Hamiltonian ----------------------------------------------------------
+ * Replaces Hamiltonian::Hamiltonian(Model&) / initialize_gpu
+ * (src/hamiltonian.cu:27-89,124-131): the arrays are the Model's row-major
+ * ELL arrays (pair (n, m) at n*max_neighbor + m, slots m >= neighbor_number[n]
+ * ignored), uploaded column-major.  Unlike the reference, the caller's
+ * arrays are not freed. */
+int lsqt_ham_create(lsqt_ctx* ctx, int n, int max_neighbor, const int* neighbor_number,
+                    const int* neighbor_list, const double* potential,
+                    const double* hopping_real, const double* hopping_imag, const double* xx,
+                    double energy_max, int layout_hint, lsqt_ham** out);
+/* Hamiltonian::~Hamiltonian (src/hamiltonian.cu:133-150) */
+int lsqt_ham_destroy(lsqt_ham* h);
+int lsqt_ham_get_info(const lsqt_ham* h, lsqt_ham_info* info);
+
+/* ---- Vector ---------------------------------------------------------------
+ * Vector(int n): zero-filled (src/vector.cu:61-71) */
+int lsqt_vec_create(lsqt_ctx* ctx, int n, lsqt_vec** out);
+/* Vector(Vector&): deep copy (src/vector.cu:92-105) */
+int lsqt_vec_clone(const lsqt_vec* src, lsqt_vec** out);
+/* ~Vector (src/vector.cu:107-116) */
+int lsqt_vec_destroy(lsqt_vec* v);
+int lsqt_vec_size(const lsqt_vec* v, int* n);
+/* Vector::copy_from_host (src/vector.cu:199-208) */
+int lsqt_vec_from_host(lsqt_vec* v, const double* re, const double* im);
+/* Vector::copy_to_host (src/vector.cu:210-219); synchronises */
+int lsqt_vec_to_host(const lsqt_vec* v, double* re, double* im);
+/* Vector::copy: dst = src (src/vector.cu:148-157) */
+int lsqt_vec_copy(lsqt_vec* dst, const lsqt_vec* src);
+/* Vector::add: dst += src (src/vector.cu:137-146) */
+int lsqt_vec_add(lsqt_vec* dst, const lsqt_vec* src);
+/* Vector::apply_sz: dst = (-1)^n src (src/vector.cu:188-197) */
+int lsqt_vec_apply_sz(lsqt_vec* dst, const lsqt_vec* src);
+/* Vector::swap: exchanges storage (src/vector.cu:221-228) */
+int lsqt_vec_swap(lsqt_vec* a, lsqt_vec* b);
+/* sum_n conj(left_n) psi_n (Vector::inner_product_1 + inner_product_2,
+ * src/vector.cu:331-344,425-436, for one vector pair); synchronises */
+int lsqt_vec_dot(const lsqt_vec* left, const lsqt_vec* psi, double* re, double* im);
+
+/* ---- operators ------------------------------------------------------------
+ * Hamiltonian::apply: out = H~ in, H~ = H / energy_max (src/hamiltonian.cu:225-237) */
+int lsqt_apply(lsqt_ham* h, const lsqt_vec* in, lsqt_vec* out);
+/* Hamiltonian::apply_commutator: out = [X, H~] in (src/hamiltonian.cu:308-320) */
+int lsqt_apply_commutator(lsqt_ham* h, const lsqt_vec* in, lsqt_vec* out);
+/* Hamiltonian::apply_current: out = V in, not scaled (src/hamiltonian.cu:387-399) */
+int lsqt_apply_current(lsqt_ham* h, const lsqt_vec* in, lsqt_vec* out);
+/* Hamiltonian::kernel_polynomial: s2 = 2 H~ s1 - s0 (src/hamiltonian.cu:970-984);
+ * s2 may be s0 */
+int lsqt_kernel_polynomial(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1, lsqt_vec* s2);
+/* Hamiltonian::chebyshev_01 (src/hamiltonian.cu:447-460) */
+int lsqt_chebyshev_01(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1, lsqt_vec* s,
+                      double bessel_0, double bessel_1, int direction);
+/* Hamiltonian::chebyshev_2 (src/hamiltonian.cu:599-614); s2 may be s0 */
+int lsqt_chebyshev_2(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1, lsqt_vec* s2,
+                     lsqt_vec* s, double bessel_m, int label);
+/* Hamiltonian::chebyshev_1x (src/hamiltonian.cu:652-662) */
+int lsqt_chebyshev_1x(lsqt_ham* h, const lsqt_vec* s1x, lsqt_vec* s, double bessel_1);
+/* Hamiltonian::chebyshev_2x (src/hamiltonian.cu:856-882); s2 may be s0 and
+ * s2x may be s0x */
+int lsqt_chebyshev_2x(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s0x, const lsqt_vec* s1,
+                      const lsqt_vec* s1x, lsqt_vec* s2, lsqt_vec* s2x, lsqt_vec* s,
+                      double bessel_m, int label);
+
+/* ---- fused algorithms (src/sigma.cu) --------------------------------------
+ * find_moments_chebyshev (src/sigma.cu:33-61): mu_m = <left|T_m(H~)|right>,
+ * m = 0..M-1, M >= 2.  The whole recursion runs on the device, each step
+ * fused with its moment product; one download of M complex values at the
+ * end.  mu_re/mu_im are host arrays of M doubles. */
+int lsqt_moments(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right, int M,
+                 double* mu_re, double* mu_im);
+/* evolve (src/sigma.cu:116-151): state <- U(direction * t) state, where
+ * time_step_scaled = dt * energy_max.  Bessel coefficients from the host's
+ * libm j0/j1/jn, stopping at the first |J_m| < 1e-15 (m >= 2). */
+int lsqt_evolve(lsqt_ham* h, int direction, double time_step_scaled, lsqt_vec* state);
+/* evolvex (src/sigma.cu:157-200): state <- [X, U(t)] state (direction +1)
+ * or [U(-t), X] state (direction -1) */
+int lsqt_evolvex(lsqt_ham* h, int direction, double time_step_scaled, lsqt_vec* state);
+/* the number of Chebyshev terms evolve/evolvex use for time_step_scaled */
+int lsqt_bessel_terms(double time_step_scaled, int* last_m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
