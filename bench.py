@@ -1,0 +1,309 @@
+"""Benchmark: Chebyshev site-updates/s of the LSQT hot path on one MI355X.
+
+Workload (one "step"): the plain FP64 DOS + MSD job on an 8.0M-site
+honeycomb lattice -- 2000 x 1000 four-orbital rectangular cells (a_cc = 1,
+t = -2.7 eV, periodic), Anderson disorder U[-1, 1] eV, one random-phase
+vector, 3000 Chebyshev moments, E_max = 10 eV, 201 energies in [-8, 8] eV,
+MSD over 20 time steps of 2 hbar/eV (Bessel-expanded evolve / evolvex).
+Per step: 21 moment sets + 59 evolutions, i.e. ~5.3e11 site-updates, all on
+the GPU through the C ABI (lsqt_moments / lsqt_evolve / lsqt_evolvex), with
+the host-side damping and Chebyshev summation of every row.  The
+Hamiltonian and the random vector are resident in HBM before timing starts.
+
+A site-update is one recursion step T_m = 2 H~ T_{m-1} - T_{m-2} at one site
+(moment product included): N (M - 1) per moment set, N (1 + terms) per
+evolve and 2 N (1 + terms) per evolvex.
+
+Prints one JSON line (see the fields below).  --gpus N > 1 is refused: this
+build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def write_config2(d, nx=2000, ny=1000, moments=3000, time_steps=20, dt=2.0, energies=201,
+                  msd=True):
+    from gpuqt_amd import inputs as gi
+    gi.write_honeycomb(d, nx, ny)
+    gi.write_para(d, anderson=2.0, msd=msd, moments=moments, energy_max=10.0)
+    gi.write_energy(d, gi.energy_grid(-8, 8, energies))
+    gi.write_time_steps(d, [dt] * time_steps)
+
+
+class HipEvents:
+    """hipEvent timing on the context's stream (libamdhip64 via ctypes)."""
+
+    def __init__(self, stream):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.stream = C.c_void_p(stream)
+        for f in ("hipEventCreate", "hipEventRecord", "hipEventSynchronize",
+                  "hipEventElapsedTime", "hipEventDestroy"):
+            getattr(self.hip, f).restype = C.c_int
+
+    def new(self):
+        ev = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(ev)) == 0
+        return ev
+
+    def record(self, ev):
+        assert self.hip.hipEventRecord(ev, self.stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def run_job(H, model, r_vec, energies, time_steps, emax, volume, M, msd, ev=None, acc=None):
+    """One DOS (+ MSD) job on the device.  Returns (site_updates, rows).
+    If ev is given, accumulates the device time of the moment sets in acc."""
+    from gpuqt_amd import hostlib, native
+    n = H.n
+    rows = []
+    updates = 0
+
+    def moments(L, R):
+        nonlocal updates
+        if ev is not None:
+            a, b = ev.new(), ev.new()
+            ev.record(a)
+        mu = H.moments(L, R, M)
+        if ev is not None:
+            ev.record(b)
+            acc["moment_ms"] += ev.elapsed_ms(a, b)
+            acc["moment_sets"] += 1
+        updates += n * (M - 1)
+        return mu
+
+    def row(mu):
+        return hostlib.correlation_row(mu, energies, emax, volume)
+
+    def evolve(direction, t, v):
+        nonlocal updates
+        H.evolve(direction, t, v)
+        updates += n * (1 + terms(t))
+
+    def evolvex(direction, t, v):
+        nonlocal updates
+        H.evolvex(direction, t, v)
+        updates += 2 * n * (1 + terms(t))
+
+    rows.append(row(moments(r_vec, r_vec)))
+    if msd:
+        state, state_x, copy = r_vec.clone(), r_vec.clone(), native.Vector(H.ctx, n)
+        t = time_steps[0] * emax
+        evolve(1, t, state)
+        evolvex(1, t, state_x)
+        for s in range(len(time_steps)):
+            rows.append(row(moments(state_x, state_x)))
+            if s < len(time_steps) - 1:
+                t = time_steps[s + 1] * emax
+                copy.copy_from(state)
+                evolvex(1, t, copy)
+                evolve(1, t, state_x)
+                state_x.add(copy)
+                evolve(1, t, state)
+        H.ctx.sync()
+        for v in (state, state_x, copy):
+            v.close()
+    return updates, np.array(rows)
+
+
+_terms_cache = {}
+
+
+def terms(t):
+    from gpuqt_amd import native
+    if t not in _terms_cache:
+        # chebyshev_2 launches = last term index - 1
+        _terms_cache[t] = native.bessel_last_term(t) - 1
+    return _terms_cache[t]
+
+
+def cpu_baseline(nx, ny, moments_sample, threads):
+    """Times the reference CPU program (compiled from source into
+    oracle/_ref) or, where it is absent, the oracle, on a bounded sample of
+    the same workload: the DOS moment set of the same lattice with fewer
+    moments.  Returns the cpu_baseline dict."""
+    import oracle
+    with tempfile.TemporaryDirectory() as tmp:
+        d = os.path.join(tmp, "cpu")
+        os.makedirs(d)
+        write_config2(d, nx=nx, ny=ny, moments=moments_sample, msd=False)
+        n_sites = 4 * nx * ny
+        updates = n_sites * (moments_sample - 1)
+        ref = oracle.reference_binary()
+        if ref is not None:
+            driver = os.path.join(tmp, "input.txt")
+            with open(driver, "w") as f:
+                f.write(d + "\n")
+            r = subprocess.run([ref, driver], capture_output=True, text=True, timeout=600)
+            m = re.search(r"Time used for finding DOS = ([0-9.eE+-]+) s", r.stdout)
+            if r.returncode != 0 or not m:
+                raise RuntimeError("reference CPU run failed")
+            secs = float(m.group(1))
+            kind, cores = "reference", 1
+        else:
+            oracle.set_threads(threads)
+            om = oracle.OracleModel(d)
+            rr = om.random_state()
+            t0 = time.perf_counter()
+            om.moments(rr, rr, moments_sample)
+            secs = time.perf_counter() - t0
+            kind, cores = "port", oracle.num_threads()
+    return {"value": updates / secs, "unit": "site-updates/s", "cores": cores, "kind": kind,
+            "sample": f"DOS moment set, {moments_sample} moments, {n_sites}-site honeycomb "
+                      f"(same lattice as the workload), {secs:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=2000)
+    ap.add_argument("--ny", type=int, default=1000)
+    ap.add_argument("--moments", type=int, default=3000)
+    ap.add_argument("--time-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-moments", type=int, default=100)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path has no "
+              f"multi-GPU mode in this build", file=sys.stderr)
+        sys.exit(2)
+
+    from gpuqt_amd import hostlib, native
+
+    with tempfile.TemporaryDirectory() as tmp:
+        d = os.path.join(tmp, "config2")
+        os.makedirs(d)
+        write_config2(d, nx=args.nx, ny=args.ny, moments=args.moments,
+                      time_steps=args.time_steps)
+        t0 = time.perf_counter()
+        # the drop-in's Model (host C++): lattice, Anderson draws, random phases
+        devnull = os.open(os.devnull, os.O_WRONLY)
+        saved = os.dup(1)
+        os.dup2(devnull, 1)
+        try:
+            model = hostlib.HostModel(d)
+            r = model.random_state()
+            arrays = model.arrays()
+        finally:
+            os.dup2(saved, 1)
+            os.close(devnull)
+            os.close(saved)
+        energies, time_steps, _ = model.inputs()
+        build_s = time.perf_counter() - t0
+        log(f"model: {model.n} sites built in {build_s:.1f} s")
+
+    ctx = native.Context(0)
+    H = native.Hamiltonian.from_arrays(ctx, arrays, model.n, model.max_neighbor,
+                                       model.energy_max)
+    del arrays
+    info = H.info()
+    rv = H.vector(*r)
+    ctx.sync()
+    log(f"Hamiltonian on device: layout {'packed' if info.layout == 2 else 'full'}, "
+        f"{info.classes} hopping classes, {info.bytes_per_site_f64:.0f} B per site-update")
+
+    M = args.moments
+    job = dict(energies=energies, time_steps=time_steps, emax=model.energy_max,
+               volume=model.volume, M=M, msd=args.time_steps > 0)
+    for w in range(args.warmup):
+        run_job(H, model, rv, **job)
+        log(f"warmup {w} done")
+
+    ev = HipEvents(ctx.stream)
+    acc = {"moment_ms": 0.0, "moment_sets": 0}
+    ctx.sync()
+    t0 = time.perf_counter()
+    total_updates = 0
+    rows = None
+    for s in range(args.steps):
+        u, rows = run_job(H, model, rv, ev=ev, acc=acc, **job)
+        total_updates += u
+        log(f"step {s} done")
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+
+    value = total_updates / elapsed
+    # dominant kernel: the fused recursion + moment-product step
+    per_launch_updates = model.n
+    avg_launch_ms = acc["moment_ms"] / max(acc["moment_sets"] * (M - 1), 1)
+    bytes_per_launch = info.bytes_per_site_f64 * per_launch_updates
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    dos = rows[0]
+    log(f"DOS(E=0) = {dos[len(dos) // 2]:.6g}, {len(rows)} rows")
+
+    result = {
+        "metric": "Chebyshev site-updates/s (FP64 KPM + Bessel evolution), HBM GB/s of the "
+                  "fused recursion kernel",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lattice, Anderson disorder and random phases of the "
+                "reference's RNG)",
+        "config": {
+            "workload": f"honeycomb {args.nx}x{args.ny} four-orbital cells "
+                        f"({model.n} sites), t=-2.7 eV, Anderson U[-1,1] eV, 1 random vector, "
+                        f"{M} moments, E_max=10 eV, 201 energies, DOS + MSD over "
+                        f"{args.time_steps} steps of dt=2 hbar/eV",
+            "sites": model.n,
+            "moments": M,
+            "site_updates_per_step": total_updates // args.steps,
+            "layout": "packed" if info.layout == 2 else "full",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_hpsi (2 H~ psi_{m-1} - psi_{m-2} fused with <L|psi_m>)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_site_update": info.bytes_per_site_f64,
+            "avg_launch_us": avg_launch_ms * 1e3,
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("CPU baseline ...")
+        result["cpu_baseline"] = cpu_baseline(args.nx, args.ny, args.cpu_moments,
+                                              args.cpu_threads)
+    H.close()
+    rv.close()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,225 @@
+// sigma.cpp -- find_dos / find_vac0 / find_vac / find_msd /
+// find_spin_polarization (src/sigma.cu:204-442) on top of the fused device
+// entry points: each moment set is one lsqt_moments call (recursion and
+// moment products on the GPU, one download of M complex moments), each
+// time step one lsqt_evolve / lsqt_evolvex call.  Damping and the Chebyshev
+// summation stay on the host as in the reference.
+#include "sigma.h"
+
+#include <cmath>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "hamiltonian.h"
+#include "lsqt_hip.h"
+#include "model.h"
+#include "vector.h"
+
+#define PI 3.141592653589793
+
+namespace lsqt_host {
+static int g_precision = 6;
+void set_output_precision(int digits)
+{
+  g_precision = digits > 0 ? digits : 6;
+}
+int output_precision()
+{
+  return g_precision;
+}
+}  // namespace lsqt_host
+
+void find_moments_chebyshev(Model& model, Hamiltonian& H, Vector& left, Vector& right,
+                            real* mu_re, real* mu_im)
+{
+  LSQT_CHECK(lsqt_moments(H.handle(), left.handle(), right.handle(), model.number_of_moments,
+                          mu_re, mu_im));
+}
+
+namespace lsqt_host {
+
+// Jackson damping, alpha = 1/(M+1) (doc/gpuqt.tex:266-267), then
+// rho(E) = 2/(pi Omega) (mu_0 + 2 sum_k mu_k T_k(E~)) / sqrt(1 - E~^2) / E_max
+// from the real parts
+void damped_chebyshev_sum(int M, real* mu_re, real* mu_im, int n_energy, const real* energy,
+                          real energy_max, real volume, real* out)
+{
+  for (int k = 0; k < M; ++k) {
+    real factor = 1.0 / (M + 1);
+    real damping =
+      (1 - k * factor) * cos(k * PI * factor) + sin(k * PI * factor) * factor / tan(PI * factor);
+    mu_re[k] *= damping;
+    if (mu_im)
+      mu_im[k] *= damping;
+  }
+  for (int e = 0; e < n_energy; ++e) {
+    real x = energy[e] / energy_max;
+    real t0 = 1.0, t1 = x, t2;
+    real sum = mu_re[1] * t1;
+    for (int k = 2; k < M; ++k) {
+      t2 = 2.0 * x * t1 - t0;
+      t0 = t1;
+      t1 = t2;
+      sum += mu_re[k] * t2;
+    }
+    sum *= 2.0;
+    sum += mu_re[0];
+    sum *= 2.0 / (PI * volume);
+    sum /= sqrt(1.0 - x * x);
+    out[e] = sum / energy_max;
+  }
+}
+
+}  // namespace lsqt_host
+
+void apply_damping(Model& model, real* inner_product_real, real* inner_product_imag)
+{
+  lsqt_host::damped_chebyshev_sum(model.number_of_moments, inner_product_real,
+                                  inner_product_imag, 0, nullptr, model.energy_max,
+                                  model.volume, nullptr);
+}
+
+void perform_chebyshev_summation(Model& model, real* inner_product_real, real*,
+                                 real* correlation_function)
+{
+  // summation only: damping was applied by apply_damping
+  const int M = model.number_of_moments;
+  for (int e = 0; e < model.number_of_energy_points; ++e) {
+    real x = model.energy[e] / model.energy_max;
+    real t0 = 1.0, t1 = x, t2;
+    real sum = inner_product_real[1] * t1;
+    for (int k = 2; k < M; ++k) {
+      t2 = 2.0 * x * t1 - t0;
+      t0 = t1;
+      t1 = t2;
+      sum += inner_product_real[k] * t2;
+    }
+    sum *= 2.0;
+    sum += inner_product_real[0];
+    sum *= 2.0 / (PI * model.volume);
+    sum /= sqrt(1.0 - x * x);
+    correlation_function[e] = sum / model.energy_max;
+  }
+}
+
+void evolve(Model&, int direction, real time_step_scaled, Hamiltonian& H, Vector& state)
+{
+  LSQT_CHECK(lsqt_evolve(H.handle(), direction, time_step_scaled, state.handle()));
+}
+
+void evolvex(Model&, int direction, real time_step_scaled, Hamiltonian& H, Vector& state)
+{
+  LSQT_CHECK(lsqt_evolvex(H.handle(), direction, time_step_scaled, state.handle()));
+}
+
+namespace {
+
+// one moment set -> one damped, summed row
+std::vector<real> correlation_row(Model& model, Hamiltonian& H, Vector& left, Vector& right)
+{
+  std::vector<real> mu_re((size_t)model.number_of_moments), mu_im((size_t)model.number_of_moments);
+  find_moments_chebyshev(model, H, left, right, mu_re.data(), mu_im.data());
+  std::vector<real> row((size_t)model.number_of_energy_points);
+  lsqt_host::damped_chebyshev_sum(model.number_of_moments, mu_re.data(), mu_im.data(),
+                                  model.number_of_energy_points, model.energy,
+                                  model.energy_max, model.volume, row.data());
+  return row;
+}
+
+std::ofstream open_output(Model& model, const std::string& name)
+{
+  std::ofstream output(model.input_dir + name, std::ios::app);
+  if (!output.is_open()) {
+    std::cout << "Error: cannot open " + model.input_dir + name << std::endl;
+    exit(1);
+  }
+  output.precision(lsqt_host::output_precision());
+  return output;
+}
+
+void write_row(std::ofstream& output, const std::vector<real>& row)
+{
+  for (real v : row)
+    output << v << " ";
+  output << std::endl;
+}
+
+}  // namespace
+
+void find_dos(Model& model, Hamiltonian& H, Vector& random_state, int flag)
+{
+  std::vector<real> row = correlation_row(model, H, random_state, random_state);
+  std::ofstream output = open_output(model, flag == 0 ? "/dos.out" : "/ldos.out");
+  write_row(output, row);
+}
+
+void find_vac0(Model& model, Hamiltonian& H, Vector& random_state)
+{
+  Vector state(model.number_of_atoms);
+  H.apply_current(random_state, state);
+  std::vector<real> row = correlation_row(model, H, state, state);
+  std::ofstream output = open_output(model, "/vac0.out");
+  write_row(output, row);
+}
+
+void find_vac(Model& model, Hamiltonian& H, Vector& random_state)
+{
+  Vector state_left(random_state);
+  Vector state_left_copy(model.number_of_atoms);
+  Vector state_right(random_state);
+  H.apply_current(state_left, state_right);
+  std::ofstream output = open_output(model, "/vac.out");
+  for (int m = 0; m < model.number_of_steps_correlation; ++m) {
+    std::cout << "- calculating VAC step " << m << std::endl;
+    H.apply_current(state_left, state_left_copy);
+    write_row(output, correlation_row(model, H, state_right, state_left_copy));
+    if (m < model.number_of_steps_correlation - 1) {
+      real time_step_scaled = model.time_step[m] * model.energy_max;
+      evolve(model, -1, time_step_scaled, H, state_left);
+      evolve(model, -1, time_step_scaled, H, state_right);
+    }
+  }
+}
+
+void find_msd(Model& model, Hamiltonian& H, Vector& random_state)
+{
+  Vector state(random_state);
+  Vector state_x(random_state);
+  Vector state_copy(model.number_of_atoms);
+  real time_step_scaled = model.time_step[0] * model.energy_max;
+  evolve(model, 1, time_step_scaled, H, state);
+  evolvex(model, 1, time_step_scaled, H, state_x);
+  std::ofstream output = open_output(model, "/msd.out");
+  for (int m = 0; m < model.number_of_steps_correlation; ++m) {
+    std::cout << "- calculating MSD step " << m << std::endl;
+    write_row(output, correlation_row(model, H, state_x, state_x));
+    if (m < model.number_of_steps_correlation - 1) {
+      time_step_scaled = model.time_step[m + 1] * model.energy_max;
+      // [X, U^m] |phi> -> [X, U^(m+1)] |phi>
+      state_copy.copy(state);
+      evolvex(model, 1, time_step_scaled, H, state_copy);
+      evolve(model, 1, time_step_scaled, H, state_x);
+      state_x.add(state_copy);
+      // U^m |phi> -> U^(m+1) |phi>
+      evolve(model, 1, time_step_scaled, H, state);
+    }
+  }
+}
+
+void find_spin_polarization(Model& model, Hamiltonian& H, Vector& random_state)
+{
+  Vector state(random_state);
+  Vector state_sz(model.number_of_atoms);
+  std::ofstream output = open_output(model, "/S.out");
+  for (int m = 0; m < model.number_of_steps_correlation; ++m) {
+    std::cout << "- calculating spin step " << m << std::endl;
+    state_sz.apply_sz(state);
+    write_row(output, correlation_row(model, H, state, state_sz));
+    if (m < model.number_of_steps_correlation - 1) {
+      real time_step_scaled = model.time_step[m] * model.energy_max;
+      evolve(model, 1, time_step_scaled, H, state);
+    }
+  }
+}

@@ -1,0 +1,141 @@
+"""ctypes binding of liblsqt_host.so (include/lsqt_host.h): the drop-in's
+Model and driver, for tests and tools.  Model construction touches no GPU;
+running a directory does."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblsqt_host.so")
+CLI_PATH = os.path.join(HERE, "lsqt_gpu")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+SIGNATURES = {
+    "lsqt_host_main": (C.c_int, [C.c_int, C.POINTER(C.c_char_p)]),
+    "lsqt_host_run": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int]),
+    "lsqt_host_set_output_precision": (None, [C.c_int]),
+    "lsqt_host_damped_sum": (C.c_int, [C.c_int, _dp, _dp, C.c_int, _dp, C.c_double,
+                                       C.c_double, _dp]),
+    "lsqt_host_model_load": (C.c_int, [C.c_char_p, C.POINTER(_vp), C.c_char_p, C.c_int]),
+    "lsqt_host_model_free": (None, [_vp]),
+    "lsqt_host_model_info": (C.c_int, [_vp, _ip, _dp]),
+    "lsqt_host_model_arrays": (C.c_int, [_vp, _ip, _ip, _dp, _dp, _dp, _dp]),
+    "lsqt_host_model_inputs": (C.c_int, [_vp, _dp, _dp, _ip]),
+    "lsqt_host_model_random_state": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+}
+
+_lib = None
+
+
+class InputError(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"host library not built: {LIB_PATH}")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class HostModel:
+    """The drop-in's Model built from an input directory."""
+
+    def __init__(self, dirname: str):
+        self.h = _vp()
+        err = C.create_string_buffer(4096)
+        if load().lsqt_host_model_load(dirname.encode(), C.byref(self.h), err, 4096):
+            raise InputError(err.value.decode())
+        ints = (C.c_int * 8)()
+        reals = (C.c_double * 2)()
+        load().lsqt_host_model_info(self.h, ints, reals)
+        (self.n, self.max_neighbor, self.number_of_moments, self.number_of_random_vectors,
+         self.number_of_energy_points, self.number_of_steps_correlation,
+         self.number_of_local_orbitals, flags) = list(ints)
+        self.calculate_vac0 = bool(flags & 1)
+        self.calculate_vac = bool(flags & 2)
+        self.calculate_msd = bool(flags & 4)
+        self.calculate_spin = bool(flags & 8)
+        self.calculate_ldos = bool(flags & 16)
+        self.energy_max, self.volume = reals[0], reals[1]
+
+    def arrays(self):
+        N, P = self.n, self.n * self.max_neighbor
+        nn = np.zeros(N, np.int32)
+        nl = np.zeros(P, np.int32)
+        pot, hr, hi, xx = np.zeros(N), np.zeros(P), np.zeros(P), np.zeros(P)
+        load().lsqt_host_model_arrays(self.h, nn.ctypes.data_as(_ip), nl.ctypes.data_as(_ip),
+                                      *(a.ctypes.data_as(_dp) for a in (pot, hr, hi, xx)))
+        return dict(neighbor_number=nn, neighbor_list=nl, potential=pot, hopping_real=hr,
+                    hopping_imag=hi, xx=xx)
+
+    def valid_pairs(self):
+        a = self.arrays()
+        mask = (np.arange(self.max_neighbor)[None, :] < a["neighbor_number"][:, None]).reshape(-1)
+        return {k: a[k][mask] for k in ("neighbor_list", "hopping_real", "hopping_imag", "xx")}
+
+    def inputs(self):
+        e = np.zeros(max(self.number_of_energy_points, 1))
+        t = np.zeros(max(self.number_of_steps_correlation, 1))
+        o = np.zeros(max(self.number_of_local_orbitals, 1), np.int32)
+        load().lsqt_host_model_inputs(self.h, e.ctypes.data_as(_dp), t.ctypes.data_as(_dp),
+                                      o.ctypes.data_as(_ip))
+        return (e[:self.number_of_energy_points], t[:self.number_of_steps_correlation],
+                o[:self.number_of_local_orbitals])
+
+    def random_state(self, orbital: int = -1):
+        re, im = np.zeros(self.n), np.zeros(self.n)
+        if load().lsqt_host_model_random_state(self.h, orbital, re.ctypes.data_as(_dp),
+                                               im.ctypes.data_as(_dp)):
+            raise InputError("bad orbital")
+        return re, im
+
+    def close(self):
+        if self.h:
+            load().lsqt_host_model_free(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def correlation_row(mu, energies, energy_max, volume):
+    """Jackson damping + Chebyshev summation of one moment set, by the
+    drop-in's host code (src/sigma.cu:76-110)."""
+    mu = np.asarray(mu)
+    re = np.ascontiguousarray(mu.real, dtype=np.float64).copy()
+    im = np.ascontiguousarray(mu.imag, dtype=np.float64).copy()
+    e = np.ascontiguousarray(energies, dtype=np.float64)
+    out = np.zeros(len(e))
+    if load().lsqt_host_damped_sum(len(re), re.ctypes.data_as(_dp), im.ctypes.data_as(_dp),
+                                   len(e), e.ctypes.data_as(_dp), float(energy_max),
+                                   float(volume), out.ctypes.data_as(_dp)):
+        raise ValueError("invalid moment set")
+    return out
+
+
+def run_cli(driver_file: str, digits: int | None = None, timeout: float | None = None,
+            env=None):
+    """Runs the lsqt_gpu program on a driver file."""
+    e = dict(os.environ if env is None else env)
+    if digits is not None:
+        e["LSQT_OUTPUT_DIGITS"] = str(digits)
+    return subprocess.run([CLI_PATH, driver_file], capture_output=True, text=True,
+                          timeout=timeout, env=e)

@@ -1,0 +1,57 @@
+/*
+ * lsqt_host.h -- C entry points of liblsqt_host.so, the host side of the
+ * drop-in (C++ classes Model / Hamiltonian / Vector and the src/sigma.cu
+ * drivers, in gpuqt_amd/host/, running on liblsqt_hip.so).
+ *
+ * Every call returns 0 on success.  Input errors return 1 and copy the
+ * reference's message into err (the reference prints it and exits,
+ * e.g. src/model.cu:128-310); device errors print a report and exit(1) as
+ * the reference's CHECK does (src/common.h:31-42).
+ */
+#ifndef LSQT_HOST_H
+#define LSQT_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* main() of the program (src/main.cu:34-56): argv[1] is the driver file,
+ * one input directory per non-empty line.  Returns the exit status. */
+int lsqt_host_main(int argc, char** argv);
+
+/* lsqt(std::string) (src/lsqt.h:24, src/lsqt.cu:135-152) for one directory */
+int lsqt_host_run(const char* input_dir, char* err, int err_len);
+
+/* apply_damping + perform_chebyshev_summation (src/sigma.cu:76-110) on one
+ * moment set: mu_re (and mu_im, if not NULL) are damped in place; out gets
+ * n_energy values of the correlation function */
+int lsqt_host_damped_sum(int M, double* mu_re, double* mu_im, int n_energy,
+                         const double* energy, double energy_max, double volume, double* out);
+
+/* significant digits of the .out rows (default 6, the reference's) */
+void lsqt_host_set_output_precision(int digits);
+
+/* Model (src/model.h:27-104), for inspection: construction with every
+ * model-phase random draw, the row-major ELL arrays, the random vectors */
+typedef struct lsqt_host_model lsqt_host_model;
+int lsqt_host_model_load(const char* input_dir, lsqt_host_model** out, char* err, int err_len);
+void lsqt_host_model_free(lsqt_host_model* m);
+/* ints: number_of_atoms, max_neighbor, number_of_moments,
+ *       number_of_random_vectors, number_of_energy_points,
+ *       number_of_steps_correlation, number_of_local_orbitals,
+ *       flags (bit 0 vac0, 1 vac, 2 msd, 3 spin, 4 ldos)
+ * reals: energy_max, volume */
+int lsqt_host_model_info(const lsqt_host_model* m, int* ints8, double* reals2);
+/* copies out the arrays (sizes from lsqt_host_model_info) */
+int lsqt_host_model_arrays(const lsqt_host_model* m, int* neighbor_number, int* neighbor_list,
+                           double* potential, double* hopping_real, double* hopping_imag,
+                           double* xx);
+int lsqt_host_model_inputs(const lsqt_host_model* m, double* energy, double* time_step,
+                           int* local_orbitals);
+/* Model::initialize_state into host arrays (consumes random draws) */
+int lsqt_host_model_random_state(lsqt_host_model* m, int orbital, double* re, double* im);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
