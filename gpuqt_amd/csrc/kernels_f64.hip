@@ -23,19 +23,19 @@
 namespace lsqt {
 
 // --- pair access ---------------------------------------------------------
-template <int LAYOUT>
+template <int LAYOUT, bool NT>
 __device__ __forceinline__ bool load_pair(const HamDev& h, const double2* s_tab, size_t q,
                                           int& j, double& a, double& b)
 {
   if constexpr (LAYOUT == kFull) {
-    j = h.nl[q];
+    j = ld<NT>(h.nl + q);
     if (j < 0)
       return false;
-    a = h.hr[q];
-    b = h.hi[q];
+    a = ld<NT>(h.hr + q);
+    b = ld<NT>(h.hi + q);
     return true;
   } else {
-    uint32_t w = h.words[q];
+    uint32_t w = ld<NT>(h.words + q);
     if (w == kEmptySlot)
       return false;
     j = (int)(w & h.idx_mask);
@@ -47,21 +47,21 @@ __device__ __forceinline__ bool load_pair(const HamDev& h, const double2* s_tab,
   }
 }
 
-template <int LAYOUT>
+template <int LAYOUT, bool NT>
 __device__ __forceinline__ bool load_pair_x(const HamDev& h, const double2* s_tab,
                                             const double* s_xx, size_t q, int& j, double& a,
                                             double& b, double& x)
 {
   if constexpr (LAYOUT == kFull) {
-    j = h.nl[q];
+    j = ld<NT>(h.nl + q);
     if (j < 0)
       return false;
-    a = h.hr[q];
-    b = h.hi[q];
-    x = h.xx[q];
+    a = ld<NT>(h.hr + q);
+    b = ld<NT>(h.hi + q);
+    x = ld<NT>(h.xx + q);
     return true;
   } else {
-    uint32_t w = h.words[q];
+    uint32_t w = ld<NT>(h.words + q);
     if (w == kEmptySlot)
       return false;
     j = (int)(w & h.idx_mask);
@@ -124,7 +124,7 @@ __device__ __forceinline__ void block_sum_store(double re, double im, double2* o
 // RECUR: out = 2 H~ psi1 - psi0 (kernel_polynomial) else out = H~ psi1 (apply)
 // IP:    partial[tile] = sum over the tile of conj(L) out
 // out may alias psi0 (each site reads its own psi0 before writing out).
-template <int LAYOUT, int KT, int SPT, bool RECUR, bool IP>
+template <int LAYOUT, int KT, int SPT, bool RECUR, bool IP, bool NT>
 __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __restrict__ psi1,
                                                  const double2* psi0, double2* out,
                                                  const double2* __restrict__ L,
@@ -143,14 +143,14 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
     int n = (tile * SPT + k) * kBlock + threadIdx.x;
     if (n < h.n) {
       double2 p1 = psi1[n];
-      double v = h.pot[n];
+      double v = ld<NT>(h.pot + n);
       double tr = v * p1.x;
       double ti = v * p1.y;
 #pragma unroll
       for (int m = 0; m < K; ++m) {
         int j;
         double a, b;
-        if (load_pair<LAYOUT>(h, s_tab, (size_t)m * h.n + n, j, a, b)) {
+        if (load_pair<LAYOUT, NT>(h, s_tab, (size_t)m * h.n + n, j, a, b)) {
           double2 c = psi1[j];
           tr += a * c.x - b * c.y;
           ti += a * c.y + b * c.x;
@@ -159,13 +159,13 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
       tr /= h.emax;
       ti /= h.emax;
       if constexpr (RECUR) {
-        double2 p0 = psi0[n];
+        double2 p0 = ld2<NT>(psi0 + n);
         tr = 2.0 * tr - p0.x;
         ti = 2.0 * ti - p0.y;
       }
-      out[n] = make_double2(tr, ti);
+      st2<false>(out + n, make_double2(tr, ti));
       if constexpr (IP) {
-        double2 l = L[n];
+        double2 l = ld2<NT>(L + n);
         acc_re += tr * l.x + ti * l.y;
         acc_im += ti * l.x - tr * l.y;
       }
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
 }
 
 // --- [X, H~] psi and V psi ----------------------------------------------
-template <int LAYOUT, int KT, bool CURRENT>
+template <int LAYOUT, int KT, bool CURRENT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_xop(HamDev h, const double2* __restrict__ in,
                                                 double2* __restrict__ out, int tiles,
                                                 int blocks)
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_xop(HamDev h, const double2* __restr
   for (int m = 0; m < K; ++m) {
     int j;
     double a, b, x;
-    if (load_pair_x<LAYOUT>(h, s_tab, s_xx, (size_t)m * h.n + n, j, a, b, x)) {
+    if (load_pair_x<LAYOUT, NT>(h, s_tab, s_xx, (size_t)m * h.n + n, j, a, b, x)) {
       double2 c = in[j];
       if constexpr (CURRENT) {
         tr += (a * c.x - b * c.y) * x;
@@ -208,9 +208,9 @@ __global__ __launch_bounds__(kBlock) void k_xop(HamDev h, const double2* __restr
     }
   }
   if constexpr (CURRENT)
-    out[n] = make_double2(+ti, -tr);
+    st2<false>(out + n, make_double2(+ti, -tr));
   else
-    out[n] = make_double2(tr / h.emax, ti / h.emax);
+    st2<false>(out + n, make_double2(tr / h.emax, ti / h.emax));
 }
 
 // the four (-i)^m phase cases of src/hamiltonian.cu:504-525
@@ -238,7 +238,7 @@ __device__ __forceinline__ void accumulate_label(double2& s, double bm, double t
 }
 
 // --- one term of U(t): s2 = 2 H~ s1 - s0, s += b (phase) s2 -------------
-template <int LAYOUT, int KT>
+template <int LAYOUT, int KT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_cheb2(HamDev h, const double2* s0,
                                                   const double2* __restrict__ s1, double2* s2,
                                                   double2* __restrict__ s, double bm, int label,
@@ -254,14 +254,14 @@ __global__ __launch_bounds__(kBlock) void k_cheb2(HamDev h, const double2* s0,
   if (n >= h.n)
     return;
   double2 p1 = s1[n];
-  double v = h.pot[n];
+  double v = ld<NT>(h.pot + n);
   double tr = v * p1.x;
   double ti = v * p1.y;
 #pragma unroll
   for (int m = 0; m < K; ++m) {
     int j;
     double a, b;
-    if (load_pair<LAYOUT>(h, s_tab, (size_t)m * h.n + n, j, a, b)) {
+    if (load_pair<LAYOUT, NT>(h, s_tab, (size_t)m * h.n + n, j, a, b)) {
       double2 c = s1[j];
       tr += a * c.x - b * c.y;
       ti += a * c.y + b * c.x;
@@ -269,17 +269,17 @@ __global__ __launch_bounds__(kBlock) void k_cheb2(HamDev h, const double2* s0,
   }
   tr /= h.emax;
   ti /= h.emax;
-  double2 p0 = s0[n];
+  double2 p0 = ld2<NT>(s0 + n);
   tr = 2.0 * tr - p0.x;
   ti = 2.0 * ti - p0.y;
-  double2 st = s[n];
+  double2 st = ld2<NT>(s + n);
   accumulate_label(st, bm, tr, ti, label);
-  s[n] = st;
-  s2[n] = make_double2(tr, ti);
+  st2<false>(s + n, st);
+  st2<false>(s2 + n, make_double2(tr, ti));
 }
 
 // --- one term of [X, U(t)]: coupled recursion of s and sx ----------------
-template <int LAYOUT, int KT>
+template <int LAYOUT, int KT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_cheb2x(HamDev h, const double2* s0,
                                                    const double2* s0x,
                                                    const double2* __restrict__ s1,
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x(HamDev h, const double2* s0,
   int n = tile * kBlock + threadIdx.x;
   if (n >= h.n)
     return;
-  double v = h.pot[n];
+  double v = ld<NT>(h.pot + n);
   double2 p1 = s1[n], p1x = s1x[n];
   double tr = v * p1.x, ti = v * p1.y;
   double txr = v * p1x.x, txi = v * p1x.y;
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x(HamDev h, const double2* s0,
   for (int m = 0; m < K; ++m) {
     int j;
     double a, b, x;
-    if (load_pair_x<LAYOUT>(h, s_tab, s_xx, (size_t)m * h.n + n, j, a, b, x)) {
+    if (load_pair_x<LAYOUT, NT>(h, s_tab, s_xx, (size_t)m * h.n + n, j, a, b, x)) {
       double2 c = s1[j], cx = s1x[j];
       tr += a * c.x - b * c.y;
       ti += a * c.y + b * c.x;
@@ -317,19 +317,19 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x(HamDev h, const double2* s0,
   }
   tr /= h.emax;
   ti /= h.emax;
-  double2 p0 = s0[n];
+  double2 p0 = ld2<NT>(s0 + n);
   tr = 2.0 * tr - p0.x;
   ti = 2.0 * ti - p0.y;
-  s2[n] = make_double2(tr, ti);
+  st2<false>(s2 + n, make_double2(tr, ti));
   txr /= h.emax;
   txi /= h.emax;
-  double2 p0x = s0x[n];
+  double2 p0x = ld2<NT>(s0x + n);
   txr = 2.0 * txr - p0x.x;
   txi = 2.0 * txi - p0x.y;
-  s2x[n] = make_double2(txr, txi);
-  double2 st = s[n];
+  st2<false>(s2x + n, make_double2(txr, txi));
+  double2 st = ld2<NT>(s + n);
   accumulate_label(st, bm, txr, txi, label);
-  s[n] = st;
+  st2<false>(s + n, st);
 }
 
 // --- element-wise kernels -------------------------------------------------
@@ -405,14 +405,11 @@ __global__ __launch_bounds__(kBlock) void k_dot(const double2* __restrict__ psi,
   if (tile >= tiles)
     return;
   double re = 0.0, im = 0.0;
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    int i = (tile * 2 + k) * kBlock + threadIdx.x;
-    if (i < n) {
-      double2 p = psi[i], l = L[i];
-      re += p.x * l.x + p.y * l.y;
-      im += p.y * l.x - p.x * l.y;
-    }
+  int i = tile * kBlock + threadIdx.x;
+  if (i < n) {
+    double2 p = psi[i], l = L[i];
+    re = p.x * l.x + p.y * l.y;
+    im = p.y * l.x - p.x * l.y;
   }
   block_sum_store(re, im, partial + tile);
 }
@@ -484,26 +481,36 @@ inline int blocks_1d(int n)
     } break;                         \
   }
 
-constexpr int kSptHpsi = 2;
+#define LSQT_NT_SWITCH(flag, NT, ...) \
+  if (flag) {                          \
+    constexpr bool NT = true;          \
+    __VA_ARGS__;                       \
+  } else {                             \
+    constexpr bool NT = false;         \
+    __VA_ARGS__;                       \
+  }
+
+constexpr int kSptHpsi = 1;
 
 void launch_hpsi(const HamDev& h, const double2* psi1, const double2* psi0, double2* out,
                  const double2* L, double2* partial, bool recur, bool ip, hipStream_t st)
 {
   Grid g = make_grid(h.n, kSptHpsi);
-  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, {
+  const bool nt = h.n >= kStreamingMinSites;
+  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, LSQT_NT_SWITCH(nt, NT, {
     if (recur && ip)
-      k_hpsi<LAY, KT, kSptHpsi, true, true>
+      k_hpsi<LAY, KT, kSptHpsi, true, true, NT>
         <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
     else if (recur)
-      k_hpsi<LAY, KT, kSptHpsi, true, false>
+      k_hpsi<LAY, KT, kSptHpsi, true, false, NT>
         <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
     else if (ip)
-      k_hpsi<LAY, KT, kSptHpsi, false, true>
+      k_hpsi<LAY, KT, kSptHpsi, false, true, NT>
         <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
     else
-      k_hpsi<LAY, KT, kSptHpsi, false, false>
+      k_hpsi<LAY, KT, kSptHpsi, false, false, NT>
         <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
-  }))
+  })))
 }
 
 int hpsi_tiles(int n)
@@ -514,22 +521,24 @@ int hpsi_tiles(int n)
 void launch_xop(const HamDev& h, const double2* in, double2* out, bool current, hipStream_t st)
 {
   Grid g = make_grid(h.n, 1);
-  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, {
+  const bool nt = h.n >= kStreamingMinSites;
+  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, LSQT_NT_SWITCH(nt, NT, {
     if (current)
-      k_xop<LAY, KT, true><<<g.blocks, kBlock, 0, st>>>(h, in, out, g.tiles, g.blocks);
+      k_xop<LAY, KT, true, NT><<<g.blocks, kBlock, 0, st>>>(h, in, out, g.tiles, g.blocks);
     else
-      k_xop<LAY, KT, false><<<g.blocks, kBlock, 0, st>>>(h, in, out, g.tiles, g.blocks);
-  }))
+      k_xop<LAY, KT, false, NT><<<g.blocks, kBlock, 0, st>>>(h, in, out, g.tiles, g.blocks);
+  })))
 }
 
 void launch_cheb2(const HamDev& h, const double2* s0, const double2* s1, double2* s2,
                   double2* s, double bm, int label, hipStream_t st)
 {
   Grid g = make_grid(h.n, 1);
-  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, {
-    k_cheb2<LAY, KT><<<g.blocks, kBlock, 0, st>>>(h, s0, s1, s2, s, bm, label, g.tiles,
-                                                    g.blocks);
-  }))
+  const bool nt = h.n >= kStreamingMinSites;
+  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, LSQT_NT_SWITCH(nt, NT, {
+    k_cheb2<LAY, KT, NT><<<g.blocks, kBlock, 0, st>>>(h, s0, s1, s2, s, bm, label, g.tiles,
+                                                        g.blocks);
+  })))
 }
 
 void launch_cheb2x(const HamDev& h, const double2* s0, const double2* s0x, const double2* s1,
@@ -537,10 +546,11 @@ void launch_cheb2x(const HamDev& h, const double2* s0, const double2* s0x, const
                    int label, hipStream_t st)
 {
   Grid g = make_grid(h.n, 1);
-  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, {
-    k_cheb2x<LAY, KT><<<g.blocks, kBlock, 0, st>>>(h, s0, s0x, s1, s1x, s2, s2x, s, bm, label,
-                                                     g.tiles, g.blocks);
-  }))
+  const bool nt = h.n >= kStreamingMinSites;
+  LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, LSQT_NT_SWITCH(nt, NT, {
+    k_cheb2x<LAY, KT, NT><<<g.blocks, kBlock, 0, st>>>(h, s0, s0x, s1, s1x, s2, s2x, s, bm,
+                                                         label, g.tiles, g.blocks);
+  })))
 }
 
 void launch_zero(double2* v, int n, hipStream_t st)
@@ -582,12 +592,12 @@ void launch_cheb1x(const double2* s1x, double2* s, double b1, int n, hipStream_t
 
 int dot_tiles(int n)
 {
-  return make_grid(n, 2).tiles;
+  return make_grid(n, 1).tiles;
 }
 
 void launch_dot(const double2* psi, const double2* L, double2* partial, int n, hipStream_t st)
 {
-  Grid g = make_grid(n, 2);
+  Grid g = make_grid(n, 1);
   k_dot<<<g.blocks, kBlock, 0, st>>>(psi, L, partial, n, g.tiles, g.blocks);
 }
 

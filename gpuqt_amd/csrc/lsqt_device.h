@@ -59,6 +59,50 @@ struct HamDevF32 {
   int layout;
 };
 
+// Streaming policy: arrays read once per step (potential, pair words,
+// psi_{m-2}, the left vector) are loaded non-temporally on large systems, so
+// that the L2 keeps the gathered psi_{m-1}; stores stay normal (measured on
+// MI355X: non-temporal stores cost ~15% on the fused step, non-temporal
+// loads gain ~18%).  Below this many sites everything fits in the caches and
+// normal loads are used.
+constexpr int kStreamingMinSites = 1 << 20;
+
+typedef double dvec2_t __attribute__((ext_vector_type(2)));
+typedef float fvec2_t __attribute__((ext_vector_type(2)));
+
+template <bool NT, typename T>
+__device__ __forceinline__ T ld(const T* p)
+{
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+template <bool NT>
+__device__ __forceinline__ double2 ld2(const double2* p)
+{
+  if constexpr (NT) {
+    dvec2_t v = __builtin_nontemporal_load(reinterpret_cast<const dvec2_t*>(p));
+    return make_double2(v.x, v.y);
+  } else {
+    return *p;
+  }
+}
+
+template <bool NT>
+__device__ __forceinline__ void st2(double2* p, double2 v)
+{
+  if constexpr (NT) {
+    dvec2_t w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<dvec2_t*>(p));
+  } else {
+    *p = v;
+  }
+}
+
 // Grid shape of a one-tile-per-workgroup launch: `tiles` tiles of
 // kBlock*spt sites, padded to a multiple of the XCD count so that the
 // round-robin dispatch gives every XCD one contiguous range of tiles.

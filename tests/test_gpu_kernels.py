@@ -212,3 +212,37 @@ def test_error_paths(gpu_ctx):
     bad["neighbor_list"][0] = m.n + 5
     with pytest.raises(native.LsqtError):
         native.Hamiltonian.from_arrays(gpu_ctx, bad, m.n, m.max_neighbor, m.energy_max)
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_streaming_path_large(gpu_ctx, tmp_path, layout):
+    """Systems above the streaming threshold (2^20 sites) take the
+    non-temporal load/store kernels: still bit-identical states."""
+    import oracle
+    from gpuqt_amd import inputs as gi
+    print(f"[gpu] streaming path 1.08M sites {layout}", flush=True)
+    d = str(tmp_path)
+    gi.write_honeycomb(d, 520, 520)
+    gi.write_para(d, anderson=1.5, moments=24, energy_max=10.0, msd=True)
+    gi.write_energy(d, [0.0])
+    gi.write_time_steps(d, [2.0])
+    m = oracle.OracleModel(d)
+    assert m.n >= 1 << 20
+    H = _ham(gpu_ctx, m, LAYOUTS[layout])
+    r = m.random_state()
+    v = H.vector(*r)
+    mu = H.moments(v, v, 24)
+    ref = m.moments(r, r, 24)
+    assert_moments(mu, ref[0] + 1j * ref[1], "moments 1.08M")
+    t = 2.0 * m.energy_max
+    for fn, ofn in ((H.evolve, m.evolve), (H.evolvex, m.evolvex)):
+        s = H.vector(*r)
+        fn(1, t, s)
+        got = s.download()
+        want = ofn(1, t, *r)
+        assert_bits(got[0], want[0], f"{fn.__name__} re")
+        assert_bits(got[1], want[1], f"{fn.__name__} im")
+    out = H.vector()
+    H.apply_current(v, out)
+    want = m.apply_current(*r)
+    assert_bits(out.download()[0], want[0], "current")
