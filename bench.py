@@ -75,11 +75,17 @@ class HipEvents:
         return ms.value
 
 
-def run_job(H, model, r_vec, energies, time_steps, emax, volume, M, msd, ev=None, acc=None):
+def run_job(H, model, r_vec, energies, time_steps, emax, volume, M, msd, ev=None, acc=None,
+            precision="f64"):
     """One DOS (+ MSD) job on the device.  Returns (site_updates, rows).
-    If ev is given, accumulates the device time of the moment sets in acc."""
+    If ev is given, accumulates the device time of the moment sets in acc.
+    precision "f32" runs the precision path (FP32-stored states)."""
     from gpuqt_amd import hostlib, native
     n = H.n
+    f32 = precision == "f32"
+    mom_fn = H.moments_f32 if f32 else H.moments
+    evo_fn = H.evolve_f32 if f32 else H.evolve
+    evx_fn = H.evolvex_f32 if f32 else H.evolvex
     rows = []
     updates = 0
 
@@ -88,7 +94,7 @@ def run_job(H, model, r_vec, energies, time_steps, emax, volume, M, msd, ev=None
         if ev is not None:
             a, b = ev.new(), ev.new()
             ev.record(a)
-        mu = H.moments(L, R, M)
+        mu = mom_fn(L, R, M)
         if ev is not None:
             ev.record(b)
             acc["moment_ms"] += ev.elapsed_ms(a, b)
@@ -101,12 +107,12 @@ def run_job(H, model, r_vec, energies, time_steps, emax, volume, M, msd, ev=None
 
     def evolve(direction, t, v):
         nonlocal updates
-        H.evolve(direction, t, v)
+        evo_fn(direction, t, v)
         updates += n * (1 + terms(t))
 
     def evolvex(direction, t, v):
         nonlocal updates
-        H.evolvex(direction, t, v)
+        evx_fn(direction, t, v)
         updates += 2 * n * (1 + terms(t))
 
     rows.append(row(moments(r_vec, r_vec)))
@@ -187,6 +193,8 @@ def main():
     ap.add_argument("--moments", type=int, default=3000)
     ap.add_argument("--time-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-f32", dest="precision_f32", action="store_false",
+                    help="skip the precision-path (FP32 states) job")
     ap.add_argument("--cpu-moments", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=16)
     args = ap.parse_args()
@@ -232,31 +240,66 @@ def main():
     M = args.moments
     job = dict(energies=energies, time_steps=time_steps, emax=model.energy_max,
                volume=model.volume, M=M, msd=args.time_steps > 0)
-    for w in range(args.warmup):
-        run_job(H, model, rv, **job)
-        log(f"warmup {w} done")
 
-    ev = HipEvents(ctx.stream)
-    acc = {"moment_ms": 0.0, "moment_sets": 0}
-    ctx.sync()
-    t0 = time.perf_counter()
-    total_updates = 0
-    rows = None
-    for s in range(args.steps):
-        u, rows = run_job(H, model, rv, ev=ev, acc=acc, **job)
-        total_updates += u
-        log(f"step {s} done")
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
+    def timed(precision):
+        for w in range(args.warmup):
+            run_job(H, model, rv, precision=precision, **job)
+            log(f"{precision} warmup {w} done")
+        ev = HipEvents(ctx.stream)
+        acc = {"moment_ms": 0.0, "moment_sets": 0}
+        ctx.sync()
+        t0 = time.perf_counter()
+        total = 0
+        rows = None
+        for s in range(args.steps):
+            u, rows = run_job(H, model, rv, ev=ev, acc=acc, precision=precision, **job)
+            total += u
+            log(f"{precision} step {s} done")
+        ctx.sync()
+        elapsed = time.perf_counter() - t0
+        avg_launch_ms = acc["moment_ms"] / max(acc["moment_sets"] * (M - 1), 1)
+        return total, elapsed, avg_launch_ms, rows
 
+    total_updates, elapsed, avg_launch_ms, rows = timed("f64")
+    rows = np.array(rows)
     value = total_updates / elapsed
-    # dominant kernel: the fused recursion + moment-product step
-    per_launch_updates = model.n
-    avg_launch_ms = acc["moment_ms"] / max(acc["moment_sets"] * (M - 1), 1)
-    bytes_per_launch = info.bytes_per_site_f64 * per_launch_updates
+    # dominant kernel: the fused recursion + moment-product step, one site
+    # update per site per launch
+    bytes_per_launch = info.bytes_per_site_f64 * model.n
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     dos = rows[0]
     log(f"DOS(E=0) = {dos[len(dos) // 2]:.6g}, {len(rows)} rows")
+
+    f32 = None
+    if args.precision_f32:
+        u32, e32, l32, rows32 = timed("f32")
+        # FP32 path: potential 4 + psi_{m-1}, psi_{m-2}, psi_m, left 4 x 8 + the
+        # same pair bytes as FP64 (which has potential 8 + 4 x 16)
+        bpu32 = 36.0 + (info.bytes_per_site_f64 - 72.0)
+        ach32 = bpu32 * model.n / (l32 * 1e-3) / 1e9
+        errs = [float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(rows32, rows)]
+        elem = []
+        for a, b in zip(rows32, rows):
+            mask = np.abs(b) > 1e-3 * np.abs(b).max()
+            elem.append(float((np.abs(a - b)[mask] / np.abs(b)[mask]).max()))
+        f32 = {
+            "value": u32 / e32,
+            "unit": "site-updates/s",
+            "ms_per_step": e32 / args.steps * 1e3,
+            "speedup_vs_f64": (u32 / e32) / value,
+            "dtype": "f32 states, f64 arithmetic and accumulation",
+            "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": ach32 / HBM_PEAK_GBS,
+                         "bytes_per_site_update": bpu32, "avg_launch_us": l32 * 1e3},
+            "error_vs_f64": {
+                "dos_normwise": errs[0],
+                "msd_normwise_max": max(errs[1:]) if len(errs) > 1 else None,
+                "all_rows_normwise": errs,
+                "elementwise_max_above_1e-3_of_row_max": max(elem),
+            },
+        }
+        log(f"f32: {u32 / e32:.3e} site-updates/s, DOS normwise error {errs[0]:.2e}, "
+            f"MSD max {max(errs):.2e}")
 
     result = {
         "metric": "Chebyshev site-updates/s (FP64 KPM + Bessel evolution), HBM GB/s of the "
@@ -295,6 +338,8 @@ def main():
             "avg_launch_us": avg_launch_ms * 1e3,
         },
     }
+    if f32 is not None:
+        result["precision_f32"] = f32
     if not args.no_cpu_baseline:
         log("CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(args.nx, args.ny, args.cpu_moments,

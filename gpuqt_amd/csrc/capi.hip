@@ -277,13 +277,15 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
   }
   LSQT_TRY(ctx_activate(ctx));
 
-  const size_t P = (size_t)n * K;
+  const int stride = pair_stride(n);
+  const size_t P = (size_t)stride * K;  // column-major pair slots, padded columns
   lsqt_ham* h = new lsqt_ham();
   h->ctx = ctx;
   HamDev& d = h->d;
   std::memset(&d, 0, sizeof(d));
   d.n = n;
   d.K = K;
+  d.stride = stride;
   d.emax = energy_max;
 
   // distinct (re, im, xx) triples, numbered by first appearance
@@ -324,7 +326,7 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
           last = key;
           last_id = id;
         }
-        words[(size_t)m * n + i] = ((uint32_t)id << ib) | (uint32_t)nl[q];
+        words[(size_t)m * stride + i] = ((uint32_t)id << ib) | (uint32_t)nl[q];
       }
     }
     if (!packed && layout_hint == LSQT_LAYOUT_PACKED) {
@@ -353,7 +355,7 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
     std::vector<double> hr_cm(P, 0.0), hi_cm(P, 0.0), xx_cm(P, 0.0);
     for (int i = 0; i < n; ++i)
       for (int m = 0; m < nn[i]; ++m) {
-        size_t q = (size_t)i * K + m, c = (size_t)m * n + i;
+        size_t q = (size_t)i * K + m, c = (size_t)m * stride + i;
         nl_cm[c] = nl[q];
         hr_cm[c] = hr[q];
         hi_cm[c] = hi[q];
@@ -795,9 +797,4 @@ extern "C" int lsqt_bessel_terms(double t, int* last_m)
   BesselTerms bt = evolve_terms(1, t);
   *last_m = 1 + (int)bt.bm.size();
   return LSQT_OK;
-}
-
-void lsqt::ham_release_extra(lsqt_ham* h)
-{
-  (void)h;
 }

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
       for (int m = 0; m < K; ++m) {
         int j;
         double a, b;
-        if (load_pair<LAYOUT, NT>(h, s_tab, (size_t)m * h.n + n, j, a, b)) {
+        if (load_pair<LAYOUT, NT>(h, s_tab, (size_t)m * h.stride + n, j, a, b)) {
           double2 c = psi1[j];
           tr += a * c.x - b * c.y;
           ti += a * c.y + b * c.x;
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_xop(HamDev h, const double2* __restr
   for (int m = 0; m < K; ++m) {
     int j;
     double a, b, x;
-    if (load_pair_x<LAYOUT, NT>(h, s_tab, s_xx, (size_t)m * h.n + n, j, a, b, x)) {
+    if (load_pair_x<LAYOUT, NT>(h, s_tab, s_xx, (size_t)m * h.stride + n, j, a, b, x)) {
       double2 c = in[j];
       if constexpr (CURRENT) {
         tr += (a * c.x - b * c.y) * x;
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb2(HamDev h, const double2* s0,
   for (int m = 0; m < K; ++m) {
     int j;
     double a, b;
-    if (load_pair<LAYOUT, NT>(h, s_tab, (size_t)m * h.n + n, j, a, b)) {
+    if (load_pair<LAYOUT, NT>(h, s_tab, (size_t)m * h.stride + n, j, a, b)) {
       double2 c = s1[j];
       tr += a * c.x - b * c.y;
       ti += a * c.y + b * c.x;
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x(HamDev h, const double2* s0,
   for (int m = 0; m < K; ++m) {
     int j;
     double a, b, x;
-    if (load_pair_x<LAYOUT, NT>(h, s_tab, s_xx, (size_t)m * h.n + n, j, a, b, x)) {
+    if (load_pair_x<LAYOUT, NT>(h, s_tab, s_xx, (size_t)m * h.stride + n, j, a, b, x)) {
       double2 c = s1[j], cx = s1x[j];
       tr += a * c.x - b * c.y;
       ti += a * c.y + b * c.x;

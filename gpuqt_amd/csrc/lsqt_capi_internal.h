@@ -57,6 +57,9 @@ void ham_release_extra(lsqt_ham* h);
 int moments_enqueue(lsqt_ham* h, const double2* left, const double2* right, int M);
 int evolve_enqueue(lsqt_ham* h, int direction, double t, double2* state);
 int evolvex_enqueue(lsqt_ham* h, int direction, double t, double2* state);
+int moments_f32_enqueue(lsqt_ham* h, const double2* left, const double2* right, int M);
+int evolve_f32_enqueue(lsqt_ham* h, int direction, double t, double2* state, int renormalise);
+int evolvex_f32_enqueue(lsqt_ham* h, int direction, double t, double2* state);
 
 }  // namespace lsqt
 

@@ -23,6 +23,7 @@ enum Layout { kFull = 0, kPackedLds = 1, kPackedGlobal = 2 };
 struct HamDev {
   int n;
   int K;
+  int stride;  // column stride of the pair arrays (n rounded up to 64)
   double emax;
   const double* pot;
   // packed layout
@@ -38,26 +39,16 @@ struct HamDev {
   const double* hi;
   const double* xx;
   int layout;
+  // FP32 copy of the potential (the precision path; null until first used)
+  const float* pot32;
 };
 
-// FP32 copies of the per-site / per-class data (the precision path)
-struct HamDevF32 {
-  int n;
-  int K;
-  float emax;
-  float inv_emax;
-  const float* pot;
-  const uint32_t* words;
-  uint32_t idx_mask;
-  int idx_bits;
-  const float2* tab;
-  const float* tab_xx;
-  int ncls;
-  const int* nl;
-  const float2* hop;   // full layout: (re, im) per pair, column-major
-  const float* xx;
-  int layout;
-};
+constexpr int kPairStrideAlign = 64;
+
+inline int pair_stride(int n)
+{
+  return (n + kPairStrideAlign - 1) / kPairStrideAlign * kPairStrideAlign;
+}
 
 // Streaming policy: arrays read once per step (potential, pair words,
 // psi_{m-2}, the left vector) are loaded non-temporally on large systems, so
@@ -69,6 +60,8 @@ constexpr int kStreamingMinSites = 1 << 20;
 
 typedef double dvec2_t __attribute__((ext_vector_type(2)));
 typedef float fvec2_t __attribute__((ext_vector_type(2)));
+typedef float fvec4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t uvec2_t __attribute__((ext_vector_type(2)));
 
 template <bool NT, typename T>
 __device__ __forceinline__ T ld(const T* p)

@@ -27,4 +27,23 @@ void launch_dot(const double2* psi, const double2* L, double2* partial, int n, h
 void launch_reduce_partials(const double2* partial, size_t stride, int tiles, int count,
                             double2* mu, hipStream_t st);
 
+// FP32-storage path (kernels_f32.hip)
+int hpsi32_tiles(int n);
+void launch_hpsi32(const HamDev& h, const float2* psi1, const float2* psi0, float2* out,
+                   const float2* L, double2* partial, bool recur, bool ip, hipStream_t st);
+void launch_comm32(const HamDev& h, const float2* in, float2* out, hipStream_t st);
+void launch_cheb2_32(const HamDev& h, const float2* s0, const float2* s1, float2* s2,
+                     double2* s, double bm, int label, hipStream_t st);
+void launch_cheb2x_32(const HamDev& h, const float2* s0, const float2* s0x, const float2* s1,
+                      const float2* s1x, float2* s2, float2* s2x, double2* s, double bm,
+                      int label, hipStream_t st);
+void launch_zero32(float2* v, int n, hipStream_t st);
+void launch_to_f32(const double2* src, float2* dst, int n, hipStream_t st);
+void launch_cheb01_32(const float2* s1, double2* s, double b0, double b1, int direction, int n,
+                      hipStream_t st);
+void launch_cheb1x_32(const float2* s1x, double2* s, double b1, int n, hipStream_t st);
+void launch_dot32(const float2* psi, const float2* L, double2* partial, int n, hipStream_t st);
+void launch_norm2(const double2* s, double2* partial, int n, hipStream_t st);
+void launch_renormalise(double2* s, const double2* norms, int n, hipStream_t st);
+
 }  // namespace lsqt

@@ -65,6 +65,9 @@ SIGNATURES = {
     "lsqt_evolve": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
     "lsqt_evolvex": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
     "lsqt_bessel_terms": (C.c_int, [C.c_double, _ip]),
+    "lsqt_moments_f32": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp, _dp]),
+    "lsqt_evolve_f32": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_int]),
+    "lsqt_evolvex_f32": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
 }
 
 _lib = None
@@ -265,6 +268,18 @@ class Hamiltonian:
 
     def evolvex(self, direction, t_scaled, state):
         _check(load().lsqt_evolvex(self.h, int(direction), float(t_scaled), state.h))
+
+    def moments_f32(self, left, right, M):
+        re, im = np.empty(M), np.empty(M)
+        _check(load().lsqt_moments_f32(self.h, left.h, right.h, int(M), _dptr(re), _dptr(im)))
+        return re + 1j * im
+
+    def evolve_f32(self, direction, t_scaled, state, renormalise=True):
+        _check(load().lsqt_evolve_f32(self.h, int(direction), float(t_scaled), state.h,
+                                      1 if renormalise else 0))
+
+    def evolvex_f32(self, direction, t_scaled, state):
+        _check(load().lsqt_evolvex_f32(self.h, int(direction), float(t_scaled), state.h))
 
     def close(self):
         if self.h:

@@ -162,6 +162,22 @@ int lsqt_evolve(lsqt_ham* h, int direction, double time_step_scaled, lsqt_vec* s
 /* evolvex (src/sigma.cu:157-200): state <- [X, U(t)] state (direction +1)
  * or [U(-t), X] state (direction -1) */
 int lsqt_evolvex(lsqt_ham* h, int direction, double time_step_scaled, lsqt_vec* state);
+/* ---- precision path --------------------------------------------------------
+ * (no counterpart in the reference: its only precision switch is the
+ * compile-time -DUSE_SP that makes everything FP32, src/common.h:22-26)
+ * Same results as lsqt_moments / lsqt_evolve / lsqt_evolvex to FP32
+ * accuracy: the Chebyshev states are stored in FP32 (half the bytes per
+ * step), every site update is computed in FP64 registers with the FP64
+ * hopping values, moments are accumulated in FP64, and the evolved state
+ * stays FP64.  Vectors cross the ABI in FP64 as above. */
+int lsqt_moments_f32(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right, int M,
+                     double* mu_re, double* mu_im);
+/* renormalise != 0 rescales the state to its norm before the step (U is
+ * unitary; this removes the FP32 recursion's drift of the norm) */
+int lsqt_evolve_f32(lsqt_ham* h, int direction, double time_step_scaled, lsqt_vec* state,
+                    int renormalise);
+int lsqt_evolvex_f32(lsqt_ham* h, int direction, double time_step_scaled, lsqt_vec* state);
+
 /* the number of Chebyshev terms evolve/evolvex use for time_step_scaled */
 int lsqt_bessel_terms(double time_step_scaled, int* last_m);
 
