@@ -162,3 +162,21 @@ def test_failure_messages(tmp_path):
             oracle.OracleModel(str(work))
         msg = str(e.value).replace(str(work), "<dir>")
         assert msg.split("\n")[0] == exp["message"], name
+
+
+@pytest.mark.slow
+def test_reference_example_localized_dos(tmp_path):
+    """The oracle reproduces the reference's checked-in fixed-seed DOS of
+    examples/fan2018cpc/lattice/localized (4.0M-site chain, W = 4, M = 500,
+    two random vectors: pins the RNG order across vectors) at 6 digits."""
+    src = os.path.join(GOLDEN, "fan2018cpc", "localized")
+    work = tmp_path / "localized"
+    shutil.copytree(src, work, ignore=shutil.ignore_patterns("expected"))
+    # DOS only: drop the MSD request
+    para = (work / "para.in").read_text().replace("calculate_msd", "")
+    (work / "para.in").write_text(para)
+    oracle.set_threads(min(8, os.cpu_count() or 1))
+    oracle.lsqt(str(work))
+    got = (work / "dos.out").read_text()
+    want = open(os.path.join(src, "expected", "dos.out")).read()
+    assert got == want
