@@ -147,6 +147,77 @@ def terms(t):
     return _terms_cache[t]
 
 
+def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
+    """Config 4: block-FP32 KPM (R vectors per sweep) against the plain FP64
+    single-vector kernel on the same system, and the DOS error of the block
+    path against FP64."""
+    from gpuqt_amd import hostlib, native
+    from gpuqt_amd import inputs as gi
+    t0 = time.perf_counter()
+    arrays, n = gi.cubic_peierls_arrays(L)
+    H = native.Hamiltonian.from_arrays(ctx, arrays, n, 6, 15.0)
+    del arrays
+    info = H.info()
+    log(f"cubic {L}^3: {n} sites, layout {'packed' if info.layout == 2 else 'full'}, "
+        f"{info.classes} classes, built in {time.perf_counter() - t0:.1f} s")
+    rng = np.random.default_rng(7)
+    ph = rng.uniform(0, 2 * np.pi, (R, n))
+    blk = native.Block(ctx, n, R)
+    blk.upload(np.cos(ph), np.sin(ph))
+    W, rows, blocked, us = H.block_tune(blk, blk)
+    log(f"lattice slab {info.slab} row {info.row}; autotuner: W={W} (VPW={R // W}) "
+        f"rows={rows} {'blocked' if blocked else 'natural'} schedule: {us:.1f} us/step")
+    v0 = H.vector(np.cos(ph[0]), np.sin(ph[0]))
+
+    def timed(fn):
+        fn()
+        ctx.sync()
+        best = 1e30
+        out = None
+        for _ in range(reps):
+            t = time.perf_counter()
+            out = fn()
+            ctx.sync()
+            best = min(best, time.perf_counter() - t)
+        return best, out
+
+    t64, mu64 = timed(lambda: H.moments(v0, v0, M))
+    t32, mu32 = timed(lambda: H.moments_f32(v0, v0, M))
+    tb, mub = timed(lambda: H.moments_block(blk, blk, M))
+    e = np.linspace(-14, 14, 201)
+    vol = float(L ** 3)
+    errs = []
+    for r in range(check_vectors):
+        ref = mu64 if r == 0 else H.moments(*(2 * [H.vector(np.cos(ph[r]), np.sin(ph[r]))]), M)
+        a = hostlib.correlation_row(mub[r], e, 15.0, vol)
+        b = hostlib.correlation_row(ref, e, 15.0, vol)
+        errs.append(float(np.abs(a - b).max() / np.abs(b).max()))
+    u64 = n * (M - 1) / t64
+    u32 = n * (M - 1) / t32
+    ub = n * R * (M - 1) / tb
+    bpu_blk = (4.0 + 4.0 * 6) / R + 32.0
+    bpu64 = info.bytes_per_site_f64
+    res = {
+        "workload": f"simple cubic {L}^3 ({n} sites), hopping -1, Peierls phase 2pi*0.01*x on "
+                    f"y-bonds, on-site U[-8.25,8.25], E_max=15, {M} moments, DOS",
+        "f64_single": {"value": u64, "unit": "site-updates/s", "bytes_per_site_update": bpu64,
+                       "achieved_GBs": u64 * bpu64 / 1e9},
+        "f32_single": {"value": u32, "unit": "site-updates/s", "speedup_vs_f64": u32 / u64},
+        "block_f32": {"R": R, "W": W, "vectors_per_wavefront": R // W, "rows_per_tile": rows,
+                      "tile_schedule": "blocked" if blocked else "natural",
+                      "value": ub, "unit": "site-updates/s", "speedup_vs_f64": ub / u64,
+                      "bytes_per_site_update": bpu_blk,
+                      "achieved_GBs": ub * bpu_blk / 1e9,
+                      "frac_of_hbm_peak": ub * bpu_blk / 1e9 / HBM_PEAK_GBS,
+                      "dos_normwise_error_vs_f64": errs},
+    }
+    log(f"block R={R}: {ub:.3e} site-updates/s = {ub / u64:.2f}x FP64 single "
+        f"({u64:.3e}); DOS error {max(errs):.2e}")
+    for o in (H, blk, v0):
+        o.close()
+    return res
+
+
 def cpu_baseline(nx, ny, moments_sample, threads):
     """Times the reference CPU program (compiled from source into
     oracle/_ref) or, where it is absent, the oracle, on a bounded sample of
@@ -195,6 +266,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", dest="precision_f32", action="store_false",
                     help="skip the precision-path (FP32 states) job")
+    ap.add_argument("--no-block", dest="block", action="store_false",
+                    help="skip the block-path (R vectors per sweep) measurement")
+    ap.add_argument("--cubic-L", type=int, default=160)
+    ap.add_argument("--block-R", type=int, default=32)
+    ap.add_argument("--block-moments", type=int, default=2048)
     ap.add_argument("--cpu-moments", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=16)
     args = ap.parse_args()
@@ -340,12 +416,15 @@ def main():
     }
     if f32 is not None:
         result["precision_f32"] = f32
+    if args.block:
+        result["batching_block_f32"] = bench_block(ctx, L=args.cubic_L, R=args.block_R,
+                                                   M=args.block_moments)
     if not args.no_cpu_baseline:
         log("CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(args.nx, args.ny, args.cpu_moments,
                                               args.cpu_threads)
-    H.close()
     rv.close()
+    H.close()
     ctx.close()
     print(json.dumps(result), flush=True)
 

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -230,6 +232,48 @@ int upload(lsqt_ham* h, const T** dst, const T* src, size_t count)
   return LSQT_OK;
 }
 
+// Finds a 3-level natural order (slabs of `slab` sites made of rows of `row`
+// sites) from the neighbour offsets |j - i|: the slab is the largest offset
+// carried by a regular fraction of the pairs (wrap-around offsets above n/2
+// excluded) that divides n, the row the largest smaller regular offset that
+// divides the slab.  Lattices numbered x-slowest (src/model_lattice.cu:162)
+// and the simple-cubic general inputs have this form; anything else gets 0.
+void detect_lattice_offsets(lsqt_ham* h, int n, int K, const int* nn, const int* nl)
+{
+  std::unordered_map<long long, long long> count;
+  long long pairs = 0;
+  for (int i = 0; i < n; ++i)
+    for (int m = 0; m < nn[i]; ++m) {
+      long long d = std::llabs((long long)nl[(size_t)i * K + m] - i);
+      if (d == 0 || 2 * d > n)
+        continue;
+      ++count[d];
+      ++pairs;
+      if (count.size() > 4096)
+        return;  // irregular structure
+    }
+  if (pairs == 0)
+    return;
+  std::vector<long long> regular;
+  for (auto& kv : count)
+    if (kv.second * 8 * K >= pairs)
+      regular.push_back(kv.first);
+  std::sort(regular.begin(), regular.end());
+  for (int a = (int)regular.size() - 1; a >= 0; --a) {
+    long long D = regular[a];
+    if (n % D != 0 || D < 64)
+      continue;
+    for (int b = a - 1; b >= 0; --b) {
+      long long d = regular[b];
+      if (D % d == 0 && d >= 8 && D / d >= 4) {
+        h->slab = (int)D;
+        h->row = (int)d;
+        return;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 int lsqt::ham_scratch(lsqt_ham* h, int count)
@@ -373,6 +417,7 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
     lsqt_ham_destroy(h);
     return rc;
   }
+  detect_lattice_offsets(h, n, K, nn, nl);
   // host copies for the FP32 / block paths, built on demand
   h->host_pot.assign(pot, pot + n);
   h->host_tab = tab;
@@ -411,6 +456,8 @@ extern "C" int lsqt_ham_get_info(const lsqt_ham* h, lsqt_ham_info* info)
   // psi_{m-2} 16 + psi_m 16 + left vector 16
   double per_slot = h->d.layout == kFull ? 20.0 : 4.0;
   info->bytes_per_site_f64 = 8.0 + per_slot * h->d.K + 64.0;
+  info->slab = h->slab;
+  info->row = h->row;
   return LSQT_OK;
 }
 

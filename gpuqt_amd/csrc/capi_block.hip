@@ -1,0 +1,480 @@
+// capi_block.hip -- the block path of the C ABI: blocks of R vectors, the
+// block moment recursion, and the autotuner that picks the kernel shape.
+//
+// Autotuner: for a Hamiltonian and R it times a few fused steps of every
+// candidate (W = vectors per lane in {1, 2, 4}, i.e. vectors-per-wavefront
+// VPW = R / W lanes per site; ROWS = sites per workgroup tile in {256, 1024,
+// 4096}) on the caller's data, keeps the fastest, and remembers it per
+// (R, z, table placement, size class) in memory and in a text cache file
+// ($LSQT_TUNE_CACHE, default $HOME/.cache/lsqt_hip_tune.txt).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "lsqt_capi_internal.h"
+#include "lsqt_hip.h"
+#include "lsqt_launch.h"
+
+using namespace lsqt;
+
+struct lsqt_block {
+  lsqt_ctx* ctx = nullptr;
+  int n = 0;
+  int R = 0;
+  float2* d = nullptr;  // [n][R]
+};
+
+namespace {
+struct BlockScratch {
+  float2* A = nullptr;
+  float2* B = nullptr;
+  size_t elems = 0;
+  double2* ring = nullptr;
+  size_t ring_cap = 0;
+  double2* mu = nullptr;
+  size_t mu_cap = 0;
+  double2* mu_host = nullptr;
+  size_t mu_host_cap = 0;
+};
+}  // namespace
+
+// a kernel shape: W vectors per lane (R/W lanes per site), tiles of `rows`
+// sites, natural or blocked tile schedule
+struct BlockShape {
+  int W = 0;
+  int rows = 0;
+  int blocked = 0;
+};
+
+struct lsqt_block_state {
+  BlockScratch s;
+  std::map<int, BlockShape> forced;  // R -> shape set by the caller
+};
+
+void lsqt::block_release(lsqt_ham* h)
+{
+  if (!h->blk)
+    return;
+  BlockScratch& s = h->blk->s;
+  for (void* p : {(void*)s.A, (void*)s.B, (void*)s.ring, (void*)s.mu})
+    if (p)
+      (void)hipFree(p);
+  if (s.mu_host)
+    (void)hipHostFree(s.mu_host);
+  delete h->blk;
+  h->blk = nullptr;
+}
+
+namespace {
+
+constexpr int kRing = 64;  // moment partials are reduced every kRing steps
+
+struct TuneKey {
+  int R, K, lds, size_class, slab;
+  bool operator<(const TuneKey& o) const
+  {
+    return std::tie(R, K, lds, size_class, slab) <
+           std::tie(o.R, o.K, o.lds, o.size_class, o.slab);
+  }
+};
+
+struct TuneVal {
+  BlockShape shape;
+  double us;
+};
+
+std::mutex g_tune_mutex;
+std::map<TuneKey, TuneVal> g_tune;
+bool g_tune_loaded = false;
+
+std::string tune_path()
+{
+  if (const char* p = getenv("LSQT_TUNE_CACHE"))
+    return p;
+  const char* home = getenv("HOME");
+  return std::string(home ? home : "/tmp") + "/.cache/lsqt_hip_tune.txt";
+}
+
+void tune_load()
+{
+  if (g_tune_loaded)
+    return;
+  g_tune_loaded = true;
+  FILE* f = fopen(tune_path().c_str(), "r");
+  if (!f)
+    return;
+  char arch[32];
+  TuneKey k;
+  TuneVal v;
+  while (fscanf(f, "%31s %d %d %d %d %d %d %d %d %lf", arch, &k.R, &k.K, &k.lds, &k.size_class,
+                &k.slab, &v.shape.W, &v.shape.rows, &v.shape.blocked, &v.us) == 10) {
+    if (std::strcmp(arch, "gfx950") == 0)
+      g_tune[k] = v;
+  }
+  fclose(f);
+}
+
+void tune_store(const TuneKey& k, const TuneVal& v)
+{
+  g_tune[k] = v;
+  const std::string path = tune_path();
+  std::error_code ec;
+  std::filesystem::create_directories(std::filesystem::path(path).parent_path(), ec);
+  FILE* f = fopen(path.c_str(), "a");
+  if (!f)
+    return;  // the cache file is an optimisation only
+  fprintf(f, "gfx950 %d %d %d %d %d %d %d %d %.3f\n", k.R, k.K, k.lds, k.size_class, k.slab,
+          v.shape.W, v.shape.rows, v.shape.blocked, v.us);
+  fclose(f);
+}
+
+int size_class(int n)
+{
+  int c = 0;
+  while ((1 << (c + 1)) <= n)
+    ++c;
+  return c;
+}
+
+bool valid_R(int R)
+{
+  return R >= 1 && R <= 256 && (R & (R - 1)) == 0;
+}
+
+bool valid_W(int R, int W)
+{
+  return (W == 1 || W == 2 || W == 4) && R % W == 0 && R / W <= 64;
+}
+
+int block_alloc(lsqt_ctx* ctx, size_t count, float2** out)
+{
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, sizeof(float2) * (count ? count : 1));
+  if (e != hipSuccess)
+    return set_error(LSQT_ERR_NOMEM, "hipMalloc(%zu float2): %s", count, hipGetErrorString(e));
+  *out = (float2*)p;
+  (void)ctx;
+  return LSQT_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// blocks
+// ---------------------------------------------------------------------------
+extern "C" int lsqt_block_create(lsqt_ctx* ctx, int n, int R, lsqt_block** out)
+{
+  if (!ctx || !out)
+    return set_error(LSQT_ERR_INVALID, "null context or output handle");
+  *out = nullptr;
+  if (n <= 0 || !valid_R(R))
+    return set_error(LSQT_ERR_INVALID, "block of %d sites x %d vectors: R must be a power of 2 "
+                     "in 1..256", n, R);
+  LSQT_TRY(ctx_activate(ctx));
+  lsqt_block* b = new lsqt_block();
+  b->ctx = ctx;
+  b->n = n;
+  b->R = R;
+  int rc = block_alloc(ctx, (size_t)n * R, &b->d);
+  if (rc != LSQT_OK) {
+    delete b;
+    return rc;
+  }
+  launch_zero32(b->d, (int)((size_t)n * R), ctx->stream);
+  *out = b;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_block_destroy(lsqt_block* b)
+{
+  if (!b)
+    return LSQT_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  (void)hipFree(b->d);
+  delete b;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_block_from_host(lsqt_block* b, const double* re, const double* im)
+{
+  if (!b || !re || !im)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  LSQT_TRY(ctx_activate(b->ctx));
+  const size_t n = (size_t)b->n, R = (size_t)b->R;
+  std::vector<float2> tmp(n * R);
+  for (size_t r = 0; r < R; ++r)
+    for (size_t i = 0; i < n; ++i)
+      tmp[i * R + r] = make_float2((float)re[r * n + i], (float)im[r * n + i]);
+  LSQT_HIP(hipStreamSynchronize(b->ctx->stream));
+  LSQT_HIP(hipMemcpy(b->d, tmp.data(), sizeof(float2) * n * R, hipMemcpyHostToDevice));
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_block_to_host(const lsqt_block* b, double* re, double* im)
+{
+  if (!b || !re || !im)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  LSQT_TRY(ctx_activate(b->ctx));
+  const size_t n = (size_t)b->n, R = (size_t)b->R;
+  std::vector<float2> tmp(n * R);
+  LSQT_HIP(hipStreamSynchronize(b->ctx->stream));
+  LSQT_HIP(hipMemcpy(tmp.data(), b->d, sizeof(float2) * n * R, hipMemcpyDeviceToHost));
+  for (size_t r = 0; r < R; ++r)
+    for (size_t i = 0; i < n; ++i) {
+      re[r * n + i] = tmp[i * R + r].x;
+      im[r * n + i] = tmp[i * R + r].y;
+    }
+  return LSQT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// block moments
+// ---------------------------------------------------------------------------
+namespace {
+
+int scratch_for(lsqt_ham* h, int R, int M, int tiles, BlockScratch** out)
+{
+  if (!h->blk)
+    h->blk = new lsqt_block_state();
+  BlockScratch& s = h->blk->s;
+  const size_t elems = (size_t)h->d.n * R;
+  if (s.elems < elems) {
+    if (s.A)
+      (void)hipFree(s.A);
+    if (s.B)
+      (void)hipFree(s.B);
+    s.A = s.B = nullptr;
+    s.elems = 0;
+    LSQT_TRY(block_alloc(h->ctx, elems, &s.A));
+    LSQT_TRY(block_alloc(h->ctx, elems, &s.B));
+    s.elems = elems;
+  }
+  LSQT_TRY(ensure_device((void**)&s.ring, &s.ring_cap,
+                         sizeof(double2) * (size_t)kRing * tiles * R));
+  LSQT_TRY(ensure_device((void**)&s.mu, &s.mu_cap, sizeof(double2) * (size_t)M * R));
+  LSQT_TRY(ensure_pinned((void**)&s.mu_host, &s.mu_host_cap, sizeof(double2) * (size_t)M * R));
+  *out = &s;
+  return LSQT_OK;
+}
+
+// enqueues steps [from, to) of the recursion; step m's partials go to ring
+// slot m % kRing, and full rings are reduced into mu
+TileMap tile_map(const lsqt_ham* h, const BlockShape& c)
+{
+  return make_tile_map(h->d.n, c.rows, c.blocked ? h->slab : 0, c.blocked ? c.rows : 0);
+}
+
+void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const float2* L, float2*& A,
+                   float2*& B, double2* ring, double2* mu, int M, int from, int to,
+                   hipStream_t st)
+{
+  const TileMap tm = tile_map(h, c);
+  const int tiles = tm.tiles;
+  const size_t slot = (size_t)tiles * R;
+  for (int m = from; m < to; ++m) {
+    double2* part = ring + (size_t)(m % kRing) * slot;
+    if (m == 0) {
+      launch_block_dot(A, L, R, tm, part, st);
+    } else if (m == 1) {
+      launch_block_step(h->d, R, c.W, tm, A, nullptr, B, L, part, false, true, st);
+    } else {
+      launch_block_step(h->d, R, c.W, tm, B, A, A, L, part, true, true, st);
+      float2* t = A;
+      A = B;
+      B = t;
+    }
+    if (mu && (m % kRing == kRing - 1 || m == M - 1)) {
+      int m0 = m - m % kRing;
+      launch_block_reduce(ring, tiles, R, M, m0, m - m0 + 1, mu, st);
+    }
+  }
+}
+
+int pick_config(lsqt_ham* h, int R, const lsqt_block* left, const lsqt_block* right,
+                BlockShape* out, double* us_out, bool force);
+
+}  // namespace
+
+static int check_block(lsqt_ham* h, const lsqt_block* a, const lsqt_block* b)
+{
+  if (!h || !a || !b)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  if (a->n != h->d.n || b->n != h->d.n || a->R != b->R || a->ctx != h->ctx || b->ctx != h->ctx)
+    return set_error(LSQT_ERR_INVALID, "block shapes do not match the Hamiltonian");
+  if (h->d.layout == kFull)
+    return set_error(LSQT_ERR_UNSUPPORTED, "the block path needs the packed layout");
+  return ctx_activate(h->ctx);
+}
+
+extern "C" int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int blocked)
+{
+  if (!h || !valid_R(R))
+    return set_error(LSQT_ERR_INVALID, "bad Hamiltonian or R");
+  if (!h->blk)
+    h->blk = new lsqt_block_state();
+  if (W == 0) {
+    h->blk->forced.erase(R);
+    return LSQT_OK;
+  }
+  if (!valid_W(R, W) || rows < 64 || rows % 64 != 0)
+    return set_error(LSQT_ERR_INVALID, "W=%d rows=%d invalid for R=%d", W, rows, R);
+  if (blocked && (h->slab == 0 || h->slab % rows != 0))
+    return set_error(LSQT_ERR_INVALID, "blocked tiles of %d sites need a lattice slab they "
+                     "divide (slab %d)", rows, h->slab);
+  BlockShape c;
+  c.W = W;
+  c.rows = rows;
+  c.blocked = blocked ? 1 : 0;
+  h->blk->forced[R] = c;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right,
+                                  int M, double* mu_re, double* mu_im)
+{
+  LSQT_TRY(check_block(h, left, right));
+  if (M < 2)
+    return set_error(LSQT_ERR_INVALID, "number of moments %d must be >= 2", M);
+  if (!mu_re || !mu_im)
+    return set_error(LSQT_ERR_INVALID, "null moment output");
+  LSQT_TRY(f32_potential(h));
+  const int R = left->R;
+  BlockShape c;
+  double us;
+  LSQT_TRY(pick_config(h, R, left, right, &c, &us, false));
+  BlockScratch* s;
+  LSQT_TRY(scratch_for(h, R, M, tile_map(h, c).tiles, &s));
+  hipStream_t st = h->ctx->stream;
+  float2* A = s->A;
+  float2* B = s->B;
+  LSQT_HIP(hipMemcpyAsync(A, right->d, sizeof(float2) * (size_t)h->d.n * R,
+                          hipMemcpyDeviceToDevice, st));
+  enqueue_steps(h, R, c, left->d, A, B, s->ring, s->mu, M, 0, M, st);
+  LSQT_HIP(hipGetLastError());
+  LSQT_HIP(hipMemcpyAsync(s->mu_host, s->mu, sizeof(double2) * (size_t)M * R,
+                          hipMemcpyDeviceToHost, st));
+  LSQT_HIP(hipStreamSynchronize(st));
+  for (size_t i = 0; i < (size_t)M * R; ++i) {
+    mu_re[i] = s->mu_host[i].x;
+    mu_im[i] = s->mu_host[i].y;
+  }
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right,
+                               int force, int* W, int* rows, int* blocked, double* us_per_step)
+{
+  LSQT_TRY(check_block(h, left, right));
+  LSQT_TRY(f32_potential(h));
+  BlockShape c;
+  double us;
+  LSQT_TRY(pick_config(h, left->R, left, right, &c, &us, force != 0));
+  if (W)
+    *W = c.W;
+  if (rows)
+    *rows = c.rows;
+  if (blocked)
+    *blocked = c.blocked;
+  if (us_per_step)
+    *us_per_step = us;
+  return LSQT_OK;
+}
+
+namespace {
+
+int pick_config(lsqt_ham* h, int R, const lsqt_block* left, const lsqt_block* right,
+                BlockShape* out, double* us_out, bool force)
+{
+  if (h->blk) {
+    auto fit = h->blk->forced.find(R);
+    if (fit != h->blk->forced.end()) {
+      *out = fit->second;
+      *us_out = 0.0;
+      return LSQT_OK;
+    }
+  }
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  tune_load();
+  // the slab of a detected lattice enters the key: blocked shapes exist only
+  // for lattices, and their best tile size depends on the slab
+  TuneKey key{R, h->d.K, h->d.layout == kPackedLds ? 1 : 0, size_class(h->d.n), h->slab};
+  auto it = g_tune.find(key);
+  if (it != g_tune.end() && !force) {
+    *out = it->second.shape;
+    *us_out = it->second.us;
+    return LSQT_OK;
+  }
+  // candidates: natural tiles of 256..4096 sites; on a detected lattice also
+  // blocked tiles of a whole number of rows that divide the slab
+  std::vector<BlockShape> cands;
+  for (int W : {1, 2, 4}) {
+    if (!valid_W(R, W))
+      continue;
+    for (int rows : {256, 1024, 4096}) {
+      if (rows < kBlock / (R / W))
+        continue;
+      BlockShape c;
+      c.W = W;
+      c.rows = rows;
+      cands.push_back(c);
+    }
+    if (h->slab > 0) {
+      for (int span = h->row; span <= 8192 && span <= h->slab; span += h->row) {
+        if (h->slab % span != 0 || span % 64 != 0 || span < 512 || span < kBlock / (R / W))
+          continue;
+        BlockShape c;
+        c.W = W;
+        c.rows = span;
+        c.blocked = 1;
+        cands.push_back(c);
+      }
+    }
+  }
+  const int steps = 6;
+  int max_tiles = 0;
+  for (const BlockShape& c : cands)
+    max_tiles = std::max(max_tiles, tile_map(h, c).tiles);
+  BlockScratch* s;
+  LSQT_TRY(scratch_for(h, R, steps, max_tiles, &s));
+  hipStream_t st = h->ctx->stream;
+  hipEvent_t e0, e1;
+  LSQT_HIP(hipEventCreate(&e0));
+  LSQT_HIP(hipEventCreate(&e1));
+  TuneVal best{BlockShape{}, 1e300};
+  for (const BlockShape& c : cands) {
+    // time a few fused steps on the caller's blocks
+    float2* A = s->A;
+    float2* B = s->B;
+    LSQT_HIP(hipMemcpyAsync(A, right->d, sizeof(float2) * (size_t)h->d.n * R,
+                            hipMemcpyDeviceToDevice, st));
+    enqueue_steps(h, R, c, left->d, A, B, s->ring, nullptr, steps, 0, 2, st);
+    LSQT_HIP(hipEventRecord(e0, st));
+    enqueue_steps(h, R, c, left->d, A, B, s->ring, nullptr, steps, 2, steps, st);
+    LSQT_HIP(hipEventRecord(e1, st));
+    LSQT_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    LSQT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    double us = 1e3 * ms / (steps - 2);
+    if (us < best.us)
+      best = TuneVal{c, us};
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  LSQT_HIP(hipGetLastError());
+  tune_store(key, best);
+  *out = best.shape;
+  *us_out = best.us;
+  return LSQT_OK;
+}
+
+}  // namespace

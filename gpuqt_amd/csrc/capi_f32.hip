@@ -21,6 +21,7 @@ struct lsqt_f32_state {
 
 void lsqt::ham_release_extra(lsqt_ham* h)
 {
+  block_release(h);
   if (!h->f32)
     return;
   if (h->f32->pot32)
@@ -74,6 +75,11 @@ static int f32_prepare(lsqt_ham* h, int nvec)
     f->norms = (double2*)d;
   }
   return LSQT_OK;
+}
+
+int lsqt::f32_potential(lsqt_ham* h)
+{
+  return f32_prepare(h, 0);
 }
 
 static int check_f32(lsqt_ham* h, std::initializer_list<const lsqt_vec*> vs)

@@ -1,0 +1,411 @@
+// kernels_block.hip -- the block path: R random vectors per sparse sweep.
+//
+// A block holds R complex vectors stored site-major with the R values of a
+// site contiguous ([site][R] float2), so that
+//   - the Hamiltonian (potential, pair words) is read once per site for all R
+//     vectors: (4 + 4z) / R bytes per site-update instead of 4 + 4z;
+//   - a neighbour gather reads R*8 contiguous bytes (256 B at R = 32).
+// VPW lanes share a site ("vectors per wavefront" = 64/VPW sites per wave),
+// each lane carrying W = R/VPW adjacent vectors (8W-byte vector loads); a
+// workgroup sweeps a tile of ROWS consecutive sites ("rows per block").  Both
+// are picked per (R, z, layout, size) by the autotuner (capi_block.hip).
+//
+// Precision as in the precision path: FP32 states and left vectors, FP64
+// hopping values and arithmetic, FP64 moment products reduced per vector by a
+// fixed shuffle / LDS tree into per-tile partials [M][tiles][R].
+#include "lsqt_device.h"
+#include "lsqt_launch.h"
+
+namespace lsqt {
+
+constexpr int kBlockLdsTableMax = 1024;
+
+template <int W>
+struct Vec {
+  float2 v[W];
+};
+
+template <int W, bool NT>
+__device__ __forceinline__ Vec<W> ldv(const float2* p)
+{
+  Vec<W> r;
+  if constexpr (W == 1) {
+    if constexpr (NT) {
+      fvec2_t t = __builtin_nontemporal_load(reinterpret_cast<const fvec2_t*>(p));
+      r.v[0] = make_float2(t.x, t.y);
+    } else {
+      r.v[0] = *p;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < W; i += 2) {
+      fvec4_t t;
+      if constexpr (NT)
+        t = __builtin_nontemporal_load(reinterpret_cast<const fvec4_t*>(p + i));
+      else
+        t = *reinterpret_cast<const fvec4_t*>(p + i);
+      r.v[i] = make_float2(t.x, t.y);
+      r.v[i + 1] = make_float2(t.z, t.w);
+    }
+  }
+  return r;
+}
+
+template <int W>
+__device__ __forceinline__ void stv(float2* p, const Vec<W>& r)
+{
+  if constexpr (W == 1) {
+    *p = r.v[0];
+  } else {
+#pragma unroll
+    for (int i = 0; i < W; i += 2) {
+      fvec4_t t;
+      t.x = r.v[i].x;
+      t.y = r.v[i].y;
+      t.z = r.v[i + 1].x;
+      t.w = r.v[i + 1].y;
+      *reinterpret_cast<fvec4_t*>(p + i) = t;
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_range(const TileMap& t, int tile, int& begin, int& end)
+{
+  if (t.slab == 0) {
+    begin = tile * t.rows;
+    end = min(begin + t.rows, t.n);
+  } else {
+    // blocked schedule: tile = (row block, slab) with slabs fastest, so the
+    // tiles that run together are the same rows of neighbouring slabs
+    const int rb = tile / t.slabs, sl = tile % t.slabs;
+    begin = sl * t.slab + rb * t.span;
+    end = begin + t.span;
+  }
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ bool decode_blk(const HamDev& h, const double2* s_tab, uint32_t w,
+                                           int& j, double& a, double& b)
+{
+  if (w == kEmptySlot)
+    return false;
+  j = (int)(w & h.idx_mask);
+  int c = (int)(w >> h.idx_bits);
+  double2 t = (LAYOUT == kPackedLds) ? s_tab[c] : h.tab[c];
+  a = t.x;
+  b = t.y;
+  return true;
+}
+
+// One Chebyshev step for R vectors (RECUR: out = 2 H~ psi1 - psi0, else
+// out = H~ psi1), fused with the per-vector moment partials when IP.
+template <int LAYOUT, int KT, int W, bool RECUR, bool IP, bool NT>
+__global__ __launch_bounds__(kBlock) void k_block_step(
+  HamDev h, int R, TileMap tm, const float2* __restrict__ psi1, const float2* psi0, float2* out,
+  const float2* __restrict__ L, double2* __restrict__ partial, int tiles, int blocks)
+{
+  __shared__ double2 s_tab[LAYOUT == kPackedLds ? kBlockLdsTableMax : 1];
+  __shared__ double s_red[2][kBlock / 64][64 * 4];  // [re/im][wave][vector] (R <= 256)
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  if constexpr (LAYOUT == kPackedLds) {
+    for (int i = threadIdx.x; i < h.ncls; i += kBlock)
+      s_tab[i] = h.tab[i];
+    __syncthreads();
+  }
+  const int K = KT > 0 ? KT : h.K;
+  const int vpw = R / W;                       // lanes per site
+  const int per_pass = kBlock / vpw;           // sites per workgroup pass
+  const int lane_site = threadIdx.x / vpw;
+  const int r0 = (threadIdx.x % vpw) * W;      // first vector of this lane
+  int begin, end;
+  tile_range(tm, tile, begin, end);
+  const double inv_scale = 1.0 / h.emax;
+  double acc_re[W], acc_im[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    acc_re[w] = acc_im[w] = 0.0;
+
+  for (int s = begin + lane_site; s < end; s += per_pass) {
+    const size_t base = (size_t)s * R + r0;
+    Vec<W> p1 = ldv<W, false>(psi1 + base);
+    double v = (double)ld<NT>(h.pot32 + s);
+    double tr[W], ti[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      tr[w] = v * (double)p1.v[w].x;
+      ti[w] = v * (double)p1.v[w].y;
+    }
+    constexpr int C = SlotBatch<KT>::C;
+    for (int m0 = 0; m0 < K; m0 += C) {
+      int j[C];
+      double a[C], b[C], x[C];
+      bool ok[C];
+      load_slots<LAYOUT, C, NT, false>(h, s_tab, nullptr, s, m0, K, j, a, b, x, ok);
+      Vec<W> c[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q)
+        c[q] = ldv<W, false>(psi1 + (size_t)j[q] * R + r0);
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        if (ok[q]) {
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            double cx = c[q].v[w].x, cy = c[q].v[w].y;
+            tr[w] += a[q] * cx - b[q] * cy;
+            ti[w] += a[q] * cy + b[q] * cx;
+          }
+        }
+      }
+    }
+    Vec<W> o;
+    if constexpr (RECUR) {
+      Vec<W> p0 = ldv<W, NT>(psi0 + base);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        o.v[w].x = (float)(2.0 * (tr[w] * inv_scale) - (double)p0.v[w].x);
+        o.v[w].y = (float)(2.0 * (ti[w] * inv_scale) - (double)p0.v[w].y);
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        o.v[w].x = (float)(tr[w] * inv_scale);
+        o.v[w].y = (float)(ti[w] * inv_scale);
+      }
+    }
+    stv<W>(out + base, o);
+    if constexpr (IP) {
+      Vec<W> l = ldv<W, NT>(L + base);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        double ox = o.v[w].x, oy = o.v[w].y, lx = l.v[w].x, ly = l.v[w].y;
+        acc_re[w] += ox * lx + oy * ly;
+        acc_im[w] += oy * lx - ox * ly;
+      }
+    }
+  }
+
+  if constexpr (IP) {
+    // lanes holding the same vectors differ in the site bits of the lane id
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      for (int off = vpw; off < 64; off <<= 1) {
+        acc_re[w] += __shfl_xor(acc_re[w], off, 64);
+        acc_im[w] += __shfl_xor(acc_im[w], off, 64);
+      }
+    }
+    if (lane < vpw) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        s_red[0][wave][r0 + w] = acc_re[w];
+        s_red[1][wave][r0 + w] = acc_im[w];
+      }
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < R; r += kBlock) {
+      double re = s_red[0][0][r], im = s_red[1][0][r];
+#pragma unroll
+      for (int wv = 1; wv < kBlock / 64; ++wv) {
+        re += s_red[0][wv][r];
+        im += s_red[1][wv][r];
+      }
+      partial[(size_t)tile * R + r] = make_double2(re, im);
+    }
+  }
+}
+
+// mu[r][m0 + i] = sum_t partial[i][t][r] in tile order, i < gridDim.x
+__global__ __launch_bounds__(kBlock) void k_block_reduce(const double2* __restrict__ partial,
+                                                         int tiles, int R, int M, int m0,
+                                                         double2* __restrict__ mu)
+{
+  __shared__ double s_re[kBlock], s_im[kBlock];
+  const int m = m0 + blockIdx.x;
+  const double2* p = partial + (size_t)blockIdx.x * tiles * R;
+  for (int r = blockIdx.y; r < R; r += gridDim.y) {
+    double re = 0.0, im = 0.0;
+    for (int t = threadIdx.x; t < tiles; t += kBlock) {
+      double2 v = p[(size_t)t * R + r];
+      re += v.x;
+      im += v.y;
+    }
+    s_re[threadIdx.x] = re;
+    s_im[threadIdx.x] = im;
+    __syncthreads();
+    for (int half = kBlock / 2; half > 0; half >>= 1) {
+      if (threadIdx.x < half) {
+        s_re[threadIdx.x] += s_re[threadIdx.x + half];
+        s_im[threadIdx.x] += s_im[threadIdx.x + half];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0)
+      mu[(size_t)r * M + m] = make_double2(s_re[0], s_im[0]);
+    __syncthreads();
+  }
+}
+
+// partial[t][r] = sum over the tile of conj(L) psi for each vector (moment 0)
+__global__ __launch_bounds__(kBlock) void k_block_dot(const float2* __restrict__ psi,
+                                                      const float2* __restrict__ L, int R,
+                                                      TileMap tm, double2* __restrict__ partial,
+                                                      int tiles, int blocks)
+{
+  __shared__ double s_re[kBlock], s_im[kBlock];
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  int b0, e0;
+  tile_range(tm, tile, b0, e0);
+  const size_t begin = (size_t)b0 * R;
+  const size_t end = (size_t)e0 * R;
+  // thread t accumulates vector t % R over the tile, for R dividing kBlock
+  double re = 0.0, im = 0.0;
+  for (size_t i = begin + threadIdx.x; i < end; i += kBlock) {
+    float2 p = psi[i], l = L[i];
+    double px = p.x, py = p.y, lx = l.x, ly = l.y;
+    re += px * lx + py * ly;
+    im += py * lx - px * ly;
+  }
+  s_re[threadIdx.x] = re;
+  s_im[threadIdx.x] = im;
+  __syncthreads();
+  for (int r = threadIdx.x; r < R; r += kBlock) {
+    double a = 0.0, b = 0.0;
+    for (int k = r; k < kBlock; k += R) {
+      a += s_re[k];
+      b += s_im[k];
+    }
+    partial[(size_t)tile * R + r] = make_double2(a, b);
+  }
+}
+
+// ==========================================================================
+// launchers
+// ==========================================================================
+// the block kernel stages class tables of up to kBlockLdsTableMax entries
+// in LDS (one table per tile of `rows` sites costs little next to R vectors)
+#define LSQTB_LAYOUT_SWITCH(h, L, ...)                                   \
+  if (h.layout == kPackedLds ||                                          \
+      (h.layout == kPackedGlobal && h.ncls <= kBlockLdsTableMax)) {      \
+    constexpr int L = kPackedLds;           \
+    __VA_ARGS__;                            \
+  } else {                                  \
+    constexpr int L = kPackedGlobal;        \
+    __VA_ARGS__;                            \
+  }
+
+#define LSQTB_K_SWITCH(kval, KT, ...) \
+  switch (kval) {                     \
+    case 3: {                         \
+      constexpr int KT = 3;           \
+      __VA_ARGS__;                    \
+    } break;                          \
+    case 4: {                         \
+      constexpr int KT = 4;           \
+      __VA_ARGS__;                    \
+    } break;                          \
+    case 6: {                         \
+      constexpr int KT = 6;           \
+      __VA_ARGS__;                    \
+    } break;                          \
+    default: {                        \
+      constexpr int KT = 0;           \
+      __VA_ARGS__;                    \
+    } break;                          \
+  }
+
+#define LSQTB_W_SWITCH(wval, W, ...) \
+  switch (wval) {                    \
+    case 1: {                        \
+      constexpr int W = 1;           \
+      __VA_ARGS__;                   \
+    } break;                         \
+    case 2: {                        \
+      constexpr int W = 2;           \
+      __VA_ARGS__;                   \
+    } break;                         \
+    default: {                       \
+      constexpr int W = 4;           \
+      __VA_ARGS__;                   \
+    } break;                         \
+  }
+
+TileMap make_tile_map(int n, int rows, int slab, int span)
+{
+  TileMap t{};
+  t.n = n;
+  t.rows = rows;
+  if (slab > 0 && span > 0 && slab % span == 0 && n % slab == 0) {
+    t.slab = slab;
+    t.span = span;
+    t.slabs = n / slab;
+    t.tiles = (slab / span) * t.slabs;
+  } else {
+    t.tiles = (n + rows - 1) / rows;
+  }
+  return t;
+}
+
+Grid block_grid(const TileMap& t)
+{
+  int blocks = (t.tiles + kXcds - 1) / kXcds * kXcds;
+  return Grid{t.tiles, blocks};
+}
+
+void launch_block_step(const HamDev& h, int R, int W, const TileMap& tm, const float2* psi1,
+                       const float2* psi0, float2* out, const float2* L, double2* partial,
+                       bool recur, bool ip, hipStream_t st)
+{
+  Grid g = block_grid(tm);
+  const bool nt = (size_t)h.n * R >= (size_t)kStreamingMinSites;
+  LSQTB_LAYOUT_SWITCH(h, LAY, LSQTB_K_SWITCH(h.K, KT, LSQTB_W_SWITCH(W, WW, {
+    if (nt) {
+      if (recur && ip)
+        k_block_step<LAY, KT, WW, true, true, true><<<g.blocks, kBlock, 0, st>>>(
+          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+      else if (ip)
+        k_block_step<LAY, KT, WW, false, true, true><<<g.blocks, kBlock, 0, st>>>(
+          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+      else if (recur)
+        k_block_step<LAY, KT, WW, true, false, true><<<g.blocks, kBlock, 0, st>>>(
+          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+      else
+        k_block_step<LAY, KT, WW, false, false, true><<<g.blocks, kBlock, 0, st>>>(
+          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+    } else {
+      if (recur && ip)
+        k_block_step<LAY, KT, WW, true, true, false><<<g.blocks, kBlock, 0, st>>>(
+          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+      else if (ip)
+        k_block_step<LAY, KT, WW, false, true, false><<<g.blocks, kBlock, 0, st>>>(
+          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+      else if (recur)
+        k_block_step<LAY, KT, WW, true, false, false><<<g.blocks, kBlock, 0, st>>>(
+          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+      else
+        k_block_step<LAY, KT, WW, false, false, false><<<g.blocks, kBlock, 0, st>>>(
+          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+    }
+  })))
+}
+
+void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& tm,
+                      double2* partial, hipStream_t st)
+{
+  Grid g = block_grid(tm);
+  k_block_dot<<<g.blocks, kBlock, 0, st>>>(psi, L, R, tm, partial, g.tiles, g.blocks);
+}
+
+void launch_block_reduce(const double2* partial, int tiles, int R, int M, int m0, int count,
+                         double2* mu, hipStream_t st)
+{
+  if (count <= 0)
+    return;
+  dim3 grid(count, R < 8 ? R : 8);
+  k_block_reduce<<<grid, kBlock, 0, st>>>(partial, tiles, R, M, m0, mu);
+}
+
+}  // namespace lsqt

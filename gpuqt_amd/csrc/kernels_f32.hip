@@ -32,30 +32,6 @@ __device__ __forceinline__ float2 ldf2(const float2* p)
   }
 }
 
-template <int LAYOUT, bool NT>
-__device__ __forceinline__ bool pair32(const HamDev& h, const double2* s_tab, size_t q, int& j,
-                                       double& a, double& b)
-{
-  if constexpr (LAYOUT == kFull) {
-    j = ld<NT>(h.nl + q);
-    if (j < 0)
-      return false;
-    a = ld<NT>(h.hr + q);
-    b = ld<NT>(h.hi + q);
-    return true;
-  } else {
-    uint32_t w = ld<NT>(h.words + q);
-    if (w == kEmptySlot)
-      return false;
-    j = (int)(w & h.idx_mask);
-    int c = (int)(w >> h.idx_bits);
-    double2 t = (LAYOUT == kPackedLds) ? s_tab[c] : h.tab[c];
-    a = t.x;
-    b = t.y;
-    return true;
-  }
-}
-
 template <int LAYOUT>
 __device__ __forceinline__ bool decode32(const HamDev& h, const double2* s_tab, uint32_t w,
                                          int& j, double& a, double& b)
@@ -68,33 +44,6 @@ __device__ __forceinline__ bool decode32(const HamDev& h, const double2* s_tab, 
   a = t.x;
   b = t.y;
   return true;
-}
-
-template <int LAYOUT, bool NT>
-__device__ __forceinline__ bool pair32x(const HamDev& h, const double2* s_tab,
-                                        const double* s_xx, size_t q, int& j, double& a,
-                                        double& b, double& x)
-{
-  if constexpr (LAYOUT == kFull) {
-    j = ld<NT>(h.nl + q);
-    if (j < 0)
-      return false;
-    a = ld<NT>(h.hr + q);
-    b = ld<NT>(h.hi + q);
-    x = ld<NT>(h.xx + q);
-    return true;
-  } else {
-    uint32_t w = ld<NT>(h.words + q);
-    if (w == kEmptySlot)
-      return false;
-    j = (int)(w & h.idx_mask);
-    int c = (int)(w >> h.idx_bits);
-    double2 t = (LAYOUT == kPackedLds) ? s_tab[c] : h.tab[c];
-    a = t.x;
-    b = t.y;
-    x = (LAYOUT == kPackedLds) ? s_xx[c] : h.tab_xx[c];
-    return true;
-  }
 }
 
 template <int LAYOUT, bool WITH_XX>
@@ -141,20 +90,28 @@ __device__ __forceinline__ void hpsi32_site(const HamDev& h, const double2* s_ta
                                             const float2* __restrict__ psi1, int n, double& tr,
                                             double& ti)
 {
+  constexpr int C = SlotBatch<KT>::C;
   const int K = KT > 0 ? KT : h.K;
   float2 p1 = psi1[n];
   double v = (double)ld<NT>(h.pot32 + n);
   tr = v * (double)p1.x;
   ti = v * (double)p1.y;
+  for (int m0 = 0; m0 < K; m0 += C) {
+    int j[C];
+    double a[C], b[C], x[C];
+    bool ok[C];
+    load_slots<LAYOUT, C, NT, false>(h, s_tab, nullptr, n, m0, K, j, a, b, x, ok);
+    float2 c[C];
 #pragma unroll
-  for (int m = 0; m < K; ++m) {
-    int j;
-    double a, b;
-    if (pair32<LAYOUT, NT>(h, s_tab, (size_t)m * h.stride + n, j, a, b)) {
-      float2 c = psi1[j];
-      double cx = c.x, cy = c.y;
-      tr += a * cx - b * cy;
-      ti += a * cy + b * cx;
+    for (int q = 0; q < C; ++q)
+      c[q] = psi1[j[q]];
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      if (ok[q]) {
+        double cx = c[q].x, cy = c[q].y;
+        tr += a[q] * cx - b[q] * cy;
+        ti += a[q] * cy + b[q] * cx;
+      }
     }
   }
   tr /= h.emax;
@@ -187,22 +144,44 @@ __global__ __launch_bounds__(kBlock) void k_hpsi32(HamDev h, const float2* __res
     fvec2_t v = ld<NT>(reinterpret_cast<const fvec2_t*>(h.pot32 + n));
     double tr0 = (double)v.x * (double)p1.x, ti0 = (double)v.x * (double)p1.y;
     double tr1 = (double)v.y * (double)p1.z, ti1 = (double)v.y * (double)p1.w;
+    constexpr int C = SlotBatch<KT>::C;
+    for (int m0 = 0; m0 < K; m0 += C) {
+      uvec2_t w[C];
 #pragma unroll
-    for (int m = 0; m < K; ++m) {
-      uvec2_t w = ld<NT>(reinterpret_cast<const uvec2_t*>(h.words + (size_t)m * h.stride + n));
-      int j;
-      double a, b;
-      if (decode32<LAYOUT>(h, s_tab, w.x, j, a, b)) {
-        float2 c = psi1[j];
-        double cx = c.x, cy = c.y;
-        tr0 += a * cx - b * cy;
-        ti0 += a * cy + b * cx;
+      for (int q = 0; q < C; ++q)
+        w[q] = (m0 + q < K)
+                 ? ld<NT>(reinterpret_cast<const uvec2_t*>(h.words + (size_t)(m0 + q) * h.stride + n))
+                 : uvec2_t{kEmptySlot, kEmptySlot};
+      int j0[C], j1[C];
+      double a0[C], b0[C], a1[C], b1[C];
+      bool ok0[C], ok1[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        ok0[q] = decode32<LAYOUT>(h, s_tab, w[q].x, j0[q], a0[q], b0[q]);
+        ok1[q] = decode32<LAYOUT>(h, s_tab, w[q].y, j1[q], a1[q], b1[q]);
+        if (!ok0[q])
+          j0[q] = n;
+        if (!ok1[q])
+          j1[q] = n;
       }
-      if (decode32<LAYOUT>(h, s_tab, w.y, j, a, b)) {
-        float2 c = psi1[j];
-        double cx = c.x, cy = c.y;
-        tr1 += a * cx - b * cy;
-        ti1 += a * cy + b * cx;
+      float2 c0[C], c1[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        c0[q] = psi1[j0[q]];
+        c1[q] = psi1[j1[q]];
+      }
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        if (ok0[q]) {
+          double cx = c0[q].x, cy = c0[q].y;
+          tr0 += a0[q] * cx - b0[q] * cy;
+          ti0 += a0[q] * cy + b0[q] * cx;
+        }
+        if (ok1[q]) {
+          double cx = c1[q].x, cy = c1[q].y;
+          tr1 += a1[q] * cx - b1[q] * cy;
+          ti1 += a1[q] * cy + b1[q] * cx;
+        }
       }
     }
     tr0 /= h.emax;
@@ -288,20 +267,28 @@ __global__ __launch_bounds__(kBlock) void k_comm32(HamDev h, const float2* __res
   if (tile >= tiles)
     return;
   stage32<LAYOUT, true>(h, s_tab, s_xx);
+  constexpr int C = SlotBatch<KT>::C;
   const int K = KT > 0 ? KT : h.K;
   int n = tile * kBlock + threadIdx.x;
   if (n >= h.n)
     return;
   double tr = 0.0, ti = 0.0;
+  for (int m0 = 0; m0 < K; m0 += C) {
+    int j[C];
+    double a[C], b[C], x[C];
+    bool ok[C];
+    load_slots<LAYOUT, C, NT, true>(h, s_tab, s_xx, n, m0, K, j, a, b, x, ok);
+    float2 c[C];
 #pragma unroll
-  for (int m = 0; m < K; ++m) {
-    int j;
-    double a, b, x;
-    if (pair32x<LAYOUT, NT>(h, s_tab, s_xx, (size_t)m * h.stride + n, j, a, b, x)) {
-      float2 c = in[j];
-      double cx = c.x, cy = c.y;
-      tr -= (a * cx - b * cy) * x;
-      ti -= (a * cy + b * cx) * x;
+    for (int q = 0; q < C; ++q)
+      c[q] = in[j[q]];
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      if (ok[q]) {
+        double cx = c[q].x, cy = c[q].y;
+        tr -= (a[q] * cx - b[q] * cy) * x[q];
+        ti -= (a[q] * cy + b[q] * cx) * x[q];
+      }
     }
   }
   out[n] = make_float2((float)(tr / h.emax), (float)(ti / h.emax));
@@ -373,6 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x_32(HamDev h, const float2* s0
   if (tile >= tiles)
     return;
   stage32<LAYOUT, true>(h, s_tab, s_xx);
+  constexpr int C = SlotBatch<KT>::C;
   const int K = KT > 0 ? KT : h.K;
   int n = tile * kBlock + threadIdx.x;
   if (n >= h.n)
@@ -381,20 +369,29 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x_32(HamDev h, const float2* s0
   float2 p1 = s1[n], p1x = s1x[n];
   double tr = v * (double)p1.x, ti = v * (double)p1.y;
   double txr = v * (double)p1x.x, txi = v * (double)p1x.y;
+  for (int m0 = 0; m0 < K; m0 += C) {
+    int j[C];
+    double a[C], b[C], x[C];
+    bool ok[C];
+    load_slots<LAYOUT, C, NT, true>(h, s_tab, s_xx, n, m0, K, j, a, b, x, ok);
+    float2 c[C], cxv[C];
 #pragma unroll
-  for (int m = 0; m < K; ++m) {
-    int j;
-    double a, b, x;
-    if (pair32x<LAYOUT, NT>(h, s_tab, s_xx, (size_t)m * h.stride + n, j, a, b, x)) {
-      float2 c = s1[j], cxv = s1x[j];
-      double cx = c.x, cy = c.y, dx = cxv.x, dy = cxv.y;
-      double hr = a * cx - b * cy, hi = a * cy + b * cx;
-      tr += hr;
-      ti += hi;
-      txr += a * dx - b * dy;
-      txi += a * dy + b * dx;
-      txr -= hr * x;
-      txi -= hi * x;
+    for (int q = 0; q < C; ++q) {
+      c[q] = s1[j[q]];
+      cxv[q] = s1x[j[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      if (ok[q]) {
+        double cx = c[q].x, cy = c[q].y, dx = cxv[q].x, dy = cxv[q].y;
+        double hr = a[q] * cx - b[q] * cy, hi = a[q] * cy + b[q] * cx;
+        tr += hr;
+        ti += hi;
+        txr += a[q] * dx - b[q] * dy;
+        txi += a[q] * dy + b[q] * dx;
+        txr -= hr * x[q];
+        txi -= hi * x[q];
+      }
     }
   }
   float2 p0 = ldf2<NT>(s0 + n), p0x = ldf2<NT>(s0x + n);

@@ -22,65 +22,7 @@
 
 namespace lsqt {
 
-// --- pair access ---------------------------------------------------------
-template <int LAYOUT, bool NT>
-__device__ __forceinline__ bool load_pair(const HamDev& h, const double2* s_tab, size_t q,
-                                          int& j, double& a, double& b)
-{
-  if constexpr (LAYOUT == kFull) {
-    j = ld<NT>(h.nl + q);
-    if (j < 0)
-      return false;
-    a = ld<NT>(h.hr + q);
-    b = ld<NT>(h.hi + q);
-    return true;
-  } else {
-    uint32_t w = ld<NT>(h.words + q);
-    if (w == kEmptySlot)
-      return false;
-    j = (int)(w & h.idx_mask);
-    int c = (int)(w >> h.idx_bits);
-    double2 t = (LAYOUT == kPackedLds) ? s_tab[c] : h.tab[c];
-    a = t.x;
-    b = t.y;
-    return true;
-  }
-}
-
-template <int LAYOUT, bool NT>
-__device__ __forceinline__ bool load_pair_x(const HamDev& h, const double2* s_tab,
-                                            const double* s_xx, size_t q, int& j, double& a,
-                                            double& b, double& x)
-{
-  if constexpr (LAYOUT == kFull) {
-    j = ld<NT>(h.nl + q);
-    if (j < 0)
-      return false;
-    a = ld<NT>(h.hr + q);
-    b = ld<NT>(h.hi + q);
-    x = ld<NT>(h.xx + q);
-    return true;
-  } else {
-    uint32_t w = ld<NT>(h.words + q);
-    if (w == kEmptySlot)
-      return false;
-    j = (int)(w & h.idx_mask);
-    int c = (int)(w >> h.idx_bits);
-    if constexpr (LAYOUT == kPackedLds) {
-      double2 t = s_tab[c];
-      a = t.x;
-      b = t.y;
-      x = s_xx[c];
-    } else {
-      double2 t = h.tab[c];
-      a = t.x;
-      b = t.y;
-      x = h.tab_xx[c];
-    }
-    return true;
-  }
-}
-
+// --- class tables in LDS -------------------------------------------------
 template <int LAYOUT, bool WITH_XX>
 __device__ __forceinline__ void stage_table(const HamDev& h, double2* s_tab, double* s_xx)
 {
@@ -91,6 +33,38 @@ __device__ __forceinline__ void stage_table(const HamDev& h, double2* s_tab, dou
         s_xx[i] = h.tab_xx[i];
     }
     __syncthreads();
+  }
+}
+
+// --- H~-type sums over the slots of site n, in slot order -----------------
+// hsum:  t += sum_m h_nm psi[j_m]                 (apply, kernel_polynomial)
+// xsum:  t -= sum_m h_nm xx_nm psi[j_m]  (or += for the current operator)
+// Every slot's loads are issued before any is used; the adds are predicated
+// so the skipped slots leave the sums bit-identical to a loop that stops at
+// neighbor_number[n] (src/hamiltonian.cu:206-215).
+template <int LAYOUT, int KT, bool NT>
+__device__ __forceinline__ void hsum(const HamDev& h, const double2* s_tab,
+                                     const double2* __restrict__ psi, int n, double& tr,
+                                     double& ti)
+{
+  constexpr int C = SlotBatch<KT>::C;
+  const int K = KT > 0 ? KT : h.K;
+  for (int m0 = 0; m0 < K; m0 += C) {
+    int j[C];
+    double a[C], b[C], x[C];
+    bool ok[C];
+    load_slots<LAYOUT, C, NT, false>(h, s_tab, nullptr, n, m0, K, j, a, b, x, ok);
+    double2 c[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      c[q] = psi[j[q]];
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      if (ok[q]) {
+        tr += a[q] * c[q].x - b[q] * c[q].y;
+        ti += a[q] * c[q].y + b[q] * c[q].x;
+      }
+    }
   }
 }
 
@@ -136,7 +110,6 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
   if (tile >= tiles)
     return;
   stage_table<LAYOUT, false>(h, s_tab, nullptr);
-  const int K = KT > 0 ? KT : h.K;
   double acc_re = 0.0, acc_im = 0.0;
 #pragma unroll
   for (int k = 0; k < SPT; ++k) {
@@ -146,16 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
       double v = ld<NT>(h.pot + n);
       double tr = v * p1.x;
       double ti = v * p1.y;
-#pragma unroll
-      for (int m = 0; m < K; ++m) {
-        int j;
-        double a, b;
-        if (load_pair<LAYOUT, NT>(h, s_tab, (size_t)m * h.stride + n, j, a, b)) {
-          double2 c = psi1[j];
-          tr += a * c.x - b * c.y;
-          ti += a * c.y + b * c.x;
-        }
-      }
+      hsum<LAYOUT, KT, NT>(h, s_tab, psi1, n, tr, ti);
       tr /= h.emax;
       ti /= h.emax;
       if constexpr (RECUR) {
@@ -187,23 +151,31 @@ __global__ __launch_bounds__(kBlock) void k_xop(HamDev h, const double2* __restr
   if (tile >= tiles)
     return;
   stage_table<LAYOUT, true>(h, s_tab, s_xx);
+  constexpr int C = SlotBatch<KT>::C;
   const int K = KT > 0 ? KT : h.K;
   int n = tile * kBlock + threadIdx.x;
   if (n >= h.n)
     return;
   double tr = 0.0, ti = 0.0;
+  for (int m0 = 0; m0 < K; m0 += C) {
+    int j[C];
+    double a[C], b[C], x[C];
+    bool ok[C];
+    load_slots<LAYOUT, C, NT, true>(h, s_tab, s_xx, n, m0, K, j, a, b, x, ok);
+    double2 c[C];
 #pragma unroll
-  for (int m = 0; m < K; ++m) {
-    int j;
-    double a, b, x;
-    if (load_pair_x<LAYOUT, NT>(h, s_tab, s_xx, (size_t)m * h.stride + n, j, a, b, x)) {
-      double2 c = in[j];
-      if constexpr (CURRENT) {
-        tr += (a * c.x - b * c.y) * x;
-        ti += (a * c.y + b * c.x) * x;
-      } else {
-        tr -= (a * c.x - b * c.y) * x;
-        ti -= (a * c.y + b * c.x) * x;
+    for (int q = 0; q < C; ++q)
+      c[q] = in[j[q]];
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      if (ok[q]) {
+        if constexpr (CURRENT) {
+          tr += (a[q] * c[q].x - b[q] * c[q].y) * x[q];
+          ti += (a[q] * c[q].y + b[q] * c[q].x) * x[q];
+        } else {
+          tr -= (a[q] * c[q].x - b[q] * c[q].y) * x[q];
+          ti -= (a[q] * c[q].y + b[q] * c[q].x) * x[q];
+        }
       }
     }
   }
@@ -249,7 +221,6 @@ __global__ __launch_bounds__(kBlock) void k_cheb2(HamDev h, const double2* s0,
   if (tile >= tiles)
     return;
   stage_table<LAYOUT, false>(h, s_tab, nullptr);
-  const int K = KT > 0 ? KT : h.K;
   int n = tile * kBlock + threadIdx.x;
   if (n >= h.n)
     return;
@@ -257,16 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb2(HamDev h, const double2* s0,
   double v = ld<NT>(h.pot + n);
   double tr = v * p1.x;
   double ti = v * p1.y;
-#pragma unroll
-  for (int m = 0; m < K; ++m) {
-    int j;
-    double a, b;
-    if (load_pair<LAYOUT, NT>(h, s_tab, (size_t)m * h.stride + n, j, a, b)) {
-      double2 c = s1[j];
-      tr += a * c.x - b * c.y;
-      ti += a * c.y + b * c.x;
-    }
-  }
+  hsum<LAYOUT, KT, NT>(h, s_tab, s1, n, tr, ti);
   tr /= h.emax;
   ti /= h.emax;
   double2 p0 = ld2<NT>(s0 + n);
@@ -293,6 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x(HamDev h, const double2* s0,
   if (tile >= tiles)
     return;
   stage_table<LAYOUT, true>(h, s_tab, s_xx);
+  constexpr int C = SlotBatch<KT>::C;
   const int K = KT > 0 ? KT : h.K;
   int n = tile * kBlock + threadIdx.x;
   if (n >= h.n)
@@ -301,18 +264,27 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x(HamDev h, const double2* s0,
   double2 p1 = s1[n], p1x = s1x[n];
   double tr = v * p1.x, ti = v * p1.y;
   double txr = v * p1x.x, txi = v * p1x.y;
+  for (int m0 = 0; m0 < K; m0 += C) {
+    int j[C];
+    double a[C], b[C], x[C];
+    bool ok[C];
+    load_slots<LAYOUT, C, NT, true>(h, s_tab, s_xx, n, m0, K, j, a, b, x, ok);
+    double2 c[C], cx[C];
 #pragma unroll
-  for (int m = 0; m < K; ++m) {
-    int j;
-    double a, b, x;
-    if (load_pair_x<LAYOUT, NT>(h, s_tab, s_xx, (size_t)m * h.stride + n, j, a, b, x)) {
-      double2 c = s1[j], cx = s1x[j];
-      tr += a * c.x - b * c.y;
-      ti += a * c.y + b * c.x;
-      txr += a * cx.x - b * cx.y;
-      txi += a * cx.y + b * cx.x;
-      txr -= (a * c.x - b * c.y) * x;
-      txi -= (a * c.y + b * c.x) * x;
+    for (int q = 0; q < C; ++q) {
+      c[q] = s1[j[q]];
+      cx[q] = s1x[j[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      if (ok[q]) {
+        tr += a[q] * c[q].x - b[q] * c[q].y;
+        ti += a[q] * c[q].y + b[q] * c[q].x;
+        txr += a[q] * cx[q].x - b[q] * cx[q].y;
+        txi += a[q] * cx[q].y + b[q] * cx[q].x;
+        txr -= (a[q] * c[q].x - b[q] * c[q].y) * x[q];
+        txi -= (a[q] * c[q].y + b[q] * c[q].x) * x[q];
+      }
     }
   }
   tr /= h.emax;

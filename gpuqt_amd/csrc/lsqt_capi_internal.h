@@ -22,7 +22,8 @@ struct lsqt_ctx {
   size_t mu_host_cap = 0;
 };
 
-struct lsqt_f32_state;  // FP32 / block-path device data (kernels_f32.hip)
+struct lsqt_f32_state;    // FP32-path device data (capi_f32.hip)
+struct lsqt_block_state;  // block-path scratch and choices (capi_block.hip)
 
 struct lsqt_ham {
   lsqt_ctx* ctx = nullptr;
@@ -34,6 +35,12 @@ struct lsqt_ham {
   std::vector<double2> host_tab;
   std::vector<double> host_tab_xx;
   lsqt_f32_state* f32 = nullptr;
+  lsqt_block_state* blk = nullptr;
+  // neighbour-offset structure of a lattice in natural order (0 when none):
+  // sites come in slabs of `slab` consecutive sites whose neighbours are at
+  // most one slab away, and slabs in rows of `row` sites (slab % row == 0)
+  int slab = 0;
+  int row = 0;
 };
 
 struct lsqt_vec {
@@ -54,6 +61,8 @@ int ctx_reserve_moments(lsqt_ctx* ctx, int M, int tiles);
 int ctx_download_moments(lsqt_ctx* ctx, int M, double* mu_re, double* mu_im);
 int ham_scratch(lsqt_ham* h, int count);
 void ham_release_extra(lsqt_ham* h);
+void block_release(lsqt_ham* h);
+int f32_potential(lsqt_ham* h);
 int moments_enqueue(lsqt_ham* h, const double2* left, const double2* right, int M);
 int evolve_enqueue(lsqt_ham* h, int direction, double t, double2* state);
 int evolvex_enqueue(lsqt_ham* h, int direction, double t, double2* state);

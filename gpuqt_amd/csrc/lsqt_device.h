@@ -96,12 +96,80 @@ __device__ __forceinline__ void st2(double2* p, double2 v)
   }
 }
 
+// Loads slots [m0, m0 + C) of site n, every load issued before any is used
+// (independent loads overlap their latencies; a slot-by-slot loop serialises
+// three dependent round trips per slot).  Slots at or beyond `count` and
+// unused slots get j = n (an in-bounds address) and ok = false; callers skip
+// their terms.  The pair arrays are padded to `stride`, so every slot load
+// below `count` is in bounds.
+template <int LAYOUT, int C, bool NT, bool WITH_XX>
+__device__ __forceinline__ void load_slots(const HamDev& h, const double2* s_tab,
+                                           const double* s_xx, int n, int m0, int count,
+                                           int (&j)[C], double (&a)[C], double (&b)[C],
+                                           double (&x)[C], bool (&ok)[C])
+{
+  if constexpr (LAYOUT == 0) {  // full layout: index, re, im (and xx) per slot
+    int jj[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      jj[q] = (m0 + q < count) ? ld<NT>(h.nl + (size_t)(m0 + q) * h.stride + n) : -1;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const bool in = m0 + q < count;
+      a[q] = in ? ld<NT>(h.hr + (size_t)(m0 + q) * h.stride + n) : 0.0;
+      b[q] = in ? ld<NT>(h.hi + (size_t)(m0 + q) * h.stride + n) : 0.0;
+      if constexpr (WITH_XX)
+        x[q] = in ? ld<NT>(h.xx + (size_t)(m0 + q) * h.stride + n) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      ok[q] = jj[q] >= 0;
+      j[q] = ok[q] ? jj[q] : n;
+    }
+  } else {  // packed: one word per slot, (re, im, xx) from the class table
+    uint32_t w[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      w[q] = (m0 + q < count) ? ld<NT>(h.words + (size_t)(m0 + q) * h.stride + n) : kEmptySlot;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      ok[q] = w[q] != kEmptySlot;
+      j[q] = ok[q] ? (int)(w[q] & h.idx_mask) : n;
+      const int c = ok[q] ? (int)(w[q] >> h.idx_bits) : 0;
+      const double2 t = (LAYOUT == 1) ? s_tab[c] : h.tab[c];
+      a[q] = t.x;
+      b[q] = t.y;
+      if constexpr (WITH_XX)
+        x[q] = (LAYOUT == 1) ? s_xx[c] : h.tab_xx[c];
+    }
+  }
+}
+
+// Slots per load batch: all K slots when K is a compile-time constant,
+// otherwise batches of 4 (in slot order).
+template <int KT>
+struct SlotBatch {
+  static constexpr int C = KT > 0 ? KT : 4;
+};
+
 // Grid shape of a one-tile-per-workgroup launch: `tiles` tiles of
 // kBlock*spt sites, padded to a multiple of the XCD count so that the
 // round-robin dispatch gives every XCD one contiguous range of tiles.
 struct Grid {
   int tiles;
   int blocks;
+};
+
+// Block-path tile schedule: natural (tile t = sites [t*rows, (t+1)*rows))
+// or blocked (slab > 0: tile t = the span of sites at offset (t / slabs)*span
+// in slab t % slabs).
+struct TileMap {
+  int n;
+  int rows;
+  int slab;
+  int span;
+  int slabs;
+  int tiles;
 };
 
 inline Grid make_grid(int n, int spt)

@@ -46,4 +46,15 @@ void launch_dot32(const float2* psi, const float2* L, double2* partial, int n, h
 void launch_norm2(const double2* s, double2* partial, int n, hipStream_t st);
 void launch_renormalise(double2* s, const double2* norms, int n, hipStream_t st);
 
+// block path (kernels_block.hip)
+TileMap make_tile_map(int n, int rows, int slab, int span);
+Grid block_grid(const TileMap& t);
+void launch_block_step(const HamDev& h, int R, int W, const TileMap& tm, const float2* psi1,
+                       const float2* psi0, float2* out, const float2* L, double2* partial,
+                       bool recur, bool ip, hipStream_t st);
+void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& tm,
+                      double2* partial, hipStream_t st);
+void launch_block_reduce(const double2* partial, int tiles, int R, int M, int m0, int count,
+                         double2* mu, hipStream_t st);
+
 }  // namespace lsqt

@@ -175,6 +175,31 @@ def cubic_peierls(L: int, *, t: float = -1.0, flux: float = 0.01):
     return neighbors, hoppings, positions
 
 
+def cubic_peierls_arrays(L: int = 160, W: float = 16.5, flux: float = 0.01, seed: int = 2024):
+    """Simple-cubic L^3 Anderson model, hopping -1, on-site U[-W/2, W/2],
+    Peierls phase 2 pi flux x on y-bonds (+y: -exp(+i 2 pi flux x), -y: its
+    conjugate), periodic, neighbour order -x, +x, -y, +y, -z, +z, site index
+    (x L + y) L + z; xx = transport (x) offset with the minimum image.
+    Returns the row-major ELL arrays the general model would build."""
+    n = L ** 3
+    idx = np.arange(n, dtype=np.int64)
+    x, y, z = idx // (L * L), (idx // L) % L, idx % L
+    site = lambda a, b, c: ((a % L) * L + (b % L)) * L + (c % L)
+    nl = np.stack([site(x - 1, y, z), site(x + 1, y, z), site(x, y - 1, z), site(x, y + 1, z),
+                   site(x, y, z - 1), site(x, y, z + 1)], axis=1).astype(np.int32)
+    ph = np.exp(2j * np.pi * flux * x)
+    hop = np.empty((n, 6), dtype=complex)
+    hop[:, [0, 1, 4, 5]] = -1.0
+    hop[:, 2] = -np.conj(ph)
+    hop[:, 3] = -ph
+    xx = np.zeros((n, 6))
+    xx[:, 0], xx[:, 1] = -1.0, 1.0
+    pot = np.random.default_rng(seed).uniform(-W / 2, W / 2, n)
+    return dict(neighbor_number=np.full(n, 6, np.int32), neighbor_list=nl.reshape(-1),
+                potential=pot, hopping_real=hop.real.reshape(-1).copy(),
+                hopping_imag=hop.imag.reshape(-1).copy(), xx=xx.reshape(-1)), n
+
+
 def anderson_potential(n: int, W: float, seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed)
     return rng.uniform(-W / 2, W / 2, size=n)

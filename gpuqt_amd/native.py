@@ -24,7 +24,7 @@ _vp = C.c_void_p
 class HamInfo(C.Structure):
     _fields_ = [("n", C.c_int), ("max_neighbor", C.c_int), ("layout", C.c_int),
                 ("classes", C.c_int), ("index_bits", C.c_int), ("energy_max", C.c_double),
-                ("bytes_per_site_f64", C.c_double)]
+                ("bytes_per_site_f64", C.c_double), ("slab", C.c_int), ("row", C.c_int)]
 
 
 # every symbol declared in include/lsqt_hip.h, with (restype, argtypes)
@@ -68,6 +68,13 @@ SIGNATURES = {
     "lsqt_moments_f32": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp, _dp]),
     "lsqt_evolve_f32": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_int]),
     "lsqt_evolvex_f32": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
+    "lsqt_block_create": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "lsqt_block_destroy": (C.c_int, [_vp]),
+    "lsqt_block_from_host": (C.c_int, [_vp, _dp, _dp]),
+    "lsqt_block_to_host": (C.c_int, [_vp, _dp, _dp]),
+    "lsqt_moments_block": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp, _dp]),
+    "lsqt_block_tune": (C.c_int, [_vp, _vp, _vp, C.c_int, _ip, _ip, _ip, _dp]),
+    "lsqt_block_set_config": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
 }
 
 _lib = None
@@ -199,6 +206,38 @@ class Vector:
             pass
 
 
+class Block:
+    """R complex vectors of n sites stored together on the device (FP32)."""
+
+    def __init__(self, ctx: Context, n: int, R: int):
+        self.ctx, self.n, self.R = ctx, n, R
+        self.h = _vp()
+        _check(load().lsqt_block_create(ctx.h, n, R, C.byref(self.h)))
+
+    def upload(self, re, im):
+        """re, im: arrays of shape (R, n)."""
+        re = np.ascontiguousarray(re, dtype=np.float64)
+        im = np.ascontiguousarray(im, dtype=np.float64)
+        assert re.shape == (self.R, self.n) and im.shape == (self.R, self.n)
+        _check(load().lsqt_block_from_host(self.h, _dptr(re), _dptr(im)))
+
+    def download(self):
+        re, im = np.empty((self.R, self.n)), np.empty((self.R, self.n))
+        _check(load().lsqt_block_to_host(self.h, _dptr(re), _dptr(im)))
+        return re, im
+
+    def close(self):
+        if self.h:
+            load().lsqt_block_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Hamiltonian:
     """Hamiltonian (src/hamiltonian.h:25-54) on the device, from row-major ELL arrays."""
 
@@ -280,6 +319,24 @@ class Hamiltonian:
 
     def evolvex_f32(self, direction, t_scaled, state):
         _check(load().lsqt_evolvex_f32(self.h, int(direction), float(t_scaled), state.h))
+
+    def moments_block(self, left, right, M):
+        """complex array (R, M) of <left_r|T_m|right_r>"""
+        R = left.R
+        re, im = np.empty(R * M), np.empty(R * M)
+        _check(load().lsqt_moments_block(self.h, left.h, right.h, int(M), _dptr(re), _dptr(im)))
+        return (re + 1j * im).reshape(R, M)
+
+    def block_tune(self, left, right, force=False):
+        """(W, rows, blocked, us per step) of the autotuned block kernel shape"""
+        W, rows, blocked, us = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        _check(load().lsqt_block_tune(self.h, left.h, right.h, 1 if force else 0, C.byref(W),
+                                      C.byref(rows), C.byref(blocked), C.byref(us)))
+        return W.value, rows.value, blocked.value, us.value
+
+    def block_set_config(self, R, W=0, rows=0, blocked=False):
+        _check(load().lsqt_block_set_config(self.h, int(R), int(W), int(rows),
+                                            1 if blocked else 0))
 
     def close(self):
         if self.h:

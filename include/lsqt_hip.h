@@ -70,6 +70,8 @@ typedef struct {
   double bytes_per_site_f64; /* compulsory HBM bytes of one FP64 Chebyshev
                                 site-update with this layout (fused moment
                                 product included) */
+  int slab;           /* detected lattice slab in sites (0: none), see the */
+  int row;            /* block tile schedules below; row: the next level    */
 } lsqt_ham_info;
 
 const char* lsqt_strerror(int status);
@@ -177,6 +179,37 @@ int lsqt_moments_f32(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right, i
 int lsqt_evolve_f32(lsqt_ham* h, int direction, double time_step_scaled, lsqt_vec* state,
                     int renormalise);
 int lsqt_evolvex_f32(lsqt_ham* h, int direction, double time_step_scaled, lsqt_vec* state);
+
+/* ---- block path --------------------------------------------------------------
+ * (no counterpart in the reference, which runs one random vector at a time,
+ * src/lsqt.cu:140-150)  A block holds R vectors of n sites (R a power of 2,
+ * at most 256) in FP32, site-major; one sparse sweep updates all R.
+ * Needs the packed layout.  Host arrays are vector-major: vector r at
+ * re[r*n .. r*n + n). */
+typedef struct lsqt_block lsqt_block;
+int lsqt_block_create(lsqt_ctx* ctx, int n, int R, lsqt_block** out);  /* zero-filled */
+int lsqt_block_destroy(lsqt_block* b);
+int lsqt_block_from_host(lsqt_block* b, const double* re, const double* im);
+int lsqt_block_to_host(const lsqt_block* b, double* re, double* im);
+/* mu[r*M + m] = <left_r|T_m(H~)|right_r>, r < R, m < M (M >= 2) */
+int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right, int M,
+                       double* mu_re, double* mu_im);
+/* The autotuner: times every kernel shape (W vectors per lane, i.e. R/W lanes
+ * per site; rows = sites per workgroup tile; blocked = tile schedule) on these
+ * blocks and keeps the fastest for (R, max_neighbor, table placement, size
+ * class, lattice slab), also in the cache file $LSQT_TUNE_CACHE (default
+ * ~/.cache/lsqt_hip_tune.txt).  force re-tunes; the choice and its time per
+ * step are returned.
+ * Tile schedules: natural (tile t = sites [t*rows, (t+1)*rows)); blocked, for
+ * Hamiltonians whose neighbour offsets show a lattice of slabs (sites [k*slab,
+ * (k+1)*slab) for the slowest lattice index k): tile (b, k) = rows sites at
+ * offset b*rows of slab k, with k fastest, so that the tiles in flight together
+ * are the same rows of neighbouring slabs and share their neighbours in cache. */
+int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right, int force,
+                    int* W, int* rows, int* blocked, double* us_per_step);
+/* fixes the shape for R on this Hamiltonian (W = 0 returns to the tuner);
+ * blocked != 0 needs a detected slab that rows divides */
+int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int blocked);
 
 /* the number of Chebyshev terms evolve/evolvex use for time_step_scaled */
 int lsqt_bessel_terms(double time_step_scaled, int* last_m);
