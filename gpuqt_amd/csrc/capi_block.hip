@@ -152,7 +152,7 @@ bool valid_R(int R)
 
 bool valid_W(int R, int W)
 {
-  return (W == 1 || W == 2 || W == 4) && R % W == 0 && R / W <= 64;
+  return (W == 1 || W == 2 || W == 4 || W == 8) && R % W == 0 && R / W <= 64;
 }
 
 int block_alloc(lsqt_ctx* ctx, size_t count, float2** out)
@@ -269,16 +269,19 @@ int scratch_for(lsqt_ham* h, int R, int M, int tiles, BlockScratch** out)
 
 // enqueues steps [from, to) of the recursion; step m's partials go to ring
 // slot m % kRing, and full rings are reduced into mu
-TileMap tile_map(const lsqt_ham* h, const BlockShape& c)
+// blocked: 0 natural, 1 blocked, 2 blocked with the column walk
+TileMap tile_map(const lsqt_ham* h, int R, const BlockShape& c)
 {
-  return make_tile_map(h->d.n, c.rows, c.blocked ? h->slab : 0, c.blocked ? c.rows : 0);
+  const int per_pass = kBlock / (R / c.W);
+  return make_tile_map(h->d.n, c.rows, c.blocked ? h->slab : 0, c.blocked ? c.rows : 0, h->row,
+                       c.blocked == 2 ? per_pass : 0);
 }
 
 void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const float2* L, float2*& A,
                    float2*& B, double2* ring, double2* mu, int M, int from, int to,
                    hipStream_t st)
 {
-  const TileMap tm = tile_map(h, c);
+  const TileMap tm = tile_map(h, R, c);
   const int tiles = tm.tiles;
   const size_t slot = (size_t)tiles * R;
   for (int m = from; m < to; ++m) {
@@ -328,13 +331,18 @@ extern "C" int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int bl
   }
   if (!valid_W(R, W) || rows < 64 || rows % 64 != 0)
     return set_error(LSQT_ERR_INVALID, "W=%d rows=%d invalid for R=%d", W, rows, R);
+  if (blocked < 0 || blocked > 2)
+    return set_error(LSQT_ERR_INVALID, "tile schedule %d is not 0, 1 or 2", blocked);
   if (blocked && (h->slab == 0 || h->slab % rows != 0))
     return set_error(LSQT_ERR_INVALID, "blocked tiles of %d sites need a lattice slab they "
                      "divide (slab %d)", rows, h->slab);
+  if (blocked == 2 && (rows % h->row != 0 || h->row % (kBlock / (R / W)) != 0))
+    return set_error(LSQT_ERR_INVALID, "the column walk needs tiles of whole rows (%d sites) "
+                     "that passes of %d sites divide", h->row, kBlock / (R / W));
   BlockShape c;
   c.W = W;
   c.rows = rows;
-  c.blocked = blocked ? 1 : 0;
+  c.blocked = blocked;
   h->blk->forced[R] = c;
   return LSQT_OK;
 }
@@ -353,7 +361,7 @@ extern "C" int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsq
   double us;
   LSQT_TRY(pick_config(h, R, left, right, &c, &us, false));
   BlockScratch* s;
-  LSQT_TRY(scratch_for(h, R, M, tile_map(h, c).tiles, &s));
+  LSQT_TRY(scratch_for(h, R, M, tile_map(h, R, c).tiles, &s));
   hipStream_t st = h->ctx->stream;
   float2* A = s->A;
   float2* B = s->B;
@@ -417,7 +425,7 @@ int pick_config(lsqt_ham* h, int R, const lsqt_block* left, const lsqt_block* ri
   // candidates: natural tiles of 256..4096 sites; on a detected lattice also
   // blocked tiles of a whole number of rows that divide the slab
   std::vector<BlockShape> cands;
-  for (int W : {1, 2, 4}) {
+  for (int W : {1, 2, 4, 8}) {
     if (!valid_W(R, W))
       continue;
     for (int rows : {256, 1024, 4096}) {
@@ -437,13 +445,17 @@ int pick_config(lsqt_ham* h, int R, const lsqt_block* left, const lsqt_block* ri
         c.rows = span;
         c.blocked = 1;
         cands.push_back(c);
+        if (h->row % (kBlock / (R / W)) == 0) {
+          c.blocked = 2;
+          cands.push_back(c);
+        }
       }
     }
   }
   const int steps = 6;
   int max_tiles = 0;
   for (const BlockShape& c : cands)
-    max_tiles = std::max(max_tiles, tile_map(h, c).tiles);
+    max_tiles = std::max(max_tiles, tile_map(h, R, c).tiles);
   BlockScratch* s;
   LSQT_TRY(scratch_for(h, R, steps, max_tiles, &s));
   hipStream_t st = h->ctx->stream;

@@ -10,9 +10,11 @@
 // workgroup sweeps a tile of ROWS consecutive sites ("rows per block").  Both
 // are picked per (R, z, layout, size) by the autotuner (capi_block.hip).
 //
-// Precision as in the precision path: FP32 states and left vectors, FP64
-// hopping values and arithmetic, FP64 moment products reduced per vector by a
-// fixed shuffle / LDS tree into per-tile partials [M][tiles][R].
+// Precision: FP32 states, left vectors, on-site energies and hopping values,
+// FP32 arithmetic for the sparse product and the three-term recursion; the
+// moment products <L_r|psi_r> are formed and summed in FP64 (a product of two
+// FP32 numbers is exact in FP64), reduced per vector by a fixed shuffle / LDS
+// tree into per-tile partials [M][tiles][R] and then in tile order.
 #include "lsqt_device.h"
 #include "lsqt_launch.h"
 
@@ -83,35 +85,73 @@ __device__ __forceinline__ void tile_range(const TileMap& t, int tile, int& begi
   }
 }
 
-template <int LAYOUT>
-__device__ __forceinline__ bool decode_blk(const HamDev& h, const double2* s_tab, uint32_t w,
-                                           int& j, double& a, double& b)
+// Packed-layout slot words of one site (all K = C slots), issued together.
+template <int C, bool NT>
+__device__ __forceinline__ void load_words(const HamDev& h, int site, uint32_t (&w)[C])
 {
-  if (w == kEmptySlot)
-    return false;
-  j = (int)(w & h.idx_mask);
-  int c = (int)(w >> h.idx_bits);
-  double2 t = (LAYOUT == kPackedLds) ? s_tab[c] : h.tab[c];
-  a = t.x;
-  b = t.y;
-  return true;
+#pragma unroll
+  for (int q = 0; q < C; ++q)
+    w[q] = ld<NT>(h.words + (size_t)q * h.stride + site);
+}
+
+// Decodes slot words into neighbour index and FP32 hopping value.  Unused
+// slots get the site itself as index (an in-bounds address) and a zero
+// hopping, so that every slot is gathered and summed without a branch (the
+// gathered values are finite, and adding +0 leaves a sum unchanged).
+template <int LAYOUT, int C>
+__device__ __forceinline__ void decode_words32(const HamDev& h, const float2* s_tab, int site,
+                                               const uint32_t (&w)[C], int (&j)[C],
+                                               float (&a)[C], float (&b)[C])
+{
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    const bool ok = w[q] != kEmptySlot;
+    j[q] = ok ? (int)(w[q] & h.idx_mask) : site;
+    const int c = ok ? (int)(w[q] >> h.idx_bits) : 0;
+    float2 t;
+    if constexpr (LAYOUT == kPackedLds) {
+      t = s_tab[c];
+    } else {
+      const double2 d = h.tab[c];
+      t = make_float2((float)d.x, (float)d.y);
+    }
+    a[q] = ok ? t.x : 0.0f;
+    b[q] = ok ? t.y : 0.0f;
+  }
+}
+
+// Slot batch [m0, m0 + C) of a site whose slot count is known only at run
+// time: all words of the batch are issued before any is decoded.
+template <int LAYOUT, int C, bool NT>
+__device__ __forceinline__ void load_slots32(const HamDev& h, const float2* s_tab, int site,
+                                             int m0, int count, int (&j)[C], float (&a)[C],
+                                             float (&b)[C])
+{
+  uint32_t w[C];
+#pragma unroll
+  for (int q = 0; q < C; ++q)
+    w[q] = (m0 + q < count) ? ld<NT>(h.words + (size_t)(m0 + q) * h.stride + site) : kEmptySlot;
+  decode_words32<LAYOUT, C>(h, s_tab, site, w, j, a, b);
 }
 
 // One Chebyshev step for R vectors (RECUR: out = 2 H~ psi1 - psi0, else
-// out = H~ psi1), fused with the per-vector moment partials when IP.
+// out = H~ psi1) in FP32 arithmetic, fused with the per-vector moment
+// partials <L_r|out_r> (FP64 products and sums) when IP.
 template <int LAYOUT, int KT, int W, bool RECUR, bool IP, bool NT>
 __global__ __launch_bounds__(kBlock) void k_block_step(
   HamDev h, int R, TileMap tm, const float2* __restrict__ psi1, const float2* psi0, float2* out,
   const float2* __restrict__ L, double2* __restrict__ partial, int tiles, int blocks)
 {
-  __shared__ double2 s_tab[LAYOUT == kPackedLds ? kBlockLdsTableMax : 1];
+  __shared__ float2 s_tab[LAYOUT == kPackedLds ? kBlockLdsTableMax : 1];
   __shared__ double s_red[2][kBlock / 64][64 * 4];  // [re/im][wave][vector] (R <= 256)
   int tile = xcd_tile(blockIdx.x, blocks);
   if (tile >= tiles)
     return;
   if constexpr (LAYOUT == kPackedLds) {
-    for (int i = threadIdx.x; i < h.ncls; i += kBlock)
-      s_tab[i] = h.tab[i];
+    for (int i = threadIdx.x; i < h.ncls; i += kBlock) {
+      const double2 d = h.tab[i];
+      s_tab[i] = make_float2((float)d.x, (float)d.y);
+    }
     __syncthreads();
   }
   const int K = KT > 0 ? KT : h.K;
@@ -121,69 +161,117 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
   const int r0 = (threadIdx.x % vpw) * W;      // first vector of this lane
   int begin, end;
   tile_range(tm, tile, begin, end);
-  const double inv_scale = 1.0 / h.emax;
+  const float inv_scale = (float)(1.0 / h.emax);
   double acc_re[W], acc_im[W];
 #pragma unroll
   for (int w = 0; w < W; ++w)
     acc_re[w] = acc_im[w] = 0.0;
 
-  for (int s = begin + lane_site; s < end; s += per_pass) {
+  // With a compile-time slot count the loop is software-pipelined: the words
+  // of the next site are loaded while this site's gathers are in flight, so a
+  // pass costs one memory round trip instead of two.  The prefetch is issued
+  // unconditionally (the last pass re-reads its own words) so that the
+  // compiler can count the loads in flight and wait only for the ones used.
+  //
+  // Pass order: consecutive sites (s += per_pass), or with the column walk
+  // (blocked tiles of whole lattice rows) the tile's rows are swept side by
+  // side, one chunk of per_pass sites of every row before the next chunk, so
+  // that neighbours one row apart are read one pass apart.
+  constexpr int C = SlotBatch<KT>::C;
+  constexpr bool kPipe = KT > 0;
+  const int by = tm.walk ? tm.span / tm.row : 1;  // rows per tile (column walk)
+  int s = begin + lane_site;
+  int wy = 0;  // row of the column walk
+  const int np = tm.walk ? by * (tm.row / per_pass) : (end - s + per_pass - 1) / per_pass;
+  auto advance = [&](int& site, int& y) {
+    if (tm.walk) {
+      if (++y == by) {
+        y = 0;
+        site += per_pass - (by - 1) * tm.row;
+      } else {
+        site += tm.row;
+      }
+    } else {
+      site += per_pass;
+    }
+  };
+  uint32_t wnext[C];
+  if constexpr (kPipe) {
+    if (np > 0)
+      load_words<C, NT>(h, s, wnext);
+  }
+  for (int p = 0; p < np; ++p) {
+    int sn = s, yn = wy;
+    advance(sn, yn);
     const size_t base = (size_t)s * R + r0;
     Vec<W> p1 = ldv<W, false>(psi1 + base);
-    double v = (double)ld<NT>(h.pot32 + s);
-    double tr[W], ti[W];
+    const float v = ld<NT>(h.pot32 + s);
+    Vec<W> p0, l;
+    if constexpr (RECUR)
+      p0 = ldv<W, NT>(psi0 + base);
+    if constexpr (IP)
+      l = ldv<W, NT>(L + base);
+    float tr[W], ti[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      tr[w] = v * (double)p1.v[w].x;
-      ti[w] = v * (double)p1.v[w].y;
+      tr[w] = v * p1.v[w].x;
+      ti[w] = v * p1.v[w].y;
     }
-    constexpr int C = SlotBatch<KT>::C;
     for (int m0 = 0; m0 < K; m0 += C) {
       int j[C];
-      double a[C], b[C], x[C];
-      bool ok[C];
-      load_slots<LAYOUT, C, NT, false>(h, s_tab, nullptr, s, m0, K, j, a, b, x, ok);
+      float a[C], b[C];
+      if constexpr (kPipe) {
+        uint32_t wcur[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q)
+          wcur[q] = wnext[q];
+        load_words<C, NT>(h, p + 1 < np ? sn : s, wnext);
+        decode_words32<LAYOUT, C>(h, s_tab, s, wcur, j, a, b);
+      } else {
+        load_slots32<LAYOUT, C, NT>(h, s_tab, s, m0, K, j, a, b);
+      }
       Vec<W> c[C];
 #pragma unroll
       for (int q = 0; q < C; ++q)
         c[q] = ldv<W, false>(psi1 + (size_t)j[q] * R + r0);
 #pragma unroll
       for (int q = 0; q < C; ++q) {
-        if (ok[q]) {
 #pragma unroll
-          for (int w = 0; w < W; ++w) {
-            double cx = c[q].v[w].x, cy = c[q].v[w].y;
-            tr[w] += a[q] * cx - b[q] * cy;
-            ti[w] += a[q] * cy + b[q] * cx;
-          }
+        for (int w = 0; w < W; ++w) {
+          // explicit FMAs: the same rounding in every kernel shape
+          const float cx = c[q].v[w].x, cy = c[q].v[w].y;
+          tr[w] = __builtin_fmaf(a[q], cx, tr[w]);
+          tr[w] = __builtin_fmaf(-b[q], cy, tr[w]);
+          ti[w] = __builtin_fmaf(a[q], cy, ti[w]);
+          ti[w] = __builtin_fmaf(b[q], cx, ti[w]);
         }
       }
     }
     Vec<W> o;
     if constexpr (RECUR) {
-      Vec<W> p0 = ldv<W, NT>(psi0 + base);
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        o.v[w].x = (float)(2.0 * (tr[w] * inv_scale) - (double)p0.v[w].x);
-        o.v[w].y = (float)(2.0 * (ti[w] * inv_scale) - (double)p0.v[w].y);
+        o.v[w].x = __builtin_fmaf(2.0f, tr[w] * inv_scale, -p0.v[w].x);
+        o.v[w].y = __builtin_fmaf(2.0f, ti[w] * inv_scale, -p0.v[w].y);
       }
     } else {
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        o.v[w].x = (float)(tr[w] * inv_scale);
-        o.v[w].y = (float)(ti[w] * inv_scale);
+        o.v[w].x = tr[w] * inv_scale;
+        o.v[w].y = ti[w] * inv_scale;
       }
     }
     stv<W>(out + base, o);
     if constexpr (IP) {
-      Vec<W> l = ldv<W, NT>(L + base);
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        double ox = o.v[w].x, oy = o.v[w].y, lx = l.v[w].x, ly = l.v[w].y;
+        const double ox = o.v[w].x, oy = o.v[w].y, lx = l.v[w].x, ly = l.v[w].y;
         acc_re[w] += ox * lx + oy * ly;
         acc_im[w] += oy * lx - ox * ly;
       }
     }
+    s = sn;
+    wy = yn;
   }
 
   if constexpr (IP) {
@@ -327,13 +415,17 @@ __global__ __launch_bounds__(kBlock) void k_block_dot(const float2* __restrict__
       constexpr int W = 2;           \
       __VA_ARGS__;                   \
     } break;                         \
-    default: {                       \
+    case 4: {                        \
       constexpr int W = 4;           \
+      __VA_ARGS__;                   \
+    } break;                         \
+    default: {                       \
+      constexpr int W = 8;           \
       __VA_ARGS__;                   \
     } break;                         \
   }
 
-TileMap make_tile_map(int n, int rows, int slab, int span)
+TileMap make_tile_map(int n, int rows, int slab, int span, int row, int walk_pass)
 {
   TileMap t{};
   t.n = n;
@@ -343,6 +435,12 @@ TileMap make_tile_map(int n, int rows, int slab, int span)
     t.span = span;
     t.slabs = n / slab;
     t.tiles = (slab / span) * t.slabs;
+    // the column walk needs tiles of whole rows cut into whole passes; any
+    // other request keeps the consecutive pass order (same sites, same sums)
+    if (walk_pass > 0 && row > 0 && span % row == 0 && row % walk_pass == 0) {
+      t.row = row;
+      t.walk = 1;
+    }
   } else {
     t.tiles = (n + rows - 1) / rows;
   }

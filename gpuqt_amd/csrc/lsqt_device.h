@@ -162,7 +162,8 @@ struct Grid {
 
 // Block-path tile schedule: natural (tile t = sites [t*rows, (t+1)*rows))
 // or blocked (slab > 0: tile t = the span of sites at offset (t / slabs)*span
-// in slab t % slabs).
+// in slab t % slabs); walk != 0 sweeps a blocked tile made of whole rows of
+// `row` sites chunk by chunk across its rows (the column walk).
 struct TileMap {
   int n;
   int rows;
@@ -170,6 +171,8 @@ struct TileMap {
   int span;
   int slabs;
   int tiles;
+  int row;
+  int walk;
 };
 
 inline Grid make_grid(int n, int spt)

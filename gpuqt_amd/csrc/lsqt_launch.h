@@ -47,7 +47,8 @@ void launch_norm2(const double2* s, double2* partial, int n, hipStream_t st);
 void launch_renormalise(double2* s, const double2* norms, int n, hipStream_t st);
 
 // block path (kernels_block.hip)
-TileMap make_tile_map(int n, int rows, int slab, int span);
+// walk_pass > 0 asks for the column walk with passes of walk_pass sites
+TileMap make_tile_map(int n, int rows, int slab, int span, int row, int walk_pass);
 Grid block_grid(const TileMap& t);
 void launch_block_step(const HamDev& h, int R, int W, const TileMap& tm, const float2* psi1,
                        const float2* psi0, float2* out, const float2* L, double2* partial,

@@ -334,9 +334,10 @@ class Hamiltonian:
                                       C.byref(rows), C.byref(blocked), C.byref(us)))
         return W.value, rows.value, blocked.value, us.value
 
-    def block_set_config(self, R, W=0, rows=0, blocked=False):
-        _check(load().lsqt_block_set_config(self.h, int(R), int(W), int(rows),
-                                            1 if blocked else 0))
+    def block_set_config(self, R, W=0, rows=0, blocked=0):
+        """blocked: 0 natural tiles, 1 (or True) blocked, 2 blocked with the
+        column walk"""
+        _check(load().lsqt_block_set_config(self.h, int(R), int(W), int(rows), int(blocked)))
 
     def close(self):
         if self.h:

@@ -200,15 +200,19 @@ int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsqt_block* ri
  * class, lattice slab), also in the cache file $LSQT_TUNE_CACHE (default
  * ~/.cache/lsqt_hip_tune.txt).  force re-tunes; the choice and its time per
  * step are returned.
- * Tile schedules: natural (tile t = sites [t*rows, (t+1)*rows)); blocked, for
- * Hamiltonians whose neighbour offsets show a lattice of slabs (sites [k*slab,
- * (k+1)*slab) for the slowest lattice index k): tile (b, k) = rows sites at
- * offset b*rows of slab k, with k fastest, so that the tiles in flight together
- * are the same rows of neighbouring slabs and share their neighbours in cache. */
+ * Tile schedules (blocked = 0, 1, 2): 0 natural (tile t = sites [t*rows,
+ * (t+1)*rows)); 1 blocked, for Hamiltonians whose neighbour offsets show a
+ * lattice of slabs (sites [k*slab, (k+1)*slab) for the slowest lattice index
+ * k) made of rows: tile (b, k) = rows sites at offset b*rows of slab k, with k
+ * fastest, so that the tiles in flight together are the same rows of
+ * neighbouring slabs and share their neighbours in cache; 2 blocked with the
+ * column walk: a tile of whole rows is swept one chunk of every row at a time,
+ * so that neighbours one row apart are also read close together. */
 int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right, int force,
                     int* W, int* rows, int* blocked, double* us_per_step);
 /* fixes the shape for R on this Hamiltonian (W = 0 returns to the tuner);
- * blocked != 0 needs a detected slab that rows divides */
+ * blocked != 0 needs a detected slab that rows divides, blocked = 2 also
+ * tiles of whole rows that passes of 256*W/R sites divide */
 int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int blocked);
 
 /* the number of Chebyshev terms evolve/evolvex use for time_step_scaled */

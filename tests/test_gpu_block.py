@@ -58,7 +58,7 @@ def test_block_shapes_agree(gpu_ctx, R):
     m, H = _setup(gpu_ctx, "config1_honeycomb_20k")
     b, re, im = _block_of(gpu_ctx, m, R)
     results = {}
-    for W in (1, 2, 4):
+    for W in (1, 2, 4, 8):
         for rows in (256, 1024, 4096):
             H.block_set_config(R, W, rows)
             results[(W, rows)] = H.moments_block(b, b, 64)
@@ -69,8 +69,8 @@ def test_block_shapes_agree(gpu_ctx, R):
                                    err_msg=str(k))
 
 
-@pytest.mark.parametrize("L", [16, 32])
-def test_block_blocked_schedule(gpu_ctx, L):
+@pytest.mark.parametrize("L,R", [(16, 16), (32, 8)])
+def test_block_blocked_schedule(gpu_ctx, L, R):
     """Lattice detection on the simple-cubic Peierls model (slab L^2, row L)
     and the blocked tile schedule against the natural one and FP64."""
     from gpuqt_amd import inputs as gi
@@ -80,7 +80,7 @@ def test_block_blocked_schedule(gpu_ctx, L):
     H = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, 6, 15.0)
     info = H.info()
     assert (info.slab, info.row) == (L * L, L)
-    R, M = 8, 96
+    M = 96
     rng = np.random.default_rng(3)
     ph = rng.uniform(0, 2 * np.pi, (R, n))
     b = native.Block(gpu_ctx, n, R)
@@ -88,13 +88,21 @@ def test_block_blocked_schedule(gpu_ctx, L):
     H.block_set_config(R, 1, 256)
     ref = H.moments_block(b, b, M)
     tol = 1e-12 * np.abs(ref).max()
-    for W in (1, 2, 4):
+    walked = 0
+    for W in (1, 2, 4, 8):
         for rows in (64, 128, 256, 512, 1024):
             if (L * L) % rows:
                 continue
-            H.block_set_config(R, W, rows, blocked=True)
-            np.testing.assert_allclose(H.moments_block(b, b, M), ref, rtol=0, atol=tol,
-                                       err_msg=f"W={W} rows={rows}")
+            for sched in (1, 2):
+                if sched == 2 and (rows % L or L % (256 * W // R)):
+                    with pytest.raises(native.LsqtError):
+                        H.block_set_config(R, W, rows, blocked=2)
+                    continue
+                walked += sched == 2
+                H.block_set_config(R, W, rows, blocked=sched)
+                np.testing.assert_allclose(H.moments_block(b, b, M), ref, rtol=0, atol=tol,
+                                           err_msg=f"W={W} rows={rows} schedule {sched}")
+    assert walked > 0
     H.block_set_config(R, 0)
     v = H.vector(np.cos(ph[R - 1]), np.sin(ph[R - 1]))
     mu64 = H.moments(v, v, M)
@@ -140,7 +148,7 @@ def test_autotuner(gpu_ctx, tmp_path, monkeypatch):
     m, H = _setup(gpu_ctx, "config1_honeycomb_20k")
     b, _, _ = _block_of(gpu_ctx, m, 16)
     W, rows, blocked, us = H.block_tune(b, b, force=True)
-    assert W in (1, 2, 4) and rows in (256, 1024, 4096) and not blocked and us > 0
+    assert W in (1, 2, 4, 8) and rows in (256, 1024, 4096) and not blocked and us > 0
     assert (tmp_path / "tune.txt").read_text().startswith("gfx950 16 3")
     assert H.block_tune(b, b) == (W, rows, blocked, us)
 
