@@ -166,7 +166,7 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     blk.upload(np.cos(ph), np.sin(ph))
     W, rows, blocked, us = H.block_tune(blk, blk)
     log(f"lattice slab {info.slab} row {info.row}; autotuner: W={W} (VPW={R // W}) "
-        f"rows={rows} {'blocked' if blocked else 'natural'} schedule: {us:.1f} us/step")
+        f"rows={rows} {('natural', 'blocked', 'column-walk')[blocked]} schedule: {us:.1f} us/step")
     v0 = H.vector(np.cos(ph[0]), np.sin(ph[0]))
 
     def timed(fn):
@@ -184,17 +184,22 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     t64, mu64 = timed(lambda: H.moments(v0, v0, M))
     t32, mu32 = timed(lambda: H.moments_f32(v0, v0, M))
     tb, mub = timed(lambda: H.moments_block(blk, blk, M))
+    ts, mus = timed(lambda: H.moments_block_self(blk, M))
     e = np.linspace(-14, 14, 201)
     vol = float(L ** 3)
-    errs = []
+    errs, errs_self = [], []
     for r in range(check_vectors):
         ref = mu64 if r == 0 else H.moments(*(2 * [H.vector(np.cos(ph[r]), np.sin(ph[r]))]), M)
-        a = hostlib.correlation_row(mub[r], e, 15.0, vol)
         b = hostlib.correlation_row(ref, e, 15.0, vol)
-        errs.append(float(np.abs(a - b).max() / np.abs(b).max()))
+        for got, out in ((mub, errs), (mus, errs_self)):
+            a = hostlib.correlation_row(got[r], e, 15.0, vol)
+            out.append(float(np.abs(a - b).max() / np.abs(b).max()))
     u64 = n * (M - 1) / t64
     u32 = n * (M - 1) / t32
     ub = n * R * (M - 1) / tb
+    steps_self = M // 2
+    us_ = n * R * steps_self / ts          # recursion site-updates/s
+    um_ = n * R * (M - 1) / ts             # moments x sites x vectors per second
     bpu_blk = (4.0 + 4.0 * 6) / R + 32.0
     bpu64 = info.bytes_per_site_f64
     res = {
@@ -204,15 +209,26 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
                        "achieved_GBs": u64 * bpu64 / 1e9},
         "f32_single": {"value": u32, "unit": "site-updates/s", "speedup_vs_f64": u32 / u64},
         "block_f32": {"R": R, "W": W, "vectors_per_wavefront": R // W, "rows_per_tile": rows,
-                      "tile_schedule": "blocked" if blocked else "natural",
+                      "tile_schedule": ("natural", "blocked", "column-walk")[blocked],
                       "value": ub, "unit": "site-updates/s", "speedup_vs_f64": ub / u64,
                       "bytes_per_site_update": bpu_blk,
                       "achieved_GBs": ub * bpu_blk / 1e9,
                       "frac_of_hbm_peak": ub * bpu_blk / 1e9 / HBM_PEAK_GBS,
                       "dos_normwise_error_vs_f64": errs},
+        "block_f32_product_identities": {
+            "R": R, "steps": steps_self, "moments": M,
+            "value": us_, "unit": "site-updates/s",
+            "bytes_per_site_update": (4.0 + 4.0 * 6) / R + 24.0,
+            "achieved_GBs": us_ * ((4.0 + 4.0 * 6) / R + 24.0) / 1e9,
+            "moment_site_updates_per_s": um_,
+            "time_to_moments_speedup_vs_f64": um_ / u64,
+            "dos_normwise_error_vs_f64": errs_self},
     }
     log(f"block R={R}: {ub:.3e} site-updates/s = {ub / u64:.2f}x FP64 single "
         f"({u64:.3e}); DOS error {max(errs):.2e}")
+    log(f"block R={R} product identities: {us_:.3e} site-updates/s in {steps_self} steps, "
+        f"{um_:.3e} moment site-updates/s = {um_ / u64:.2f}x FP64 single; "
+        f"DOS error {max(errs_self):.2e}")
     for o in (H, blk, v0):
         o.close()
     return res

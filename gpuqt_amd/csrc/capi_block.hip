@@ -289,9 +289,9 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const float2* L, flo
     if (m == 0) {
       launch_block_dot(A, L, R, tm, part, st);
     } else if (m == 1) {
-      launch_block_step(h->d, R, c.W, tm, A, nullptr, B, L, part, false, true, st);
+      launch_block_step(h->d, R, c.W, tm, A, nullptr, B, L, part, nullptr, false, false, st);
     } else {
-      launch_block_step(h->d, R, c.W, tm, B, A, A, L, part, true, true, st);
+      launch_block_step(h->d, R, c.W, tm, B, A, A, L, part, nullptr, true, false, st);
       float2* t = A;
       A = B;
       B = t;
@@ -299,6 +299,41 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const float2* L, flo
     if (mu && (m % kRing == kRing - 1 || m == M - 1)) {
       int m0 = m - m % kRing;
       launch_block_reduce(ring, tiles, R, M, m0, m - m0 + 1, mu, st);
+    }
+  }
+}
+
+// The self-product moments of A's vectors (A is overwritten):
+//   S_0 = <v|v>;  step m >= 1 makes psi_m (psi_0 = v) and adds
+//   S_{2m-1} = <psi_{m-1}|psi_m>, S_{2m} = <psi_m|psi_m>;
+// the moments follow from T_{2m} = 2 T_m^2 - T_0, T_{2m+1} = 2 T_m T_{m+1} - T_1
+// (H Hermitian): mu_0 = S_0, mu_1 = S_1, mu_k = 2 S_k - mu_{k mod 2}.  Moment k
+// >= 1 goes to ring slot (k - 1) mod kRing; M/2 steps give moments 0..M-1.
+void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, float2*& A, float2*& B,
+                        double2* ring, double2* mu, int M, hipStream_t st)
+{
+  const TileMap tm = tile_map(h, R, c);
+  const int tiles = tm.tiles;
+  const size_t slot = (size_t)tiles * R;
+  launch_block_dot(A, A, R, tm, ring, st);
+  launch_block_reduce(ring, tiles, R, M, 0, 1, mu, st);
+  const int steps = M / 2;
+  for (int m = 1; m <= steps; ++m) {
+    double2* odd = ring + (size_t)((2 * m - 2) % kRing) * slot;
+    double2* even = ring + (size_t)((2 * m - 1) % kRing) * slot;
+    if (m == 1) {
+      launch_block_step(h->d, R, c.W, tm, A, nullptr, B, nullptr, odd, even, false, true, st);
+    } else {
+      launch_block_step(h->d, R, c.W, tm, B, A, A, nullptr, odd, even, true, true, st);
+      float2* t = A;
+      A = B;
+      B = t;
+    }
+    const int j = 2 * m - 1;  // ring slot of moment 2m
+    if (j % kRing == kRing - 1 || m == steps) {
+      const int k0 = j - j % kRing + 1;
+      const int count = std::min(j - (k0 - 1) + 1, M - k0);
+      launch_block_reduce(ring, tiles, R, M, k0, count, mu, st);
     }
   }
 }
@@ -375,6 +410,47 @@ extern "C" int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsq
   for (size_t i = 0; i < (size_t)M * R; ++i) {
     mu_re[i] = s->mu_host[i].x;
     mu_im[i] = s->mu_host[i].y;
+  }
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, double* mu_re,
+                                       double* mu_im)
+{
+  LSQT_TRY(check_block(h, v, v));
+  if (M < 2)
+    return set_error(LSQT_ERR_INVALID, "number of moments %d must be >= 2", M);
+  if (!mu_re || !mu_im)
+    return set_error(LSQT_ERR_INVALID, "null moment output");
+  LSQT_TRY(f32_potential(h));
+  const int R = v->R;
+  BlockShape c;
+  double us;
+  LSQT_TRY(pick_config(h, R, v, v, &c, &us, false));
+  BlockScratch* s;
+  LSQT_TRY(scratch_for(h, R, M, tile_map(h, R, c).tiles, &s));
+  hipStream_t st = h->ctx->stream;
+  float2* A = s->A;
+  float2* B = s->B;
+  LSQT_HIP(hipMemcpyAsync(A, v->d, sizeof(float2) * (size_t)h->d.n * R, hipMemcpyDeviceToDevice,
+                          st));
+  enqueue_self_steps(h, R, c, A, B, s->ring, s->mu, M, st);
+  LSQT_HIP(hipGetLastError());
+  LSQT_HIP(hipMemcpyAsync(s->mu_host, s->mu, sizeof(double2) * (size_t)M * R,
+                          hipMemcpyDeviceToHost, st));
+  LSQT_HIP(hipStreamSynchronize(st));
+  for (int r = 0; r < R; ++r) {
+    const double2* S = s->mu_host + (size_t)r * M;
+    double* re = mu_re + (size_t)r * M;
+    double* im = mu_im + (size_t)r * M;
+    re[0] = S[0].x;
+    im[0] = S[0].y;
+    re[1] = S[1].x;
+    im[1] = S[1].y;
+    for (int k = 2; k < M; ++k) {
+      re[k] = 2.0 * S[k].x - re[k % 2];
+      im[k] = 2.0 * S[k].y - im[k % 2];
+    }
   }
   return LSQT_OK;
 }

@@ -134,13 +134,57 @@ __device__ __forceinline__ void load_slots32(const HamDev& h, const float2* s_ta
   decode_words32<LAYOUT, C>(h, s_tab, site, w, j, a, b);
 }
 
+// Per-vector tile sums of the moment products (FP64), by a fixed tree: lanes
+// holding the same vectors differ in the site bits of the lane id (shuffles),
+// then the workgroup's waves in order (LDS).  Ends with a barrier, so that
+// s_red can be reused.
+template <int W>
+__device__ __forceinline__ void tile_sums(double (&acc_re)[W], double (&acc_im)[W], int vpw, int R,
+                                          int r0, double (*s_red)[kBlock / 64][64 * 4],
+                                          double2* __restrict__ dst)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    for (int off = vpw; off < 64; off <<= 1) {
+      acc_re[w] += __shfl_xor(acc_re[w], off, 64);
+      acc_im[w] += __shfl_xor(acc_im[w], off, 64);
+    }
+  }
+  if (lane < vpw) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      s_red[0][wave][r0 + w] = acc_re[w];
+      s_red[1][wave][r0 + w] = acc_im[w];
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < R; r += kBlock) {
+    double re = s_red[0][0][r], im = s_red[1][0][r];
+#pragma unroll
+    for (int wv = 1; wv < kBlock / 64; ++wv) {
+      re += s_red[0][wv][r];
+      im += s_red[1][wv][r];
+    }
+    dst[r] = make_double2(re, im);
+  }
+  __syncthreads();
+}
+
+// Moment products fused into a block step.
+enum BlockProducts {
+  kLeft = 1,  // <L_r|out_r> into partial
+  kSelf = 2,  // <psi1_r|out_r> into partial, <out_r|out_r> into partial2
+};
+
 // One Chebyshev step for R vectors (RECUR: out = 2 H~ psi1 - psi0, else
 // out = H~ psi1) in FP32 arithmetic, fused with the per-vector moment
-// partials <L_r|out_r> (FP64 products and sums) when IP.
-template <int LAYOUT, int KT, int W, bool RECUR, bool IP, bool NT>
+// products of mode IPM (FP64 products and sums, per-tile partials).
+template <int LAYOUT, int KT, int W, bool RECUR, int IPM, bool NT>
 __global__ __launch_bounds__(kBlock) void k_block_step(
   HamDev h, int R, TileMap tm, const float2* __restrict__ psi1, const float2* psi0, float2* out,
-  const float2* __restrict__ L, double2* __restrict__ partial, int tiles, int blocks)
+  const float2* __restrict__ L, double2* __restrict__ partial, double2* __restrict__ partial2,
+  int tiles, int blocks)
 {
   __shared__ float2 s_tab[LAYOUT == kPackedLds ? kBlockLdsTableMax : 1];
   __shared__ double s_red[2][kBlock / 64][64 * 4];  // [re/im][wave][vector] (R <= 256)
@@ -162,10 +206,10 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
   int begin, end;
   tile_range(tm, tile, begin, end);
   const float inv_scale = (float)(1.0 / h.emax);
-  double acc_re[W], acc_im[W];
+  double acc_re[W], acc_im[W], acc_nn[W];
 #pragma unroll
   for (int w = 0; w < W; ++w)
-    acc_re[w] = acc_im[w] = 0.0;
+    acc_re[w] = acc_im[w] = acc_nn[w] = 0.0;
 
   // With a compile-time slot count the loop is software-pipelined: the words
   // of the next site are loaded while this site's gathers are in flight, so a
@@ -209,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
     Vec<W> p0, l;
     if constexpr (RECUR)
       p0 = ldv<W, NT>(psi0 + base);
-    if constexpr (IP)
+    if constexpr (IPM == kLeft)
       l = ldv<W, NT>(L + base);
     float tr[W], ti[W];
 #pragma unroll
@@ -262,45 +306,27 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
       }
     }
     stv<W>(out + base, o);
-    if constexpr (IP) {
+    if constexpr (IPM == kSelf)
+      l = p1;  // <psi1|out>: psi1 was loaded for the on-site term
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        const double ox = o.v[w].x, oy = o.v[w].y, lx = l.v[w].x, ly = l.v[w].y;
-        acc_re[w] += ox * lx + oy * ly;
-        acc_im[w] += oy * lx - ox * ly;
-      }
+    for (int w = 0; w < W; ++w) {
+      const double ox = o.v[w].x, oy = o.v[w].y, lx = l.v[w].x, ly = l.v[w].y;
+      acc_re[w] += ox * lx + oy * ly;
+      acc_im[w] += oy * lx - ox * ly;
+      if constexpr (IPM == kSelf)
+        acc_nn[w] += ox * ox + oy * oy;
     }
     s = sn;
     wy = yn;
   }
 
-  if constexpr (IP) {
-    // lanes holding the same vectors differ in the site bits of the lane id
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  tile_sums<W>(acc_re, acc_im, vpw, R, r0, s_red, partial + (size_t)tile * R);
+  if constexpr (IPM == kSelf) {
+    double zero[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      for (int off = vpw; off < 64; off <<= 1) {
-        acc_re[w] += __shfl_xor(acc_re[w], off, 64);
-        acc_im[w] += __shfl_xor(acc_im[w], off, 64);
-      }
-    }
-    if (lane < vpw) {
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        s_red[0][wave][r0 + w] = acc_re[w];
-        s_red[1][wave][r0 + w] = acc_im[w];
-      }
-    }
-    __syncthreads();
-    for (int r = threadIdx.x; r < R; r += kBlock) {
-      double re = s_red[0][0][r], im = s_red[1][0][r];
-#pragma unroll
-      for (int wv = 1; wv < kBlock / 64; ++wv) {
-        re += s_red[0][wv][r];
-        im += s_red[1][wv][r];
-      }
-      partial[(size_t)tile * R + r] = make_double2(re, im);
-    }
+    for (int w = 0; w < W; ++w)
+      zero[w] = 0.0;
+    tile_sums<W>(acc_nn, zero, vpw, R, r0, s_red, partial2 + (size_t)tile * R);
   }
 }
 
@@ -455,39 +481,27 @@ Grid block_grid(const TileMap& t)
 
 void launch_block_step(const HamDev& h, int R, int W, const TileMap& tm, const float2* psi1,
                        const float2* psi0, float2* out, const float2* L, double2* partial,
-                       bool recur, bool ip, hipStream_t st)
+                       double2* partial2, bool recur, bool self, hipStream_t st)
 {
   Grid g = block_grid(tm);
   const bool nt = (size_t)h.n * R >= (size_t)kStreamingMinSites;
+#define LSQTB_STEP(RC, MODE, NTV)                                                 \
+  k_block_step<LAY, KT, WW, RC, MODE, NTV><<<g.blocks, kBlock, 0, st>>>(          \
+    h, R, tm, psi1, psi0, out, L, partial, partial2, g.tiles, g.blocks)
   LSQTB_LAYOUT_SWITCH(h, LAY, LSQTB_K_SWITCH(h.K, KT, LSQTB_W_SWITCH(W, WW, {
     if (nt) {
-      if (recur && ip)
-        k_block_step<LAY, KT, WW, true, true, true><<<g.blocks, kBlock, 0, st>>>(
-          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
-      else if (ip)
-        k_block_step<LAY, KT, WW, false, true, true><<<g.blocks, kBlock, 0, st>>>(
-          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
-      else if (recur)
-        k_block_step<LAY, KT, WW, true, false, true><<<g.blocks, kBlock, 0, st>>>(
-          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+      if (self)
+        recur ? LSQTB_STEP(true, kSelf, true) : LSQTB_STEP(false, kSelf, true);
       else
-        k_block_step<LAY, KT, WW, false, false, true><<<g.blocks, kBlock, 0, st>>>(
-          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+        recur ? LSQTB_STEP(true, kLeft, true) : LSQTB_STEP(false, kLeft, true);
     } else {
-      if (recur && ip)
-        k_block_step<LAY, KT, WW, true, true, false><<<g.blocks, kBlock, 0, st>>>(
-          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
-      else if (ip)
-        k_block_step<LAY, KT, WW, false, true, false><<<g.blocks, kBlock, 0, st>>>(
-          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
-      else if (recur)
-        k_block_step<LAY, KT, WW, true, false, false><<<g.blocks, kBlock, 0, st>>>(
-          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+      if (self)
+        recur ? LSQTB_STEP(true, kSelf, false) : LSQTB_STEP(false, kSelf, false);
       else
-        k_block_step<LAY, KT, WW, false, false, false><<<g.blocks, kBlock, 0, st>>>(
-          h, R, tm, psi1, psi0, out, L, partial, g.tiles, g.blocks);
+        recur ? LSQTB_STEP(true, kLeft, false) : LSQTB_STEP(false, kLeft, false);
     }
   })))
+#undef LSQTB_STEP
 }
 
 void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& tm,

@@ -50,9 +50,11 @@ void launch_renormalise(double2* s, const double2* norms, int n, hipStream_t st)
 // walk_pass > 0 asks for the column walk with passes of walk_pass sites
 TileMap make_tile_map(int n, int rows, int slab, int span, int row, int walk_pass);
 Grid block_grid(const TileMap& t);
+// self: <psi1|out> into partial and <out|out> into partial2 (L unused);
+// otherwise <L|out> into partial
 void launch_block_step(const HamDev& h, int R, int W, const TileMap& tm, const float2* psi1,
                        const float2* psi0, float2* out, const float2* L, double2* partial,
-                       bool recur, bool ip, hipStream_t st);
+                       double2* partial2, bool recur, bool self, hipStream_t st);
 void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& tm,
                       double2* partial, hipStream_t st);
 void launch_block_reduce(const double2* partial, int tiles, int R, int M, int m0, int count,

@@ -73,6 +73,7 @@ SIGNATURES = {
     "lsqt_block_from_host": (C.c_int, [_vp, _dp, _dp]),
     "lsqt_block_to_host": (C.c_int, [_vp, _dp, _dp]),
     "lsqt_moments_block": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp, _dp]),
+    "lsqt_moments_block_self": (C.c_int, [_vp, _vp, C.c_int, _dp, _dp]),
     "lsqt_block_tune": (C.c_int, [_vp, _vp, _vp, C.c_int, _ip, _ip, _ip, _dp]),
     "lsqt_block_set_config": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
 }
@@ -325,6 +326,14 @@ class Hamiltonian:
         R = left.R
         re, im = np.empty(R * M), np.empty(R * M)
         _check(load().lsqt_moments_block(self.h, left.h, right.h, int(M), _dptr(re), _dptr(im)))
+        return (re + 1j * im).reshape(R, M)
+
+    def moments_block_self(self, v, M):
+        """complex array (R, M) of <v_r|T_m|v_r> by the product identities
+        (M/2 steps)"""
+        R = v.R
+        re, im = np.empty(R * M), np.empty(R * M)
+        _check(load().lsqt_moments_block_self(self.h, v.h, int(M), _dptr(re), _dptr(im)))
         return (re + 1j * im).reshape(R, M)
 
     def block_tune(self, left, right, force=False):

@@ -194,6 +194,13 @@ int lsqt_block_to_host(const lsqt_block* b, double* re, double* im);
 /* mu[r*M + m] = <left_r|T_m(H~)|right_r>, r < R, m < M (M >= 2) */
 int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right, int M,
                        double* mu_re, double* mu_im);
+/* mu[r*M + m] = <v_r|T_m(H~)|v_r> (the DOS moments) in M/2 steps, from the
+ * product identities T_{2m} = 2 T_m^2 - T_0 and T_{2m+1} = 2 T_m T_{m+1} - T_1
+ * (H Hermitian): step m adds <psi_{m-1}|psi_m> and <psi_m|psi_m>, and no left
+ * block is read.  The same numbers as lsqt_moments_block(h, v, v, ...) up to
+ * FP32 rounding of the states. */
+int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, double* mu_re,
+                            double* mu_im);
 /* The autotuner: times every kernel shape (W vectors per lane, i.e. R/W lanes
  * per site; rows = sites per workgroup tile; blocked = tile schedule) on these
  * blocks and keeps the fastest for (R, max_neighbor, table placement, size

@@ -111,6 +111,31 @@ def test_block_blocked_schedule(gpu_ctx, L, R):
         o.close()
 
 
+@pytest.mark.parametrize("case,R", [("cube_peierls", 8), ("config1_honeycomb_20k", 32),
+                                    ("square_vacancy_ldos", 4)])
+def test_block_self_products(gpu_ctx, case, R):
+    """DOS moments from the product identities (M/2 steps) against the direct
+    block moments and the FP64 single-vector path, for M across the 64-slot
+    partial ring's boundaries."""
+    print(f"[gpu] block self products {case} R={R}", flush=True)
+    m, H = _setup(gpu_ctx, case)
+    b, re, im = _block_of(gpu_ctx, m, R)
+    Mmax = 400
+    direct = H.moments_block(b, b, Mmax)
+    for M in (2, 3, 4, 64, 65, 66, 129, 130, 131, Mmax):
+        mu = H.moments_block_self(b, M)
+        assert mu.shape == (R, M)
+        scale = np.abs(direct[:, :1]).max()
+        np.testing.assert_allclose(mu, direct[:, :M], rtol=0, atol=2e-5 * scale,
+                                   err_msg=f"M={M}")
+    for r in (0, R - 1):
+        v = H.vector(re[r], im[r])
+        ref = H.moments(v, v, Mmax)
+        a, w = _row(m, mu[r].real.astype(complex)), _row(m, ref)
+        err = np.abs(a - w).max() / np.abs(w).max()
+        assert err <= 1e-6, f"vector {r}: DOS normwise {err:.2e}"
+
+
 def test_block_left_right_differ(gpu_ctx):
     """<L_r|T_m|R_r> with L != R (the VAC shape), against the FP64 path"""
     m, H = _setup(gpu_ctx, "cube_peierls")
