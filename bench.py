@@ -35,6 +35,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "hbm_traffic_probe.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` (template arguments without spaces)
+    from the committed FETCH_SIZE/WRITE_SIZE profile of the default
+    workloads, scaled by the plain-copy calibration; None if absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -197,6 +214,10 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     u64 = n * (M - 1) / t64
     u32 = n * (M - 1) / t32
     ub = n * R * (M - 1) / tb
+    # measured HBM bytes per launch, where the profile covers this workload
+    profiled = L == 160 and R == 32
+    traffic_left = profiled_traffic(f"k_block_step<1,6,{W},true,1,true>") if profiled else None
+    traffic_self = profiled_traffic(f"k_block_step<1,6,{W},true,2,true>") if profiled else None
     steps_self = M // 2
     us_ = n * R * steps_self / ts          # recursion site-updates/s
     um_ = n * R * (M - 1) / ts             # moments x sites x vectors per second
@@ -214,12 +235,16 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
                       "bytes_per_site_update": bpu_blk,
                       "achieved_GBs": ub * bpu_blk / 1e9,
                       "frac_of_hbm_peak": ub * bpu_blk / 1e9 / HBM_PEAK_GBS,
+                      "bytes_per_launch_algorithmic": bpu_blk * n * R,
+                      "traffic_per_launch": traffic_left,
                       "dos_normwise_error_vs_f64": errs},
         "block_f32_product_identities": {
             "R": R, "steps": steps_self, "moments": M,
             "value": us_, "unit": "site-updates/s",
             "bytes_per_site_update": (4.0 + 4.0 * 6) / R + 24.0,
             "achieved_GBs": us_ * ((4.0 + 4.0 * 6) / R + 24.0) / 1e9,
+            "frac_of_hbm_peak": us_ * ((4.0 + 4.0 * 6) / R + 24.0) / 1e9 / HBM_PEAK_GBS,
+            "traffic_per_launch": traffic_self,
             "moment_site_updates_per_s": um_,
             "time_to_moments_speedup_vs_f64": um_ / u64,
             "dos_normwise_error_vs_f64": errs_self},
@@ -232,6 +257,36 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     for o in (H, blk, v0):
         o.close()
     return res
+
+
+def probe(ctx, H, rv, args, moments=10):
+    """Short runs of the hot kernels for counter profiles: a plain copy of
+    one vector (k_copy: n x 16 B read and written, the counters' scale), the
+    FP64 and FP32 fused moment steps on the honeycomb workload, and the block
+    steps on the simple-cubic one."""
+    from gpuqt_amd import inputs as gi
+    from gpuqt_amd import native
+    w = H.vector()
+    w.copy_from(rv)
+    ctx.sync()
+    log(f"probe: copy of {H.n} x 16 B")
+    H.moments(rv, rv, moments)
+    H.moments_f32(rv, rv, moments)
+    ctx.sync()
+    log(f"probe: {moments} FP64 and FP32 moments on {H.n} sites")
+    if args.block:
+        arrays, n = gi.cubic_peierls_arrays(args.cubic_L)
+        Hc = native.Hamiltonian.from_arrays(ctx, arrays, n, 6, 15.0)
+        del arrays
+        R = args.block_R
+        ph = np.random.default_rng(7).uniform(0, 2 * np.pi, (R, n))
+        blk = native.Block(ctx, n, R)
+        blk.upload(np.cos(ph), np.sin(ph))
+        Hc.block_tune(blk, blk)
+        Hc.moments_block(blk, blk, moments)
+        Hc.moments_block_self(blk, 2 * moments)
+        ctx.sync()
+        log(f"probe: block R={R} moments on {n} sites")
 
 
 def cpu_baseline(nx, ny, moments_sample, threads):
@@ -289,6 +344,9 @@ def main():
     ap.add_argument("--block-moments", type=int, default=2048)
     ap.add_argument("--cpu-moments", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--probe", action="store_true",
+                    help="run each hot kernel a few times on the workloads' inputs (for "
+                         "hardware-counter profiles) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path has no "
@@ -326,6 +384,9 @@ def main():
     info = H.info()
     rv = H.vector(*r)
     ctx.sync()
+    if args.probe:
+        probe(ctx, H, rv, args)
+        return
     log(f"Hamiltonian on device: layout {'packed' if info.layout == 2 else 'full'}, "
         f"{info.classes} hopping classes, {info.bytes_per_site_f64:.0f} B per site-update")
 
@@ -359,6 +420,9 @@ def main():
     # update per site per launch
     bytes_per_launch = info.bytes_per_site_f64 * model.n
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    default_workload = (args.nx, args.ny) == (2000, 1000) and info.layout == 2 \
+        and info.max_neighbor == 3
+    traffic = profiled_traffic("k_hpsi<1,3,1,true,true,true>") if default_workload else None
     dos = rows[0]
     log(f"DOS(E=0) = {dos[len(dos) // 2]:.6g}, {len(rows)} rows")
 
@@ -425,7 +489,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on this workload, "
+                              "calibrated on a plain copy (profiles/hbm_traffic_probe.md)"
+                              if traffic else None,
+            "bytes_per_launch_algorithmic": bytes_per_launch,
             "bytes_per_site_update": info.bytes_per_site_f64,
             "avg_launch_us": avg_launch_ms * 1e3,
         },
