@@ -9,6 +9,7 @@
 #include "lsqt.h"
 #include "lsqt_host.h"
 #include "model.h"
+#include "postprocess.h"
 #include "sigma.h"
 #include "vector.h"
 
@@ -40,8 +41,40 @@ void print_welcome()
 
 }  // namespace
 
+namespace {
+
+// `lsqt_gpu --sigma <driver-file>`: the conductivity of every listed
+// directory from its existing output rows
+int postprocess_all(const char* driver)
+{
+  std::ifstream input(driver);
+  if (!input.is_open()) {
+    std::cout << "Failed to open " << driver << std::endl;
+    return 1;
+  }
+  std::string directory;
+  while (std::getline(input, directory)) {
+    if (directory == "")
+      continue;
+    try {
+      lsqt_host::Conductivity c = lsqt_host::postprocess_directory(directory);
+      std::cout << "Conductivity for " << directory << ": " << c.Nt << " time steps x " << c.Ne
+                << " energies" << (c.sigma_vac.empty() ? "" : ", sigma_vac.out")
+                << (c.sigma_msd.empty() ? "" : ", sigma_msd.out") << std::endl;
+    } catch (const lsqt_host::InputError& e) {
+      std::cout << e.what() << std::endl;
+      return 1;
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
 extern "C" int lsqt_host_main(int argc, char** argv)
 {
+  if (argc == 3 && std::strcmp(argv[1], "--sigma") == 0)
+    return postprocess_all(argv[2]);
   print_welcome();
   if (argc != 2) {
     std::cout << "Usage: src/gpuqt input.txt" << std::endl;
@@ -97,6 +130,39 @@ extern "C" int lsqt_host_damped_sum(int M, double* mu_re, double* mu_im, int n_e
   if (M < 2 || !mu_re || (n_energy > 0 && (!energy || !out)))
     return 1;
   lsqt_host::damped_chebyshev_sum(M, mu_re, mu_im, n_energy, energy, energy_max, volume, out);
+  return 0;
+}
+
+extern "C" int lsqt_host_conductivity(int n_energy, int n_time, const double* time_step,
+                                      const double* dos, int dos_rows, const double* vac,
+                                      int vac_rows, const double* msd, int msd_rows,
+                                      double* sigma_vac, double* sigma_msd, double* vac_true,
+                                      double* msd_true, char* err, int err_len)
+{
+  try {
+    lsqt_host::Conductivity c = lsqt_host::conductivity(n_energy, n_time, time_step, dos,
+                                                        dos_rows, vac, vac_rows, msd, msd_rows);
+    auto put = [](const std::vector<double>& v, double* out) {
+      if (out && !v.empty())
+        std::memcpy(out, v.data(), sizeof(double) * v.size());
+    };
+    put(c.sigma_vac, sigma_vac);
+    put(c.sigma_msd, sigma_msd);
+    put(c.vac, vac_true);
+    put(c.msd, msd_true);
+  } catch (const lsqt_host::InputError& e) {
+    return report(e, err, err_len);
+  }
+  return 0;
+}
+
+extern "C" int lsqt_host_postprocess(const char* input_dir, char* err, int err_len)
+{
+  try {
+    lsqt_host::postprocess_directory(input_dir);
+  } catch (const lsqt_host::InputError& e) {
+    return report(e, err, err_len);
+  }
   return 0;
 }
 

@@ -29,6 +29,10 @@ SIGNATURES = {
     "lsqt_host_model_arrays": (C.c_int, [_vp, _ip, _ip, _dp, _dp, _dp, _dp]),
     "lsqt_host_model_inputs": (C.c_int, [_vp, _dp, _dp, _ip]),
     "lsqt_host_model_random_state": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "lsqt_host_conductivity": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int,
+                                         _dp, C.c_int, _dp, _dp, _dp, _dp, C.c_char_p,
+                                         C.c_int]),
+    "lsqt_host_postprocess": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int]),
 }
 
 _lib = None
@@ -131,11 +135,57 @@ def correlation_row(mu, energies, energy_max, volume):
     return out
 
 
+def conductivity(time_step, dos=None, vac=None, msd=None):
+    """sigma from the VAC and MSD rows by the host code
+    (lsqt_host_conductivity): returns dict of [Nt, Ne] arrays sigma_vac,
+    sigma_msd, vac, msd (those that the inputs allow)."""
+    ts = np.ascontiguousarray(time_step, dtype=np.float64)
+    arrs = {}
+    for k, a in (("dos", dos), ("vac", vac), ("msd", msd)):
+        arrs[k] = None if a is None else np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64)
+    Ne = next(a.shape[1] for a in arrs.values() if a is not None)
+    Nt = len(ts)
+    out = {k: np.zeros((Nt, Ne)) for k in ("sigma_vac", "sigma_msd", "vac", "msd")}
+    err = C.create_string_buffer(4096)
+
+    def ptr(a):
+        return None if a is None else a.ctypes.data_as(_dp)
+
+    def rows(a):
+        return 0 if a is None else a.shape[0]
+
+    rc = load().lsqt_host_conductivity(Ne, Nt, ptr(ts), ptr(arrs["dos"]), rows(arrs["dos"]),
+                                       ptr(arrs["vac"]), rows(arrs["vac"]), ptr(arrs["msd"]),
+                                       rows(arrs["msd"]), ptr(out["sigma_vac"]),
+                                       ptr(out["sigma_msd"]), ptr(out["vac"]), ptr(out["msd"]),
+                                       err, len(err))
+    if rc:
+        raise InputError(err.value.decode())
+    keep = {}
+    if vac is not None:
+        keep["sigma_vac"] = out["sigma_vac"]
+        if dos is not None:
+            keep["vac"] = out["vac"]
+    if msd is not None:
+        keep["sigma_msd"] = out["sigma_msd"]
+        if dos is not None:
+            keep["msd"] = out["msd"]
+    return keep
+
+
+def postprocess(dirname: str):
+    """Writes dirname/sigma_vac.out and sigma_msd.out (lsqt_host_postprocess)."""
+    err = C.create_string_buffer(4096)
+    if load().lsqt_host_postprocess(dirname.encode(), err, len(err)):
+        raise InputError(err.value.decode())
+
+
 def run_cli(driver_file: str, digits: int | None = None, timeout: float | None = None,
-            env=None):
-    """Runs the lsqt_gpu program on a driver file."""
+            env=None, sigma: bool = False):
+    """Runs the lsqt_gpu program on a driver file (sigma: the --sigma
+    post-processing mode)."""
     e = dict(os.environ if env is None else env)
     if digits is not None:
         e["LSQT_OUTPUT_DIGITS"] = str(digits)
-    return subprocess.run([CLI_PATH, driver_file], capture_output=True, text=True,
-                          timeout=timeout, env=e)
+    argv = [CLI_PATH, "--sigma", driver_file] if sigma else [CLI_PATH, driver_file]
+    return subprocess.run(argv, capture_output=True, text=True, timeout=timeout, env=e)

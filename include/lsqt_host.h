@@ -28,6 +28,25 @@ int lsqt_host_run(const char* input_dir, char* err, int err_len);
 int lsqt_host_damped_sum(int M, double* mu_re, double* mu_im, int n_energy,
                          const double* energy, double energy_max, double volume, double* out);
 
+/* Conductivity from the correlation rows (the post-processing of the
+ * reference's example scripts, examples/fan2018cpc/lattice/diffusive/
+ * plot_results.m:16-44): dos_rows rows of n_energy DOS values, vac_rows /
+ * msd_rows rows (whole blocks of n_time rows, one block per random vector;
+ * NULL or 0 skips), time_step[n_time].  Outputs [n_time][n_energy] (NULL
+ * skips): sigma_vac(k) = 2 pi * trapezoid integral of the averaged VAC over
+ * t = 0, dt0, dt0 + dt1, ...; sigma_msd(k) = pi (MSD(k) - MSD(k-1)) / dt(k),
+ * MSD(-1) = 0 (units e^2/h); vac_true / msd_true = averages / averaged DOS
+ * (need dos_rows > 0). */
+int lsqt_host_conductivity(int n_energy, int n_time, const double* time_step, const double* dos,
+                           int dos_rows, const double* vac, int vac_rows, const double* msd,
+                           int msd_rows, double* sigma_vac, double* sigma_msd, double* vac_true,
+                           double* msd_true, char* err, int err_len);
+/* the same from input_dir's energy.in, time_step.in, dos.out, vac.out and
+ * msd.out, written to input_dir/sigma_vac.out and sigma_msd.out (n_time rows
+ * of n_energy values in the .out format, replacing earlier files); also
+ * `lsqt_gpu --sigma <driver-file>` for every listed directory */
+int lsqt_host_postprocess(const char* input_dir, char* err, int err_len);
+
 /* significant digits of the .out rows (default 6, the reference's) */
 void lsqt_host_set_output_precision(int digits);
 
