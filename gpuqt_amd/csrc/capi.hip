@@ -426,6 +426,36 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
   return LSQT_OK;
 }
 
+extern "C" int lsqt_ham_create_segmented(lsqt_ctx* ctx, int segments, const int* starts, int n,
+                                         int K, const int* nn, const int* nl, const double* pot,
+                                         const double* hr, const double* hi, const double* xx,
+                                         double energy_max, lsqt_ham** out)
+{
+  if (!out || !starts || !nn || !nl)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (segments < 1 || starts[0] != 0 || starts[segments] != n)
+    return set_error(LSQT_ERR_INVALID, "segment starts must run from 0 to n = %d", n);
+  for (int k = 0; k < segments; ++k) {
+    if (starts[k + 1] <= starts[k])
+      return set_error(LSQT_ERR_INVALID, "segment %d is empty or reversed", k);
+    for (int i = starts[k]; i < starts[k + 1]; ++i)
+      for (int m = 0; m < nn[i] && m < K; ++m) {
+        const int j = nl[(size_t)i * K + m];
+        if (j < starts[k] || j >= starts[k + 1])
+          return set_error(LSQT_ERR_INVALID, "site %d of segment %d has neighbour %d outside "
+                           "its segment", i, k, j);
+      }
+  }
+  lsqt_ham* h = nullptr;
+  LSQT_TRY(lsqt_ham_create(ctx, n, K, nn, nl, pot, hr, hi, xx, energy_max,
+                           LSQT_LAYOUT_PACKED, &h));
+  h->seg_starts.assign(starts, starts + segments + 1);
+  h->slab = h->row = 0;  // lattice tile schedules do not cross segments
+  *out = h;
+  return LSQT_OK;
+}
+
 extern "C" int lsqt_ham_destroy(lsqt_ham* h)
 {
   if (!h)

@@ -20,6 +20,7 @@
 #include <tuple>
 #include <vector>
 
+#include "bessel.h"
 #include "lsqt_capi_internal.h"
 #include "lsqt_hip.h"
 #include "lsqt_launch.h"
@@ -35,9 +36,15 @@ struct lsqt_block {
 
 namespace {
 struct BlockScratch {
-  float2* A = nullptr;
+  float2* A = nullptr;  // moment recursion ping-pong, evolution temporaries
   float2* B = nullptr;
   size_t elems = 0;
+  float2* F[3] = {nullptr, nullptr, nullptr};  // left vector; [X, U] temporaries
+  size_t f_elems = 0;
+  double2* S[3] = {nullptr, nullptr, nullptr};  // FP64 states of the time drivers
+  size_t s_elems = 0;
+  double2* norms = nullptr;  // [2][segments][R]
+  size_t norms_cap = 0;
   double2* ring = nullptr;
   size_t ring_cap = 0;
   double2* mu = nullptr;
@@ -55,9 +62,19 @@ struct BlockShape {
   int blocked = 0;
 };
 
+// tiles of a segmented Hamiltonian for one tile size: `rows` sites per tile,
+// never across a segment boundary
+struct SegTable {
+  int tiles = 0;
+  int2* table = nullptr;
+  int* tile_seg = nullptr;
+  int* seg_tiles = nullptr;
+};
+
 struct lsqt_block_state {
   BlockScratch s;
   std::map<int, BlockShape> forced;  // R -> shape set by the caller
+  std::map<int, SegTable> seg;       // rows -> tile table
 };
 
 void lsqt::block_release(lsqt_ham* h)
@@ -65,9 +82,16 @@ void lsqt::block_release(lsqt_ham* h)
   if (!h->blk)
     return;
   BlockScratch& s = h->blk->s;
-  for (void* p : {(void*)s.A, (void*)s.B, (void*)s.ring, (void*)s.mu})
+  for (void* p : {(void*)s.A, (void*)s.B, (void*)s.ring, (void*)s.mu, (void*)s.F[0],
+                  (void*)s.F[1], (void*)s.F[2], (void*)s.S[0], (void*)s.S[1], (void*)s.S[2],
+                  (void*)s.norms})
     if (p)
       (void)hipFree(p);
+  for (auto& kv : h->blk->seg)
+    for (void* p : {(void*)kv.second.table, (void*)kv.second.tile_seg,
+                    (void*)kv.second.seg_tiles})
+      if (p)
+        (void)hipFree(p);
   if (s.mu_host)
     (void)hipHostFree(s.mu_host);
   delete h->blk;
@@ -242,6 +266,11 @@ extern "C" int lsqt_block_to_host(const lsqt_block* b, double* re, double* im)
 // ---------------------------------------------------------------------------
 namespace {
 
+int segments(const lsqt_ham* h)
+{
+  return h->seg_starts.empty() ? 1 : (int)h->seg_starts.size() - 1;
+}
+
 int scratch_for(lsqt_ham* h, int R, int M, int tiles, BlockScratch** out)
 {
   if (!h->blk)
@@ -261,27 +290,81 @@ int scratch_for(lsqt_ham* h, int R, int M, int tiles, BlockScratch** out)
   }
   LSQT_TRY(ensure_device((void**)&s.ring, &s.ring_cap,
                          sizeof(double2) * (size_t)kRing * tiles * R));
-  LSQT_TRY(ensure_device((void**)&s.mu, &s.mu_cap, sizeof(double2) * (size_t)M * R));
-  LSQT_TRY(ensure_pinned((void**)&s.mu_host, &s.mu_host_cap, sizeof(double2) * (size_t)M * R));
+  const size_t outs = (size_t)M * R * segments(h);
+  LSQT_TRY(ensure_device((void**)&s.mu, &s.mu_cap, sizeof(double2) * outs));
+  LSQT_TRY(ensure_pinned((void**)&s.mu_host, &s.mu_host_cap, sizeof(double2) * outs));
   *out = &s;
   return LSQT_OK;
 }
 
 // enqueues steps [from, to) of the recursion; step m's partials go to ring
 // slot m % kRing, and full rings are reduced into mu
-// blocked: 0 natural, 1 blocked, 2 blocked with the column walk
-TileMap tile_map(const lsqt_ham* h, int R, const BlockShape& c)
+// the tile table of a segmented Hamiltonian for tiles of `rows` sites
+int seg_table(lsqt_ham* h, int rows, SegTable** out)
 {
-  const int per_pass = kBlock / (R / c.W);
-  return make_tile_map(h->d.n, c.rows, c.blocked ? h->slab : 0, c.blocked ? c.rows : 0, h->row,
-                       c.blocked == 2 ? per_pass : 0);
+  if (!h->blk)
+    h->blk = new lsqt_block_state();
+  auto it = h->blk->seg.find(rows);
+  if (it == h->blk->seg.end()) {
+    std::vector<int2> table;
+    std::vector<int> tile_seg, seg_tiles;
+    const int S = segments(h);
+    for (int k = 0; k < S; ++k) {
+      seg_tiles.push_back((int)table.size());
+      for (int b = h->seg_starts[k]; b < h->seg_starts[k + 1]; b += rows) {
+        table.push_back(make_int2(b, std::min(b + rows, h->seg_starts[k + 1])));
+        tile_seg.push_back(k);
+      }
+    }
+    seg_tiles.push_back((int)table.size());
+    SegTable t;
+    t.tiles = (int)table.size();
+    size_t cap = 0;
+    LSQT_TRY(ensure_device((void**)&t.table, &cap, sizeof(int2) * table.size()));
+    cap = 0;
+    LSQT_TRY(ensure_device((void**)&t.tile_seg, &cap, sizeof(int) * tile_seg.size()));
+    cap = 0;
+    LSQT_TRY(ensure_device((void**)&t.seg_tiles, &cap, sizeof(int) * seg_tiles.size()));
+    LSQT_HIP(hipMemcpy(t.table, table.data(), sizeof(int2) * table.size(),
+                       hipMemcpyHostToDevice));
+    LSQT_HIP(hipMemcpy(t.tile_seg, tile_seg.data(), sizeof(int) * tile_seg.size(),
+                       hipMemcpyHostToDevice));
+    LSQT_HIP(hipMemcpy(t.seg_tiles, seg_tiles.data(), sizeof(int) * seg_tiles.size(),
+                       hipMemcpyHostToDevice));
+    it = h->blk->seg.emplace(rows, t).first;
+  }
+  *out = &it->second;
+  return LSQT_OK;
 }
 
-void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const float2* L, float2*& A,
-                   float2*& B, double2* ring, double2* mu, int M, int from, int to,
+// blocked: 0 natural, 1 blocked, 2 blocked with the column walk; segmented
+// Hamiltonians use their tile table (natural tiles inside each segment)
+int tile_map(lsqt_ham* h, int R, const BlockShape& c, TileMap* tm)
+{
+  if (!h->seg_starts.empty()) {
+    SegTable* t;
+    LSQT_TRY(seg_table(h, c.rows, &t));
+    TileMap m{};
+    m.n = h->d.n;
+    m.rows = c.rows;
+    m.tiles = t->tiles;
+    m.table = t->table;
+    m.tile_seg = t->tile_seg;
+    m.seg_tiles = t->seg_tiles;
+    m.segs = segments(h);
+    *tm = m;
+    return LSQT_OK;
+  }
+  const int per_pass = kBlock / (R / c.W);
+  *tm = make_tile_map(h->d.n, c.rows, c.blocked ? h->slab : 0, c.blocked ? c.rows : 0, h->row,
+                      c.blocked == 2 ? per_pass : 0);
+  return LSQT_OK;
+}
+
+void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, const float2* L,
+                   float2*& A, float2*& B, double2* ring, double2* mu, int M, int from, int to,
                    hipStream_t st)
 {
-  const TileMap tm = tile_map(h, R, c);
   const int tiles = tm.tiles;
   const size_t slot = (size_t)tiles * R;
   for (int m = from; m < to; ++m) {
@@ -298,7 +381,7 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const float2* L, flo
     }
     if (mu && (m % kRing == kRing - 1 || m == M - 1)) {
       int m0 = m - m % kRing;
-      launch_block_reduce(ring, tiles, R, M, m0, m - m0 + 1, mu, st);
+      launch_block_reduce(ring, tm, R, M, m0, m - m0 + 1, mu, st);
     }
   }
 }
@@ -309,14 +392,13 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const float2* L, flo
 // the moments follow from T_{2m} = 2 T_m^2 - T_0, T_{2m+1} = 2 T_m T_{m+1} - T_1
 // (H Hermitian): mu_0 = S_0, mu_1 = S_1, mu_k = 2 S_k - mu_{k mod 2}.  Moment k
 // >= 1 goes to ring slot (k - 1) mod kRing; M/2 steps give moments 0..M-1.
-void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, float2*& A, float2*& B,
-                        double2* ring, double2* mu, int M, hipStream_t st)
+void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, float2*& A,
+                        float2*& B, double2* ring, double2* mu, int M, hipStream_t st)
 {
-  const TileMap tm = tile_map(h, R, c);
   const int tiles = tm.tiles;
   const size_t slot = (size_t)tiles * R;
   launch_block_dot(A, A, R, tm, ring, st);
-  launch_block_reduce(ring, tiles, R, M, 0, 1, mu, st);
+  launch_block_reduce(ring, tm, R, M, 0, 1, mu, st);
   const int steps = M / 2;
   for (int m = 1; m <= steps; ++m) {
     double2* odd = ring + (size_t)((2 * m - 2) % kRing) * slot;
@@ -333,13 +415,13 @@ void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, float2*& A, flo
     if (j % kRing == kRing - 1 || m == steps) {
       const int k0 = j - j % kRing + 1;
       const int count = std::min(j - (k0 - 1) + 1, M - k0);
-      launch_block_reduce(ring, tiles, R, M, k0, count, mu, st);
+      launch_block_reduce(ring, tm, R, M, k0, count, mu, st);
     }
   }
 }
 
-int pick_config(lsqt_ham* h, int R, const lsqt_block* left, const lsqt_block* right,
-                BlockShape* out, double* us_out, bool force);
+int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, BlockShape* out,
+                double* us_out, bool force);
 
 }  // namespace
 
@@ -382,67 +464,40 @@ extern "C" int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int bl
   return LSQT_OK;
 }
 
-extern "C" int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right,
-                                  int M, double* mu_re, double* mu_im)
-{
-  LSQT_TRY(check_block(h, left, right));
-  if (M < 2)
-    return set_error(LSQT_ERR_INVALID, "number of moments %d must be >= 2", M);
-  if (!mu_re || !mu_im)
-    return set_error(LSQT_ERR_INVALID, "null moment output");
-  LSQT_TRY(f32_potential(h));
-  const int R = left->R;
-  BlockShape c;
-  double us;
-  LSQT_TRY(pick_config(h, R, left, right, &c, &us, false));
-  BlockScratch* s;
-  LSQT_TRY(scratch_for(h, R, M, tile_map(h, R, c).tiles, &s));
-  hipStream_t st = h->ctx->stream;
-  float2* A = s->A;
-  float2* B = s->B;
-  LSQT_HIP(hipMemcpyAsync(A, right->d, sizeof(float2) * (size_t)h->d.n * R,
-                          hipMemcpyDeviceToDevice, st));
-  enqueue_steps(h, R, c, left->d, A, B, s->ring, s->mu, M, 0, M, st);
-  LSQT_HIP(hipGetLastError());
-  LSQT_HIP(hipMemcpyAsync(s->mu_host, s->mu, sizeof(double2) * (size_t)M * R,
-                          hipMemcpyDeviceToHost, st));
-  LSQT_HIP(hipStreamSynchronize(st));
-  for (size_t i = 0; i < (size_t)M * R; ++i) {
-    mu_re[i] = s->mu_host[i].x;
-    mu_im[i] = s->mu_host[i].y;
-  }
-  return LSQT_OK;
-}
+namespace {
 
-extern "C" int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, double* mu_re,
-                                       double* mu_im)
+// One moment set for R vectors of every segment: <left_r|T_m|right_r> (or the
+// self products of `right` by the product identities), downloaded into
+// mu[(k*R + r)*M + m].  A and B of the scratch are overwritten.
+int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, BlockScratch* s,
+               const float2* left, const float2* right, bool self, int M, double* mu_re,
+               double* mu_im)
 {
-  LSQT_TRY(check_block(h, v, v));
-  if (M < 2)
-    return set_error(LSQT_ERR_INVALID, "number of moments %d must be >= 2", M);
-  if (!mu_re || !mu_im)
-    return set_error(LSQT_ERR_INVALID, "null moment output");
-  LSQT_TRY(f32_potential(h));
-  const int R = v->R;
-  BlockShape c;
-  double us;
-  LSQT_TRY(pick_config(h, R, v, v, &c, &us, false));
-  BlockScratch* s;
-  LSQT_TRY(scratch_for(h, R, M, tile_map(h, R, c).tiles, &s));
   hipStream_t st = h->ctx->stream;
   float2* A = s->A;
   float2* B = s->B;
-  LSQT_HIP(hipMemcpyAsync(A, v->d, sizeof(float2) * (size_t)h->d.n * R, hipMemcpyDeviceToDevice,
-                          st));
-  enqueue_self_steps(h, R, c, A, B, s->ring, s->mu, M, st);
+  if (A != right)
+    LSQT_HIP(hipMemcpyAsync(A, right, sizeof(float2) * (size_t)h->d.n * R,
+                            hipMemcpyDeviceToDevice, st));
+  if (self)
+    enqueue_self_steps(h, R, c, tm, A, B, s->ring, s->mu, M, st);
+  else
+    enqueue_steps(h, R, c, tm, left, A, B, s->ring, s->mu, M, 0, M, st);
   LSQT_HIP(hipGetLastError());
-  LSQT_HIP(hipMemcpyAsync(s->mu_host, s->mu, sizeof(double2) * (size_t)M * R,
-                          hipMemcpyDeviceToHost, st));
+  const size_t count = (size_t)M * R * segments(h);
+  LSQT_HIP(hipMemcpyAsync(s->mu_host, s->mu, sizeof(double2) * count, hipMemcpyDeviceToHost, st));
   LSQT_HIP(hipStreamSynchronize(st));
-  for (int r = 0; r < R; ++r) {
-    const double2* S = s->mu_host + (size_t)r * M;
-    double* re = mu_re + (size_t)r * M;
-    double* im = mu_im + (size_t)r * M;
+  for (size_t v = 0; v < count / M; ++v) {
+    const double2* S = s->mu_host + v * M;
+    double* re = mu_re + v * M;
+    double* im = mu_im + v * M;
+    if (!self) {
+      for (int k = 0; k < M; ++k) {
+        re[k] = S[k].x;
+        im[k] = S[k].y;
+      }
+      continue;
+    }
     re[0] = S[0].x;
     im[0] = S[0].y;
     re[1] = S[1].x;
@@ -455,6 +510,262 @@ extern "C" int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, 
   return LSQT_OK;
 }
 
+// shape, tile map and scratch (M moments) for R vectors
+int block_setup(lsqt_ham* h, int R, const float2* left, const float2* right, int M,
+                BlockShape* c, TileMap* tm, BlockScratch** s)
+{
+  LSQT_TRY(f32_potential(h));
+  double us;
+  LSQT_TRY(pick_config(h, R, left, right, c, &us, false));
+  LSQT_TRY(tile_map(h, R, *c, tm));
+  LSQT_TRY(scratch_for(h, R, M, tm->tiles, s));
+  return LSQT_OK;
+}
+
+}  // namespace
+
+extern "C" int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right,
+                                  int M, double* mu_re, double* mu_im)
+{
+  LSQT_TRY(check_block(h, left, right));
+  if (M < 2)
+    return set_error(LSQT_ERR_INVALID, "number of moments %d must be >= 2", M);
+  if (!mu_re || !mu_im)
+    return set_error(LSQT_ERR_INVALID, "null moment output");
+  BlockShape c;
+  TileMap tm;
+  BlockScratch* s;
+  LSQT_TRY(block_setup(h, left->R, left->d, right->d, M, &c, &tm, &s));
+  return moment_set(h, left->R, c, tm, s, left->d, right->d, false, M, mu_re, mu_im);
+}
+
+extern "C" int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, double* mu_re,
+                                       double* mu_im)
+{
+  LSQT_TRY(check_block(h, v, v));
+  if (M < 2)
+    return set_error(LSQT_ERR_INVALID, "number of moments %d must be >= 2", M);
+  if (!mu_re || !mu_im)
+    return set_error(LSQT_ERR_INVALID, "null moment output");
+  BlockShape c;
+  TileMap tm;
+  BlockScratch* s;
+  LSQT_TRY(block_setup(h, v->R, v->d, v->d, M, &c, &tm, &s));
+  return moment_set(h, v->R, c, tm, s, nullptr, v->d, true, M, mu_re, mu_im);
+}
+
+// ---------------------------------------------------------------------------
+// time evolution of blocks and the VAC / MSD drivers
+// ---------------------------------------------------------------------------
+namespace {
+
+int evo_scratch(lsqt_ham* h, int R, BlockScratch* s)
+{
+  const size_t elems = (size_t)h->d.n * R;
+  if (s->f_elems < elems) {
+    for (float2*& p : s->F) {
+      if (p)
+        (void)hipFree(p);
+      p = nullptr;
+    }
+    s->f_elems = 0;
+    for (float2*& p : s->F)
+      LSQT_TRY(block_alloc(h->ctx, elems, &p));
+    s->f_elems = elems;
+  }
+  if (s->s_elems < elems) {
+    for (double2*& p : s->S) {
+      if (p)
+        (void)hipFree(p);
+      p = nullptr;
+    }
+    s->s_elems = 0;
+    for (double2*& p : s->S) {
+      void* q = nullptr;
+      hipError_t e = hipMalloc(&q, sizeof(double2) * elems);
+      if (e != hipSuccess)
+        return set_error(LSQT_ERR_NOMEM, "hipMalloc(%zu double2): %s", elems,
+                         hipGetErrorString(e));
+      p = (double2*)q;
+    }
+    s->s_elems = elems;
+  }
+  LSQT_TRY(ensure_device((void**)&s->norms, &s->norms_cap,
+                         sizeof(double2) * 2 * (size_t)segments(h) * R));
+  return LSQT_OK;
+}
+
+// |S_r|^2 per segment into norms slot `slot` (ring slot 0 as partials)
+void block_norms(lsqt_ham* h, int R, const TileMap& tm, BlockScratch* s, const double2* S,
+                 int slot)
+{
+  hipStream_t st = h->ctx->stream;
+  launch_block_norm64(S, R, tm, s->ring, st);
+  launch_block_reduce(s->ring, tm, R, 1, 0, 1, s->norms + (size_t)slot * segments(h) * R, st);
+}
+
+// S <- U(direction t) S for every vector (src/sigma.cu:116-151 with FP32
+// recursion states and FP64 accumulation), re-normalised to the norm before
+// the step when `renormalise`
+void block_evolve(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, BlockScratch* s,
+                  int direction, double t, double2* S, bool renormalise)
+{
+  hipStream_t st = h->ctx->stream;
+  const int nR = (int)((size_t)h->d.n * R);
+  float2* A = s->A;
+  float2* B = s->B;
+  if (renormalise)
+    block_norms(h, R, tm, s, S, 0);
+  BesselTerms bt = evolve_terms(direction, t);
+  launch_to_f32(S, A, nR, st);
+  launch_block_evo(h->d, R, c.W, tm, 0, nullptr, A, B, nullptr, nullptr, nullptr, nullptr, 0.0,
+                   0, st);
+  launch_cheb01_32(B, S, bt.b0, bt.b1, direction, nR, st);
+  for (size_t k = 0; k < bt.bm.size(); ++k) {
+    launch_block_evo(h->d, R, c.W, tm, 1, A, B, A, nullptr, nullptr, nullptr, S, bt.bm[k],
+                     bt.label[k], st);
+    std::swap(A, B);
+  }
+  if (renormalise) {
+    block_norms(h, R, tm, s, S, 1);
+    launch_block_renorm(S, R, tm, s->norms, s->norms + (size_t)segments(h) * R, st);
+  }
+}
+
+// S <- [X, U(t)] S (src/sigma.cu:157-200)
+void block_evolvex(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, BlockScratch* s,
+                   int direction, double t, double2* S)
+{
+  hipStream_t st = h->ctx->stream;
+  const int nR = (int)((size_t)h->d.n * R);
+  float2 *A = s->A, *B = s->B, *Ax = s->F[1], *Bx = s->F[2];
+  BesselTerms bt = evolvex_terms(direction, t);
+  launch_to_f32(S, A, nR, st);
+  launch_zero32(Ax, nR, st);
+  launch_block_evo(h->d, R, c.W, tm, 0, nullptr, A, B, nullptr, nullptr, nullptr, nullptr, 0.0,
+                   0, st);
+  launch_block_evo(h->d, R, c.W, tm, 2, nullptr, A, Bx, nullptr, nullptr, nullptr, nullptr, 0.0,
+                   0, st);
+  launch_cheb1x_32(Bx, S, bt.b1, nR, st);
+  for (size_t k = 0; k < bt.bm.size(); ++k) {
+    launch_block_evo(h->d, R, c.W, tm, 4, A, B, A, Ax, Bx, Ax, S, bt.bm[k], bt.label[k], st);
+    std::swap(A, B);
+    std::swap(Ax, Bx);
+  }
+}
+
+// host vectors (vector-major [R][n]) into a site-major FP64 block
+int upload_states(lsqt_ham* h, int R, const double* re, const double* im, double2* S)
+{
+  const size_t n = (size_t)h->d.n;
+  std::vector<double2> tmp(n * R);
+  for (size_t r = 0; r < (size_t)R; ++r)
+    for (size_t i = 0; i < n; ++i)
+      tmp[i * R + r] = make_double2(re[r * n + i], im[r * n + i]);
+  LSQT_HIP(hipStreamSynchronize(h->ctx->stream));
+  LSQT_HIP(hipMemcpy(S, tmp.data(), sizeof(double2) * n * R, hipMemcpyHostToDevice));
+  return LSQT_OK;
+}
+
+int check_driver(lsqt_ham* h, int R, const double* re, const double* im, int M, int n_steps,
+                 const double* steps, const double* mu_re, const double* mu_im)
+{
+  if (!h || !re || !im || !steps || !mu_re || !mu_im)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  if (!valid_R(R))
+    return set_error(LSQT_ERR_INVALID, "R = %d must be a power of 2 in 1..256", R);
+  if (M < 2 || n_steps < 1)
+    return set_error(LSQT_ERR_INVALID, "M = %d (>= 2) and %d time steps (>= 1)", M, n_steps);
+  if (h->d.layout == kFull)
+    return set_error(LSQT_ERR_UNSUPPORTED, "the block path needs the packed layout");
+  return ctx_activate(h->ctx);
+}
+
+}  // namespace
+
+extern "C" int lsqt_block_vac(lsqt_ham* h, int R, const double* re, const double* im, int M,
+                              int n_steps, const double* time_step_scaled, double* mu_re,
+                              double* mu_im)
+{
+  LSQT_TRY(check_driver(h, R, re, im, M, n_steps, time_step_scaled, mu_re, mu_im));
+  BlockScratch* s;
+  LSQT_TRY(scratch_for(h, R, 2, 1, &s));
+  LSQT_TRY(evo_scratch(h, R, s));
+  double2* left = s->S[0];   // U(-t) phi
+  double2* right = s->S[1];  // U(-t) V phi
+  LSQT_TRY(upload_states(h, R, re, im, left));
+  hipStream_t st = h->ctx->stream;
+  const int nR = (int)((size_t)h->d.n * R);
+  float2* L = s->F[0];
+  // the shape is tuned on the random vectors themselves
+  launch_to_f32(left, L, nR, st);
+  BlockShape c;
+  TileMap tm;
+  LSQT_TRY(block_setup(h, R, L, L, M, &c, &tm, &s));
+  LSQT_TRY(evo_scratch(h, R, s));
+  // state_right = V state_left (src/sigma.cu:293)
+  launch_to_f32(left, s->B, nR, st);
+  launch_block_evo(h->d, R, c.W, tm, 3, nullptr, s->B, s->A, nullptr, nullptr, nullptr, nullptr,
+                   0.0, 0, st);
+  launch_to_f64(s->A, right, nR, st);
+  const size_t per_step = (size_t)M * R * segments(h);
+  for (int m = 0; m < n_steps; ++m) {
+    // state_left_copy = V state_left; moments <state_right|T|state_left_copy>
+    launch_to_f32(left, s->B, nR, st);
+    launch_block_evo(h->d, R, c.W, tm, 3, nullptr, s->B, s->A, nullptr, nullptr, nullptr,
+                     nullptr, 0.0, 0, st);
+    launch_to_f32(right, L, nR, st);
+    LSQT_TRY(moment_set(h, R, c, tm, s, L, s->A, false, M, mu_re + m * per_step,
+                        mu_im + m * per_step));
+    if (m < n_steps - 1) {
+      block_evolve(h, R, c, tm, s, -1, time_step_scaled[m], left, true);
+      block_evolve(h, R, c, tm, s, -1, time_step_scaled[m], right, true);
+    }
+  }
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double* im, int M,
+                              int n_steps, const double* time_step_scaled, double* mu_re,
+                              double* mu_im)
+{
+  LSQT_TRY(check_driver(h, R, re, im, M, n_steps, time_step_scaled, mu_re, mu_im));
+  BlockScratch* s;
+  LSQT_TRY(scratch_for(h, R, 2, 1, &s));
+  LSQT_TRY(evo_scratch(h, R, s));
+  double2* state = s->S[0];
+  double2* state_x = s->S[1];
+  double2* state_copy = s->S[2];
+  LSQT_TRY(upload_states(h, R, re, im, state));
+  hipStream_t st = h->ctx->stream;
+  const int nR = (int)((size_t)h->d.n * R);
+  launch_to_f32(state, s->F[0], nR, st);
+  BlockShape c;
+  TileMap tm;
+  LSQT_TRY(block_setup(h, R, s->F[0], s->F[0], M, &c, &tm, &s));
+  LSQT_TRY(evo_scratch(h, R, s));
+  launch_copy(state_x, state, nR, st);
+  block_evolve(h, R, c, tm, s, 1, time_step_scaled[0], state, true);
+  block_evolvex(h, R, c, tm, s, 1, time_step_scaled[0], state_x);
+  const size_t per_step = (size_t)M * R * segments(h);
+  for (int m = 0; m < n_steps; ++m) {
+    launch_to_f32(state_x, s->A, nR, st);
+    LSQT_TRY(moment_set(h, R, c, tm, s, nullptr, s->A, true, M, mu_re + m * per_step,
+                        mu_im + m * per_step));
+    if (m < n_steps - 1) {
+      const double t = time_step_scaled[m + 1];
+      launch_copy(state_copy, state, nR, st);
+      block_evolvex(h, R, c, tm, s, 1, t, state_copy);
+      block_evolve(h, R, c, tm, s, 1, t, state_x, true);
+      launch_add(state_x, state_copy, nR, st);
+      block_evolve(h, R, c, tm, s, 1, t, state, true);
+    }
+  }
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
 extern "C" int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right,
                                int force, int* W, int* rows, int* blocked, double* us_per_step)
 {
@@ -462,7 +773,7 @@ extern "C" int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_b
   LSQT_TRY(f32_potential(h));
   BlockShape c;
   double us;
-  LSQT_TRY(pick_config(h, left->R, left, right, &c, &us, force != 0));
+  LSQT_TRY(pick_config(h, left->R, left->d, right->d, &c, &us, force != 0));
   if (W)
     *W = c.W;
   if (rows)
@@ -476,8 +787,8 @@ extern "C" int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_b
 
 namespace {
 
-int pick_config(lsqt_ham* h, int R, const lsqt_block* left, const lsqt_block* right,
-                BlockShape* out, double* us_out, bool force)
+int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, BlockShape* out,
+                double* us_out, bool force)
 {
   if (h->blk) {
     auto fit = h->blk->forced.find(R);
@@ -530,8 +841,11 @@ int pick_config(lsqt_ham* h, int R, const lsqt_block* left, const lsqt_block* ri
   }
   const int steps = 6;
   int max_tiles = 0;
-  for (const BlockShape& c : cands)
-    max_tiles = std::max(max_tiles, tile_map(h, R, c).tiles);
+  for (const BlockShape& c : cands) {
+    TileMap tm;
+    LSQT_TRY(tile_map(h, R, c, &tm));
+    max_tiles = std::max(max_tiles, tm.tiles);
+  }
   BlockScratch* s;
   LSQT_TRY(scratch_for(h, R, steps, max_tiles, &s));
   hipStream_t st = h->ctx->stream;
@@ -543,11 +857,13 @@ int pick_config(lsqt_ham* h, int R, const lsqt_block* left, const lsqt_block* ri
     // time a few fused steps on the caller's blocks
     float2* A = s->A;
     float2* B = s->B;
-    LSQT_HIP(hipMemcpyAsync(A, right->d, sizeof(float2) * (size_t)h->d.n * R,
+    TileMap tm;
+    LSQT_TRY(tile_map(h, R, c, &tm));
+    LSQT_HIP(hipMemcpyAsync(A, right, sizeof(float2) * (size_t)h->d.n * R,
                             hipMemcpyDeviceToDevice, st));
-    enqueue_steps(h, R, c, left->d, A, B, s->ring, nullptr, steps, 0, 2, st);
+    enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, steps, 0, 2, st);
     LSQT_HIP(hipEventRecord(e0, st));
-    enqueue_steps(h, R, c, left->d, A, B, s->ring, nullptr, steps, 2, steps, st);
+    enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, steps, 2, steps, st);
     LSQT_HIP(hipEventRecord(e1, st));
     LSQT_HIP(hipEventSynchronize(e1));
     float ms = 0.f;

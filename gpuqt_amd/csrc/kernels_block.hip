@@ -73,7 +73,11 @@ __device__ __forceinline__ void stv(float2* p, const Vec<W>& r)
 
 __device__ __forceinline__ void tile_range(const TileMap& t, int tile, int& begin, int& end)
 {
-  if (t.slab == 0) {
+  if (t.table) {
+    const int2 e = t.table[tile];
+    begin = e.x;
+    end = e.y;
+  } else if (t.slab == 0) {
     begin = tile * t.rows;
     end = min(begin + t.rows, t.n);
   } else {
@@ -330,17 +334,24 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
   }
 }
 
-// mu[r][m0 + i] = sum_t partial[i][t][r] in tile order, i < gridDim.x
+// mu[(k*R + r)*M + m0 + i] = sum over the tiles t of segment k of
+// partial[i][t][r], in tile order, i < gridDim.x, k = blockIdx.z (one
+// segment of all tiles without a segment table)
 __global__ __launch_bounds__(kBlock) void k_block_reduce(const double2* __restrict__ partial,
                                                          int tiles, int R, int M, int m0,
+                                                         const int* __restrict__ seg_tiles,
                                                          double2* __restrict__ mu)
 {
   __shared__ double s_re[kBlock], s_im[kBlock];
   const int m = m0 + blockIdx.x;
+  const int k = blockIdx.z;
+  const int t0 = seg_tiles ? seg_tiles[k] : 0;
+  const int t1 = seg_tiles ? seg_tiles[k + 1] : tiles;
   const double2* p = partial + (size_t)blockIdx.x * tiles * R;
+  mu += (size_t)k * R * M;
   for (int r = blockIdx.y; r < R; r += gridDim.y) {
     double re = 0.0, im = 0.0;
-    for (int t = threadIdx.x; t < tiles; t += kBlock) {
+    for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
       double2 v = p[(size_t)t * R + r];
       re += v.x;
       im += v.y;
@@ -393,6 +404,241 @@ __global__ __launch_bounds__(kBlock) void k_block_dot(const float2* __restrict__
       b += s_im[k];
     }
     partial[(size_t)tile * R + r] = make_double2(a, b);
+  }
+}
+
+// ==========================================================================
+// time evolution of blocks: FP32 states and recursion, FP64 accumulated state
+// ==========================================================================
+enum BlockEvo {
+  kApplyH = 0,   // out = H~ in
+  kCheb2 = 1,    // s2 = 2 H~ s1 - s0; S += bm * phase(s2)
+  kComm = 2,     // out = [X, H~] in = -sum t xx in_j / E_max
+  kCurrent = 3,  // out = V in = -i sum t xx in_j
+  kCheb2x = 4,   // s2 = 2 H~ s1 - s0; s2x = 2 (H~ s1x + [X, H~] s1) - s0x; S += bm * phase(s2x)
+};
+
+// the four (-i)^m phase cases of src/hamiltonian.cu:504-525 in FP64
+__device__ __forceinline__ void accumulate_phase(double2& s, double bm, double tr, double ti,
+                                                 int label)
+{
+  switch (label) {
+    case 1:
+      s.x += bm * tr;
+      s.y += bm * ti;
+      break;
+    case 2:
+      s.x -= bm * tr;
+      s.y -= bm * ti;
+      break;
+    case 3:
+      s.x += bm * ti;
+      s.y -= bm * tr;
+      break;
+    case 4:
+      s.x -= bm * ti;
+      s.y += bm * tr;
+      break;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void accumulate_block(double2* S, const Vec<W>& o, double bm, int label)
+{
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    double2 v = S[w];
+    accumulate_phase(v, bm, (double)o.v[w].x, (double)o.v[w].y, label);
+    S[w] = v;
+  }
+}
+
+template <int LAYOUT, int KT, int W, int OP, bool NT>
+__global__ __launch_bounds__(kBlock) void k_block_evo(
+  HamDev h, int R, TileMap tm, const float2* s0, const float2* __restrict__ s1, float2* s2,
+  const float2* s0x, const float2* __restrict__ s1x, float2* s2x, double2* __restrict__ S,
+  double bm, int label, int tiles, int blocks)
+{
+  constexpr bool kXX = OP == kComm || OP == kCurrent || OP == kCheb2x;
+  constexpr bool kOnsite = OP == kApplyH || OP == kCheb2 || OP == kCheb2x;
+  __shared__ float2 s_tab[LAYOUT == kPackedLds ? kBlockLdsTableMax : 1];
+  __shared__ float s_xx[LAYOUT == kPackedLds && kXX ? kBlockLdsTableMax : 1];
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  if constexpr (LAYOUT == kPackedLds) {
+    for (int i = threadIdx.x; i < h.ncls; i += kBlock) {
+      const double2 d = h.tab[i];
+      s_tab[i] = make_float2((float)d.x, (float)d.y);
+      if constexpr (kXX)
+        s_xx[i] = (float)h.tab_xx[i];
+    }
+    __syncthreads();
+  }
+  constexpr int C = SlotBatch<KT>::C;
+  const int K = KT > 0 ? KT : h.K;
+  const int vpw = R / W;
+  const int per_pass = kBlock / vpw;
+  const int lane_site = threadIdx.x / vpw;
+  const int r0 = (threadIdx.x % vpw) * W;
+  int begin, end;
+  tile_range(tm, tile, begin, end);
+  const float inv_scale = (float)(1.0 / h.emax);
+  for (int s = begin + lane_site; s < end; s += per_pass) {
+    const size_t base = (size_t)s * R + r0;
+    float tr[W], ti[W], xr[W], xi[W];
+    if constexpr (kOnsite) {
+      const float v = ld<NT>(h.pot32 + s);
+      Vec<W> p1 = ldv<W, false>(s1 + base);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        tr[w] = v * p1.v[w].x;
+        ti[w] = v * p1.v[w].y;
+      }
+      if constexpr (OP == kCheb2x) {
+        Vec<W> p1x = ldv<W, false>(s1x + base);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          xr[w] = v * p1x.v[w].x;
+          xi[w] = v * p1x.v[w].y;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        tr[w] = ti[w] = 0.0f;
+    }
+    for (int m0 = 0; m0 < K; m0 += C) {
+      uint32_t wd[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q)
+        wd[q] = (m0 + q < K) ? ld<NT>(h.words + (size_t)(m0 + q) * h.stride + s) : kEmptySlot;
+      int j[C];
+      float a[C], b[C], x[C];
+      decode_words32<LAYOUT, C>(h, s_tab, s, wd, j, a, b);
+      if constexpr (kXX) {
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+          const int c = wd[q] != kEmptySlot ? (int)(wd[q] >> h.idx_bits) : 0;
+          x[q] = (LAYOUT == kPackedLds) ? s_xx[c] : (float)h.tab_xx[c];
+        }
+      }
+      Vec<W> c[C], cx[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        c[q] = ldv<W, false>(s1 + (size_t)j[q] * R + r0);
+        if constexpr (OP == kCheb2x)
+          cx[q] = ldv<W, false>(s1x + (size_t)j[q] * R + r0);
+      }
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const float cr = c[q].v[w].x, ci = c[q].v[w].y;
+          // t * in_j (a zero hopping for an absent slot adds nothing)
+          const float hr = __builtin_fmaf(a[q], cr, -b[q] * ci);
+          const float hi = __builtin_fmaf(a[q], ci, b[q] * cr);
+          if constexpr (OP == kApplyH || OP == kCheb2) {
+            tr[w] += hr;
+            ti[w] += hi;
+          } else if constexpr (OP == kComm) {
+            tr[w] = __builtin_fmaf(-hr, x[q], tr[w]);
+            ti[w] = __builtin_fmaf(-hi, x[q], ti[w]);
+          } else if constexpr (OP == kCurrent) {
+            tr[w] = __builtin_fmaf(hr, x[q], tr[w]);
+            ti[w] = __builtin_fmaf(hi, x[q], ti[w]);
+          } else {  // kCheb2x
+            tr[w] += hr;
+            ti[w] += hi;
+            const float dr = cx[q].v[w].x, di = cx[q].v[w].y;
+            xr[w] += __builtin_fmaf(a[q], dr, -b[q] * di);
+            xi[w] += __builtin_fmaf(a[q], di, b[q] * dr);
+            xr[w] = __builtin_fmaf(-hr, x[q], xr[w]);
+            xi[w] = __builtin_fmaf(-hi, x[q], xi[w]);
+          }
+        }
+      }
+    }
+    Vec<W> o;
+    if constexpr (OP == kApplyH || OP == kComm) {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        o.v[w] = make_float2(tr[w] * inv_scale, ti[w] * inv_scale);
+      stv<W>(s2 + base, o);
+    } else if constexpr (OP == kCurrent) {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        o.v[w] = make_float2(ti[w], -tr[w]);
+      stv<W>(s2 + base, o);
+    } else {
+      Vec<W> p0 = ldv<W, NT>(s0 + base);
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        o.v[w] = make_float2(__builtin_fmaf(2.0f, tr[w] * inv_scale, -p0.v[w].x),
+                             __builtin_fmaf(2.0f, ti[w] * inv_scale, -p0.v[w].y));
+      stv<W>(s2 + base, o);
+      if constexpr (OP == kCheb2) {
+        accumulate_block<W>(S + base, o, bm, label);
+      } else {
+        Vec<W> p0x = ldv<W, NT>(s0x + base), ox;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          ox.v[w] = make_float2(__builtin_fmaf(2.0f, xr[w] * inv_scale, -p0x.v[w].x),
+                                __builtin_fmaf(2.0f, xi[w] * inv_scale, -p0x.v[w].y));
+        stv<W>(s2x + base, ox);
+        accumulate_block<W>(S + base, ox, bm, label);
+      }
+    }
+  }
+}
+
+// partial[t][r] = (sum over the tile of |S_r|^2, 0), FP64 state
+__global__ __launch_bounds__(kBlock) void k_block_norm64(const double2* __restrict__ S, int R,
+                                                         TileMap tm, double2* __restrict__ partial,
+                                                         int tiles, int blocks)
+{
+  __shared__ double s_re[kBlock];
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  int b0, e0;
+  tile_range(tm, tile, b0, e0);
+  double acc = 0.0;
+  for (size_t i = (size_t)b0 * R + threadIdx.x; i < (size_t)e0 * R; i += kBlock) {
+    const double2 v = S[i];
+    acc += v.x * v.x + v.y * v.y;
+  }
+  s_re[threadIdx.x] = acc;
+  __syncthreads();
+  for (int r = threadIdx.x; r < R; r += kBlock) {
+    double a = 0.0;
+    for (int k = r; k < kBlock; k += R)
+      a += s_re[k];
+    partial[(size_t)tile * R + r] = make_double2(a, 0.0);
+  }
+}
+
+// S_r *= sqrt(before_r / after_r) per segment and vector (U is unitary: the
+// norm before an evolution step is the exact norm after it)
+__global__ __launch_bounds__(kBlock) void k_block_renorm(double2* __restrict__ S, int R, TileMap tm,
+                                                         const double2* __restrict__ before,
+                                                         const double2* __restrict__ after,
+                                                         int tiles, int blocks)
+{
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  int b0, e0;
+  tile_range(tm, tile, b0, e0);
+  const int seg = tm.tile_seg ? tm.tile_seg[tile] : 0;
+  for (size_t i = (size_t)b0 * R + threadIdx.x; i < (size_t)e0 * R; i += kBlock) {
+    const int r = (int)(i % (size_t)R);
+    const double a = after[(size_t)seg * R + r].x, b = before[(size_t)seg * R + r].x;
+    const double f = a > 0.0 ? sqrt(b / a) : 1.0;
+    double2 v = S[i];
+    v.x *= f;
+    v.y *= f;
+    S[i] = v;
   }
 }
 
@@ -511,13 +757,67 @@ void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& 
   k_block_dot<<<g.blocks, kBlock, 0, st>>>(psi, L, R, tm, partial, g.tiles, g.blocks);
 }
 
-void launch_block_reduce(const double2* partial, int tiles, int R, int M, int m0, int count,
-                         double2* mu, hipStream_t st)
+void launch_block_reduce(const double2* partial, const TileMap& tm, int R, int M, int m0,
+                         int count, double2* mu, hipStream_t st)
 {
   if (count <= 0)
     return;
-  dim3 grid(count, R < 8 ? R : 8);
-  k_block_reduce<<<grid, kBlock, 0, st>>>(partial, tiles, R, M, m0, mu);
+  const int segs = tm.seg_tiles ? tm.segs : 1;
+  dim3 grid(count, R < 8 ? R : 8, segs);
+  k_block_reduce<<<grid, kBlock, 0, st>>>(partial, tm.tiles, R, M, m0, tm.seg_tiles, mu);
+}
+
+void launch_block_evo(const HamDev& h, int R, int W, const TileMap& tm, int op,
+                      const float2* s0, const float2* s1, float2* s2, const float2* s0x,
+                      const float2* s1x, float2* s2x, double2* S, double bm, int label,
+                      hipStream_t st)
+{
+  Grid g = block_grid(tm);
+  const bool nt = (size_t)h.n * R >= (size_t)kStreamingMinSites;
+#define LSQTB_EVO(OPV, NTV)                                                              \
+  k_block_evo<LAY, KT, WW, OPV, NTV><<<g.blocks, kBlock, 0, st>>>(                       \
+    h, R, tm, s0, s1, s2, s0x, s1x, s2x, S, bm, label, g.tiles, g.blocks)
+#define LSQTB_EVO_OP(NTV)                  \
+  switch (op) {                            \
+    case kApplyH:                          \
+      LSQTB_EVO(kApplyH, NTV);             \
+      break;                               \
+    case kCheb2:                           \
+      LSQTB_EVO(kCheb2, NTV);              \
+      break;                               \
+    case kComm:                            \
+      LSQTB_EVO(kComm, NTV);               \
+      break;                               \
+    case kCurrent:                         \
+      LSQTB_EVO(kCurrent, NTV);            \
+      break;                               \
+    default:                               \
+      LSQTB_EVO(kCheb2x, NTV);             \
+      break;                               \
+  }
+  LSQTB_LAYOUT_SWITCH(h, LAY, LSQTB_K_SWITCH(h.K, KT, LSQTB_W_SWITCH(W, WW, {
+    if (nt) {
+      LSQTB_EVO_OP(true)
+    } else {
+      LSQTB_EVO_OP(false)
+    }
+  })))
+#undef LSQTB_EVO_OP
+#undef LSQTB_EVO
+}
+
+void launch_block_norm64(const double2* S, int R, const TileMap& tm, double2* partial,
+                         hipStream_t st)
+{
+  Grid g = block_grid(tm);
+  k_block_norm64<<<g.blocks, kBlock, 0, st>>>(S, R, tm, partial, g.tiles, g.blocks);
+}
+
+void launch_block_renorm(double2* S, int R, const TileMap& tm, const double2* before,
+                         const double2* after, hipStream_t st)
+{
+  Grid g = block_grid(tm);
+  k_block_renorm<<<g.blocks, kBlock, 0, st>>>(S, R, tm, before, after, g.tiles, g.blocks);
 }
 
 }  // namespace lsqt

@@ -418,6 +418,16 @@ __global__ __launch_bounds__(kBlock) void k_to_f32(const double2* __restrict__ s
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_to_f64(const float2* __restrict__ src,
+                                                   double2* __restrict__ dst, int n)
+{
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    float2 v = src[i];
+    dst[i] = make_double2((double)v.x, (double)v.y);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_zero32(float2* v, int n)
 {
   int i = blockIdx.x * kBlock + threadIdx.x;
@@ -636,6 +646,12 @@ void launch_to_f32(const double2* src, float2* dst, int n, hipStream_t st)
 {
   if (n > 0)
     k_to_f32<<<blocks_1d32(n), kBlock, 0, st>>>(src, dst, n);
+}
+
+void launch_to_f64(const float2* src, double2* dst, int n, hipStream_t st)
+{
+  if (n > 0)
+    k_to_f64<<<blocks_1d32(n), kBlock, 0, st>>>(src, dst, n);
 }
 
 void launch_zero32(float2* v, int n, hipStream_t st)

@@ -41,6 +41,9 @@ struct lsqt_ham {
   // most one slab away, and slabs in rows of `row` sites (slab % row == 0)
   int slab = 0;
   int row = 0;
+  // segmented Hamiltonian: independent systems on sites [seg_starts[k],
+  // seg_starts[k+1]); empty for one system
+  std::vector<int> seg_starts;
 };
 
 struct lsqt_vec {

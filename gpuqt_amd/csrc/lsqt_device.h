@@ -163,7 +163,10 @@ struct Grid {
 // Block-path tile schedule: natural (tile t = sites [t*rows, (t+1)*rows))
 // or blocked (slab > 0: tile t = the span of sites at offset (t / slabs)*span
 // in slab t % slabs); walk != 0 sweeps a blocked tile made of whole rows of
-// `row` sites chunk by chunk across its rows (the column walk).
+// `row` sites chunk by chunk across its rows (the column walk).  With a tile
+// table (segmented Hamiltonians: independent systems side by side) tile t is
+// sites [table[t].x, table[t].y) of segment tile_seg[t], and the tiles of
+// segment k are seg_tiles[k] .. seg_tiles[k+1]-1.
 struct TileMap {
   int n;
   int rows;
@@ -173,6 +176,10 @@ struct TileMap {
   int tiles;
   int row;
   int walk;
+  const int2* table;
+  const int* tile_seg;
+  const int* seg_tiles;
+  int segs;
 };
 
 inline Grid make_grid(int n, int spt)

@@ -39,6 +39,7 @@ void launch_cheb2x_32(const HamDev& h, const float2* s0, const float2* s0x, cons
                       int label, hipStream_t st);
 void launch_zero32(float2* v, int n, hipStream_t st);
 void launch_to_f32(const double2* src, float2* dst, int n, hipStream_t st);
+void launch_to_f64(const float2* src, double2* dst, int n, hipStream_t st);
 void launch_cheb01_32(const float2* s1, double2* s, double b0, double b1, int direction, int n,
                       hipStream_t st);
 void launch_cheb1x_32(const float2* s1x, double2* s, double b1, int n, hipStream_t st);
@@ -57,7 +58,21 @@ void launch_block_step(const HamDev& h, int R, int W, const TileMap& tm, const f
                        double2* partial2, bool recur, bool self, hipStream_t st);
 void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& tm,
                       double2* partial, hipStream_t st);
-void launch_block_reduce(const double2* partial, int tiles, int R, int M, int m0, int count,
-                         double2* mu, hipStream_t st);
+// block time evolution (op: 0 H~, 1 Chebyshev term of U, 2 [X, H~], 3 V,
+// 4 Chebyshev term of [X, U]); S is the FP64 accumulated state [n][R]
+void launch_block_evo(const HamDev& h, int R, int W, const TileMap& tm, int op,
+                      const float2* s0, const float2* s1, float2* s2, const float2* s0x,
+                      const float2* s1x, float2* s2x, double2* S, double bm, int label,
+                      hipStream_t st);
+// partial[t][r] = (|S_r|^2 over tile t, 0)
+void launch_block_norm64(const double2* S, int R, const TileMap& tm, double2* partial,
+                         hipStream_t st);
+// S_r *= sqrt(before[k][r] / after[k][r]) in segment k
+void launch_block_renorm(double2* S, int R, const TileMap& tm, const double2* before,
+                         const double2* after, hipStream_t st);
+// mu[(k*R + r)*M + m0 + i] for segment k, i < count (partials of moment
+// m0 + i at slot i)
+void launch_block_reduce(const double2* partial, const TileMap& tm, int R, int M, int m0,
+                         int count, double2* mu, hipStream_t st);
 
 }  // namespace lsqt

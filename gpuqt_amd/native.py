@@ -40,6 +40,10 @@ SIGNATURES = {
     "lsqt_ham_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp,
                                   C.c_double, C.c_int, C.POINTER(_vp)]),
     "lsqt_ham_destroy": (C.c_int, [_vp]),
+    "lsqt_ham_create_segmented": (C.c_int, [_vp, C.c_int, _ip, C.c_int, C.c_int, _ip, _ip, _dp,
+                                            _dp, _dp, _dp, C.c_double, C.POINTER(_vp)]),
+    "lsqt_block_vac": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "lsqt_block_msd": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]),
     "lsqt_ham_get_info": (C.c_int, [_vp, C.POINTER(HamInfo)]),
     "lsqt_vec_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "lsqt_vec_clone": (C.c_int, [_vp, C.POINTER(_vp)]),
@@ -243,7 +247,8 @@ class Hamiltonian:
     """Hamiltonian (src/hamiltonian.h:25-54) on the device, from row-major ELL arrays."""
 
     def __init__(self, ctx: Context, n, max_neighbor, neighbor_number, neighbor_list, potential,
-                 hopping_real, hopping_imag, xx, energy_max, layout=LAYOUT_AUTO):
+                 hopping_real, hopping_imag, xx, energy_max, layout=LAYOUT_AUTO,
+                 segment_starts=None):
         self.ctx = ctx
         self.n = int(n)
         nn = np.ascontiguousarray(neighbor_number, dtype=np.int32)
@@ -252,15 +257,25 @@ class Hamiltonian:
                 for a in (potential, hopping_real, hopping_imag, xx)]
         assert nn.shape == (n,) and nl.shape == (n * max_neighbor,)
         self.h = _vp()
-        _check(load().lsqt_ham_create(ctx.h, int(n), int(max_neighbor), _iptr(nn), _iptr(nl),
-                                      *(_dptr(a) for a in arrs), float(energy_max), int(layout),
-                                      C.byref(self.h)))
+        self.segments = 1
+        if segment_starts is None:
+            _check(load().lsqt_ham_create(ctx.h, int(n), int(max_neighbor), _iptr(nn),
+                                          _iptr(nl), *(_dptr(a) for a in arrs),
+                                          float(energy_max), int(layout), C.byref(self.h)))
+        else:
+            st = np.ascontiguousarray(segment_starts, dtype=np.int32)
+            self.segments = len(st) - 1
+            _check(load().lsqt_ham_create_segmented(ctx.h, self.segments, _iptr(st), int(n),
+                                                    int(max_neighbor), _iptr(nn), _iptr(nl),
+                                                    *(_dptr(a) for a in arrs),
+                                                    float(energy_max), C.byref(self.h)))
 
     @classmethod
-    def from_arrays(cls, ctx, arrays, n, max_neighbor, energy_max, layout=LAYOUT_AUTO):
+    def from_arrays(cls, ctx, arrays, n, max_neighbor, energy_max, layout=LAYOUT_AUTO,
+                    segment_starts=None):
         return cls(ctx, n, max_neighbor, arrays["neighbor_number"], arrays["neighbor_list"],
                    arrays["potential"], arrays["hopping_real"], arrays["hopping_imag"],
-                   arrays["xx"], energy_max, layout)
+                   arrays["xx"], energy_max, layout, segment_starts)
 
     def info(self) -> HamInfo:
         inf = HamInfo()
@@ -322,19 +337,44 @@ class Hamiltonian:
         _check(load().lsqt_evolvex_f32(self.h, int(direction), float(t_scaled), state.h))
 
     def moments_block(self, left, right, M):
-        """complex array (R, M) of <left_r|T_m|right_r>"""
+        """complex array (R, M) of <left_r|T_m|right_r> ((segments, R, M) on a
+        segmented Hamiltonian)"""
         R = left.R
-        re, im = np.empty(R * M), np.empty(R * M)
+        S = self.segments
+        re, im = np.empty(S * R * M), np.empty(S * R * M)
         _check(load().lsqt_moments_block(self.h, left.h, right.h, int(M), _dptr(re), _dptr(im)))
-        return (re + 1j * im).reshape(R, M)
+        mu = (re + 1j * im).reshape(S, R, M)
+        return mu[0] if S == 1 else mu
 
     def moments_block_self(self, v, M):
         """complex array (R, M) of <v_r|T_m|v_r> by the product identities
         (M/2 steps)"""
         R = v.R
-        re, im = np.empty(R * M), np.empty(R * M)
+        S = self.segments
+        re, im = np.empty(S * R * M), np.empty(S * R * M)
         _check(load().lsqt_moments_block_self(self.h, v.h, int(M), _dptr(re), _dptr(im)))
-        return (re + 1j * im).reshape(R, M)
+        mu = (re + 1j * im).reshape(S, R, M)
+        return mu[0] if S == 1 else mu
+
+    def _block_driver(self, fn, re, im, M, time_steps_scaled):
+        re = np.ascontiguousarray(re, dtype=np.float64)
+        im = np.ascontiguousarray(im, dtype=np.float64)
+        R = re.shape[0]
+        assert re.shape == (R, self.n) and im.shape == (R, self.n)
+        ts = np.ascontiguousarray(time_steps_scaled, dtype=np.float64)
+        Nt = len(ts)
+        size = Nt * self.segments * R * M
+        mr, mi = np.empty(size), np.empty(size)
+        _check(fn(self.h, R, _dptr(re), _dptr(im), int(M), Nt, _dptr(ts), _dptr(mr), _dptr(mi)))
+        return (mr + 1j * mi).reshape(Nt, self.segments, R, M)
+
+    def block_vac(self, re, im, M, time_steps_scaled):
+        """complex (Nt, segments, R, M): find_vac's moment sets for R vectors"""
+        return self._block_driver(load().lsqt_block_vac, re, im, M, time_steps_scaled)
+
+    def block_msd(self, re, im, M, time_steps_scaled):
+        """complex (Nt, segments, R, M): find_msd's moment sets for R vectors"""
+        return self._block_driver(load().lsqt_block_msd, re, im, M, time_steps_scaled)
 
     def block_tune(self, left, right, force=False):
         """(W, rows, blocked, us per step) of the autotuned block kernel shape"""

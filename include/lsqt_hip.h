@@ -100,6 +100,18 @@ int lsqt_ham_create(lsqt_ctx* ctx, int n, int max_neighbor, const int* neighbor_
                     double energy_max, int layout_hint, lsqt_ham** out);
 /* Hamiltonian::~Hamiltonian (src/hamiltonian.cu:133-150) */
 int lsqt_ham_destroy(lsqt_ham* h);
+/* (no counterpart in the reference, which runs one system per input
+ * directory)  A block-diagonal Hamiltonian of `segments` independent systems
+ * side by side: system k on sites [starts[k], starts[k+1]), starts[0] = 0,
+ * starts[segments] = n, every neighbour inside its own system (checked), one
+ * energy_max for all.  Packed layout.  The block entry points then return
+ * moments per system: S = segments times as many, segment-major.  Used for
+ * many disorder realisations of uneven size in one launch per step. */
+int lsqt_ham_create_segmented(lsqt_ctx* ctx, int segments, const int* starts, int n,
+                              int max_neighbor, const int* neighbor_number,
+                              const int* neighbor_list, const double* potential,
+                              const double* hopping_real, const double* hopping_imag,
+                              const double* xx, double energy_max, lsqt_ham** out);
 int lsqt_ham_get_info(const lsqt_ham* h, lsqt_ham_info* info);
 
 /* ---- Vector ---------------------------------------------------------------
@@ -191,7 +203,8 @@ int lsqt_block_create(lsqt_ctx* ctx, int n, int R, lsqt_block** out);  /* zero-f
 int lsqt_block_destroy(lsqt_block* b);
 int lsqt_block_from_host(lsqt_block* b, const double* re, const double* im);
 int lsqt_block_to_host(const lsqt_block* b, double* re, double* im);
-/* mu[r*M + m] = <left_r|T_m(H~)|right_r>, r < R, m < M (M >= 2) */
+/* mu[r*M + m] = <left_r|T_m(H~)|right_r>, r < R, m < M (M >= 2); on a
+ * segmented Hamiltonian mu[(k*R + r)*M + m] for segment k */
 int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right, int M,
                        double* mu_re, double* mu_im);
 /* mu[r*M + m] = <v_r|T_m(H~)|v_r> (the DOS moments) in M/2 steps, from the
@@ -201,6 +214,19 @@ int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsqt_block* ri
  * FP32 rounding of the states. */
 int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, double* mu_re,
                             double* mu_im);
+/* Time-correlation drivers for R vectors at once (FP32 recursion states,
+ * FP64 accumulated and re-normalised evolved states, FP64 moments):
+ * re/im hold the random vectors vector-major ([R][n], FP64, as
+ * Model::initialize_state makes them), time_step_scaled[t] = dt_t *
+ * energy_max, and mu[((t*S + k)*R + r)*M + m] receives moment m of time step
+ * t, segment k (S = 1 unless the Hamiltonian is segmented), vector r.
+ *   lsqt_block_vac: find_vac (src/sigma.cu:278-329): <U(-t) V phi|T_m|V U(-t) phi>
+ *   lsqt_block_msd: find_msd (src/sigma.cu:333-393): <[X,U(t)] phi|T_m|[X,U(t)] phi>,
+ *                   by the product identities (M/2 recursion steps) */
+int lsqt_block_vac(lsqt_ham* h, int R, const double* re, const double* im, int M, int n_steps,
+                   const double* time_step_scaled, double* mu_re, double* mu_im);
+int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double* im, int M, int n_steps,
+                   const double* time_step_scaled, double* mu_re, double* mu_im);
 /* The autotuner: times every kernel shape (W vectors per lane, i.e. R/W lanes
  * per site; rows = sites per workgroup tile; blocked = tile schedule) on these
  * blocks and keeps the fastest for (R, max_neighbor, table placement, size

@@ -1,0 +1,177 @@
+"""Time-correlation drivers of the block path (VAC and MSD for R vectors at
+once, FP32 recursion states, FP64 evolved states and moments) against the
+oracle's find_vac / find_msd per vector, and segmented Hamiltonians (several
+independent systems side by side in one launch per step) against the same
+systems run one by one (GPU).
+
+Stated tolerances: every VAC / MSD row (Jackson-damped Chebyshev sum) within
+2e-5 of the FP64 oracle's row, normwise over the row; segmented and
+one-by-one runs of the same systems agree to 1e-9 of the moment scale (the
+same per-site arithmetic; only the grouping of the sums over tiles differs)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import CASES_DIR
+
+pytestmark = pytest.mark.gpu
+
+ROW_TOL = 2e-5
+
+
+def _model(case):
+    import oracle
+    return oracle.OracleModel(os.path.join(CASES_DIR, case))
+
+
+def _ham(ctx, m, **kw):
+    from gpuqt_amd import native
+    return native.Hamiltonian.from_arrays(ctx, m.arrays(), m.n, m.max_neighbor, m.energy_max,
+                                          **kw)
+
+
+def _rows(m, mu):
+    from gpuqt_amd import hostlib
+    return np.array([hostlib.correlation_row(x, m.energy, m.energy_max, m.volume) for x in mu])
+
+
+def _vectors(m, R):
+    vecs = [m.random_state() for _ in range(R)]
+    return np.array([v[0] for v in vecs]), np.array([v[1] for v in vecs]), vecs
+
+
+def _normwise(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("case,R", [("honeycomb_small", 4), ("cube_peierls", 8),
+                                    ("honeycomb_transport_y", 2), ("square_vacancy_ldos", 4),
+                                    ("config1_honeycomb_20k", 4)])
+def test_block_vac_against_oracle(gpu_ctx, case, R):
+    print(f"[gpu] block VAC {case} R={R}", flush=True)
+    m = _model(case)
+    H = _ham(gpu_ctx, m)
+    re, im, vecs = _vectors(m, R)
+    ts = m.time_step * m.energy_max
+    mu = H.block_vac(re, im, m.number_of_moments, ts)
+    assert mu.shape == (len(ts), 1, R, m.number_of_moments)
+    worst = 0.0
+    for r in range(R):
+        want_rows, _ = m.find_vac(vecs[r])
+        got_rows = _rows(m, mu[:, 0, r])
+        for t in range(len(ts)):
+            worst = max(worst, _normwise(got_rows[t], want_rows[t]))
+    print(f"    worst VAC row error {worst:.2e}", flush=True)
+    assert worst <= ROW_TOL
+
+
+@pytest.mark.parametrize("case,R", [("honeycomb_small", 4), ("cube_peierls", 8),
+                                    ("strip_open", 2), ("charged_only", 4),
+                                    ("config1_honeycomb_20k", 4)])
+def test_block_msd_against_oracle(gpu_ctx, case, R):
+    print(f"[gpu] block MSD {case} R={R}", flush=True)
+    m = _model(case)
+    H = _ham(gpu_ctx, m)
+    re, im, vecs = _vectors(m, R)
+    ts = m.time_step * m.energy_max
+    mu = H.block_msd(re, im, m.number_of_moments, ts)
+    worst = 0.0
+    for r in range(R):
+        want_rows, _ = m.find_msd(vecs[r])
+        got_rows = _rows(m, mu[:, 0, r])
+        for t in range(len(ts)):
+            worst = max(worst, _normwise(got_rows[t], want_rows[t]))
+    print(f"    worst MSD row error {worst:.2e}", flush=True)
+    assert worst <= ROW_TOL
+
+
+def _concat(models):
+    """row-major ELL arrays of independent systems side by side"""
+    K = max(m.max_neighbor for m in models)
+    starts = [0]
+    parts = {k: [] for k in ("neighbor_number", "neighbor_list", "potential", "hopping_real",
+                             "hopping_imag", "xx")}
+    for m in models:
+        a = m.arrays()
+        n, k = m.n, m.max_neighbor
+        nn = a["neighbor_number"]
+        pad = lambda x, fill: np.hstack([x.reshape(n, k), np.full((n, K - k), fill, x.dtype)])
+        nl = pad(a["neighbor_list"], 0)
+        valid = np.arange(K)[None, :] < nn[:, None]
+        nl = np.where(valid, nl + starts[-1], 0)
+        parts["neighbor_number"].append(nn)
+        parts["neighbor_list"].append(nl.reshape(-1))
+        parts["potential"].append(a["potential"])
+        for key in ("hopping_real", "hopping_imag", "xx"):
+            parts[key].append(pad(a[key], 0.0).reshape(-1))
+        starts.append(starts[-1] + n)
+    arrays = {k: np.concatenate(v) for k, v in parts.items()}
+    return arrays, starts, K
+
+
+def test_segmented_matches_one_by_one(gpu_ctx):
+    """Three honeycomb systems of different sizes (one energy_max): block
+    moments, product-identity moments, VAC and MSD of the segmented
+    Hamiltonian equal those of each system alone."""
+    import oracle
+    from gpuqt_amd import inputs as gi
+    from gpuqt_amd import native
+    print("[gpu] segmented block drivers", flush=True)
+    import tempfile
+    models = []
+    with tempfile.TemporaryDirectory() as tmp:
+        for i, (nx, ny) in enumerate(((6, 4), (11, 3), (4, 9))):
+            d = os.path.join(tmp, f"s{i}")
+            os.makedirs(d)
+            gi.write_honeycomb(d, nx, ny)
+            gi.write_para(d, anderson=1.0 + i, moments=48, energy_max=10.0, msd=True, vac=True)
+            gi.write_energy(d, gi.energy_grid(-8, 8, 21))
+            gi.write_time_steps(d, [1.0, 0.5])
+            models.append(oracle.OracleModel(d))
+    arrays, starts, K = _concat(models)
+    n = starts[-1]
+    Hs = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, K, 10.0, segment_starts=starts)
+    assert Hs.segments == 3
+    R, M = 4, 48
+    rng = np.random.default_rng(9)
+    ph = rng.uniform(0, 2 * np.pi, (R, n))
+    re, im = np.cos(ph), np.sin(ph)
+    ts = np.array([1.0, 0.5]) * 10.0
+    W, rows = 1, 256
+    Hs.block_set_config(R, W, rows)
+    b = native.Block(gpu_ctx, n, R)
+    b.upload(re, im)
+    seg = {"moments": Hs.moments_block(b, b, M), "self": Hs.moments_block_self(b, M),
+           "vac": Hs.block_vac(re, im, M, ts), "msd": Hs.block_msd(re, im, M, ts)}
+    for k, m in enumerate(models):
+        lo, hi = starts[k], starts[k + 1]
+        H = _ham(gpu_ctx, m)
+        H.block_set_config(R, W, rows)
+        bk = native.Block(gpu_ctx, m.n, R)
+        bk.upload(re[:, lo:hi], im[:, lo:hi])
+        one = {"moments": H.moments_block(bk, bk, M), "self": H.moments_block_self(bk, M),
+               "vac": H.block_vac(re[:, lo:hi], im[:, lo:hi], M, ts)[:, 0],
+               "msd": H.block_msd(re[:, lo:hi], im[:, lo:hi], M, ts)[:, 0]}
+        for key in one:
+            got = seg[key][k] if key in ("moments", "self") else seg[key][:, k]
+            want = one[key]
+            scale = np.abs(want).max()
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-9 * scale,
+                                       err_msg=f"system {k} {key}")
+        for o in (bk, H):
+            o.close()
+    for o in (b, Hs):
+        o.close()
+
+
+def test_segmented_errors(gpu_ctx):
+    from gpuqt_amd import native
+    m = _model("honeycomb_small")
+    a = m.arrays()
+    with pytest.raises(native.LsqtError, match="outside"):
+        native.Hamiltonian.from_arrays(gpu_ctx, a, m.n, m.max_neighbor, m.energy_max,
+                                       segment_starts=[0, m.n // 2, m.n])
+    with pytest.raises(native.LsqtError):
+        native.Hamiltonian.from_arrays(gpu_ctx, a, m.n, m.max_neighbor, m.energy_max,
+                                       segment_starts=[0, m.n + 1])
