@@ -259,6 +259,167 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     return res
 
 
+def fp64_vac_msd(H, re, im, M, ts):
+    """find_vac and find_msd of one vector on the FP64 single-vector path
+    (src/sigma.cu:278-393 over the C ABI): the moment sets of every step."""
+    from gpuqt_amd import native
+    n = H.n
+    vac, msd = [], []
+    left, right, copy = H.vector(re, im), native.Vector(H.ctx, n), native.Vector(H.ctx, n)
+    H.apply_current(left, right)
+    for m in range(len(ts)):
+        H.apply_current(left, copy)
+        vac.append(H.moments(right, copy, M))
+        if m < len(ts) - 1:
+            H.evolve(-1, ts[m], left)
+            H.evolve(-1, ts[m], right)
+    state, sx = H.vector(re, im), H.vector(re, im)
+    H.evolve(1, ts[0], state)
+    H.evolvex(1, ts[0], sx)
+    for m in range(len(ts)):
+        msd.append(H.moments(sx, sx, M))
+        if m < len(ts) - 1:
+            copy.copy_from(state)
+            H.evolvex(1, ts[m + 1], copy)
+            H.evolve(1, ts[m + 1], sx)
+            sx.add(copy)
+            H.evolve(1, ts[m + 1], state)
+    for v in (left, right, copy, state, sx):
+        v.close()
+    return np.array(vac), np.array(msd)
+
+
+def batch_cells(count, lo=200, hi=1500, seed=5):
+    """Cell counts of the uneven batch: (Nc1, Nc2) uniform in [lo, hi]^2 from
+    a fixed seed; a four-orbital cell holds two two-site cells, so Nx = Nc1,
+    Ny = Nc2 / 2 (Nc2 rounded up to even)."""
+    c = np.random.default_rng(seed).integers(lo, hi + 1, size=(count, 2))
+    return [(int(a), int((b + 1) // 2)) for a, b in c]
+
+
+def bench_batch(ctx, realisations=8, R=8, M=1500, n_steps=16, dt=2.0, emax=12.0,
+                threads=16, check=True):
+    """Uneven batch: honeycomb realisations of uneven size (a_cc = 1.42 A,
+    charged impurities at 0.1 %, amplitudes U[-2.8, 2.8] eV, xi = 4.26 A),
+    R random vectors each, VAC + MSD over n_steps steps of dt, all
+    realisations side by side in one segmented Hamiltonian: one launch per
+    recursion step for the whole batch, moment sets replayed as hipGraphs."""
+    import concurrent.futures as cf
+    from gpuqt_amd import hostlib, native
+    from gpuqt_amd import inputs as gi
+    cells = batch_cells(64)[:realisations]
+    t0 = time.perf_counter()
+    with tempfile.TemporaryDirectory() as tmp:
+        dirs = []
+        for i, (nx, ny) in enumerate(cells):
+            d = os.path.join(tmp, f"r{i}")
+            os.makedirs(d)
+            n = 4 * nx * ny
+            gi.write_honeycomb(d, nx, ny, ax=2.4595, ay=4.26)
+            gi.write_para(d, charged=(int(round(0.001 * n)), 5.6, 4.26), moments=M,
+                          energy_max=emax, vac=True, msd=True, random_vectors=R)
+            gi.write_energy(d, gi.energy_grid(-8, 8, 201))
+            gi.write_time_steps(d, [dt] * n_steps)
+            dirs.append(d)
+
+        def build(d):
+            m = hostlib.HostModel(d)
+            vecs = [m.random_state() for _ in range(R)]
+            return m, m.arrays(), vecs
+
+        devnull = os.open(os.devnull, os.O_WRONLY)
+        saved = os.dup(1)
+        os.dup2(devnull, 1)
+        try:
+            with cf.ThreadPoolExecutor(max(1, min(threads, len(dirs)))) as pool:
+                built = list(pool.map(build, dirs))
+        finally:
+            os.dup2(saved, 1)
+            os.close(devnull)
+            os.close(saved)
+    sizes = [m.n for m, _, _ in built]
+    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(starts[-1])
+    K = max(m.max_neighbor for m, _, _ in built)
+    arrays = {"neighbor_number": np.concatenate([a["neighbor_number"] for _, a, _ in built]),
+              "potential": np.concatenate([a["potential"] for _, a, _ in built])}
+    nl, rest = [], {k: [] for k in ("hopping_real", "hopping_imag", "xx")}
+    for k, (m, a, _) in enumerate(built):
+        assert m.max_neighbor == K
+        valid = np.arange(K)[None, :] < a["neighbor_number"][:, None]
+        nl.append(np.where(valid, a["neighbor_list"].reshape(m.n, K) + starts[k], 0).reshape(-1))
+        for key in rest:
+            rest[key].append(a[key])
+    arrays["neighbor_list"] = np.concatenate(nl).astype(np.int32)
+    for key in rest:
+        arrays[key] = np.concatenate(rest[key])
+    re = np.concatenate([np.array([v[0] for v in vecs]) for _, _, vecs in built], axis=1)
+    im = np.concatenate([np.array([v[1] for v in vecs]) for _, _, vecs in built], axis=1)
+    build_s = time.perf_counter() - t0
+    log(f"batch: {realisations} realisations, {min(sizes)}..{max(sizes)} sites, {n} in all, "
+        f"built in {build_s:.1f} s")
+    H = native.Hamiltonian.from_arrays(ctx, arrays, n, K, emax,
+                                       segment_starts=starts.astype(np.int32))
+    del arrays
+    ts = np.full(n_steps, dt * emax)
+    # untimed: tuning and graph capture on a short run of the same shape
+    H.block_vac(re, im, M, ts[:1])
+    H.block_msd(re, im, M, ts[:1])
+    ctx.sync()
+    t = time.perf_counter()
+    vac = H.block_vac(re, im, M, ts)
+    t_vac = time.perf_counter() - t
+    t = time.perf_counter()
+    msd = H.block_msd(re, im, M, ts)
+    t_msd = time.perf_counter() - t
+    elapsed = t_vac + t_msd
+    K_t = terms(dt * emax)
+    # recursion site-updates per vector and site: VAC (M - 1) per step, MSD
+    # M / 2 per step (product identities); evolve 1 + K, evolvex 2 (1 + K)
+    per_site = (n_steps * (M - 1) + 2 * (n_steps - 1) * (1 + K_t)
+                + n_steps * (M // 2) + 3 * (1 + K_t) + (n_steps - 1) * 4 * (1 + K_t))
+    updates = float(n) * R * per_site
+    moment_updates = float(n) * R * 2 * n_steps * (M - 1)
+    log(f"batch VAC {t_vac:.1f} s + MSD {t_msd:.1f} s: {updates / elapsed:.3e} site-updates/s, "
+        f"{moment_updates / elapsed:.3e} moment site-updates/s")
+    res = {
+        "workload": f"{realisations} honeycomb realisations of the 64-realisation uneven batch "
+                    f"(cells U[200,1500]^2 from seed 5; {min(sizes)}..{max(sizes)} sites, {n} "
+                    f"in all), charged impurities 0.1 % U[-2.8,2.8] eV xi=4.26 A, {R} vectors "
+                    f"each, {M} moments, VAC + MSD over {n_steps} steps of dt={dt}, E_max={emax}",
+        "realisations": realisations, "sites": n, "R": R,
+        "value": updates / elapsed, "unit": "site-updates/s",
+        "moment_site_updates_per_s": moment_updates / elapsed,
+        "seconds": {"vac": t_vac, "msd": t_msd, "build": build_s},
+        "launches": "one per recursion step for all realisations (segmented Hamiltonian); "
+                    "moment sets replayed from captured hipGraphs",
+    }
+    if check:
+        # the smallest realisation, vector 0, against the FP64 single-vector path
+        k = int(np.argmin(sizes))
+        m, a, vecs = built[k]
+        H1 = native.Hamiltonian.from_arrays(ctx, a, m.n, K, emax)
+        v64, m64 = fp64_vac_msd(H1, vecs[0][0], vecs[0][1], M, ts)
+        energies, _, _ = m.inputs()
+        row = lambda mu: hostlib.correlation_row(mu, energies, emax, m.volume)
+        errs = {}
+        for name, got, want in (("vac", vac[:, k, 0], v64), ("msd", msd[:, k, 0], m64)):
+            g = np.array([row(x) for x in got])
+            w = np.array([row(x) for x in want])
+            errs[f"{name}_rows_normwise_max"] = float(
+                max(np.abs(a_ - b_).max() / np.abs(b_).max() for a_, b_ in zip(g, w)))
+            sg = hostlib.conductivity(ts / emax, **{name: g})[f"sigma_{name}"]
+            sw = hostlib.conductivity(ts / emax, **{name: w})[f"sigma_{name}"]
+            errs[f"sigma_{name}_normwise_max"] = float(
+                max(np.abs(a_ - b_).max() / np.abs(b_).max()
+                    for a_, b_ in zip(sg, sw) if np.abs(b_).max() > 0))
+        res["error_vs_f64_smallest_realisation"] = errs
+        log(f"batch errors vs FP64 (realisation {k}, {m.n} sites, vector 0): {errs}")
+        H1.close()
+    H.close()
+    return res
+
+
 def probe(ctx, H, rv, args, moments=10):
     """Short runs of the hot kernels for counter profiles: a plain copy of
     one vector (k_copy: n x 16 B read and written, the counters' scale), the
@@ -344,6 +505,11 @@ def main():
     ap.add_argument("--block-moments", type=int, default=2048)
     ap.add_argument("--cpu-moments", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-batch", dest="batch", action="store_false",
+                    help="skip the uneven-batch (VAC + MSD over many realisations) section")
+    ap.add_argument("--batch-realisations", type=int, default=8)
+    ap.add_argument("--batch-moments", type=int, default=1500)
+    ap.add_argument("--batch-steps", type=int, default=16)
     ap.add_argument("--probe", action="store_true",
                     help="run each hot kernel a few times on the workloads' inputs (for "
                          "hardware-counter profiles) and exit")
@@ -438,6 +604,12 @@ def main():
         for a, b in zip(rows32, rows):
             mask = np.abs(b) > 1e-3 * np.abs(b).max()
             elem.append(float((np.abs(a - b)[mask] / np.abs(b)[mask]).max()))
+        sigma_err = None
+        if len(rows) > 1:
+            s64 = hostlib.conductivity(time_steps, msd=rows[1:])["sigma_msd"]
+            s32 = hostlib.conductivity(time_steps, msd=np.array(rows32)[1:])["sigma_msd"]
+            sigma_err = float(max(np.abs(a - b).max() / np.abs(b).max()
+                                  for a, b in zip(s32, s64) if np.abs(b).max() > 0))
         f32 = {
             "value": u32 / e32,
             "unit": "site-updates/s",
@@ -452,6 +624,7 @@ def main():
                 "msd_normwise_max": max(errs[1:]) if len(errs) > 1 else None,
                 "all_rows_normwise": errs,
                 "elementwise_max_above_1e-3_of_row_max": max(elem),
+                "sigma_msd_normwise_max": sigma_err,
             },
         }
         log(f"f32: {u32 / e32:.3e} site-updates/s, DOS normwise error {errs[0]:.2e}, "
@@ -503,6 +676,10 @@ def main():
     if args.block:
         result["batching_block_f32"] = bench_block(ctx, L=args.cubic_L, R=args.block_R,
                                                    M=args.block_moments)
+    if args.batch:
+        result["batching_uneven"] = bench_batch(ctx, realisations=args.batch_realisations,
+                                                M=args.batch_moments,
+                                                n_steps=args.batch_steps)
     if not args.no_cpu_baseline:
         log("CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(args.nx, args.ny, args.cpu_moments,

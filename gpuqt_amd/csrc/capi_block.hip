@@ -71,10 +71,31 @@ struct SegTable {
   int* seg_tiles = nullptr;
 };
 
+// a captured moment set: every argument its kernels were recorded with
+struct GraphKey {
+  int self, M, R, W, rows, blocked, tiles;
+  const void *left, *A, *B, *ring, *mu;
+  bool operator<(const GraphKey& o) const
+  {
+    return std::tie(self, M, R, W, rows, blocked, tiles, left, A, B, ring, mu) <
+           std::tie(o.self, o.M, o.R, o.W, o.rows, o.blocked, o.tiles, o.left, o.A, o.B, o.ring,
+                    o.mu);
+  }
+};
+
 struct lsqt_block_state {
   BlockScratch s;
   std::map<int, BlockShape> forced;  // R -> shape set by the caller
   std::map<int, SegTable> seg;       // rows -> tile table
+  // moment sets are captured once and replayed as hipGraphs (LSQT_GRAPHS=0
+  // or lsqt_block_set_graphs(h, 0) launches them one by one instead)
+  bool graphs = true;
+  std::map<GraphKey, hipGraphExec_t> graph;
+  lsqt_block_state()
+  {
+    const char* e = getenv("LSQT_GRAPHS");
+    graphs = !(e && e[0] == '0');
+  }
 };
 
 void lsqt::block_release(lsqt_ham* h)
@@ -92,6 +113,8 @@ void lsqt::block_release(lsqt_ham* h)
                     (void*)kv.second.seg_tiles})
       if (p)
         (void)hipFree(p);
+  for (auto& kv : h->blk->graph)
+    (void)hipGraphExecDestroy(kv.second);
   if (s.mu_host)
     (void)hipHostFree(s.mu_host);
   delete h->blk;
@@ -479,10 +502,36 @@ int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Block
   if (A != right)
     LSQT_HIP(hipMemcpyAsync(A, right, sizeof(float2) * (size_t)h->d.n * R,
                             hipMemcpyDeviceToDevice, st));
-  if (self)
-    enqueue_self_steps(h, R, c, tm, A, B, s->ring, s->mu, M, st);
-  else
-    enqueue_steps(h, R, c, tm, left, A, B, s->ring, s->mu, M, 0, M, st);
+  auto enqueue = [&]() {
+    float2* a = A;
+    float2* b = B;
+    if (self)
+      enqueue_self_steps(h, R, c, tm, a, b, s->ring, s->mu, M, st);
+    else
+      enqueue_steps(h, R, c, tm, left, a, b, s->ring, s->mu, M, 0, M, st);
+  };
+  if (h->blk->graphs) {
+    const GraphKey key{self ? 1 : 0, M, R, c.W, c.rows, c.blocked, tm.tiles, left, A, B,
+                       s->ring, s->mu};
+    auto it = h->blk->graph.find(key);
+    if (it == h->blk->graph.end()) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t exec = nullptr;
+      LSQT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      enqueue();
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (e != hipSuccess)
+        return set_error(LSQT_ERR_HIP, "capturing a moment set: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess)
+        return set_error(LSQT_ERR_HIP, "instantiating a moment set: %s", hipGetErrorString(e));
+      it = h->blk->graph.emplace(key, exec).first;
+    }
+    LSQT_HIP(hipGraphLaunch(it->second, st));
+  } else {
+    enqueue();
+  }
   LSQT_HIP(hipGetLastError());
   const size_t count = (size_t)M * R * segments(h);
   LSQT_HIP(hipMemcpyAsync(s->mu_host, s->mu, sizeof(double2) * count, hipMemcpyDeviceToHost, st));
@@ -763,6 +812,16 @@ extern "C" int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double
     }
   }
   LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_block_set_graphs(lsqt_ham* h, int enable)
+{
+  if (!h)
+    return set_error(LSQT_ERR_INVALID, "null Hamiltonian");
+  if (!h->blk)
+    h->blk = new lsqt_block_state();
+  h->blk->graphs = enable != 0;
   return LSQT_OK;
 }
 

@@ -42,6 +42,7 @@ SIGNATURES = {
     "lsqt_ham_destroy": (C.c_int, [_vp]),
     "lsqt_ham_create_segmented": (C.c_int, [_vp, C.c_int, _ip, C.c_int, C.c_int, _ip, _ip, _dp,
                                             _dp, _dp, _dp, C.c_double, C.POINTER(_vp)]),
+    "lsqt_block_set_graphs": (C.c_int, [_vp, C.c_int]),
     "lsqt_block_vac": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]),
     "lsqt_block_msd": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]),
     "lsqt_ham_get_info": (C.c_int, [_vp, C.POINTER(HamInfo)]),
@@ -367,6 +368,9 @@ class Hamiltonian:
         mr, mi = np.empty(size), np.empty(size)
         _check(fn(self.h, R, _dptr(re), _dptr(im), int(M), Nt, _dptr(ts), _dptr(mr), _dptr(mi)))
         return (mr + 1j * mi).reshape(Nt, self.segments, R, M)
+
+    def block_set_graphs(self, enable: bool):
+        _check(load().lsqt_block_set_graphs(self.h, 1 if enable else 0))
 
     def block_vac(self, re, im, M, time_steps_scaled):
         """complex (Nt, segments, R, M): find_vac's moment sets for R vectors"""

@@ -227,6 +227,10 @@ int lsqt_block_vac(lsqt_ham* h, int R, const double* re, const double* im, int M
                    const double* time_step_scaled, double* mu_re, double* mu_im);
 int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double* im, int M, int n_steps,
                    const double* time_step_scaled, double* mu_re, double* mu_im);
+/* Block moment sets are captured once per shape and buffer set and replayed
+ * as hipGraphs (default; also LSQT_GRAPHS=0 in the environment turns it
+ * off): enable = 0 launches their kernels one by one. */
+int lsqt_block_set_graphs(lsqt_ham* h, int enable);
 /* The autotuner: times every kernel shape (W vectors per lane, i.e. R/W lanes
  * per site; rows = sites per workgroup tile; blocked = tile schedule) on these
  * blocks and keeps the fastest for (R, max_neighbor, table placement, size

@@ -175,3 +175,31 @@ def test_segmented_errors(gpu_ctx):
     with pytest.raises(native.LsqtError):
         native.Hamiltonian.from_arrays(gpu_ctx, a, m.n, m.max_neighbor, m.energy_max,
                                        segment_starts=[0, m.n + 1])
+
+
+def test_graph_replays_match_eager(gpu_ctx):
+    """Moment sets replayed from captured hipGraphs equal the same kernels
+    launched one by one, bit for bit, on every replay, with eager work in
+    between."""
+    from gpuqt_amd import native
+    print("[gpu] hipGraph replays", flush=True)
+    m = _model("cube_peierls")
+    H = _ham(gpu_ctx, m)
+    R, M = 8, 130
+    re, im, _ = _vectors(m, R)
+    b = native.Block(gpu_ctx, m.n, R)
+    b.upload(re, im)
+    ts = m.time_step * m.energy_max
+    H.block_set_graphs(False)
+    eager = (H.moments_block(b, b, M), H.moments_block_self(b, M),
+             H.block_vac(re, im, 40, ts), H.block_msd(re, im, 40, ts))
+    for rep in range(3):
+        H.block_set_graphs(True)
+        got = (H.moments_block(b, b, M), H.moments_block_self(b, M),
+               H.block_vac(re, im, 40, ts), H.block_msd(re, im, 40, ts))
+        for g, e in zip(got, eager):
+            np.testing.assert_array_equal(g, e, err_msg=f"replay {rep}")
+        H.block_set_graphs(False)
+        np.testing.assert_array_equal(H.moments_block(b, b, M), eager[0])
+    for o in (b, H):
+        o.close()
