@@ -6,6 +6,7 @@
 #include <iostream>
 #include <string>
 
+#include "general_io.h"
 #include "lsqt.h"
 #include "lsqt_host.h"
 #include "model.h"
@@ -150,6 +151,29 @@ extern "C" int lsqt_host_conductivity(int n_energy, int n_time, const double* ti
     put(c.sigma_msd, sigma_msd);
     put(c.vac, vac_true);
     put(c.msd, msd_true);
+  } catch (const lsqt_host::InputError& e) {
+    return report(e, err, err_len);
+  }
+  return 0;
+}
+
+extern "C" int lsqt_host_write_general(const char* dir, int n, int max_neighbor,
+                                       const int* neighbor_number, const int* neighbor_list,
+                                       const double* x, double box, double volume,
+                                       const double* potential, const double* hopping_real,
+                                       const double* hopping_imag, char* err, int err_len)
+{
+  try {
+    if (!dir || n <= 0 || max_neighbor <= 0 || !neighbor_number || !neighbor_list || !x ||
+        !hopping_real)
+      throw lsqt_host::InputError("Error: invalid arguments for the general-model files");
+    for (int i = 0; i < n; ++i)
+      if (neighbor_number[i] < 0 || neighbor_number[i] > max_neighbor)
+        throw lsqt_host::InputError("Error: site " + std::to_string(i) +
+                                    " has more neighbors than " +
+                                    std::to_string(max_neighbor));
+    lsqt_host::write_general(dir, n, max_neighbor, neighbor_number, neighbor_list, x, box,
+                             volume, potential, hopping_real, hopping_imag);
   } catch (const lsqt_host::InputError& e) {
     return report(e, err, err_len);
   }

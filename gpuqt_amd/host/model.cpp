@@ -22,11 +22,13 @@
 
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
 #include <sstream>
 
+#include "text_reader.h"
 #include "vector.h"
 
 #define PI 3.141592653589793
@@ -340,32 +342,124 @@ void Model::initialize_model_general()
   initialize_hopping();
 }
 
-void Model::initialize_neighbor()
+// Each file of the general model is parsed by one template over the input
+// type: first from a TextReader (mapped file, exact fast conversion); if that
+// meets a token outside the plain number grammar, again from std::ifstream,
+// so that unusual inputs behave exactly as with the stream (see
+// text_reader.h).  Messages are printed once, after the parse.
+namespace {
+
+template <class In>
+void parse_neighbors(In& input, const std::string& filename, int& n_atoms, int& max_nb,
+                     std::vector<int>& nn, std::vector<int>& nl)
 {
-  std::string filename = input_dir + "/neighbor.in";
-  std::ifstream input(filename);
-  if (!input.is_open())
-    throw InputError("Error: cannot open " + filename);
-  print_started_reading(filename);
-  input >> number_of_atoms >> max_neighbor;
-  if (number_of_atoms <= 0 || max_neighbor <= 0 ||
-      (long long)number_of_atoms * max_neighbor > INT_MAX)
+  input >> n_atoms >> max_nb;
+  if (n_atoms <= 0 || max_nb <= 0 || (long long)n_atoms * max_nb > INT_MAX)
     throw InputError("Error: invalid header in " + filename);
-  number_of_pairs = number_of_atoms * max_neighbor;
-  neighbor_number = new_zeroed<int>((size_t)number_of_atoms);
-  neighbor_list = new_zeroed<int>((size_t)number_of_pairs);
-  for (int n = 0; n < number_of_atoms; ++n) {
-    input >> neighbor_number[n];
-    if (neighbor_number[n] < 0 || neighbor_number[n] > max_neighbor)
+  nn.assign((size_t)n_atoms, 0);
+  nl.assign((size_t)n_atoms * max_nb, 0);
+  for (int n = 0; n < n_atoms; ++n) {
+    input >> nn[n];
+    if (nn[n] < 0 || nn[n] > max_nb)
       throw InputError("Error: site " + std::to_string(n) + " has more neighbors than " +
-                       std::to_string(max_neighbor));
-    for (int m = 0; m < neighbor_number[n]; ++m) {
-      int& j = neighbor_list[(size_t)n * max_neighbor + m];
+                       std::to_string(max_nb));
+    for (int m = 0; m < nn[n]; ++m) {
+      int& j = nl[(size_t)n * max_nb + m];
       input >> j;
-      if (j < 0 || j >= number_of_atoms)
+      if (j < 0 || j >= n_atoms)
         throw InputError("Error: neighbor index out of range at site " + std::to_string(n));
     }
   }
+}
+
+template <class In>
+void parse_values(In& input, std::vector<real>& v)
+{
+  for (real& x : v)
+    input >> x;
+}
+
+template <class In>
+int parse_hopping_kind(In& input)
+{
+  std::string first;
+  input >> first;
+  if (first == "complex")
+    return 1;
+  if (first == "real")
+    return 2;
+  return 0;
+}
+
+template <class In>
+void parse_hoppings(In& input, int type, int n_atoms, int max_nb, const int* nn,
+                    std::vector<real>& hr, std::vector<real>& hi)
+{
+  hr.assign((size_t)n_atoms * max_nb, 0.0);
+  hi.assign((size_t)n_atoms * max_nb, 0.0);
+  for (int n = 0; n < n_atoms; ++n)
+    for (int m = 0; m < nn[n]; ++m) {
+      size_t q = (size_t)n * max_nb + m;
+      input >> hr[q];
+      if (type == 1)
+        input >> hi[q];
+    }
+}
+
+// runs parse(reader) on the mapped file and, if it met an unusual token,
+// parse(stream) instead; false if the file does not exist
+template <class F>
+bool parse_file(const std::string& filename, F parse)
+{
+  // LSQT_STREAM_READER=1 parses with the stream only (to check the fast path)
+  const char* stream_only = std::getenv("LSQT_STREAM_READER");
+  if (!(stream_only && stream_only[0] == '1')) {
+    lsqt_host::TextReader fast(filename);
+    if (!fast.is_open())
+      return false;
+    bool done = false;
+    try {
+      parse(fast);
+      done = !fast.unusual();
+    } catch (const InputError&) {
+      if (!fast.unusual())
+        throw;
+    }
+    if (done)
+      return true;
+  }
+  std::ifstream input(filename);
+  if (!input.is_open())
+    return false;
+  parse(input);
+  return true;
+}
+
+}  // namespace
+
+void Model::initialize_neighbor()
+{
+  std::string filename = input_dir + "/neighbor.in";
+  std::vector<int> nn, nl;
+  int n_atoms = 0, max_nb = 0;
+  bool found = false;
+  {
+    // the file must open before anything is printed (the reference's order)
+    std::ifstream probe(filename);
+    found = probe.is_open();
+  }
+  if (!found)
+    throw InputError("Error: cannot open " + filename);
+  print_started_reading(filename);
+  if (!parse_file(filename, [&](auto& in) { parse_neighbors(in, filename, n_atoms, max_nb, nn, nl); }))
+    throw InputError("Error: cannot open " + filename);
+  number_of_atoms = n_atoms;
+  max_neighbor = max_nb;
+  number_of_pairs = number_of_atoms * max_neighbor;
+  neighbor_number = new_zeroed<int>((size_t)number_of_atoms);
+  neighbor_list = new_zeroed<int>((size_t)number_of_pairs);
+  std::memcpy(neighbor_number, nn.data(), sizeof(int) * nn.size());
+  std::memcpy(neighbor_list, nl.data(), sizeof(int) * nl.size());
   std::cout << "- Number of orbitals is " << number_of_atoms << std::endl;
   std::cout << "- Maximum neighbor number is " << max_neighbor << std::endl;
   print_finished_reading(filename);
@@ -383,15 +477,21 @@ static real reduce_distance(real d, real box)
 void Model::initialize_positions()
 {
   std::string filename = input_dir + "/position.in";
-  std::ifstream input(filename);
-  if (!input.is_open())
-    throw InputError("Error: cannot open " + filename);
+  {
+    std::ifstream probe(filename);
+    if (!probe.is_open())
+      throw InputError("Error: cannot open " + filename);
+  }
   print_started_reading(filename);
-  real box;
-  input >> box >> volume;
-  std::vector<real> x((size_t)number_of_atoms, 0.0);
-  for (int i = 0; i < number_of_atoms; ++i)
-    input >> x[i];
+  real box = 0.0, vol = 0.0;
+  std::vector<real> x;
+  parse_file(filename, [&](auto& in) {
+    box = vol = 0.0;
+    in >> box >> vol;
+    x.assign((size_t)number_of_atoms, 0.0);
+    parse_values(in, x);
+  });
+  volume = vol;
   std::cout << "- Box length along the transport direction is " << box << std::endl;
   std::cout << "- System volume is " << volume << std::endl;
   xx = new_zeroed<real>((size_t)number_of_pairs);
@@ -407,8 +507,11 @@ void Model::initialize_potential()
 {
   std::string filename = input_dir + "/potential.in";
   print_started_reading(filename);
-  std::ifstream input(filename);
-  bool nonzero = input.is_open();
+  std::vector<real> v((size_t)number_of_atoms, 0.0);
+  bool nonzero = parse_file(filename, [&](auto& in) {
+    v.assign((size_t)number_of_atoms, 0.0);
+    parse_values(in, v);
+  });
   if (!nonzero) {
     std::cout << "- Could not open " + filename << std::endl;
     std::cout << "- Assuming zero on-site potential" << std::endl;
@@ -416,9 +519,7 @@ void Model::initialize_potential()
     std::cout << "- On-site potential will be read in" << std::endl;
   }
   potential = new_zeroed<real>((size_t)number_of_atoms);
-  if (nonzero)
-    for (int n = 0; n < number_of_atoms; ++n)
-      input >> potential[n];
+  std::memcpy(potential, v.data(), sizeof(real) * v.size());
   print_finished_reading(filename);
 }
 
@@ -426,39 +527,30 @@ void Model::initialize_hopping()
 {
   std::string filename = input_dir + "/hopping.in";
   print_started_reading(filename);
-  std::ifstream input(filename);
   // 1: complex, 2: real, 3: uniform -1 (no file)
   int type = 3;
-  if (!input.is_open()) {
-    std::cout << "- Could not open " + filename << std::endl;
-    std::cout << "- Assuming uniform hoppings with strength -1" << std::endl;
-  } else {
-    std::string first;
-    input >> first;
-    if (first == "complex") {
-      type = 1;
-      std::cout << "- Hoppings have imaginary part" << std::endl;
-    } else if (first == "real") {
-      type = 2;
-      std::cout << "- Hoppings are real" << std::endl;
-    } else {
+  std::vector<real> hr, hi;
+  bool found = parse_file(filename, [&](auto& in) {
+    type = parse_hopping_kind(in);
+    if (type == 0)
       throw InputError("- Hoppings can only be real or complex");
-    }
-  }
+    parse_hoppings(in, type, number_of_atoms, max_neighbor, neighbor_number, hr, hi);
+  });
   hopping_real = new_zeroed<real>((size_t)number_of_pairs);
   hopping_imag = new_zeroed<real>((size_t)number_of_pairs);
-  for (int n = 0; n < number_of_atoms; ++n)
-    for (int m = 0; m < neighbor_number[n]; ++m) {
-      size_t q = (size_t)n * max_neighbor + m;
-      if (type < 3)
-        input >> hopping_real[q];
-      else
-        hopping_real[q] = -1.0;
-      if (type == 1)
-        input >> hopping_imag[q];
-      else
-        hopping_imag[q] = 0.0;
-    }
+  if (!found) {
+    type = 3;
+    std::cout << "- Could not open " + filename << std::endl;
+    std::cout << "- Assuming uniform hoppings with strength -1" << std::endl;
+    for (int n = 0; n < number_of_atoms; ++n)
+      for (int m = 0; m < neighbor_number[n]; ++m)
+        hopping_real[(size_t)n * max_neighbor + m] = -1.0;
+  } else {
+    std::cout << (type == 1 ? "- Hoppings have imaginary part" : "- Hoppings are real")
+              << std::endl;
+    std::memcpy(hopping_real, hr.data(), sizeof(real) * hr.size());
+    std::memcpy(hopping_imag, hi.data(), sizeof(real) * hi.size());
+  }
   print_finished_reading(filename);
 }
 

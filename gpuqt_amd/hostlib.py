@@ -33,6 +33,9 @@ SIGNATURES = {
                                          _dp, C.c_int, _dp, _dp, _dp, _dp, C.c_char_p,
                                          C.c_int]),
     "lsqt_host_postprocess": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int]),
+    "lsqt_host_write_general": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _ip, _ip, _dp,
+                                          C.c_double, C.c_double, _dp, _dp, _dp, C.c_char_p,
+                                          C.c_int]),
 }
 
 _lib = None
@@ -171,6 +174,27 @@ def conductivity(time_step, dos=None, vac=None, msd=None):
         if dos is not None:
             keep["msd"] = out["msd"]
     return keep
+
+
+def write_general(dirname: str, arrays, n, max_neighbor, x, box, volume, complex_hoppings=True,
+                  potential=True):
+    """Writes the general-model input files from row-major ELL arrays
+    (lsqt_host_write_general)."""
+    nn = np.ascontiguousarray(arrays["neighbor_number"], dtype=np.int32)
+    nl = np.ascontiguousarray(arrays["neighbor_list"], dtype=np.int32)
+    xs = np.ascontiguousarray(x, dtype=np.float64)
+    hr = np.ascontiguousarray(arrays["hopping_real"], dtype=np.float64)
+    hi = np.ascontiguousarray(arrays["hopping_imag"], dtype=np.float64)
+    pot = np.ascontiguousarray(arrays["potential"], dtype=np.float64)
+    assert nn.shape == (n,) and nl.shape == (n * max_neighbor,) and xs.shape == (n,)
+    err = C.create_string_buffer(4096)
+    ip = lambda a: a.ctypes.data_as(_ip)
+    dp = lambda a: a.ctypes.data_as(_dp)
+    if load().lsqt_host_write_general(dirname.encode(), int(n), int(max_neighbor), ip(nn), ip(nl),
+                                      dp(xs), float(box), float(volume),
+                                      dp(pot) if potential else None, dp(hr),
+                                      dp(hi) if complex_hoppings else None, err, len(err)):
+        raise InputError(err.value.decode())
 
 
 def postprocess(dirname: str):

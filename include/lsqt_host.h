@@ -47,6 +47,17 @@ int lsqt_host_conductivity(int n_energy, int n_time, const double* time_step, co
  * `lsqt_gpu --sigma <driver-file>` for every listed directory */
 int lsqt_host_postprocess(const char* input_dir, char* err, int err_len);
 
+/* Writes the general-model input files (read by src/model_general.cu:32-185)
+ * into dir: neighbor.in, position.in (box, volume, x[n]), potential.in
+ * (skipped if potential is NULL), hopping.in ("complex", or "real" if
+ * hopping_imag is NULL), numbers in the shortest form that reads back to the
+ * same bits.  The arrays are the Model's row-major ELL arrays. */
+int lsqt_host_write_general(const char* dir, int n, int max_neighbor,
+                            const int* neighbor_number, const int* neighbor_list,
+                            const double* x, double box, double volume, const double* potential,
+                            const double* hopping_real, const double* hopping_imag, char* err,
+                            int err_len);
+
 /* significant digits of the .out rows (default 6, the reference's) */
 void lsqt_host_set_output_precision(int digits);
 
