@@ -3,6 +3,7 @@
 // evolution drivers that keep the whole recursion on the device.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -420,6 +421,143 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
   detect_lattice_offsets(h, n, K, nn, nl);
   // host copies for the FP32 / block paths, built on demand
   h->host_pot.assign(pot, pot + n);
+  h->host_tab = tab;
+  h->host_tab_xx = tab_xx;
+  *out = h;
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
+                                       const double* potential, double energy_max,
+                                       lsqt_ham** out)
+{
+  if (!ctx || !L || !potential || !out)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (!L->orbital_x || !L->hop_count || !L->hop_cell || !L->hop_orbital || !L->hop_re ||
+      !L->hop_im)
+    return set_error(LSQT_ERR_INVALID, "null lattice table");
+  if (!(energy_max > 0.0))
+    return set_error(LSQT_ERR_INVALID, "energy_max must be > 0");
+  const int No = L->orbitals, K = L->max_neighbor;
+  if (No <= 0 || K <= 0 || L->transport < 0 || L->transport > 2)
+    return set_error(LSQT_ERR_INVALID, "orbitals, max_neighbor or transport direction invalid");
+  long long nll = No;
+  for (int d = 0; d < 3; ++d) {
+    if (L->cells[d] <= 0 || (L->pbc[d] != 0 && L->pbc[d] != 1))
+      return set_error(LSQT_ERR_INVALID, "cells and boundaries must be > 0 and 0/1");
+    nll *= L->cells[d];
+  }
+  if (nll > INT_MAX)
+    return set_error(LSQT_ERR_INVALID, "%lld sites exceed the int32 site index", nll);
+  const int n = (int)nll;
+  for (int m = 0; m < No; ++m) {
+    if (L->hop_count[m] < 0 || L->hop_count[m] > K)
+      return set_error(LSQT_ERR_INVALID, "orbital %d has %d hoppings (max %d)", m,
+                       L->hop_count[m], K);
+    for (int i = 0; i < L->hop_count[m]; ++i) {
+      const int k = m * K + i;
+      if (L->hop_orbital[k] < 0 || L->hop_orbital[k] >= No)
+        return set_error(LSQT_ERR_INVALID, "hopping %d of orbital %d: orbital %d out of range",
+                         i, m, L->hop_orbital[k]);
+      for (int d = 0; d < 3; ++d)
+        if (L->pbc[d] && std::abs(L->hop_cell[3 * k + d]) > L->cells[d])
+          return set_error(LSQT_ERR_INVALID, "hopping %d of orbital %d reaches beyond one "
+                           "period in direction %d", i, m, d);
+    }
+  }
+  const int ib = index_bits_for(n);
+  const long long classes = (long long)No * K;
+  if (ib >= 32 || classes >= (1ll << (32 - ib)) || classes > kPackedMaxClasses)
+    return set_error(LSQT_ERR_UNSUPPORTED, "%lld hopping classes do not fit the packed words",
+                     classes);
+  LSQT_TRY(ctx_activate(ctx));
+  // class table: value and transport offset of every (orbital, hopping)
+  std::vector<double2> tab((size_t)classes, make_double2(0.0, 0.0));
+  std::vector<double> tab_xx((size_t)classes, 0.0);
+  std::vector<int4> hop((size_t)classes, make_int4(0, 0, 0, 0));
+  const int t = L->transport;
+  for (int m = 0; m < No; ++m)
+    for (int i = 0; i < L->hop_count[m]; ++i) {
+      const int k = m * K + i;
+      tab[k] = make_double2(L->hop_re[k], L->hop_im[k]);
+      double x12 = L->lattice_constant[t] * L->hop_cell[3 * k + t];
+      x12 += L->orbital_x[L->hop_orbital[k]] - L->orbital_x[m];
+      tab_xx[k] = x12;
+      hop[k] = make_int4(L->hop_cell[3 * k], L->hop_cell[3 * k + 1], L->hop_cell[3 * k + 2],
+                         L->hop_orbital[k]);
+    }
+  const int stride = pair_stride(n);
+  lsqt_ham* h = new lsqt_ham();
+  h->ctx = ctx;
+  HamDev& d = h->d;
+  std::memset(&d, 0, sizeof(d));
+  d.n = n;
+  d.K = K;
+  d.stride = stride;
+  d.emax = energy_max;
+  d.layout = tab.size() <= (size_t)kLdsTableMax ? kPackedLds : kPackedGlobal;
+  d.idx_bits = ib;
+  d.idx_mask = (1u << ib) - 1u;
+  d.ncls = (int)classes;
+  const int* dev_count = nullptr;
+  const int4* dev_hop = nullptr;
+  std::vector<int> counts(L->hop_count, L->hop_count + No);
+  int rc = upload(h, &d.pot, potential, (size_t)n);
+  if (rc == LSQT_OK)
+    rc = upload(h, &d.tab, tab.data(), tab.size());
+  if (rc == LSQT_OK)
+    rc = upload(h, &d.tab_xx, tab_xx.data(), tab_xx.size());
+  if (rc == LSQT_OK)
+    rc = upload(h, &dev_count, counts.data(), counts.size());
+  if (rc == LSQT_OK)
+    rc = upload(h, &dev_hop, hop.data(), hop.size());
+  uint32_t* words = nullptr;
+  if (rc == LSQT_OK) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(uint32_t) * (size_t)stride * K);
+    if (e != hipSuccess) {
+      rc = set_error(LSQT_ERR_NOMEM, "hipMalloc(words): %s", hipGetErrorString(e));
+    } else {
+      h->allocations.push_back(p);
+      words = (uint32_t*)p;
+    }
+  }
+  if (rc == LSQT_OK) {
+    LatticeDev ld{};
+    for (int q = 0; q < 3; ++q) {
+      ld.cells[q] = L->cells[q];
+      ld.pbc[q] = L->pbc[q];
+    }
+    ld.orbitals = No;
+    ld.K = K;
+    ld.hop_count = dev_count;
+    ld.hop = dev_hop;
+    // the padding columns past n stay marked empty
+    hipError_t e = hipMemsetD32Async((hipDeviceptr_t)words, (int)kEmptySlot,
+                                     (size_t)stride * K, ctx->stream);
+    if (e == hipSuccess) {
+      launch_lattice_words(ld, words, stride, ib, n, ctx->stream);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+      rc = set_error(LSQT_ERR_HIP, "building the lattice words: %s", hipGetErrorString(e));
+    d.words = words;
+  }
+  if (rc != LSQT_OK) {
+    lsqt_ham_destroy(h);
+    return rc;
+  }
+  // block tile schedules: slabs of one x cell, rows of one y cell
+  const long long slab = (long long)L->cells[1] * L->cells[2] * No;
+  const long long row = (long long)L->cells[2] * No;
+  if (L->cells[0] > 1 && slab >= 64 && row >= 8 && slab / row >= 4) {
+    h->slab = (int)slab;
+    h->row = (int)row;
+  }
+  h->host_pot.assign(potential, potential + n);
   h->host_tab = tab;
   h->host_tab_xx = tab_xx;
   *out = h;

@@ -182,6 +182,18 @@ struct TileMap {
   int segs;
 };
 
+// A lattice model for device-side construction: cells[3] x orbitals sites,
+// K hopping slots per site; hop[m*K + i] = (dx, dy, dz, target orbital) of
+// hopping i of orbital m, hop_count[m] of them.
+struct LatticeDev {
+  int cells[3];
+  int pbc[3];
+  int orbitals;
+  int K;
+  const int* hop_count;
+  const int4* hop;
+};
+
 inline Grid make_grid(int n, int spt)
 {
   int per_tile = kBlock * spt;

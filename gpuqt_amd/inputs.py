@@ -110,6 +110,40 @@ def write_lattice(dirname: str, cells: Sequence[int], pbc: Sequence[int], transp
     _write(os.path.join(dirname, "lattice.in"), "\n".join(lines) + "\n")
 
 
+def read_lattice(path: str) -> dict:
+    """The contents of a lattice.in (src/model_lattice.cu:165-341) as the
+    tables of lsqt_ham_create_lattice."""
+    tok = open(path).read().split()
+    pos = 0
+
+    def take(kind, count=1):
+        nonlocal pos
+        vals = [kind(t) for t in tok[pos:pos + count]]
+        pos += count
+        return vals if count > 1 else vals[0]
+
+    cells = take(int, 3)
+    pbc = take(int, 3)
+    transport = take(int)
+    a = take(float, 3)
+    No, K = take(int), take(int)
+    xyz = np.array([take(float, 3) for _ in range(No)])
+    count = np.zeros(No, dtype=np.int32)
+    cell = np.zeros((No, K, 3), dtype=np.int32)
+    orb = np.zeros((No, K), dtype=np.int32)
+    re = np.zeros((No, K))
+    im = np.zeros((No, K))
+    for m in range(No):
+        count[m] = take(int)
+        for i in range(count[m]):
+            cell[m, i] = take(int, 3)
+            orb[m, i] = take(int)
+            re[m, i], im[m, i] = take(float), take(float)
+    return {"cells": cells, "pbc": pbc, "transport": transport, "lattice_constant": a,
+            "orbitals": No, "max_neighbor": K, "orbital_x": xyz[:, 0], "hop_count": count,
+            "hop_cell": cell, "hop_orbital": orb, "hop_re": re, "hop_im": im}
+
+
 def write_honeycomb(dirname: str, nx: int, ny: int, *, pbc=(1, 1, 0), transport: int = 0,
                     t: float = -2.7, ax: float = 1.7321, ay: float = 3.0) -> None:
     hops = [[(a, b, c, o, t, 0.0) for (a, b, c, o) in row] for row in HONEYCOMB_HOPS]

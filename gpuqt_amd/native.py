@@ -27,6 +27,14 @@ class HamInfo(C.Structure):
                 ("bytes_per_site_f64", C.c_double), ("slab", C.c_int), ("row", C.c_int)]
 
 
+class Lattice(C.Structure):
+    """lsqt_lattice: a lattice model for construction on the device"""
+    _fields_ = [("cells", C.c_int * 3), ("pbc", C.c_int * 3), ("transport", C.c_int),
+                ("lattice_constant", C.c_double * 3), ("orbitals", C.c_int),
+                ("orbital_x", _dp), ("max_neighbor", C.c_int), ("hop_count", _ip),
+                ("hop_cell", _ip), ("hop_orbital", _ip), ("hop_re", _dp), ("hop_im", _dp)]
+
+
 # every symbol declared in include/lsqt_hip.h, with (restype, argtypes)
 SIGNATURES = {
     "lsqt_strerror": (C.c_char_p, [C.c_int]),
@@ -40,6 +48,8 @@ SIGNATURES = {
     "lsqt_ham_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp,
                                   C.c_double, C.c_int, C.POINTER(_vp)]),
     "lsqt_ham_destroy": (C.c_int, [_vp]),
+    "lsqt_ham_create_lattice": (C.c_int, [_vp, C.POINTER(Lattice), _dp, C.c_double,
+                                          C.POINTER(_vp)]),
     "lsqt_ham_create_segmented": (C.c_int, [_vp, C.c_int, _ip, C.c_int, C.c_int, _ip, _ip, _dp,
                                             _dp, _dp, _dp, C.c_double, C.POINTER(_vp)]),
     "lsqt_block_set_graphs": (C.c_int, [_vp, C.c_int]),
@@ -270,6 +280,40 @@ class Hamiltonian:
                                                     int(max_neighbor), _iptr(nn), _iptr(nl),
                                                     *(_dptr(a) for a in arrs),
                                                     float(energy_max), C.byref(self.h)))
+
+    @classmethod
+    def from_lattice(cls, ctx, lattice: dict, potential, energy_max):
+        """Builds a pristine lattice model on the device (lsqt_ham_create_lattice);
+        `lattice` as returned by gpuqt_amd.inputs.read_lattice."""
+        No, K = lattice["orbitals"], lattice["max_neighbor"]
+        keep = {
+            "x": np.ascontiguousarray(lattice["orbital_x"], dtype=np.float64),
+            "count": np.ascontiguousarray(lattice["hop_count"], dtype=np.int32),
+            "cell": np.ascontiguousarray(lattice["hop_cell"], dtype=np.int32).reshape(-1),
+            "orb": np.ascontiguousarray(lattice["hop_orbital"], dtype=np.int32).reshape(-1),
+            "re": np.ascontiguousarray(lattice["hop_re"], dtype=np.float64).reshape(-1),
+            "im": np.ascontiguousarray(lattice["hop_im"], dtype=np.float64).reshape(-1),
+        }
+        assert keep["count"].shape == (No,) and keep["orb"].shape == (No * K,)
+        L = Lattice()
+        L.cells[:] = [int(c) for c in lattice["cells"]]
+        L.pbc[:] = [int(c) for c in lattice["pbc"]]
+        L.transport = int(lattice["transport"])
+        L.lattice_constant[:] = [float(a) for a in lattice["lattice_constant"]]
+        L.orbitals, L.max_neighbor = int(No), int(K)
+        L.orbital_x, L.hop_count = _dptr(keep["x"]), _iptr(keep["count"])
+        L.hop_cell, L.hop_orbital = _iptr(keep["cell"]), _iptr(keep["orb"])
+        L.hop_re, L.hop_im = _dptr(keep["re"]), _dptr(keep["im"])
+        pot = np.ascontiguousarray(potential, dtype=np.float64)
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.n = int(np.prod(lattice["cells"]) * No)
+        assert pot.shape == (self.n,)
+        self.segments = 1
+        self.h = _vp()
+        _check(load().lsqt_ham_create_lattice(ctx.h, C.byref(L), _dptr(pot), float(energy_max),
+                                              C.byref(self.h)))
+        return self
 
     @classmethod
     def from_arrays(cls, ctx, arrays, n, max_neighbor, energy_max, layout=LAYOUT_AUTO,

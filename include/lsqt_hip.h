@@ -100,6 +100,36 @@ int lsqt_ham_create(lsqt_ctx* ctx, int n, int max_neighbor, const int* neighbor_
                     double energy_max, int layout_hint, lsqt_ham** out);
 /* Hamiltonian::~Hamiltonian (src/hamiltonian.cu:133-150) */
 int lsqt_ham_destroy(lsqt_ham* h);
+
+/* A lattice model (the contents of lattice.in, src/model_lattice.cu:165-341)
+ * for construction on the device.  Sites n = ((nx*Ny + ny)*Nz + nz)*orbitals
+ * + m.  Hopping i of orbital m (i < hop_count[m] <= max_neighbor) goes to
+ * cell offset hop_cell[3*(m*max_neighbor + i) + 0..2], orbital
+ * hop_orbital[m*max_neighbor + i], with value hop_re + i hop_im at the same
+ * index. */
+typedef struct {
+  int cells[3];
+  int pbc[3];
+  int transport;               /* 0, 1, 2 */
+  double lattice_constant[3];
+  int orbitals;
+  const double* orbital_x;     /* [orbitals]: x of each orbital in its cell */
+  int max_neighbor;
+  const int* hop_count;        /* [orbitals] */
+  const int* hop_cell;         /* [orbitals][max_neighbor][3] */
+  const int* hop_orbital;      /* [orbitals][max_neighbor] */
+  const double* hop_re;        /* [orbitals][max_neighbor] */
+  const double* hop_im;
+} lsqt_lattice;
+/* Builds the Hamiltonian of a pristine lattice (no vacancies) on the device
+ * (the neighbour expansion of src/model_lattice.cu:342-386): the same
+ * neighbour order, open-boundary skips, single periodic wrap and transport
+ * offsets x12 = a[transport] * offset[transport] + x[orbital'] - x[m] as the
+ * Model's arrays, so every result equals that of lsqt_ham_create on those
+ * arrays bit for bit.  potential[n] comes from the host (the disorder
+ * draws of the Model).  Packed layout, one class per (orbital, hopping). */
+int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* lattice, const double* potential,
+                            double energy_max, lsqt_ham** out);
 /* (no counterpart in the reference, which runs one system per input
  * directory)  A block-diagonal Hamiltonian of `segments` independent systems
  * side by side: system k on sites [starts[k], starts[k+1]), starts[0] = 0,
