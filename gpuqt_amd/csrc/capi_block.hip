@@ -815,6 +815,24 @@ extern "C" int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double
   return LSQT_OK;
 }
 
+extern "C" int lsqt_block_apply(lsqt_ham* h, int op, const lsqt_block* in, lsqt_block* out)
+{
+  LSQT_TRY(check_block(h, in, out));
+  if (op < 0 || op > 2)
+    return set_error(LSQT_ERR_INVALID, "operator %d is not 0 (H), 1 ([X,H]) or 2 (V)", op);
+  if (in == out)
+    return set_error(LSQT_ERR_INVALID, "input and output blocks must differ");
+  BlockShape c;
+  TileMap tm;
+  BlockScratch* s;
+  LSQT_TRY(block_setup(h, in->R, in->d, in->d, 2, &c, &tm, &s));
+  static const int kOp[3] = {0, 2, 3};  // k_block_evo operators: H~, [X, H~], V
+  launch_block_evo(h->d, in->R, c.W, tm, kOp[op], nullptr, in->d, out->d, nullptr, nullptr,
+                   nullptr, nullptr, 0.0, 0, h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
 extern "C" int lsqt_block_set_graphs(lsqt_ham* h, int enable)
 {
   if (!h)

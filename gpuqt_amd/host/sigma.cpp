@@ -6,6 +6,7 @@
 // summation stay on the host as in the reference.
 #include "sigma.h"
 
+#include <chrono>
 #include <cmath>
 #include <fstream>
 #include <iostream>
@@ -29,13 +30,38 @@ int output_precision()
 {
   return g_precision;
 }
+
+static Path g_path = Path::f64;
+Path path()
+{
+  return g_path;
+}
+void set_path(Path p)
+{
+  g_path = p;
+}
+Path path_from_environment()
+{
+  const char* e = getenv("LSQT_PATH");
+  if (!e || !*e || std::string(e) == "f64")
+    return Path::f64;
+  if (std::string(e) == "f32")
+    return Path::f32;
+  if (std::string(e) == "block")
+    return Path::block;
+  throw InputError(std::string("Error: LSQT_PATH must be f64, f32 or block, not ") + e);
+}
 }  // namespace lsqt_host
 
 void find_moments_chebyshev(Model& model, Hamiltonian& H, Vector& left, Vector& right,
                             real* mu_re, real* mu_im)
 {
-  LSQT_CHECK(lsqt_moments(H.handle(), left.handle(), right.handle(), model.number_of_moments,
-                          mu_re, mu_im));
+  if (lsqt_host::path() == lsqt_host::Path::f64)
+    LSQT_CHECK(lsqt_moments(H.handle(), left.handle(), right.handle(), model.number_of_moments,
+                            mu_re, mu_im));
+  else
+    LSQT_CHECK(lsqt_moments_f32(H.handle(), left.handle(), right.handle(),
+                                model.number_of_moments, mu_re, mu_im));
 }
 
 namespace lsqt_host {
@@ -106,12 +132,18 @@ void perform_chebyshev_summation(Model& model, real* inner_product_real, real*,
 
 void evolve(Model&, int direction, real time_step_scaled, Hamiltonian& H, Vector& state)
 {
-  LSQT_CHECK(lsqt_evolve(H.handle(), direction, time_step_scaled, state.handle()));
+  if (lsqt_host::path() == lsqt_host::Path::f64)
+    LSQT_CHECK(lsqt_evolve(H.handle(), direction, time_step_scaled, state.handle()));
+  else
+    LSQT_CHECK(lsqt_evolve_f32(H.handle(), direction, time_step_scaled, state.handle(), 1));
 }
 
 void evolvex(Model&, int direction, real time_step_scaled, Hamiltonian& H, Vector& state)
 {
-  LSQT_CHECK(lsqt_evolvex(H.handle(), direction, time_step_scaled, state.handle()));
+  if (lsqt_host::path() == lsqt_host::Path::f64)
+    LSQT_CHECK(lsqt_evolvex(H.handle(), direction, time_step_scaled, state.handle()));
+  else
+    LSQT_CHECK(lsqt_evolvex_f32(H.handle(), direction, time_step_scaled, state.handle()));
 }
 
 namespace {
@@ -223,3 +255,97 @@ void find_spin_polarization(Model& model, Hamiltonian& H, Vector& random_state)
     }
   }
 }
+
+namespace lsqt_host {
+
+namespace {
+
+// the rows of R moment sets (mu[r*M + m]) for the first `used` vectors
+std::vector<std::vector<real>> rows_of(Model& model, std::vector<real>& mu_re,
+                                       std::vector<real>& mu_im, int used)
+{
+  const int M = model.number_of_moments;
+  std::vector<std::vector<real>> rows((size_t)used,
+                                      std::vector<real>((size_t)model.number_of_energy_points));
+  for (int r = 0; r < used; ++r)
+    damped_chebyshev_sum(M, mu_re.data() + (size_t)r * M, mu_im.data() + (size_t)r * M,
+                         model.number_of_energy_points, model.energy, model.energy_max,
+                         model.volume, rows[r].data());
+  return rows;
+}
+
+struct BlockHandle {
+  lsqt_block* b = nullptr;
+  ~BlockHandle() { lsqt_block_destroy(b); }
+};
+
+}  // namespace
+
+void find_block_quantities(Model& model, Hamiltonian& H, int R, int used,
+                           const std::vector<real>& re, const std::vector<real>& im)
+{
+  const int M = model.number_of_moments;
+  const int n = model.number_of_atoms;
+  const int Nt = model.number_of_steps_correlation;
+  auto t0 = std::chrono::steady_clock::now();
+  auto lap = [&](const char* what) {
+    auto t1 = std::chrono::steady_clock::now();
+    std::cout << "- Time used for finding " << what << " of " << used
+              << " random vectors = " << std::chrono::duration<double>(t1 - t0).count() << " s"
+              << std::endl;
+    t0 = t1;
+  };
+  std::vector<real> mu_re((size_t)R * M), mu_im((size_t)R * M);
+  BlockHandle phi, tmp;
+  LSQT_CHECK(lsqt_block_create(default_context(), n, R, &phi.b));
+  LSQT_CHECK(lsqt_block_from_host(phi.b, re.data(), im.data()));
+  // DOS: self products (src/sigma.cu:204-239)
+  LSQT_CHECK(lsqt_moments_block_self(H.handle(), phi.b, M, mu_re.data(), mu_im.data()));
+  {
+    auto rows = rows_of(model, mu_re, mu_im, used);
+    std::ofstream output = open_output(model, "/dos.out");
+    for (auto& row : rows)
+      write_row(output, row);
+  }
+  lap("DOS");
+  if (model.calculate_vac0) {  // <V phi|T_m|V phi> (src/sigma.cu:243-274)
+    LSQT_CHECK(lsqt_block_create(default_context(), n, R, &tmp.b));
+    LSQT_CHECK(lsqt_block_apply(H.handle(), 2, phi.b, tmp.b));
+    LSQT_CHECK(lsqt_moments_block_self(H.handle(), tmp.b, M, mu_re.data(), mu_im.data()));
+    auto rows = rows_of(model, mu_re, mu_im, used);
+    std::ofstream output = open_output(model, "/vac0.out");
+    for (auto& row : rows)
+      write_row(output, row);
+    lap("VAC0");
+  }
+  if ((model.calculate_vac || model.calculate_msd) && Nt > 0) {
+    std::vector<real> steps((size_t)Nt);
+    for (int t = 0; t < Nt; ++t)
+      steps[t] = model.time_step[t] * model.energy_max;
+    std::vector<real> all_re((size_t)Nt * R * M), all_im((size_t)Nt * R * M);
+    for (int q = 0; q < 2; ++q) {
+      const bool vac = q == 0;
+      if (!(vac ? model.calculate_vac : model.calculate_msd))
+        continue;
+      LSQT_CHECK((vac ? lsqt_block_vac : lsqt_block_msd)(H.handle(), R, re.data(), im.data(), M,
+                                                         Nt, steps.data(), all_re.data(),
+                                                         all_im.data()));
+      std::ofstream output = open_output(model, vac ? "/vac.out" : "/msd.out");
+      // the reference writes vector by vector, each with its Nt rows
+      std::vector<std::vector<std::vector<real>>> by_step;
+      for (int t = 0; t < Nt; ++t) {
+        std::vector<real> sr(all_re.begin() + (size_t)t * R * M,
+                             all_re.begin() + (size_t)(t + 1) * R * M);
+        std::vector<real> si(all_im.begin() + (size_t)t * R * M,
+                             all_im.begin() + (size_t)(t + 1) * R * M);
+        by_step.push_back(rows_of(model, sr, si, used));
+      }
+      for (int r = 0; r < used; ++r)
+        for (int t = 0; t < Nt; ++t)
+          write_row(output, by_step[t][r]);
+      lap(vac ? "VAC" : "MSD");
+    }
+  }
+}
+
+}  // namespace lsqt_host

@@ -35,4 +35,23 @@ void damped_chebyshev_sum(int M, real* mu_re, real* mu_im, int n_energy, const r
 // writes with) unless set otherwise.
 void set_output_precision(int digits);
 int output_precision();
+
+// Which device path the drivers use (environment LSQT_PATH):
+//   f64   (default) the reference's arithmetic, bit-faithful states;
+//   f32   FP32-stored recursion states, FP64 arithmetic, moments and
+//         re-normalised evolved states (lsqt_moments_f32, lsqt_evolve_f32);
+//   block all random vectors of a directory at once (lsqt_moments_block_self,
+//         lsqt_block_vac / msd): FP32 recursion, FP64 moments and evolved
+//         states; the per-vector leftovers (spin, LDOS) use f32.
+enum class Path { f64, f32, block };
+Path path();
+void set_path(Path p);
+Path path_from_environment();
+
+// The block path's drivers for one directory: DOS, VAC0, VAC and MSD of R
+// vectors (host arrays [R][n]; the first `used` are the random vectors, the
+// rest zero padding up to a power of 2), appending rows to the .out files in
+// the reference's order (vector by vector; Nt rows per vector for VAC/MSD).
+void find_block_quantities(Model& model, Hamiltonian& H, int R, int used,
+                           const std::vector<real>& re, const std::vector<real>& im);
 }  // namespace lsqt_host

@@ -53,6 +53,7 @@ SIGNATURES = {
     "lsqt_ham_create_segmented": (C.c_int, [_vp, C.c_int, _ip, C.c_int, C.c_int, _ip, _ip, _dp,
                                             _dp, _dp, _dp, C.c_double, C.POINTER(_vp)]),
     "lsqt_block_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "lsqt_block_apply": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "lsqt_block_vac": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]),
     "lsqt_block_msd": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]),
     "lsqt_ham_get_info": (C.c_int, [_vp, C.POINTER(HamInfo)]),
@@ -412,6 +413,10 @@ class Hamiltonian:
         mr, mi = np.empty(size), np.empty(size)
         _check(fn(self.h, R, _dptr(re), _dptr(im), int(M), Nt, _dptr(ts), _dptr(mr), _dptr(mi)))
         return (mr + 1j * mi).reshape(Nt, self.segments, R, M)
+
+    def block_apply(self, op: int, bin, bout):
+        """bout = op(bin): 0 H~, 1 [X, H~], 2 V"""
+        _check(load().lsqt_block_apply(self.h, int(op), bin.h, bout.h))
 
     def block_set_graphs(self, enable: bool):
         _check(load().lsqt_block_set_graphs(self.h, 1 if enable else 0))

@@ -244,6 +244,10 @@ int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsqt_block* ri
  * FP32 rounding of the states. */
 int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, double* mu_re,
                             double* mu_im);
+/* out = op(in) for every vector of a block: op 0 = H~ (apply), 1 = [X, H~]
+ * (apply_commutator), 2 = V (apply_current); FP32 arithmetic
+ * (src/hamiltonian.cu:225-237, 308-320, 387-399) */
+int lsqt_block_apply(lsqt_ham* h, int op, const lsqt_block* in, lsqt_block* out);
 /* Time-correlation drivers for R vectors at once (FP32 recursion states,
  * FP64 accumulated and re-normalised evolved states, FP64 moments):
  * re/im hold the random vectors vector-major ([R][n], FP64, as

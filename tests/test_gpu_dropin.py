@@ -107,3 +107,39 @@ def test_append_and_ldos(tmp_path):
     assert r.returncode == 0
     rows = read_out(work / "ldos.out")
     assert rows.shape[0] == 6 and np.array_equal(rows[:3], rows[3:])
+
+
+@pytest.mark.parametrize("path,tol", [("f32", 1e-5), ("block", 2e-5)])
+@pytest.mark.parametrize("case", golden_cases())
+def test_precision_and_block_paths(tmp_path, case, path, tol):
+    """LSQT_PATH=f32 (FP32-stored states) and LSQT_PATH=block (all random
+    vectors at once) through the unchanged program: the same .out files with
+    the same rows, each within `tol` of the reference's row (normwise)."""
+    src = os.path.join(CASES_DIR, case)
+    work = tmp_path / case
+    shutil.copytree(src, work, ignore=shutil.ignore_patterns("expected", "dump.npz"))
+    driver = tmp_path / "input.txt"
+    driver.write_text(f"{work}\n")
+    print(f"[gpu] lsqt_gpu LSQT_PATH={path} {case}", flush=True)
+    env = dict(os.environ, LSQT_PATH=path)
+    r = hostlib.run_cli(str(driver), digits=17, timeout=600, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    expected = sorted(os.listdir(os.path.join(src, "expected")))
+    assert sorted(f for f in os.listdir(work) if f.endswith(".out")) == expected
+    for name in expected:
+        got = read_out(work / name)
+        want = read_out(os.path.join(src, "expected", name))
+        assert got.shape == want.shape, name
+        for k in range(want.shape[0]):
+            err = np.abs(got[k] - want[k]).max() / np.abs(want[k]).max()
+            assert err <= tol, f"{case}/{name} row {k}: {err:.2e}"
+
+
+def test_unknown_path_is_an_input_error(tmp_path):
+    src = os.path.join(CASES_DIR, "honeycomb_small")
+    work = tmp_path / "w"
+    shutil.copytree(src, work, ignore=shutil.ignore_patterns("expected", "dump.npz"))
+    driver = tmp_path / "input.txt"
+    driver.write_text(f"{work}\n")
+    r = hostlib.run_cli(str(driver), timeout=120, env=dict(os.environ, LSQT_PATH="f16"))
+    assert r.returncode == 1 and "LSQT_PATH" in r.stdout
