@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -226,6 +227,9 @@ extern "C" int lsqt_block_create(lsqt_ctx* ctx, int n, int R, lsqt_block** out)
   if (n <= 0 || !valid_R(R))
     return set_error(LSQT_ERR_INVALID, "block of %d sites x %d vectors: R must be a power of 2 "
                      "in 1..256", n, R);
+  if ((size_t)n * R > (size_t)INT_MAX)
+    return set_error(LSQT_ERR_INVALID, "block of %d sites x %d vectors exceeds 2^31 values", n,
+                     R);
   LSQT_TRY(ctx_activate(ctx));
   lsqt_block* b = new lsqt_block();
   b->ctx = ctx;
@@ -725,6 +729,8 @@ int check_driver(lsqt_ham* h, int R, const double* re, const double* im, int M, 
     return set_error(LSQT_ERR_INVALID, "R = %d must be a power of 2 in 1..256", R);
   if (M < 2 || n_steps < 1)
     return set_error(LSQT_ERR_INVALID, "M = %d (>= 2) and %d time steps (>= 1)", M, n_steps);
+  if ((size_t)h->d.n * R > (size_t)INT_MAX)
+    return set_error(LSQT_ERR_INVALID, "%d sites x %d vectors exceeds 2^31 values", h->d.n, R);
   if (h->d.layout == kFull)
     return set_error(LSQT_ERR_UNSUPPORTED, "the block path needs the packed layout");
   return ctx_activate(h->ctx);
