@@ -12,7 +12,7 @@ namespace lsqt {
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kXcds = 8;             // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;  // packed word of an unused ELL slot
-constexpr int kLdsTableMax = 64;     // packed tables up to this size are staged in LDS
+constexpr int kLdsTableMax = 512;    // packed tables up to this size are staged in LDS
 
 enum Layout { kFull = 0, kPackedLds = 1, kPackedGlobal = 2 };
 
