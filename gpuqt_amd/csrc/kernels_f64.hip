@@ -462,7 +462,11 @@ inline int blocks_1d(int n)
     __VA_ARGS__;                       \
   }
 
-constexpr int kSptHpsi = 1;
+// sites per thread of the fused step (a build-time tuning knob; 1 measured best)
+#ifndef LSQT_SPT_HPSI
+#define LSQT_SPT_HPSI 1
+#endif
+constexpr int kSptHpsi = LSQT_SPT_HPSI;
 
 void launch_hpsi(const HamDev& h, const double2* psi1, const double2* psi0, double2* out,
                  const double2* L, double2* partial, bool recur, bool ip, hipStream_t st)
