@@ -153,6 +153,9 @@ extern "C" int lsqt_ctx_destroy(lsqt_ctx* ctx)
 {
   if (!ctx)
     return LSQT_OK;
+  if (ctx->live > 0)
+    return set_error(LSQT_ERR_INVALID, "%d Hamiltonians, vectors or blocks still use this "
+                     "context; destroy them first", ctx->live);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->partial)
@@ -326,6 +329,7 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
   const size_t P = (size_t)stride * K;  // column-major pair slots, padded columns
   lsqt_ham* h = new lsqt_ham();
   h->ctx = ctx;
+  ++ctx->live;
   HamDev& d = h->d;
   std::memset(&d, 0, sizeof(d));
   d.n = n;
@@ -375,6 +379,7 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
       }
     }
     if (!packed && layout_hint == LSQT_LAYOUT_PACKED) {
+      --ctx->live;
       delete h;
       return set_error(LSQT_ERR_UNSUPPORTED,
                        "packed layout needs at most %lld distinct hopping triples", max_classes);
@@ -490,6 +495,7 @@ extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
   const int stride = pair_stride(n);
   lsqt_ham* h = new lsqt_ham();
   h->ctx = ctx;
+  ++ctx->live;
   HamDev& d = h->d;
   std::memset(&d, 0, sizeof(d));
   d.n = n;
@@ -606,6 +612,7 @@ extern "C" int lsqt_ham_destroy(lsqt_ham* h)
     if (p)
       (void)hipFree(p);
   ham_release_extra(h);
+  --h->ctx->live;
   delete h;
   return LSQT_OK;
 }
@@ -644,12 +651,17 @@ extern "C" int lsqt_vec_create(lsqt_ctx* ctx, int n, lsqt_vec** out)
   hipError_t e = hipMalloc(&p, sizeof(double2) * (size_t)n);
   if (e != hipSuccess)
     return set_error(LSQT_ERR_NOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  launch_zero((double2*)p, n, ctx->stream);
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    return set_error(LSQT_ERR_HIP, "zeroing a vector: %s", hipGetErrorString(e));
+  }
   lsqt_vec* v = new lsqt_vec();
   v->ctx = ctx;
+  ++ctx->live;
   v->n = n;
   v->d = (double2*)p;
-  launch_zero(v->d, n, ctx->stream);
-  LSQT_HIP(hipGetLastError());
   *out = v;
   return LSQT_OK;
 }
@@ -671,6 +683,7 @@ extern "C" int lsqt_vec_destroy(lsqt_vec* v)
   (void)hipSetDevice(v->ctx->device);
   (void)hipStreamSynchronize(v->ctx->stream);
   (void)hipFree(v->d);
+  --v->ctx->live;
   delete v;
   return LSQT_OK;
 }

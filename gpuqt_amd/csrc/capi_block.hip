@@ -61,6 +61,7 @@ struct BlockShape {
   int W = 0;
   int rows = 0;
   int blocked = 0;
+  int depth = 1;  // passes in flight per wave: 1, or 2 (double-buffered loop)
 };
 
 // tiles of a segmented Hamiltonian for one tile size: `rows` sites per tile,
@@ -74,13 +75,13 @@ struct SegTable {
 
 // a captured moment set: every argument its kernels were recorded with
 struct GraphKey {
-  int self, M, R, W, rows, blocked, tiles;
+  int self, M, R, W, rows, blocked, depth, tiles;
   const void *left, *A, *B, *ring, *mu;
   bool operator<(const GraphKey& o) const
   {
-    return std::tie(self, M, R, W, rows, blocked, tiles, left, A, B, ring, mu) <
-           std::tie(o.self, o.M, o.R, o.W, o.rows, o.blocked, o.tiles, o.left, o.A, o.B, o.ring,
-                    o.mu);
+    return std::tie(self, M, R, W, rows, blocked, depth, tiles, left, A, B, ring, mu) <
+           std::tie(o.self, o.M, o.R, o.W, o.rows, o.blocked, o.depth, o.tiles, o.left, o.A,
+                    o.B, o.ring, o.mu);
   }
 };
 
@@ -103,6 +104,11 @@ void lsqt::block_release(lsqt_ham* h)
 {
   if (!h->blk)
     return;
+  // graphs first: nothing of theirs may run or refer to memory freed below
+  (void)hipStreamSynchronize(h->ctx->stream);
+  for (auto& kv : h->blk->graph)
+    (void)hipGraphExecDestroy(kv.second);
+  h->blk->graph.clear();
   BlockScratch& s = h->blk->s;
   for (void* p : {(void*)s.A, (void*)s.B, (void*)s.ring, (void*)s.mu, (void*)s.F[0],
                   (void*)s.F[1], (void*)s.F[2], (void*)s.S[0], (void*)s.S[1], (void*)s.S[2],
@@ -114,8 +120,6 @@ void lsqt::block_release(lsqt_ham* h)
                     (void*)kv.second.seg_tiles})
       if (p)
         (void)hipFree(p);
-  for (auto& kv : h->blk->graph)
-    (void)hipGraphExecDestroy(kv.second);
   if (s.mu_host)
     (void)hipHostFree(s.mu_host);
   delete h->blk;
@@ -163,8 +167,9 @@ void tune_load()
   char arch[32];
   TuneKey k;
   TuneVal v;
-  while (fscanf(f, "%31s %d %d %d %d %d %d %d %d %lf", arch, &k.R, &k.K, &k.lds, &k.size_class,
-                &k.slab, &v.shape.W, &v.shape.rows, &v.shape.blocked, &v.us) == 10) {
+  while (fscanf(f, "%31s %d %d %d %d %d %d %d %d %d %lf", arch, &k.R, &k.K, &k.lds,
+                &k.size_class, &k.slab, &v.shape.W, &v.shape.rows, &v.shape.blocked,
+                &v.shape.depth, &v.us) == 11) {
     if (std::strcmp(arch, "gfx950") == 0)
       g_tune[k] = v;
   }
@@ -180,8 +185,8 @@ void tune_store(const TuneKey& k, const TuneVal& v)
   FILE* f = fopen(path.c_str(), "a");
   if (!f)
     return;  // the cache file is an optimisation only
-  fprintf(f, "gfx950 %d %d %d %d %d %d %d %d %.3f\n", k.R, k.K, k.lds, k.size_class, k.slab,
-          v.shape.W, v.shape.rows, v.shape.blocked, v.us);
+  fprintf(f, "gfx950 %d %d %d %d %d %d %d %d %d %.3f\n", k.R, k.K, k.lds, k.size_class, k.slab,
+          v.shape.W, v.shape.rows, v.shape.blocked, v.shape.depth, v.us);
   fclose(f);
 }
 
@@ -233,10 +238,12 @@ extern "C" int lsqt_block_create(lsqt_ctx* ctx, int n, int R, lsqt_block** out)
   LSQT_TRY(ctx_activate(ctx));
   lsqt_block* b = new lsqt_block();
   b->ctx = ctx;
+  ++ctx->live;
   b->n = n;
   b->R = R;
   int rc = block_alloc(ctx, (size_t)n * R, &b->d);
   if (rc != LSQT_OK) {
+    --ctx->live;
     delete b;
     return rc;
   }
@@ -252,6 +259,7 @@ extern "C" int lsqt_block_destroy(lsqt_block* b)
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
   (void)hipFree(b->d);
+  --b->ctx->live;
   delete b;
   return LSQT_OK;
 }
@@ -399,9 +407,11 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, c
     if (m == 0) {
       launch_block_dot(A, L, R, tm, part, st);
     } else if (m == 1) {
-      launch_block_step(h->d, R, c.W, tm, A, nullptr, B, L, part, nullptr, false, false, st);
+      launch_block_step(h->d, R, c.W, c.depth == 2, tm, A, nullptr, B, L, part, nullptr, false,
+                        false, st);
     } else {
-      launch_block_step(h->d, R, c.W, tm, B, A, A, L, part, nullptr, true, false, st);
+      launch_block_step(h->d, R, c.W, c.depth == 2, tm, B, A, A, L, part, nullptr, true, false,
+                        st);
       float2* t = A;
       A = B;
       B = t;
@@ -431,9 +441,11 @@ void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& 
     double2* odd = ring + (size_t)((2 * m - 2) % kRing) * slot;
     double2* even = ring + (size_t)((2 * m - 1) % kRing) * slot;
     if (m == 1) {
-      launch_block_step(h->d, R, c.W, tm, A, nullptr, B, nullptr, odd, even, false, true, st);
+      launch_block_step(h->d, R, c.W, c.depth == 2, tm, A, nullptr, B, nullptr, odd, even, false,
+                        true, st);
     } else {
-      launch_block_step(h->d, R, c.W, tm, B, A, A, nullptr, odd, even, true, true, st);
+      launch_block_step(h->d, R, c.W, c.depth == 2, tm, B, A, A, nullptr, odd, even, true, true,
+                        st);
       float2* t = A;
       A = B;
       B = t;
@@ -463,7 +475,8 @@ static int check_block(lsqt_ham* h, const lsqt_block* a, const lsqt_block* b)
   return ctx_activate(h->ctx);
 }
 
-extern "C" int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int blocked)
+extern "C" int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int blocked,
+                                     int depth)
 {
   if (!h || !valid_R(R))
     return set_error(LSQT_ERR_INVALID, "bad Hamiltonian or R");
@@ -477,6 +490,8 @@ extern "C" int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int bl
     return set_error(LSQT_ERR_INVALID, "W=%d rows=%d invalid for R=%d", W, rows, R);
   if (blocked < 0 || blocked > 2)
     return set_error(LSQT_ERR_INVALID, "tile schedule %d is not 0, 1 or 2", blocked);
+  if (depth != 1 && depth != 2)
+    return set_error(LSQT_ERR_INVALID, "depth %d is not 1 or 2", depth);
   if (blocked && (h->slab == 0 || h->slab % rows != 0))
     return set_error(LSQT_ERR_INVALID, "blocked tiles of %d sites need a lattice slab they "
                      "divide (slab %d)", rows, h->slab);
@@ -487,6 +502,7 @@ extern "C" int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int bl
   c.W = W;
   c.rows = rows;
   c.blocked = blocked;
+  c.depth = depth;
   h->blk->forced[R] = c;
   return LSQT_OK;
 }
@@ -515,8 +531,8 @@ int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Block
       enqueue_steps(h, R, c, tm, left, a, b, s->ring, s->mu, M, 0, M, st);
   };
   if (h->blk->graphs) {
-    const GraphKey key{self ? 1 : 0, M, R, c.W, c.rows, c.blocked, tm.tiles, left, A, B,
-                       s->ring, s->mu};
+    const GraphKey key{self ? 1 : 0, M,    R, c.W,    c.rows, c.blocked, c.depth,
+                       tm.tiles,     left, A, B,      s->ring, s->mu};
     auto it = h->blk->graph.find(key);
     if (it == h->blk->graph.end()) {
       hipGraph_t g = nullptr;
@@ -850,7 +866,8 @@ extern "C" int lsqt_block_set_graphs(lsqt_ham* h, int enable)
 }
 
 extern "C" int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right,
-                               int force, int* W, int* rows, int* blocked, double* us_per_step)
+                               int force, int* W, int* rows, int* blocked, int* depth,
+                               double* us_per_step)
 {
   LSQT_TRY(check_block(h, left, right));
   LSQT_TRY(f32_potential(h));
@@ -863,6 +880,8 @@ extern "C" int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_b
     *rows = c.rows;
   if (blocked)
     *blocked = c.blocked;
+  if (depth)
+    *depth = c.depth;
   if (us_per_step)
     *us_per_step = us;
   return LSQT_OK;
@@ -920,6 +939,15 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
           cands.push_back(c);
         }
       }
+    }
+  }
+  // every shape also with the double-buffered loop (compile-time slot counts)
+  if (h->d.K == 3 || h->d.K == 4 || h->d.K == 6) {
+    const size_t single = cands.size();
+    for (size_t i = 0; i < single; ++i) {
+      BlockShape c = cands[i];
+      c.depth = 2;
+      cands.push_back(c);
     }
   }
   const int steps = 6;

@@ -184,7 +184,7 @@ enum BlockProducts {
 // One Chebyshev step for R vectors (RECUR: out = 2 H~ psi1 - psi0, else
 // out = H~ psi1) in FP32 arithmetic, fused with the per-vector moment
 // products of mode IPM (FP64 products and sums, per-tile partials).
-template <int LAYOUT, int KT, int W, bool RECUR, int IPM, bool NT>
+template <int LAYOUT, int KT, int W, bool RECUR, int IPM, bool NT, bool DB>
 __global__ __launch_bounds__(kBlock) void k_block_step(
   HamDev h, int R, TileMap tm, const float2* __restrict__ psi1, const float2* psi0, float2* out,
   const float2* __restrict__ L, double2* __restrict__ partial, double2* __restrict__ partial2,
@@ -243,6 +243,94 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
       site += per_pass;
     }
   };
+  if constexpr (DB && kPipe) {
+    // Double buffering: the loads of pass p+1 (own rows, psi0, left rows,
+    // gathers) are issued before pass p is computed, the slot words two
+    // passes ahead.  Past the last pass the first site of the tile stands in
+    // (its loads are in bounds and never used).
+    struct Pass {
+      Vec<W> p1, p0, l;
+      Vec<W> c[C];
+      float v;
+      float a[C], b[C];
+    };
+    const int first = s;
+    auto issue = [&](int site, const uint32_t (&wd)[C], Pass& P) {
+      int j[C];
+      decode_words32<LAYOUT, C>(h, s_tab, site, wd, j, P.a, P.b);
+      const size_t base = (size_t)site * R + r0;
+      P.p1 = ldv<W, false>(psi1 + base);
+      P.v = ld<NT>(h.pot32 + site);
+      if constexpr (RECUR)
+        P.p0 = ldv<W, NT>(psi0 + base);
+      if constexpr (IPM == kLeft)
+        P.l = ldv<W, NT>(L + base);
+#pragma unroll
+      for (int q = 0; q < C; ++q)
+        P.c[q] = ldv<W, false>(psi1 + (size_t)j[q] * R + r0);
+    };
+    if (np > 0) {
+      int s1 = s, y1 = wy;
+      advance(s1, y1);
+      int s2 = s1, y2 = y1;
+      advance(s2, y2);
+      uint32_t wd[C];
+      load_words<C, NT>(h, s, wd);
+      Pass cur;
+      issue(s, wd, cur);
+      load_words<C, NT>(h, np > 1 ? s1 : first, wd);
+      for (int p = 0; p < np; ++p) {
+        Pass nxt;
+        issue(p + 1 < np ? s1 : first, wd, nxt);
+        load_words<C, NT>(h, p + 2 < np ? s2 : first, wd);
+        const size_t base = (size_t)s * R + r0;
+        float tr[W], ti[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          tr[w] = cur.v * cur.p1.v[w].x;
+          ti[w] = cur.v * cur.p1.v[w].y;
+        }
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            const float cx = cur.c[q].v[w].x, cy = cur.c[q].v[w].y;
+            tr[w] = __builtin_fmaf(cur.a[q], cx, tr[w]);
+            tr[w] = __builtin_fmaf(-cur.b[q], cy, tr[w]);
+            ti[w] = __builtin_fmaf(cur.a[q], cy, ti[w]);
+            ti[w] = __builtin_fmaf(cur.b[q], cx, ti[w]);
+          }
+        }
+        Vec<W> o;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          if constexpr (RECUR) {
+            o.v[w].x = __builtin_fmaf(2.0f, tr[w] * inv_scale, -cur.p0.v[w].x);
+            o.v[w].y = __builtin_fmaf(2.0f, ti[w] * inv_scale, -cur.p0.v[w].y);
+          } else {
+            o.v[w].x = tr[w] * inv_scale;
+            o.v[w].y = ti[w] * inv_scale;
+          }
+        }
+        stv<W>(out + base, o);
+        const Vec<W>& l = IPM == kSelf ? cur.p1 : cur.l;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const double ox = o.v[w].x, oy = o.v[w].y, lx = l.v[w].x, ly = l.v[w].y;
+          acc_re[w] += ox * lx + oy * ly;
+          acc_im[w] += oy * lx - ox * ly;
+          if constexpr (IPM == kSelf)
+            acc_nn[w] += ox * ox + oy * oy;
+        }
+        cur = nxt;
+        s = s1;
+        wy = y1;
+        s1 = s2;
+        y1 = y2;
+        advance(s2, y2);
+      }
+    }
+  } else {
   uint32_t wnext[C];
   if constexpr (kPipe) {
     if (np > 0)
@@ -322,6 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
     }
     s = sn;
     wy = yn;
+  }
   }
 
   tile_sums<W>(acc_re, acc_im, vpw, R, r0, s_red, partial + (size_t)tile * R);
@@ -725,28 +814,36 @@ Grid block_grid(const TileMap& t)
   return Grid{t.tiles, blocks};
 }
 
-void launch_block_step(const HamDev& h, int R, int W, const TileMap& tm, const float2* psi1,
-                       const float2* psi0, float2* out, const float2* L, double2* partial,
-                       double2* partial2, bool recur, bool self, hipStream_t st)
+void launch_block_step(const HamDev& h, int R, int W, bool db, const TileMap& tm,
+                       const float2* psi1, const float2* psi0, float2* out, const float2* L,
+                       double2* partial, double2* partial2, bool recur, bool self,
+                       hipStream_t st)
 {
   Grid g = block_grid(tm);
   const bool nt = (size_t)h.n * R >= (size_t)kStreamingMinSites;
-#define LSQTB_STEP(RC, MODE, NTV)                                                 \
-  k_block_step<LAY, KT, WW, RC, MODE, NTV><<<g.blocks, kBlock, 0, st>>>(          \
+#define LSQTB_STEP(RC, MODE, NTV, DBV)                                            \
+  k_block_step<LAY, KT, WW, RC, MODE, NTV, DBV><<<g.blocks, kBlock, 0, st>>>(     \
     h, R, tm, psi1, psi0, out, L, partial, partial2, g.tiles, g.blocks)
+#define LSQTB_STEP_DB(DBV)                                                                  \
+  if (nt) {                                                                                 \
+    if (self)                                                                               \
+      recur ? LSQTB_STEP(true, kSelf, true, DBV) : LSQTB_STEP(false, kSelf, true, DBV);     \
+    else                                                                                    \
+      recur ? LSQTB_STEP(true, kLeft, true, DBV) : LSQTB_STEP(false, kLeft, true, DBV);     \
+  } else {                                                                                  \
+    if (self)                                                                               \
+      recur ? LSQTB_STEP(true, kSelf, false, DBV) : LSQTB_STEP(false, kSelf, false, DBV);   \
+    else                                                                                    \
+      recur ? LSQTB_STEP(true, kLeft, false, DBV) : LSQTB_STEP(false, kLeft, false, DBV);   \
+  }
   LSQTB_LAYOUT_SWITCH(h, LAY, LSQTB_K_SWITCH(h.K, KT, LSQTB_W_SWITCH(W, WW, {
-    if (nt) {
-      if (self)
-        recur ? LSQTB_STEP(true, kSelf, true) : LSQTB_STEP(false, kSelf, true);
-      else
-        recur ? LSQTB_STEP(true, kLeft, true) : LSQTB_STEP(false, kLeft, true);
+    if (db && KT > 0) {
+      LSQTB_STEP_DB(true)
     } else {
-      if (self)
-        recur ? LSQTB_STEP(true, kSelf, false) : LSQTB_STEP(false, kSelf, false);
-      else
-        recur ? LSQTB_STEP(true, kLeft, false) : LSQTB_STEP(false, kLeft, false);
+      LSQTB_STEP_DB(false)
     }
   })))
+#undef LSQTB_STEP_DB
 #undef LSQTB_STEP
 }
 

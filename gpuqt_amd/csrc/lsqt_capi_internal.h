@@ -20,6 +20,9 @@ struct lsqt_ctx {
   size_t mu_cap = 0;
   double2* mu_host = nullptr;
   size_t mu_host_cap = 0;
+  // Hamiltonians, vectors and blocks that use this context's device and
+  // stream; the context cannot be destroyed while any is alive
+  int live = 0;
 };
 
 struct lsqt_f32_state;    // FP32-path device data (capi_f32.hip)

@@ -56,9 +56,11 @@ TileMap make_tile_map(int n, int rows, int slab, int span, int row, int walk_pas
 Grid block_grid(const TileMap& t);
 // self: <psi1|out> into partial and <out|out> into partial2 (L unused);
 // otherwise <L|out> into partial
-void launch_block_step(const HamDev& h, int R, int W, const TileMap& tm, const float2* psi1,
-                       const float2* psi0, float2* out, const float2* L, double2* partial,
-                       double2* partial2, bool recur, bool self, hipStream_t st);
+// db: double-buffered pass loop (compile-time slot counts only)
+void launch_block_step(const HamDev& h, int R, int W, bool db, const TileMap& tm,
+                       const float2* psi1, const float2* psi0, float2* out, const float2* L,
+                       double2* partial, double2* partial2, bool recur, bool self,
+                       hipStream_t st);
 void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& tm,
                       double2* partial, hipStream_t st);
 // block time evolution (op: 0 H~, 1 Chebyshev term of U, 2 [X, H~], 3 V,

@@ -6,8 +6,10 @@ cannot be used, calls raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -90,8 +92,8 @@ SIGNATURES = {
     "lsqt_block_to_host": (C.c_int, [_vp, _dp, _dp]),
     "lsqt_moments_block": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp, _dp]),
     "lsqt_moments_block_self": (C.c_int, [_vp, _vp, C.c_int, _dp, _dp]),
-    "lsqt_block_tune": (C.c_int, [_vp, _vp, _vp, C.c_int, _ip, _ip, _ip, _dp]),
-    "lsqt_block_set_config": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "lsqt_block_tune": (C.c_int, [_vp, _vp, _vp, C.c_int, _ip, _ip, _ip, _ip, _dp]),
+    "lsqt_block_set_config": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
 }
 
 _lib = None
@@ -130,6 +132,24 @@ def _iptr(a):
     return a.ctypes.data_as(_ip)
 
 
+# Live handles, closed at interpreter exit in dependency order (vectors and
+# blocks, then Hamiltonians, then contexts) while this module is intact, so
+# that no handle outlives the context whose stream and device it uses.
+_live = {"data": weakref.WeakSet(), "ham": weakref.WeakSet(), "ctx": weakref.WeakSet()}
+
+
+def close_all():
+    for kind in ("data", "ham", "ctx"):
+        for obj in list(_live[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+atexit.register(close_all)
+
+
 def device_count() -> int:
     n = C.c_int(0)
     rc = load().lsqt_device_count(C.byref(n))
@@ -140,6 +160,7 @@ class Context:
     def __init__(self, device: int = 0):
         self.h = _vp()
         _check(load().lsqt_ctx_create(device, C.byref(self.h)))
+        _live["ctx"].add(self)
 
     @property
     def stream(self):
@@ -171,6 +192,7 @@ class Vector:
         else:
             self.h = _vp()
             _check(load().lsqt_vec_create(ctx.h, n, C.byref(self.h)))
+        _live["data"].add(self)
 
     @classmethod
     def from_host(cls, ctx, re, im):
@@ -230,6 +252,7 @@ class Block:
         self.ctx, self.n, self.R = ctx, n, R
         self.h = _vp()
         _check(load().lsqt_block_create(ctx.h, n, R, C.byref(self.h)))
+        _live["data"].add(self)
 
     def upload(self, re, im):
         """re, im: arrays of shape (R, n)."""
@@ -281,6 +304,7 @@ class Hamiltonian:
                                                     int(max_neighbor), _iptr(nn), _iptr(nl),
                                                     *(_dptr(a) for a in arrs),
                                                     float(energy_max), C.byref(self.h)))
+        _live["ham"].add(self)
 
     @classmethod
     def from_lattice(cls, ctx, lattice: dict, potential, energy_max):
@@ -314,6 +338,7 @@ class Hamiltonian:
         self.h = _vp()
         _check(load().lsqt_ham_create_lattice(ctx.h, C.byref(L), _dptr(pot), float(energy_max),
                                               C.byref(self.h)))
+        _live["ham"].add(self)
         return self
 
     @classmethod
@@ -430,16 +455,20 @@ class Hamiltonian:
         return self._block_driver(load().lsqt_block_msd, re, im, M, time_steps_scaled)
 
     def block_tune(self, left, right, force=False):
-        """(W, rows, blocked, us per step) of the autotuned block kernel shape"""
-        W, rows, blocked, us = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        """(W, rows, blocked, us per step) of the autotuned block kernel shape
+        (the depth is in block_depth after the call)"""
+        W, rows, blocked, depth, us = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
         _check(load().lsqt_block_tune(self.h, left.h, right.h, 1 if force else 0, C.byref(W),
-                                      C.byref(rows), C.byref(blocked), C.byref(us)))
+                                      C.byref(rows), C.byref(blocked), C.byref(depth),
+                                      C.byref(us)))
+        self.block_depth = depth.value
         return W.value, rows.value, blocked.value, us.value
 
-    def block_set_config(self, R, W=0, rows=0, blocked=0):
+    def block_set_config(self, R, W=0, rows=0, blocked=0, depth=1):
         """blocked: 0 natural tiles, 1 (or True) blocked, 2 blocked with the
-        column walk"""
-        _check(load().lsqt_block_set_config(self.h, int(R), int(W), int(rows), int(blocked)))
+        column walk; depth: passes in flight per wave (1 or 2)"""
+        _check(load().lsqt_block_set_config(self.h, int(R), int(W), int(rows), int(blocked),
+                                            int(depth)))
 
     def close(self):
         if self.h:

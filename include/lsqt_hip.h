@@ -271,6 +271,7 @@ int lsqt_block_set_graphs(lsqt_ham* h, int enable);
  * class, lattice slab), also in the cache file $LSQT_TUNE_CACHE (default
  * ~/.cache/lsqt_hip_tune.txt).  force re-tunes; the choice and its time per
  * step are returned.
+ * Shapes also differ in depth (1 or 2 passes in flight per wave).
  * Tile schedules (blocked = 0, 1, 2): 0 natural (tile t = sites [t*rows,
  * (t+1)*rows)); 1 blocked, for Hamiltonians whose neighbour offsets show a
  * lattice of slabs (sites [k*slab, (k+1)*slab) for the slowest lattice index
@@ -280,11 +281,13 @@ int lsqt_block_set_graphs(lsqt_ham* h, int enable);
  * column walk: a tile of whole rows is swept one chunk of every row at a time,
  * so that neighbours one row apart are also read close together. */
 int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right, int force,
-                    int* W, int* rows, int* blocked, double* us_per_step);
+                    int* W, int* rows, int* blocked, int* depth, double* us_per_step);
 /* fixes the shape for R on this Hamiltonian (W = 0 returns to the tuner);
  * blocked != 0 needs a detected slab that rows divides, blocked = 2 also
- * tiles of whole rows that passes of 256*W/R sites divide */
-int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int blocked);
+ * tiles of whole rows that passes of 256*W/R sites divide; depth = passes in
+ * flight per wave (2: the loads of the next pass are issued before the
+ * current one is computed; used for max_neighbor 3, 4, 6) */
+int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int blocked, int depth);
 
 /* the number of Chebyshev terms evolve/evolvex use for time_step_scaled */
 int lsqt_bessel_terms(double time_step_scaled, int* last_m);

@@ -32,6 +32,10 @@ def _block_of(ctx, m, R):
     return b, re, im
 
 
+def scale_of(mu):
+    return np.abs(mu).max()
+
+
 def _row(m, mu):
     from gpuqt_amd import hostlib
     return hostlib.correlation_row(mu, m.energy, m.energy_max, m.volume)
@@ -60,10 +64,11 @@ def test_block_shapes_agree(gpu_ctx, R):
     results = {}
     for W in (1, 2, 4, 8):
         for rows in (256, 1024, 4096):
-            H.block_set_config(R, W, rows)
-            results[(W, rows)] = H.moments_block(b, b, 64)
+            for depth in (1, 2):
+                H.block_set_config(R, W, rows, depth=depth)
+                results[(W, rows, depth)] = H.moments_block(b, b, 64)
     H.block_set_config(R, 0)
-    ref = results[(1, 256)]
+    ref = results[(1, 256, 1)]
     for k, mu in results.items():
         np.testing.assert_allclose(mu, ref, rtol=0, atol=1e-12 * np.abs(ref).max(),
                                    err_msg=str(k))
@@ -88,6 +93,12 @@ def test_block_blocked_schedule(gpu_ctx, L, R):
     H.block_set_config(R, 1, 256)
     ref = H.moments_block(b, b, M)
     tol = 1e-12 * np.abs(ref).max()
+    for depth in (1, 2):
+        H.block_set_config(R, 2, 256, depth=depth)
+        np.testing.assert_allclose(H.moments_block(b, b, M), ref, rtol=0, atol=tol,
+                                   err_msg=f"depth {depth}")
+        np.testing.assert_allclose(H.moments_block_self(b, M), ref, rtol=0, atol=1e-5 * scale_of(ref),
+                                   err_msg=f"self, depth {depth}")
     walked = 0
     for W in (1, 2, 4, 8):
         for rows in (64, 128, 256, 512, 1024):

@@ -246,3 +246,23 @@ def test_streaming_path_large(gpu_ctx, tmp_path, layout):
     H.apply_current(v, out)
     want = m.apply_current(*r)
     assert_bits(out.download()[0], want[0], "current")
+
+
+def test_context_outlives_its_objects():
+    """A context refuses destruction while a Hamiltonian, vector or block
+    uses it (no handle may outlive its device stream); it goes once they are
+    destroyed."""
+    import oracle
+    from gpuqt_amd import native
+    lib = native.load()
+    m = oracle.OracleModel(os.path.join(CASES_DIR, "honeycomb_small"))
+    ctx = native.Context(0)
+    H = native.Hamiltonian.from_arrays(ctx, m.arrays(), m.n, m.max_neighbor, m.energy_max)
+    v = H.vector()
+    b = native.Block(ctx, m.n, 4)
+    for obj in (b, v, H):
+        assert lib.lsqt_ctx_destroy(ctx.h) == 1  # LSQT_ERR_INVALID
+        assert "still use this context" in lib.lsqt_last_error().decode()
+        obj.close()
+    assert lib.lsqt_ctx_destroy(ctx.h) == 0
+    ctx.h = native._vp()
