@@ -553,8 +553,8 @@ def main():
                          "hardware-counter profiles) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path has no "
-              f"multi-GPU mode in this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the LSQT path "
+              f"here has no multi-GPU mode", file=sys.stderr)
         sys.exit(2)
 
     from gpuqt_amd import hostlib, native
