@@ -264,18 +264,44 @@ extern "C" int lsqt_block_destroy(lsqt_block* b)
   return LSQT_OK;
 }
 
+namespace {
+
+// a device buffer of `bytes` for one layout change, freed by the destructor
+struct Staging {
+  void* p = nullptr;
+  ~Staging()
+  {
+    if (p)
+      (void)hipFree(p);
+  }
+  int alloc(size_t bytes)
+  {
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+      return set_error(LSQT_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    return LSQT_OK;
+  }
+};
+
+}  // namespace
+
 extern "C" int lsqt_block_from_host(lsqt_block* b, const double* re, const double* im)
 {
   if (!b || !re || !im)
     return set_error(LSQT_ERR_INVALID, "null argument");
   LSQT_TRY(ctx_activate(b->ctx));
-  const size_t n = (size_t)b->n, R = (size_t)b->R;
-  std::vector<float2> tmp(n * R);
-  for (size_t r = 0; r < R; ++r)
-    for (size_t i = 0; i < n; ++i)
-      tmp[i * R + r] = make_float2((float)re[r * n + i], (float)im[r * n + i]);
-  LSQT_HIP(hipStreamSynchronize(b->ctx->stream));
-  LSQT_HIP(hipMemcpy(b->d, tmp.data(), sizeof(float2) * n * R, hipMemcpyHostToDevice));
+  const size_t count = (size_t)b->n * b->R;
+  Staging tmp;
+  LSQT_TRY(tmp.alloc(2 * sizeof(double) * count));
+  double* d_re = (double*)tmp.p;
+  double* d_im = d_re + count;
+  hipStream_t st = b->ctx->stream;
+  LSQT_HIP(hipStreamSynchronize(st));
+  LSQT_HIP(hipMemcpy(d_re, re, sizeof(double) * count, hipMemcpyHostToDevice));
+  LSQT_HIP(hipMemcpy(d_im, im, sizeof(double) * count, hipMemcpyHostToDevice));
+  launch_site_major(d_re, d_im, b->n, b->R, b->d, st);
+  LSQT_HIP(hipGetLastError());
+  LSQT_HIP(hipStreamSynchronize(st));
   return LSQT_OK;
 }
 
@@ -284,15 +310,17 @@ extern "C" int lsqt_block_to_host(const lsqt_block* b, double* re, double* im)
   if (!b || !re || !im)
     return set_error(LSQT_ERR_INVALID, "null argument");
   LSQT_TRY(ctx_activate(b->ctx));
-  const size_t n = (size_t)b->n, R = (size_t)b->R;
-  std::vector<float2> tmp(n * R);
-  LSQT_HIP(hipStreamSynchronize(b->ctx->stream));
-  LSQT_HIP(hipMemcpy(tmp.data(), b->d, sizeof(float2) * n * R, hipMemcpyDeviceToHost));
-  for (size_t r = 0; r < R; ++r)
-    for (size_t i = 0; i < n; ++i) {
-      re[r * n + i] = tmp[i * R + r].x;
-      im[r * n + i] = tmp[i * R + r].y;
-    }
+  const size_t count = (size_t)b->n * b->R;
+  Staging tmp;
+  LSQT_TRY(tmp.alloc(2 * sizeof(double) * count));
+  double* d_re = (double*)tmp.p;
+  double* d_im = d_re + count;
+  hipStream_t st = b->ctx->stream;
+  launch_vector_major(b->d, b->n, b->R, d_re, d_im, st);
+  LSQT_HIP(hipGetLastError());
+  LSQT_HIP(hipStreamSynchronize(st));
+  LSQT_HIP(hipMemcpy(re, d_re, sizeof(double) * count, hipMemcpyDeviceToHost));
+  LSQT_HIP(hipMemcpy(im, d_im, sizeof(double) * count, hipMemcpyDeviceToHost));
   return LSQT_OK;
 }
 
@@ -724,15 +752,20 @@ void block_evolvex(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, B
 }
 
 // host vectors (vector-major [R][n]) into a site-major FP64 block
-int upload_states(lsqt_ham* h, int R, const double* re, const double* im, double2* S)
+// host vectors (vector-major [R][n]) into a site-major FP64 block, through
+// `staging` (n*R double2 of device memory, overwritten)
+int upload_states(lsqt_ham* h, int R, const double* re, const double* im, double2* S,
+                  double2* staging)
 {
-  const size_t n = (size_t)h->d.n;
-  std::vector<double2> tmp(n * R);
-  for (size_t r = 0; r < (size_t)R; ++r)
-    for (size_t i = 0; i < n; ++i)
-      tmp[i * R + r] = make_double2(re[r * n + i], im[r * n + i]);
-  LSQT_HIP(hipStreamSynchronize(h->ctx->stream));
-  LSQT_HIP(hipMemcpy(S, tmp.data(), sizeof(double2) * n * R, hipMemcpyHostToDevice));
+  const size_t count = (size_t)h->d.n * R;
+  double* d_re = (double*)staging;
+  double* d_im = d_re + count;
+  hipStream_t st = h->ctx->stream;
+  LSQT_HIP(hipStreamSynchronize(st));
+  LSQT_HIP(hipMemcpy(d_re, re, sizeof(double) * count, hipMemcpyHostToDevice));
+  LSQT_HIP(hipMemcpy(d_im, im, sizeof(double) * count, hipMemcpyHostToDevice));
+  launch_site_major(d_re, d_im, h->d.n, R, S, st);
+  LSQT_HIP(hipGetLastError());
   return LSQT_OK;
 }
 
@@ -764,7 +797,7 @@ extern "C" int lsqt_block_vac(lsqt_ham* h, int R, const double* re, const double
   LSQT_TRY(evo_scratch(h, R, s));
   double2* left = s->S[0];   // U(-t) phi
   double2* right = s->S[1];  // U(-t) V phi
-  LSQT_TRY(upload_states(h, R, re, im, left));
+  LSQT_TRY(upload_states(h, R, re, im, left, s->S[2]));
   hipStream_t st = h->ctx->stream;
   const int nR = (int)((size_t)h->d.n * R);
   float2* L = s->F[0];
@@ -808,7 +841,7 @@ extern "C" int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double
   double2* state = s->S[0];
   double2* state_x = s->S[1];
   double2* state_copy = s->S[2];
-  LSQT_TRY(upload_states(h, R, re, im, state));
+  LSQT_TRY(upload_states(h, R, re, im, state, s->S[2]));
   hipStream_t st = h->ctx->stream;
   const int nR = (int)((size_t)h->d.n * R);
   launch_to_f32(state, s->F[0], nR, st);

@@ -731,6 +731,37 @@ __global__ __launch_bounds__(kBlock) void k_block_renorm(double2* __restrict__ S
   }
 }
 
+// Layout changes between the host's vector-major arrays (vector r at
+// [r*n, (r+1)*n)) and site-major blocks ([n][R]): one thread per value,
+// coalesced on the vector-major side.
+template <class T2>
+__global__ __launch_bounds__(kBlock) void k_site_major(const double* __restrict__ re,
+                                                      const double* __restrict__ im, int n,
+                                                      int R, T2* __restrict__ dst)
+{
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n * R)
+    return;
+  const int r = idx / n, i = idx - r * n;
+  T2 v;
+  v.x = re[idx];
+  v.y = im[idx];
+  dst[(size_t)i * R + r] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_vector_major(const float2* __restrict__ src, int n,
+                                                        int R, double* __restrict__ re,
+                                                        double* __restrict__ im)
+{
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n * R)
+    return;
+  const int r = idx / n, i = idx - r * n;
+  const float2 v = src[(size_t)i * R + r];
+  re[idx] = v.x;
+  im[idx] = v.y;
+}
+
 // ==========================================================================
 // launchers
 // ==========================================================================
@@ -909,6 +940,27 @@ void launch_block_renorm(double2* S, int R, const TileMap& tm, const double2* be
 {
   Grid g = block_grid(tm);
   k_block_renorm<<<g.blocks, kBlock, 0, st>>>(S, R, tm, before, after, g.tiles, g.blocks);
+}
+
+void launch_site_major(const double* re, const double* im, int n, int R, double2* dst,
+                       hipStream_t st)
+{
+  const int total = n * R;
+  k_site_major<double2><<<(total + kBlock - 1) / kBlock, kBlock, 0, st>>>(re, im, n, R, dst);
+}
+
+void launch_site_major(const double* re, const double* im, int n, int R, float2* dst,
+                       hipStream_t st)
+{
+  const int total = n * R;
+  k_site_major<float2><<<(total + kBlock - 1) / kBlock, kBlock, 0, st>>>(re, im, n, R, dst);
+}
+
+void launch_vector_major(const float2* src, int n, int R, double* re, double* im,
+                         hipStream_t st)
+{
+  const int total = n * R;
+  k_vector_major<<<(total + kBlock - 1) / kBlock, kBlock, 0, st>>>(src, n, R, re, im);
 }
 
 }  // namespace lsqt

@@ -75,6 +75,14 @@ void launch_block_norm64(const double2* S, int R, const TileMap& tm, double2* pa
 // S_r *= sqrt(before[k][r] / after[k][r]) in segment k
 void launch_block_renorm(double2* S, int R, const TileMap& tm, const double2* before,
                          const double2* after, hipStream_t st);
+// vector-major host layout ([R][n] re and im) <-> site-major blocks ([n][R]);
+// n * R < 2^31
+void launch_site_major(const double* re, const double* im, int n, int R, double2* dst,
+                       hipStream_t st);
+void launch_site_major(const double* re, const double* im, int n, int R, float2* dst,
+                       hipStream_t st);
+void launch_vector_major(const float2* src, int n, int R, double* re, double* im,
+                         hipStream_t st);
 // mu[(k*R + r)*M + m0 + i] for segment k, i < count (partials of moment
 // m0 + i at slot i)
 void launch_block_reduce(const double2* partial, const TileMap& tm, int R, int M, int m0,
