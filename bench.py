@@ -217,9 +217,9 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     # measured HBM bytes per launch, where the profile covers this workload
     profiled = L == 160 and R == 32
     db = "true" if getattr(H, "block_depth", 1) == 2 else "false"
-    traffic_left = (profiled_traffic(f"k_block_step<1,6,{W},true,1,true,{db}>")
+    traffic_left = (profiled_traffic(f"k_block_step<1,6,{W},true,1,false,{db}>")
                     if profiled else None)
-    traffic_self = (profiled_traffic(f"k_block_step<1,6,{W},true,2,true,{db}>")
+    traffic_self = (profiled_traffic(f"k_block_step<1,6,{W},true,2,false,{db}>")
                     if profiled else None)
     steps_self = M // 2
     us_ = n * R * steps_self / ts          # recursion site-updates/s
