@@ -446,9 +446,11 @@ def probe(ctx, H, rv, args, moments=10):
         ph = np.random.default_rng(7).uniform(0, 2 * np.pi, (R, n))
         blk = native.Block(ctx, n, R)
         blk.upload(np.cos(ph), np.sin(ph))
-        Hc.block_tune(blk, blk)
-        Hc.moments_block(blk, blk, moments)
-        Hc.moments_block_self(blk, 2 * moments)
+        W, rows, blocked, _ = Hc.block_tune(blk, blk)
+        for depth in (1, 2):  # both loop depths, whichever the tuner picks
+            Hc.block_set_config(R, W, rows, blocked=blocked, depth=depth)
+            Hc.moments_block(blk, blk, moments)
+            Hc.moments_block_self(blk, 2 * moments)
         ctx.sync()
         log(f"probe: block R={R} moments on {n} sites")
 
