@@ -5,6 +5,7 @@
 #include <fstream>
 #include <iostream>
 #include <string>
+#include <vector>
 
 #include "general_io.h"
 #include "lsqt.h"
@@ -89,6 +90,40 @@ extern "C" int lsqt_host_main(int argc, char** argv)
     return 1;
   }
   std::string directory;
+  lsqt_host::Path path;
+  try {
+    path = lsqt_host::path_from_environment();
+  } catch (const lsqt_host::InputError& e) {
+    std::cout << e.what() << std::endl;
+    return 1;
+  }
+  if (path == lsqt_host::Path::batch) {
+    std::vector<std::string> directories;
+    while (std::getline(input, directory))
+      if (directory != "")
+        directories.push_back(directory);
+    std::cout << std::endl;
+    std::cout << "===============================================================\n";
+    std::cout << "Run LSQT simulation for " << directories.size()
+              << " directories together (batch path)" << std::endl;
+    std::cout << "===============================================================\n";
+    auto t0 = std::chrono::steady_clock::now();
+    try {
+      lsqt_batch(directories);
+    } catch (const lsqt_host::InputError& e) {
+      std::cout << e.what() << std::endl;
+      return 1;
+    }
+    double used =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::cout << std::endl;
+    std::cout << "===============================================================\n";
+    std::cout << "Total time used for " << directories.size() << " directories = " << used
+              << " s" << std::endl;
+    std::cout << "===============================================================\n";
+    lsqt_host::release_default_context();
+    return 0;
+  }
   while (std::getline(input, directory)) {
     if (directory == "")
       continue;

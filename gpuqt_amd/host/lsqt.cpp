@@ -4,10 +4,14 @@
 #include "lsqt.h"
 
 #include <chrono>
+#include <algorithm>
+#include <climits>
 #include <iostream>
+#include <memory>
 #include <vector>
 
 #include "hamiltonian.h"
+#include "lsqt_hip.h"
 #include "model.h"
 #include "sigma.h"
 #include "vector.h"
@@ -56,7 +60,9 @@ void lsqt(std::string input_directory)
   Model model(input_directory);
   Hamiltonian H(model);
   Vector random_state(model.number_of_atoms);
-  if (lsqt_host::path() == lsqt_host::Path::block && model.number_of_random_vectors > 0) {
+  const bool block = lsqt_host::path() == lsqt_host::Path::block ||
+                     lsqt_host::path() == lsqt_host::Path::batch;
+  if (block && model.number_of_random_vectors > 0) {
     // every random vector of the directory at once; the RNG draws are the
     // same as vector by vector (nothing else draws in between)
     const int used = model.number_of_random_vectors;
@@ -70,7 +76,8 @@ void lsqt(std::string input_directory)
     for (int i = 0; i < used; ++i)
       model.initialize_state_host(re.data() + i * n, im.data() + i * n, -1);
     banner("Started  simulation with " + std::to_string(used) + " random vectors (block path)");
-    lsqt_host::find_block_quantities(model, H, R, used, re, im);
+    std::vector<lsqt_host::BlockMember> members{{&model, 0, used}};
+    lsqt_host::find_block_quantities(members, H.handle(), R, re, im);
     if (model.calculate_spin) {
       timed("spin polarization", [&] {
         for (int i = 0; i < used; ++i) {
@@ -111,4 +118,130 @@ void lsqt_vectors(Model& model, Hamiltonian& H, Vector& random_state)
       timed("spin polarization", [&] { find_spin_polarization(model, H, random_state); });
     banner_end("Finished simulation with random vector number " + std::to_string(i));
   }
+}
+
+namespace {
+
+int pow2_at_least(int v)
+{
+  int R = 1;
+  while (R < v)
+    R *= 2;
+  return R;
+}
+
+// settings that systems of one batch must share
+std::vector<double> batch_key(const Model& m)
+{
+  std::vector<double> k = {(double)m.number_of_moments, m.energy_max,
+                           (double)m.number_of_steps_correlation,
+                           (double)m.calculate_vac0, (double)m.calculate_vac,
+                           (double)m.calculate_msd,
+                           (double)pow2_at_least(m.number_of_random_vectors)};
+  for (int t = 0; t < m.number_of_steps_correlation; ++t)
+    k.push_back(m.time_step[t]);
+  return k;
+}
+
+// one segmented Hamiltonian for the members' models (arrays released after)
+lsqt_ham* segmented_hamiltonian(std::vector<lsqt_host::BlockMember>& members, int n_total)
+{
+  int K = 0;
+  for (auto& mb : members)
+    K = std::max(K, mb.model->max_neighbor);
+  std::vector<int> nn((size_t)n_total), nl((size_t)n_total * K, 0), starts;
+  std::vector<real> pot((size_t)n_total), hr((size_t)n_total * K, 0.0),
+    hi((size_t)n_total * K, 0.0), xx((size_t)n_total * K, 0.0);
+  for (auto& mb : members) {
+    Model& m = *mb.model;
+    starts.push_back(mb.start);
+    for (int i = 0; i < m.number_of_atoms; ++i) {
+      const size_t g = (size_t)(mb.start + i);
+      nn[g] = m.neighbor_number[i];
+      pot[g] = m.potential[i];
+      for (int q = 0; q < m.neighbor_number[i]; ++q) {
+        const size_t src = (size_t)i * m.max_neighbor + q, dst = g * K + q;
+        nl[dst] = m.neighbor_list[src] + mb.start;
+        hr[dst] = m.hopping_real[src];
+        hi[dst] = m.hopping_imag[src];
+        xx[dst] = m.xx[src];
+      }
+    }
+  }
+  starts.push_back(n_total);
+  lsqt_ham* h = nullptr;
+  LSQT_CHECK(lsqt_ham_create_segmented(lsqt_host::default_context(), (int)members.size(),
+                                       starts.data(), n_total, K, nn.data(), nl.data(),
+                                       pot.data(), hr.data(), hi.data(), xx.data(),
+                                       members[0].model->energy_max, &h));
+  for (auto& mb : members)
+    mb.model->release_hamiltonian_arrays();
+  return h;
+}
+
+void run_batch(std::vector<lsqt_host::BlockMember>& members, int n_total, int R)
+{
+  banner("Started  simulation of " + std::to_string(members.size()) +
+         " systems together (batch path, " + std::to_string(n_total) + " sites)");
+  lsqt_ham* h = segmented_hamiltonian(members, n_total);
+  // every system's random vectors, drawn in its own RNG order
+  std::vector<real> re((size_t)n_total * R, 0.0), im((size_t)n_total * R, 0.0);
+  for (auto& mb : members)
+    for (int r = 0; r < mb.used; ++r)
+      mb.model->initialize_state_host(re.data() + (size_t)r * n_total + mb.start,
+                                      im.data() + (size_t)r * n_total + mb.start, -1);
+  lsqt_host::find_block_quantities(members, h, R, re, im);
+  lsqt_ham_destroy(h);
+  banner_end("Finished simulation of " + std::to_string(members.size()) + " systems");
+}
+
+}  // namespace
+
+void lsqt_batch(const std::vector<std::string>& directories)
+{
+  lsqt_host::set_path(lsqt_host::Path::batch);
+  std::vector<std::unique_ptr<Model>> models;
+  std::vector<std::string> alone;
+  for (const std::string& d : directories) {
+    std::unique_ptr<Model> m(new Model(d));
+    if (m->calculate_spin || m->calculate_ldos || m->number_of_random_vectors <= 0 ||
+        m->number_of_random_vectors > 256)
+      alone.push_back(d);  // run by itself below
+    else
+      models.push_back(std::move(m));
+  }
+  // systems with the same settings, in driver-file order within each group
+  std::vector<std::vector<double>> keys;
+  std::vector<std::vector<Model*>> groups;
+  for (auto& m : models) {
+    auto k = batch_key(*m);
+    size_t g = 0;
+    while (g < keys.size() && keys[g] != k)
+      ++g;
+    if (g == keys.size()) {
+      keys.push_back(k);
+      groups.emplace_back();
+    }
+    groups[g].push_back(m.get());
+  }
+  for (auto& group : groups) {
+    const int R = pow2_at_least(group[0]->number_of_random_vectors);
+    std::vector<lsqt_host::BlockMember> members;
+    long long n_total = 0;
+    for (Model* m : group) {
+      // one launch covers at most 2^31 values of a block
+      if (!members.empty() && (n_total + m->number_of_atoms) * R > INT_MAX) {
+        run_batch(members, (int)n_total, R);
+        members.clear();
+        n_total = 0;
+      }
+      members.push_back({m, (int)n_total, m->number_of_random_vectors});
+      n_total += m->number_of_atoms;
+    }
+    if (!members.empty())
+      run_batch(members, (int)n_total, R);
+  }
+  models.clear();
+  for (const std::string& d : alone)
+    lsqt(d);
 }

@@ -49,7 +49,9 @@ Path path_from_environment()
     return Path::f32;
   if (std::string(e) == "block")
     return Path::block;
-  throw InputError(std::string("Error: LSQT_PATH must be f64, f32 or block, not ") + e);
+  if (std::string(e) == "batch")
+    return Path::batch;
+  throw InputError(std::string("Error: LSQT_PATH must be f64, f32, block or batch, not ") + e);
 }
 }  // namespace lsqt_host
 
@@ -281,68 +283,84 @@ struct BlockHandle {
 
 }  // namespace
 
-void find_block_quantities(Model& model, Hamiltonian& H, int R, int used,
+void find_block_quantities(std::vector<BlockMember>& members, lsqt_ham* h, int R,
                            const std::vector<real>& re, const std::vector<real>& im)
 {
-  const int M = model.number_of_moments;
-  const int n = model.number_of_atoms;
-  const int Nt = model.number_of_steps_correlation;
+  Model& first = *members[0].model;
+  const int M = first.number_of_moments;
+  const int Nt = first.number_of_steps_correlation;
+  const int S = (int)members.size();
+  lsqt_ham_info info;
+  LSQT_CHECK(lsqt_ham_get_info(h, &info));
+  const int n = info.n;
+  int used = 0;
+  for (auto& mb : members)
+    used += mb.used;
   auto t0 = std::chrono::steady_clock::now();
   auto lap = [&](const char* what) {
     auto t1 = std::chrono::steady_clock::now();
-    std::cout << "- Time used for finding " << what << " of " << used
-              << " random vectors = " << std::chrono::duration<double>(t1 - t0).count() << " s"
-              << std::endl;
+    std::cout << "- Time used for finding " << what << " of " << used << " random vectors in "
+              << S << (S == 1 ? " system" : " systems") << " = "
+              << std::chrono::duration<double>(t1 - t0).count() << " s" << std::endl;
     t0 = t1;
   };
-  std::vector<real> mu_re((size_t)R * M), mu_im((size_t)R * M);
+  // the rows of member k from moment sets mu[(k*R + r)*M + m]
+  auto write_rows = [&](const char* name, std::vector<real>& mu_re, std::vector<real>& mu_im) {
+    for (int k = 0; k < S; ++k) {
+      Model& model = *members[k].model;
+      std::vector<real> sr(mu_re.begin() + (size_t)k * R * M,
+                           mu_re.begin() + (size_t)(k + 1) * R * M);
+      std::vector<real> si(mu_im.begin() + (size_t)k * R * M,
+                           mu_im.begin() + (size_t)(k + 1) * R * M);
+      auto rows = rows_of(model, sr, si, members[k].used);
+      std::ofstream output = open_output(model, name);
+      for (auto& row : rows)
+        write_row(output, row);
+    }
+  };
+  std::vector<real> mu_re((size_t)S * R * M), mu_im((size_t)S * R * M);
   BlockHandle phi, tmp;
   LSQT_CHECK(lsqt_block_create(default_context(), n, R, &phi.b));
   LSQT_CHECK(lsqt_block_from_host(phi.b, re.data(), im.data()));
   // DOS: self products (src/sigma.cu:204-239)
-  LSQT_CHECK(lsqt_moments_block_self(H.handle(), phi.b, M, mu_re.data(), mu_im.data()));
-  {
-    auto rows = rows_of(model, mu_re, mu_im, used);
-    std::ofstream output = open_output(model, "/dos.out");
-    for (auto& row : rows)
-      write_row(output, row);
-  }
+  LSQT_CHECK(lsqt_moments_block_self(h, phi.b, M, mu_re.data(), mu_im.data()));
+  write_rows("/dos.out", mu_re, mu_im);
   lap("DOS");
-  if (model.calculate_vac0) {  // <V phi|T_m|V phi> (src/sigma.cu:243-274)
+  if (first.calculate_vac0) {  // <V phi|T_m|V phi> (src/sigma.cu:243-274)
     LSQT_CHECK(lsqt_block_create(default_context(), n, R, &tmp.b));
-    LSQT_CHECK(lsqt_block_apply(H.handle(), 2, phi.b, tmp.b));
-    LSQT_CHECK(lsqt_moments_block_self(H.handle(), tmp.b, M, mu_re.data(), mu_im.data()));
-    auto rows = rows_of(model, mu_re, mu_im, used);
-    std::ofstream output = open_output(model, "/vac0.out");
-    for (auto& row : rows)
-      write_row(output, row);
+    LSQT_CHECK(lsqt_block_apply(h, 2, phi.b, tmp.b));
+    LSQT_CHECK(lsqt_moments_block_self(h, tmp.b, M, mu_re.data(), mu_im.data()));
+    write_rows("/vac0.out", mu_re, mu_im);
     lap("VAC0");
   }
-  if ((model.calculate_vac || model.calculate_msd) && Nt > 0) {
+  if ((first.calculate_vac || first.calculate_msd) && Nt > 0) {
     std::vector<real> steps((size_t)Nt);
     for (int t = 0; t < Nt; ++t)
-      steps[t] = model.time_step[t] * model.energy_max;
-    std::vector<real> all_re((size_t)Nt * R * M), all_im((size_t)Nt * R * M);
+      steps[t] = first.time_step[t] * first.energy_max;
+    const size_t per_step = (size_t)S * R * M;
+    std::vector<real> all_re((size_t)Nt * per_step), all_im((size_t)Nt * per_step);
     for (int q = 0; q < 2; ++q) {
       const bool vac = q == 0;
-      if (!(vac ? model.calculate_vac : model.calculate_msd))
+      if (!(vac ? first.calculate_vac : first.calculate_msd))
         continue;
-      LSQT_CHECK((vac ? lsqt_block_vac : lsqt_block_msd)(H.handle(), R, re.data(), im.data(), M,
-                                                         Nt, steps.data(), all_re.data(),
+      LSQT_CHECK((vac ? lsqt_block_vac : lsqt_block_msd)(h, R, re.data(), im.data(), M, Nt,
+                                                         steps.data(), all_re.data(),
                                                          all_im.data()));
-      std::ofstream output = open_output(model, vac ? "/vac.out" : "/msd.out");
-      // the reference writes vector by vector, each with its Nt rows
-      std::vector<std::vector<std::vector<real>>> by_step;
-      for (int t = 0; t < Nt; ++t) {
-        std::vector<real> sr(all_re.begin() + (size_t)t * R * M,
-                             all_re.begin() + (size_t)(t + 1) * R * M);
-        std::vector<real> si(all_im.begin() + (size_t)t * R * M,
-                             all_im.begin() + (size_t)(t + 1) * R * M);
-        by_step.push_back(rows_of(model, sr, si, used));
+      for (int k = 0; k < S; ++k) {
+        Model& model = *members[k].model;
+        // the reference writes vector by vector, each with its Nt rows
+        std::vector<std::vector<std::vector<real>>> by_step;
+        for (int t = 0; t < Nt; ++t) {
+          const size_t at = (size_t)t * per_step + (size_t)k * R * M;
+          std::vector<real> sr(all_re.begin() + at, all_re.begin() + at + (size_t)R * M);
+          std::vector<real> si(all_im.begin() + at, all_im.begin() + at + (size_t)R * M);
+          by_step.push_back(rows_of(model, sr, si, members[k].used));
+        }
+        std::ofstream output = open_output(model, vac ? "/vac.out" : "/msd.out");
+        for (int r = 0; r < members[k].used; ++r)
+          for (int t = 0; t < Nt; ++t)
+            write_row(output, by_step[t][r]);
       }
-      for (int r = 0; r < used; ++r)
-        for (int t = 0; t < Nt; ++t)
-          write_row(output, by_step[t][r]);
       lap(vac ? "VAC" : "MSD");
     }
   }

@@ -10,6 +10,7 @@
 class Model;
 class Hamiltonian;
 class Vector;
+struct lsqt_ham;
 
 void find_dos(Model&, Hamiltonian&, Vector&, int);
 void find_vac0(Model&, Hamiltonian&, Vector&);
@@ -37,21 +38,36 @@ void set_output_precision(int digits);
 int output_precision();
 
 // Which device path the drivers use (environment LSQT_PATH):
+//   batch all directories of the driver file with the same moments, time
+//         steps, energy_max, flags and vector count at once, as independent
+//         systems of one segmented Hamiltonian on the block path (others,
+//         and those asking for spin or LDOS, run one by one on the block
+//         path);
 //   f64   (default) the reference's arithmetic, bit-faithful states;
 //   f32   FP32-stored recursion states, FP64 arithmetic, moments and
 //         re-normalised evolved states (lsqt_moments_f32, lsqt_evolve_f32);
 //   block all random vectors of a directory at once (lsqt_moments_block_self,
 //         lsqt_block_vac / msd): FP32 recursion, FP64 moments and evolved
 //         states; the per-vector leftovers (spin, LDOS) use f32.
-enum class Path { f64, f32, block };
+enum class Path { f64, f32, block, batch };
 Path path();
 void set_path(Path p);
 Path path_from_environment();
 
-// The block path's drivers for one directory: DOS, VAC0, VAC and MSD of R
-// vectors (host arrays [R][n]; the first `used` are the random vectors, the
-// rest zero padding up to a power of 2), appending rows to the .out files in
-// the reference's order (vector by vector; Nt rows per vector for VAC/MSD).
-void find_block_quantities(Model& model, Hamiltonian& H, int R, int used,
+// One system of a block run: its Model, whose sites are [start, start + n)
+// of the (possibly segmented) Hamiltonian, and its number of random vectors.
+struct BlockMember {
+  Model* model;
+  int start;
+  int used;
+};
+// The block path's drivers: DOS, VAC0, VAC and MSD of R vectors for every
+// member at once (host arrays [R][n] over all members' sites; member k's
+// first `used` vectors are its random vectors, the rest zero padding), each
+// member's rows appended to its own .out files in the reference's order
+// (vector by vector; Nt rows per vector for VAC/MSD).  All members share the
+// moments, time steps, energy_max and flags of members[0].  h is segmented
+// with one segment per member when there is more than one.
+void find_block_quantities(std::vector<BlockMember>& members, lsqt_ham* h, int R,
                            const std::vector<real>& re, const std::vector<real>& im);
 }  // namespace lsqt_host

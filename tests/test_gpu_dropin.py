@@ -143,3 +143,70 @@ def test_unknown_path_is_an_input_error(tmp_path):
     driver.write_text(f"{work}\n")
     r = hostlib.run_cli(str(driver), timeout=120, env=dict(os.environ, LSQT_PATH="f16"))
     assert r.returncode == 1 and "LSQT_PATH" in r.stdout
+
+
+def test_batch_path_all_golden_cases(tmp_path):
+    """LSQT_PATH=batch with every golden case in one driver file: the
+    compatible ones run together, the others (spin, LDOS, other settings)
+    one by one; every directory's .out files hold the reference's rows
+    within 2e-5 (normwise)."""
+    cases = golden_cases()
+    dirs = []
+    for case in cases:
+        work = tmp_path / case
+        shutil.copytree(os.path.join(CASES_DIR, case), work,
+                        ignore=shutil.ignore_patterns("expected", "dump.npz"))
+        dirs.append(work)
+    driver = tmp_path / "input.txt"
+    driver.write_text("".join(f"{d}\n" for d in dirs))
+    print("[gpu] lsqt_gpu LSQT_PATH=batch, all golden cases", flush=True)
+    r = hostlib.run_cli(str(driver), digits=17, timeout=900,
+                        env=dict(os.environ, LSQT_PATH="batch"))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for case, work in zip(cases, dirs):
+        src = os.path.join(CASES_DIR, case)
+        expected = sorted(os.listdir(os.path.join(src, "expected")))
+        assert sorted(f for f in os.listdir(work) if f.endswith(".out")) == expected, case
+        for name in expected:
+            got, want = read_out(work / name), read_out(os.path.join(src, "expected", name))
+            assert got.shape == want.shape, f"{case}/{name}"
+            for k in range(want.shape[0]):
+                err = np.abs(got[k] - want[k]).max() / np.abs(want[k]).max()
+                assert err <= 2e-5, f"{case}/{name} row {k}: {err:.2e}"
+
+
+def test_batch_path_uneven_realisations(tmp_path):
+    """Honeycomb realisations of different sizes with the same settings (one
+    directory listed twice) through LSQT_PATH=batch against the default FP64
+    program on copies of the same directories."""
+    from gpuqt_amd import inputs as gi
+    dirs = []
+    for i, (nx, ny) in enumerate(((9, 5), (14, 7), (6, 12))):
+        d = tmp_path / f"r{i}"
+        d.mkdir()
+        gi.write_honeycomb(str(d), nx, ny)
+        gi.write_para(str(d), anderson=1.5, moments=96, energy_max=10.0, vac=True, msd=True,
+                      vac0=True, random_vectors=3)
+        gi.write_energy(str(d), gi.energy_grid(-8, 8, 41))
+        gi.write_time_steps(str(d), [0.5, 1.0, 0.75])
+        dirs.append(d)
+    listed = dirs + [dirs[1]]
+    copies = []
+    for d in dirs:
+        c = tmp_path / f"{d.name}_f64"
+        shutil.copytree(d, c)
+        copies.append(c)
+    for path, where in (("batch", listed), ("f64", copies + [copies[1]])):
+        driver = tmp_path / f"input_{path}.txt"
+        driver.write_text("".join(f"{d}\n" for d in where))
+        print(f"[gpu] lsqt_gpu LSQT_PATH={path}, 3 realisations", flush=True)
+        r = hostlib.run_cli(str(driver), digits=17, timeout=600,
+                            env=dict(os.environ, LSQT_PATH=path))
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for d, c in zip(dirs, copies):
+        for name in ("dos.out", "vac0.out", "vac.out", "msd.out"):
+            got, want = read_out(d / name), read_out(c / name)
+            assert got.shape == want.shape, f"{d.name}/{name}"
+            for k in range(want.shape[0]):
+                err = np.abs(got[k] - want[k]).max() / np.abs(want[k]).max()
+                assert err <= 2e-5, f"{d.name}/{name} row {k}: {err:.2e}"
