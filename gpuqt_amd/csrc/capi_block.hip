@@ -870,6 +870,41 @@ extern "C" int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double
   return LSQT_OK;
 }
 
+extern "C" int lsqt_block_spin(lsqt_ham* h, int R, const double* re, const double* im, int M,
+                               int n_steps, const double* time_step_scaled, double* mu_re,
+                               double* mu_im)
+{
+  LSQT_TRY(check_driver(h, R, re, im, M, n_steps, time_step_scaled, mu_re, mu_im));
+  if (!h->seg_starts.empty())
+    return set_error(LSQT_ERR_UNSUPPORTED, "the spin driver takes one system (S_z alternates "
+                     "over the sites of a system)");
+  BlockScratch* s;
+  LSQT_TRY(scratch_for(h, R, 2, 1, &s));
+  LSQT_TRY(evo_scratch(h, R, s));
+  double2* state = s->S[0];
+  LSQT_TRY(upload_states(h, R, re, im, state, s->S[2]));
+  hipStream_t st = h->ctx->stream;
+  const int nR = (int)((size_t)h->d.n * R);
+  float2* L = s->F[0];
+  launch_to_f32(state, L, nR, st);
+  BlockShape c;
+  TileMap tm;
+  LSQT_TRY(block_setup(h, R, L, L, M, &c, &tm, &s));
+  LSQT_TRY(evo_scratch(h, R, s));
+  const size_t per_step = (size_t)M * R;
+  for (int m = 0; m < n_steps; ++m) {
+    // <state|T_m|S_z state> (src/sigma.cu:395-442)
+    launch_to_f32(state, L, nR, st);
+    launch_block_sz(L, h->d.n, R, s->A, st);
+    LSQT_TRY(moment_set(h, R, c, tm, s, L, s->A, false, M, mu_re + m * per_step,
+                        mu_im + m * per_step));
+    if (m < n_steps - 1)
+      block_evolve(h, R, c, tm, s, 1, time_step_scaled[m], state, true);
+  }
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
 extern "C" int lsqt_block_apply(lsqt_ham* h, int op, const lsqt_block* in, lsqt_block* out)
 {
   LSQT_TRY(check_block(h, in, out));

@@ -762,6 +762,18 @@ __global__ __launch_bounds__(kBlock) void k_vector_major(const float2* __restric
   im[idx] = v.y;
 }
 
+// dst = S_z src for every vector: site i keeps its sign when i is even and
+// changes it when odd (src/vector.cu:159-172)
+__global__ __launch_bounds__(kBlock) void k_block_sz(const float2* __restrict__ src, int n, int R,
+                                                    float2* __restrict__ dst)
+{
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n * R)
+    return;
+  const float2 v = src[idx];
+  dst[idx] = ((idx / R) % 2 == 0) ? v : make_float2(-v.x, -v.y);
+}
+
 // ==========================================================================
 // launchers
 // ==========================================================================
@@ -961,6 +973,12 @@ void launch_vector_major(const float2* src, int n, int R, double* re, double* im
 {
   const int total = n * R;
   k_vector_major<<<(total + kBlock - 1) / kBlock, kBlock, 0, st>>>(src, n, R, re, im);
+}
+
+void launch_block_sz(const float2* src, int n, int R, float2* dst, hipStream_t st)
+{
+  const int total = n * R;
+  k_block_sz<<<(total + kBlock - 1) / kBlock, kBlock, 0, st>>>(src, n, R, dst);
 }
 
 }  // namespace lsqt

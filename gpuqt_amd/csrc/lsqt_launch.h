@@ -83,6 +83,8 @@ void launch_site_major(const double* re, const double* im, int n, int R, float2*
                        hipStream_t st);
 void launch_vector_major(const float2* src, int n, int R, double* re, double* im,
                          hipStream_t st);
+// dst = S_z src (sign of the odd sites flipped) for every vector of a block
+void launch_block_sz(const float2* src, int n, int R, float2* dst, hipStream_t st);
 // mu[(k*R + r)*M + m0 + i] for segment k, i < count (partials of moment
 // m0 + i at slot i)
 void launch_block_reduce(const double2* partial, const TileMap& tm, int R, int M, int m0,

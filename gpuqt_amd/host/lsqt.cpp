@@ -78,14 +78,6 @@ void lsqt(std::string input_directory)
     banner("Started  simulation with " + std::to_string(used) + " random vectors (block path)");
     std::vector<lsqt_host::BlockMember> members{{&model, 0, used}};
     lsqt_host::find_block_quantities(members, H.handle(), R, re, im);
-    if (model.calculate_spin) {
-      timed("spin polarization", [&] {
-        for (int i = 0; i < used; ++i) {
-          random_state.copy_from_host(re.data() + i * n, im.data() + i * n);
-          find_spin_polarization(model, H, random_state);
-        }
-      });
-    }
     banner_end("Finished simulation with " + std::to_string(used) + " random vectors");
   } else {
     lsqt_vectors(model, H, random_state);
@@ -93,6 +85,10 @@ void lsqt(std::string input_directory)
   if (model.calculate_ldos) {
     banner("Started LDOS calculation");
     timed("LDOS", [&] {
+      if (block) {
+        lsqt_host::find_block_ldos(model, H.handle());
+        return;
+      }
       for (int i = 0; i < model.number_of_local_orbitals; ++i) {
         model.initialize_state(random_state, model.local_orbitals[i]);
         find_dos(model, H, random_state, 1);

@@ -7,6 +7,7 @@
 #include "sigma.h"
 
 #include <chrono>
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <iostream>
@@ -333,19 +334,23 @@ void find_block_quantities(std::vector<BlockMember>& members, lsqt_ham* h, int R
     write_rows("/vac0.out", mu_re, mu_im);
     lap("VAC0");
   }
-  if ((first.calculate_vac || first.calculate_msd) && Nt > 0) {
+  if ((first.calculate_vac || first.calculate_msd || (first.calculate_spin && S == 1)) &&
+      Nt > 0) {
     std::vector<real> steps((size_t)Nt);
     for (int t = 0; t < Nt; ++t)
       steps[t] = first.time_step[t] * first.energy_max;
     const size_t per_step = (size_t)S * R * M;
     std::vector<real> all_re((size_t)Nt * per_step), all_im((size_t)Nt * per_step);
-    for (int q = 0; q < 2; ++q) {
-      const bool vac = q == 0;
-      if (!(vac ? first.calculate_vac : first.calculate_msd))
+    for (int q = 0; q < 3; ++q) {
+      const bool wanted[3] = {first.calculate_vac, first.calculate_msd,
+                              first.calculate_spin && S == 1};
+      if (!wanted[q])
         continue;
-      LSQT_CHECK((vac ? lsqt_block_vac : lsqt_block_msd)(h, R, re.data(), im.data(), M, Nt,
-                                                         steps.data(), all_re.data(),
-                                                         all_im.data()));
+      static const char* kFile[3] = {"/vac.out", "/msd.out", "/S.out"};
+      static const char* kWhat[3] = {"VAC", "MSD", "spin polarization"};
+      auto driver = q == 0 ? lsqt_block_vac : q == 1 ? lsqt_block_msd : lsqt_block_spin;
+      LSQT_CHECK(driver(h, R, re.data(), im.data(), M, Nt, steps.data(), all_re.data(),
+                        all_im.data()));
       for (int k = 0; k < S; ++k) {
         Model& model = *members[k].model;
         // the reference writes vector by vector, each with its Nt rows
@@ -356,13 +361,39 @@ void find_block_quantities(std::vector<BlockMember>& members, lsqt_ham* h, int R
           std::vector<real> si(all_im.begin() + at, all_im.begin() + at + (size_t)R * M);
           by_step.push_back(rows_of(model, sr, si, members[k].used));
         }
-        std::ofstream output = open_output(model, vac ? "/vac.out" : "/msd.out");
+        std::ofstream output = open_output(model, kFile[q]);
         for (int r = 0; r < members[k].used; ++r)
           for (int t = 0; t < Nt; ++t)
             write_row(output, by_step[t][r]);
       }
-      lap(vac ? "VAC" : "MSD");
+      lap(kWhat[q]);
     }
+  }
+}
+
+void find_block_ldos(Model& model, lsqt_ham* h)
+{
+  const int M = model.number_of_moments;
+  const int n = model.number_of_atoms;
+  const int count = model.number_of_local_orbitals;
+  std::ofstream output = open_output(model, "/ldos.out");
+  for (int first = 0; first < count; first += 256) {
+    const int used = std::min(256, count - first);
+    int R = 1;
+    while (R < used)
+      R *= 2;
+    // the reference's LDOS vectors: sqrt(N) on one orbital (src/model.cu:79-109)
+    std::vector<real> re((size_t)n * R, 0.0), im((size_t)n * R, 0.0);
+    for (int r = 0; r < used; ++r)
+      model.initialize_state_host(re.data() + (size_t)r * n, im.data() + (size_t)r * n,
+                                  model.local_orbitals[first + r]);
+    std::vector<real> mu_re((size_t)R * M), mu_im((size_t)R * M);
+    BlockHandle phi;
+    LSQT_CHECK(lsqt_block_create(default_context(), n, R, &phi.b));
+    LSQT_CHECK(lsqt_block_from_host(phi.b, re.data(), im.data()));
+    LSQT_CHECK(lsqt_moments_block_self(h, phi.b, M, mu_re.data(), mu_im.data()));
+    for (auto& row : rows_of(model, mu_re, mu_im, used))
+      write_row(output, row);
   }
 }
 

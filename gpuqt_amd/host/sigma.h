@@ -70,4 +70,7 @@ struct BlockMember {
 // with one segment per member when there is more than one.
 void find_block_quantities(std::vector<BlockMember>& members, lsqt_ham* h, int R,
                            const std::vector<real>& re, const std::vector<real>& im);
+// Spin polarization (S.out) is included for a single member.  The LDOS of
+// all local orbitals (up to 256 per block) on the block path:
+void find_block_ldos(Model& model, lsqt_ham* h);
 }  // namespace lsqt_host

@@ -58,6 +58,7 @@ SIGNATURES = {
     "lsqt_block_apply": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "lsqt_block_vac": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]),
     "lsqt_block_msd": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "lsqt_block_spin": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]),
     "lsqt_ham_get_info": (C.c_int, [_vp, C.POINTER(HamInfo)]),
     "lsqt_vec_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "lsqt_vec_clone": (C.c_int, [_vp, C.POINTER(_vp)]),
@@ -453,6 +454,10 @@ class Hamiltonian:
     def block_msd(self, re, im, M, time_steps_scaled):
         """complex (Nt, segments, R, M): find_msd's moment sets for R vectors"""
         return self._block_driver(load().lsqt_block_msd, re, im, M, time_steps_scaled)
+
+    def block_spin(self, re, im, M, time_steps_scaled):
+        """complex (Nt, 1, R, M): find_spin_polarization's moment sets for R vectors"""
+        return self._block_driver(load().lsqt_block_spin, re, im, M, time_steps_scaled)
 
     def block_tune(self, left, right, force=False):
         """(W, rows, blocked, us per step) of the autotuned block kernel shape

@@ -261,6 +261,10 @@ int lsqt_block_vac(lsqt_ham* h, int R, const double* re, const double* im, int M
                    const double* time_step_scaled, double* mu_re, double* mu_im);
 int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double* im, int M, int n_steps,
                    const double* time_step_scaled, double* mu_re, double* mu_im);
+/*   lsqt_block_spin: find_spin_polarization (src/sigma.cu:395-442):
+ *                   <U(t) phi|T_m|S_z U(t) phi> (one system only) */
+int lsqt_block_spin(lsqt_ham* h, int R, const double* re, const double* im, int M, int n_steps,
+                    const double* time_step_scaled, double* mu_re, double* mu_im);
 /* Block moment sets are captured once per shape and buffer set and replayed
  * as hipGraphs (default; also LSQT_GRAPHS=0 in the environment turns it
  * off): enable = 0 launches their kernels one by one. */

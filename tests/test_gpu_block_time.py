@@ -86,6 +86,27 @@ def test_block_msd_against_oracle(gpu_ctx, case, R):
     assert worst <= ROW_TOL
 
 
+@pytest.mark.parametrize("R", [1, 4])
+def test_block_spin_against_oracle(gpu_ctx, R):
+    """find_spin_polarization for R vectors at once: S_z flips the sign of
+    every odd site of the recursion block, states evolve forward in time."""
+    print(f"[gpu] block spin general_real_spin R={R}", flush=True)
+    m = _model("general_real_spin")
+    H = _ham(gpu_ctx, m)
+    re, im, vecs = _vectors(m, R)
+    ts = m.time_step * m.energy_max
+    mu = H.block_spin(re, im, m.number_of_moments, ts)
+    assert mu.shape == (len(ts), 1, R, m.number_of_moments)
+    worst = 0.0
+    for r in range(R):
+        want_rows, _ = m.find_spin(vecs[r])
+        got_rows = _rows(m, mu[:, 0, r])
+        for t in range(len(ts)):
+            worst = max(worst, _normwise(got_rows[t], want_rows[t]))
+    print(f"    worst spin row error {worst:.2e}", flush=True)
+    assert worst <= ROW_TOL
+
+
 def _concat(models):
     """row-major ELL arrays of independent systems side by side"""
     K = max(m.max_neighbor for m in models)
