@@ -164,6 +164,10 @@ def terms(t):
     return _terms_cache[t]
 
 
+# block-step tile schedules (BlockShape.blocked in capi_block.hip)
+SCHEDULES = ("natural", "blocked", "column-walk", "sliding-window")
+
+
 def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     """Config 4: block-FP32 KPM (R vectors per sweep) against the plain FP64
     single-vector kernel on the same system, and the DOS error of the block
@@ -183,7 +187,7 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     blk.upload(np.cos(ph), np.sin(ph))
     W, rows, blocked, us = H.block_tune(blk, blk)
     log(f"lattice slab {info.slab} row {info.row}; autotuner: W={W} (VPW={R // W}) "
-        f"rows={rows} {('natural', 'blocked', 'column-walk')[blocked]} schedule: {us:.1f} us/step")
+        f"rows={rows} {SCHEDULES[blocked]} schedule: {us:.1f} us/step")
     v0 = H.vector(np.cos(ph[0]), np.sin(ph[0]))
 
     def timed(fn):
@@ -217,10 +221,10 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     # measured HBM bytes per launch, where the profile covers this workload
     profiled = L == 160 and R == 32
     db = "true" if getattr(H, "block_depth", 1) == 2 else "false"
-    traffic_left = (profiled_traffic(f"k_block_step<1,6,{W},true,1,false,{db}>")
-                    if profiled else None)
-    traffic_self = (profiled_traffic(f"k_block_step<1,6,{W},true,2,false,{db}>")
-                    if profiled else None)
+    kname = ((lambda mode: f"k_block_win<1,6,{W},true,{mode}>") if blocked == 3 else
+             (lambda mode: f"k_block_step<1,6,{W},true,{mode},false,{db}>"))
+    traffic_left = profiled_traffic(kname(1)) if profiled else None
+    traffic_self = profiled_traffic(kname(2)) if profiled else None
     steps_self = M // 2
     us_ = n * R * steps_self / ts          # recursion site-updates/s
     um_ = n * R * (M - 1) / ts             # moments x sites x vectors per second
@@ -233,7 +237,8 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
                        "achieved_GBs": u64 * bpu64 / 1e9},
         "f32_single": {"value": u32, "unit": "site-updates/s", "speedup_vs_f64": u32 / u64},
         "block_f32": {"R": R, "W": W, "vectors_per_wavefront": R // W, "rows_per_tile": rows,
-                      "tile_schedule": ("natural", "blocked", "column-walk")[blocked],
+                      "tile_schedule": SCHEDULES[blocked],
+                      "tile_rows": ("lattice rows" if blocked == 3 else "sites"),
                       "value": ub, "unit": "site-updates/s", "speedup_vs_f64": ub / u64,
                       "bytes_per_site_update": bpu_blk,
                       "achieved_GBs": ub * bpu_blk / 1e9,
@@ -447,10 +452,26 @@ def probe(ctx, H, rv, args, moments=10):
         blk = native.Block(ctx, n, R)
         blk.upload(np.cos(ph), np.sin(ph))
         W, rows, blocked, _ = Hc.block_tune(blk, blk)
-        for depth in (1, 2):  # both loop depths, whichever the tuner picks
-            Hc.block_set_config(R, W, rows, blocked=blocked, depth=depth)
+        # the tuned shape, and the sliding window and the column walk at both
+        # loop depths with W = 2, whichever the tuner picks
+        shapes = [(W, rows, blocked, getattr(Hc, "block_depth", 1))]
+        info = Hc.info()
+        if info.slab and info.row:
+            ny = info.slab // info.row
+            by = max(d for d in range(1, 33) if ny % d == 0)
+            shapes.append((2, by, 3, 1))
+            span = next((s for s in range(info.row, info.slab + 1, info.row)
+                         if info.slab % s == 0 and s % 64 == 0 and s >= 512), 0)
+            if span:
+                shapes += [(2, span, 2, 1), (2, span, 2, 2)]
+        for (w, r, b, depth) in shapes:
+            try:
+                Hc.block_set_config(R, w, r, blocked=b, depth=depth)
+            except native.LsqtError:
+                continue
             Hc.moments_block(blk, blk, moments)
             Hc.moments_block_self(blk, 2 * moments)
+        Hc.block_set_config(R, 0)
         ctx.sync()
         log(f"probe: block R={R} moments on {n} sites")
 

@@ -57,6 +57,10 @@ struct BlockScratch {
 
 // a kernel shape: W vectors per lane (R/W lanes per site), tiles of `rows`
 // sites, natural or blocked tile schedule
+// Tile schedule of the block step.  blocked: 0 natural tiles of `rows`
+// sites, 1 blocked tiles (rows at one offset of every lattice slab), 2
+// blocked with the column walk, 3 the sliding window (tiles of `rows`
+// lattice rows of one chunk; see make_window_map).
 struct BlockShape {
   int W = 0;
   int rows = 0;
@@ -419,6 +423,14 @@ int tile_map(lsqt_ham* h, int R, const BlockShape& c, TileMap* tm)
     return LSQT_OK;
   }
   const int per_pass = kBlock / (R / c.W);
+  if (c.blocked == 3) {
+    if (!window_fits(h->d.n, h->slab, h->row, c.rows, R, c.W))
+      return set_error(LSQT_ERR_INVALID, "the sliding window does not fit this lattice "
+                       "(slab %d, row %d, %d rows per tile, R=%d, W=%d)", h->slab, h->row,
+                       c.rows, R, c.W);
+    *tm = make_window_map(h->d.n, h->slab, h->row, c.rows, per_pass);
+    return LSQT_OK;
+  }
   *tm = make_tile_map(h->d.n, c.rows, c.blocked ? h->slab : 0, c.blocked ? c.rows : 0, h->row,
                       c.blocked == 2 ? per_pass : 0);
   return LSQT_OK;
@@ -514,12 +526,27 @@ extern "C" int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int bl
     h->blk->forced.erase(R);
     return LSQT_OK;
   }
-  if (!valid_W(R, W) || rows < 64 || rows % 64 != 0)
-    return set_error(LSQT_ERR_INVALID, "W=%d rows=%d invalid for R=%d", W, rows, R);
-  if (blocked < 0 || blocked > 2)
-    return set_error(LSQT_ERR_INVALID, "tile schedule %d is not 0, 1 or 2", blocked);
+  if (blocked < 0 || blocked > 3)
+    return set_error(LSQT_ERR_INVALID, "tile schedule %d is not 0, 1, 2 or 3", blocked);
   if (depth != 1 && depth != 2)
     return set_error(LSQT_ERR_INVALID, "depth %d is not 1 or 2", depth);
+  if (blocked == 3) {
+    if (!valid_W(R, W) || depth != 1 || h->d.layout == kFull ||
+        !(h->d.K == 3 || h->d.K == 4 || h->d.K == 6) ||
+        !window_fits(h->d.n, h->slab, h->row, rows, R, W))
+      return set_error(LSQT_ERR_INVALID, "the sliding window needs a lattice of >= 3 rows per "
+                       "slab that %d rows per tile divide, rows of whole chunks, W <= 4, 3, 4 "
+                       "or 6 slots and depth 1 (slab %d, row %d, W=%d, R=%d)", rows, h->slab,
+                       h->row, W, R);
+    BlockShape c;
+    c.W = W;
+    c.rows = rows;
+    c.blocked = 3;
+    h->blk->forced[R] = c;
+    return LSQT_OK;
+  }
+  if (!valid_W(R, W) || rows < 64 || rows % 64 != 0)
+    return set_error(LSQT_ERR_INVALID, "W=%d rows=%d invalid for R=%d", W, rows, R);
   if (blocked && (h->slab == 0 || h->slab % rows != 0))
     return set_error(LSQT_ERR_INVALID, "blocked tiles of %d sites need a lattice slab they "
                      "divide (slab %d)", rows, h->slab);
@@ -1016,6 +1043,29 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
       BlockShape c = cands[i];
       c.depth = 2;
       cands.push_back(c);
+    }
+    // the sliding window: tiles of about 8, 16 and 32 lattice rows
+    if (h->slab > 0 && h->row > 0 && h->slab % h->row == 0) {
+      const int ny = h->slab / h->row;
+      for (int W : {1, 2, 4}) {
+        if (!valid_W(R, W))
+          continue;
+        int last = 0;
+        for (int want : {8, 16, 32}) {
+          int by = 0;  // the divisor of ny closest to `want` from below
+          for (int d = want; d >= 4 && by == 0; --d)
+            if (ny % d == 0)
+              by = d;
+          if (by == 0 || by == last || !window_fits(h->d.n, h->slab, h->row, by, R, W))
+            continue;
+          last = by;
+          BlockShape c;
+          c.W = W;
+          c.rows = by;
+          c.blocked = 3;
+          cands.push_back(c);
+        }
+      }
     }
   }
   const int steps = 6;

@@ -15,6 +15,8 @@
 // moment products <L_r|psi_r> are formed and summed in FP64 (a product of two
 // FP32 numbers is exact in FP64), reduced per vector by a fixed shuffle / LDS
 // tree into per-tile partials [M][tiles][R] and then in tile order.
+#include <algorithm>
+
 #include "lsqt_device.h"
 #include "lsqt_launch.h"
 
@@ -437,6 +439,216 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
   }
 }
 
+// The block step on a lattice with the sliding window of WindowMap: the
+// workgroup sweeps the rows y0 .. y0 + by - 1 of one chunk of one slab; rows
+// y - 1, y, y + 1 of psi1 (chunk plus one site either side) sit in a ring
+// of four LDS slots, and row y + 2 is loaded while row y is computed.
+// Neighbours inside the window (same slab, rows y - 1 .. y + 1, periodic
+// wrap in y and z) are read from LDS, all others (the neighbouring slabs and
+// anything the lattice does not predict) from global memory, so any
+// Hamiltonian gives the same values as k_block_step; only how often a row
+// of psi1 crosses the L1 / L2 path changes (once per row instead of once per
+// neighbour).  Same arithmetic and products as k_block_step; per-tile sums
+// over the tile's by x chunk sites.
+template <int LAYOUT, int KT, int W, bool RECUR, int IPM>
+__global__ __launch_bounds__(kBlock) void k_block_win(
+  HamDev h, int R, TileMap tm, const float2* __restrict__ psi1, const float2* psi0, float2* out,
+  const float2* __restrict__ L, double2* __restrict__ partial, double2* __restrict__ partial2,
+  int tiles, int blocks)
+{
+  static_assert(KT > 0, "the window kernel needs a compile-time slot count");
+  __shared__ float2 s_tab[LAYOUT == kPackedLds ? kBlockLdsTableMax : 1];
+  // the window ring during the sweep, the tile sums after it
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  float2* s_win = reinterpret_cast<float2*>(s_dyn);
+  auto s_red = reinterpret_cast<double (*)[kBlock / 64][64 * 4]>(s_dyn);
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  if constexpr (LAYOUT == kPackedLds) {
+    for (int i = threadIdx.x; i < h.ncls; i += kBlock) {
+      const double2 d = h.tab[i];
+      s_tab[i] = make_float2((float)d.x, (float)d.y);
+    }
+  }
+  const WindowMap& wm = tm.win;
+  const int vpw = R / W;                 // lanes per site
+  const int P = wm.chunk;                // sites per pass (kBlock / vpw)
+  const int lane_site = threadIdx.x / vpw;
+  const int r0 = (threadIdx.x % vpw) * W;
+  const int span = (P + 2) * R;          // float2 per LDS slot
+  const int x = tile % wm.slabs;
+  const int u = tile / wm.slabs;
+  const int z0 = (u % wm.chunks) * P;
+  const int y0 = (u / wm.chunks) * wm.by;
+  const int xbase = x * wm.slab;
+  // site at position d (0 .. P + 1) of the window row y (any integer row)
+  auto win_site = [&](int y, int d) {
+    int yy = y;
+    if (yy < 0)
+      yy += wm.ny;
+    else if (yy >= wm.ny)
+      yy -= wm.ny;
+    int z = z0 - 1 + d;
+    if (z < 0)
+      z += wm.row;
+    else if (z >= wm.row)
+      z -= wm.row;
+    return xbase + yy * wm.row + z;
+  };
+  // a window row is (P + 2) * vpw items of W vectors: item i = (d, lane
+  // group i % vpw); thread t loads item t and, for t < 2 vpw, item kBlock + t
+  const int d_a = threadIdx.x / vpw, g_a = threadIdx.x % vpw;
+  const bool has_b = threadIdx.x < 2 * vpw;
+  const int d_b = (kBlock + threadIdx.x) / vpw, g_b = (kBlock + threadIdx.x) % vpw;
+  auto fetch = [&](int y, Vec<W>& va, Vec<W>& vb) {
+    va = ldv<W, false>(psi1 + (size_t)win_site(y, d_a) * R + g_a * W);
+    if (has_b)
+      vb = ldv<W, false>(psi1 + (size_t)win_site(y, d_b) * R + g_b * W);
+  };
+  auto put = [&](int slot, const Vec<W>& va, const Vec<W>& vb) {
+    stv<W>(s_win + slot * span + d_a * R + g_a * W, va);
+    if (has_b)
+      stv<W>(s_win + slot * span + d_b * R + g_b * W, vb);
+  };
+  auto lds_row = [&](int pos) {
+    Vec<W> v;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      v.v[w] = s_win[pos + w];
+    return v;
+  };
+  {
+    Vec<W> a0, b0, a1, b1, a2, b2;
+    fetch(y0 - 1, a0, b0);
+    fetch(y0, a1, b1);
+    fetch(y0 + 1, a2, b2);
+    put(0, a0, b0);
+    put(1, a1, b1);
+    put(2, a2, b2);
+  }
+  __syncthreads();
+
+  const float inv_scale = (float)(1.0 / h.emax);
+  double acc_re[W], acc_im[W], acc_nn[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    acc_re[w] = acc_im[w] = acc_nn[w] = 0.0;
+  constexpr int C = KT;
+  int s = xbase + y0 * wm.row + z0 + lane_site;
+  uint32_t wnext[C];
+  load_words<C, false>(h, s, wnext);
+  for (int p = 0; p < wm.by; ++p) {
+    const int y = y0 + p;
+    const bool more = p + 1 < wm.by;
+    Vec<W> na, nb;
+    if (more)
+      fetch(y + 2, na, nb);
+    const int sn = more ? s + wm.row : s;
+    const size_t base = (size_t)s * R + r0;
+    const float v = h.pot32[s];
+    Vec<W> p0, l;
+    if constexpr (RECUR)
+      p0 = ldv<W, true>(psi0 + base);
+    if constexpr (IPM == kLeft)
+      l = ldv<W, true>(L + base);
+    int j[C];
+    float a[C], b[C];
+    {
+      uint32_t wcur[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q)
+        wcur[q] = wnext[q];
+      load_words<C, false>(h, sn, wnext);
+      decode_words32<LAYOUT, C>(h, s_tab, s, wcur, j, a, b);
+    }
+    // window position of each neighbour (-1: outside, read from global)
+    int pos[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      pos[q] = -1;
+      const int rr = j[q] - xbase;
+      if ((unsigned)rr < (unsigned)wm.slab) {
+        const int yj = div_by(rr, wm.row_div);
+        int dy = yj - y;
+        if (dy == wm.ny - 1)
+          dy = -1;
+        else if (dy == 1 - wm.ny)
+          dy = 1;
+        int dz = rr - yj * wm.row - z0 + 1;  // -row < dz <= row
+        if (dz < 0)
+          dz += wm.row;
+        else if (dz >= wm.row)
+          dz -= wm.row;
+        if (dy >= -1 && dy <= 1 && dz < P + 2)
+          pos[q] = ((p + 1 + dy) & 3) * span + dz * R + r0;
+      }
+    }
+    Vec<W> c[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      if (pos[q] < 0)
+        c[q] = ldv<W, false>(psi1 + (size_t)j[q] * R + r0);
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      if (pos[q] >= 0)
+        c[q] = lds_row(pos[q]);
+    const Vec<W> p1 = lds_row(((p + 1) & 3) * span + (lane_site + 1) * R + r0);
+    float tr[W], ti[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      tr[w] = v * p1.v[w].x;
+      ti[w] = v * p1.v[w].y;
+    }
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const float cx = c[q].v[w].x, cy = c[q].v[w].y;
+        tr[w] = __builtin_fmaf(a[q], cx, tr[w]);
+        tr[w] = __builtin_fmaf(-b[q], cy, tr[w]);
+        ti[w] = __builtin_fmaf(a[q], cy, ti[w]);
+        ti[w] = __builtin_fmaf(b[q], cx, ti[w]);
+      }
+    }
+    Vec<W> o;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      if constexpr (RECUR) {
+        o.v[w].x = __builtin_fmaf(2.0f, tr[w] * inv_scale, -p0.v[w].x);
+        o.v[w].y = __builtin_fmaf(2.0f, ti[w] * inv_scale, -p0.v[w].y);
+      } else {
+        o.v[w].x = tr[w] * inv_scale;
+        o.v[w].y = ti[w] * inv_scale;
+      }
+    }
+    stv<W, true>(out + base, o);
+    if constexpr (IPM == kSelf)
+      l = p1;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const double ox = o.v[w].x, oy = o.v[w].y, lx = l.v[w].x, ly = l.v[w].y;
+      acc_re[w] += ox * lx + oy * ly;
+      acc_im[w] += oy * lx - ox * ly;
+      if constexpr (IPM == kSelf)
+        acc_nn[w] += ox * ox + oy * oy;
+    }
+    if (more)
+      put((p + 3) & 3, na, nb);
+    __syncthreads();
+    s = sn;
+  }
+
+  tile_sums<W>(acc_re, acc_im, vpw, R, r0, s_red, partial + (size_t)tile * R);
+  if constexpr (IPM == kSelf) {
+    double zero[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      zero[w] = 0.0;
+    tile_sums<W>(acc_nn, zero, vpw, R, r0, s_red, partial2 + (size_t)tile * R);
+  }
+}
+
 // mu[(k*R + r)*M + m0 + i] = sum over the tiles t of segment k of
 // partial[i][t][r], in tile order, i < gridDim.x, k = blockIdx.z (one
 // segment of all tiles without a segment table)
@@ -843,6 +1055,22 @@ __global__ __launch_bounds__(kBlock) void k_block_sz(const float2* __restrict__ 
     } break;                         \
   }
 
+#define LSQTB_WIN_W_SWITCH(wval, W, ...) \
+  switch (wval) {                        \
+    case 1: {                            \
+      constexpr int W = 1;               \
+      __VA_ARGS__;                       \
+    } break;                             \
+    case 2: {                            \
+      constexpr int W = 2;               \
+      __VA_ARGS__;                       \
+    } break;                             \
+    default: {                           \
+      constexpr int W = 4;               \
+      __VA_ARGS__;                       \
+    } break;                             \
+  }
+
 TileMap make_tile_map(int n, int rows, int slab, int span, int row, int walk_pass)
 {
   TileMap t{};
@@ -865,6 +1093,30 @@ TileMap make_tile_map(int n, int rows, int slab, int span, int row, int walk_pas
   return t;
 }
 
+TileMap make_window_map(int n, int slab, int row, int by, int chunk)
+{
+  TileMap t = make_tile_map(n, by * chunk, 0, 0, 0, 0);
+  WindowMap& w = t.win;
+  w.slab = slab;
+  w.row = row;
+  w.slabs = n / slab;
+  w.chunks = row / chunk;
+  w.ny = slab / row;
+  w.by = by;
+  w.chunk = chunk;
+  w.row_div = make_divisor(row);
+  return t;
+}
+
+bool window_fits(int n, int slab, int row, int by, int R, int W)
+{
+  if (W > 4 || slab <= 0 || row < 2 || by < 1)
+    return false;
+  const int chunk = kBlock / (R / W);
+  return n % slab == 0 && slab % row == 0 && slab / row >= 3 && (slab / row) % by == 0 &&
+         row % chunk == 0;
+}
+
 Grid block_grid(const TileMap& t)
 {
   int blocks = (t.tiles + kXcds - 1) / kXcds * kXcds;
@@ -877,8 +1129,39 @@ void launch_block_step(const HamDev& h, int R, int W, bool db, const TileMap& tm
                        hipStream_t st)
 {
   Grid g = block_grid(tm);
-  // ordinary loads throughout: non-temporal loads of the psi0 and left
-  // streams measured 4-6 % slower here (160^3 cube, R = 32)
+  if (tm.win.by > 0 && W <= 4 && (h.K == 3 || h.K == 4 || h.K == 6)) {
+    // ring of four window rows; the tile sums reuse the same memory
+    const size_t lds = std::max(sizeof(float2) * 4 * (size_t)(tm.win.chunk + 2) * R,
+                                sizeof(double) * 2 * (kBlock / 64) * 64 * 4);
+#define LSQTB_WIN(RC, MODE)                                                                 \
+  k_block_win<LAY, KT, WW, RC, MODE><<<g.blocks, kBlock, lds, st>>>(h, R, tm, psi1, psi0, out, \
+                                                                   L, partial, partial2,       \
+                                                                   g.tiles, g.blocks)
+#define LSQTB_WIN_MODE                                                  \
+  if (self)                                                             \
+    recur ? LSQTB_WIN(true, kSelf) : LSQTB_WIN(false, kSelf);           \
+  else                                                                  \
+    recur ? LSQTB_WIN(true, kLeft) : LSQTB_WIN(false, kLeft);
+    LSQTB_LAYOUT_SWITCH(h, LAY, {
+      switch (h.K) {
+        case 3: {
+          constexpr int KT = 3;
+          LSQTB_WIN_W_SWITCH(W, WW, LSQTB_WIN_MODE)
+        } break;
+        case 4: {
+          constexpr int KT = 4;
+          LSQTB_WIN_W_SWITCH(W, WW, LSQTB_WIN_MODE)
+        } break;
+        default: {
+          constexpr int KT = 6;
+          LSQTB_WIN_W_SWITCH(W, WW, LSQTB_WIN_MODE)
+        } break;
+      }
+    })
+#undef LSQTB_WIN_MODE
+#undef LSQTB_WIN
+    return;
+  }
 #define LSQTB_STEP(RC, MODE, DBV)                                                 \
   k_block_step<LAY, KT, WW, RC, MODE, false, DBV><<<g.blocks, kBlock, 0, st>>>(   \
     h, R, tm, psi1, psi0, out, L, partial, partial2, g.tiles, g.blocks)

@@ -167,6 +167,41 @@ struct Grid {
 // table (segmented Hamiltonians: independent systems side by side) tile t is
 // sites [table[t].x, table[t].y) of segment tile_seg[t], and the tiles of
 // segment k are seg_tiles[k] .. seg_tiles[k+1]-1.
+// Division of 0 <= v < 2^31 by an invariant divisor 2 <= d < 2^31 with one
+// high multiply and a shift (Granlund and Montgomery, "Division by invariant
+// integers using multiplication", PLDI 1994, section 4): with
+// l = ceil(log2 d) and m = ceil(2^(31+l) / d) < 2^32, v / d equals
+// umulhi(v, m) >> (l - 1), since m d - 2^(31+l) < d keeps the error of
+// v m / 2^(31+l) below 1/d.
+struct Divisor {
+  uint32_t m;
+  int shift;
+};
+
+inline Divisor make_divisor(int d)
+{
+  int l = 0;
+  while ((1ll << l) < d)
+    ++l;
+  const unsigned long long p = 1ull << (31 + l);
+  return Divisor{(uint32_t)((p + (unsigned long long)d - 1) / (unsigned long long)d), l - 1};
+}
+
+__device__ __forceinline__ int div_by(int v, const Divisor& q)
+{
+  return (int)(__umulhi((uint32_t)v, q.m) >> q.shift);
+}
+
+// Sliding-window geometry of a lattice (sites x * slab + y * row + z): a tile
+// is `by` consecutive rows y of one slab x, restricted to one chunk of
+// `chunk` consecutive z, swept row by row with rows y-1, y, y+1 of psi_{m-1}
+// (chunk plus one site either side, periodic in y and z) held in LDS.
+// by == 0: no window.
+struct WindowMap {
+  int slab, row, slabs, chunks, ny, by, chunk;
+  Divisor row_div;
+};
+
 struct TileMap {
   int n;
   int rows;
@@ -180,6 +215,7 @@ struct TileMap {
   const int* tile_seg;
   const int* seg_tiles;
   int segs;
+  WindowMap win;
 };
 
 // A lattice model for device-side construction: cells[3] x orbitals sites,

@@ -53,6 +53,12 @@ void launch_renormalise(double2* s, const double2* norms, int n, hipStream_t st)
 // block path (kernels_block.hip)
 // walk_pass > 0 asks for the column walk with passes of walk_pass sites
 TileMap make_tile_map(int n, int rows, int slab, int span, int row, int walk_pass);
+// Sliding-window schedule (block steps only): tiles of `by` rows of one chunk
+// of kBlock / (R / W) sites; the other block kernels see natural tiles of
+// by * chunk sites (the same tile count).
+TileMap make_window_map(int n, int slab, int row, int by, int chunk);
+// whether the window schedule applies to this lattice, tile height and shape
+bool window_fits(int n, int slab, int row, int by, int R, int W);
 Grid block_grid(const TileMap& t);
 // self: <psi1|out> into partial and <out|out> into partial2 (L unused);
 // otherwise <L|out> into partial

@@ -122,6 +122,82 @@ def test_block_blocked_schedule(gpu_ctx, L, R):
         o.close()
 
 
+def _lattice3d(Lx, Ly, Lz, open_y=False, seed=5):
+    """x-slowest 3D lattice (site (x Ly + y) Lz + z), neighbours -x, +x, -y,
+    +y, -z, +z (periodic; open in y if asked: absent slots at the end),
+    hopping -1 with a Peierls phase 2 pi 0.013 x on y-bonds, on-site
+    U[-4, 4]; row-major ELL arrays."""
+    n = Lx * Ly * Lz
+    idx = np.arange(n)
+    x, y, z = idx // (Ly * Lz), (idx // Lz) % Ly, idx % Lz
+    site = lambda a, b, c: ((a % Lx) * Ly + (b % Ly)) * Lz + (c % Lz)
+    ph = np.exp(2j * np.pi * 0.013 * x)
+    cand = [(site(x - 1, y, z), np.full(n, -1.0 + 0j), np.ones(n, bool)),
+            (site(x + 1, y, z), np.full(n, -1.0 + 0j), np.ones(n, bool)),
+            (site(x, y - 1, z), -np.conj(ph), (y > 0) if open_y else np.ones(n, bool)),
+            (site(x, y + 1, z), -ph, (y < Ly - 1) if open_y else np.ones(n, bool)),
+            (site(x, y, z - 1), np.full(n, -1.0 + 0j), np.ones(n, bool)),
+            (site(x, y, z + 1), np.full(n, -1.0 + 0j), np.ones(n, bool))]
+    nn = np.zeros(n, np.int32)
+    nl = np.zeros((n, 6), np.int32)
+    hop = np.zeros((n, 6), complex)
+    for j, t, ok in cand:
+        nl[ok, nn[ok]] = j[ok]
+        hop[ok, nn[ok]] = t[ok]
+        nn[ok] += 1
+    pot = np.random.default_rng(seed).uniform(-4, 4, n)
+    return dict(neighbor_number=nn, neighbor_list=nl.reshape(-1), potential=pot,
+                hopping_real=hop.real.reshape(-1).copy(), hopping_imag=hop.imag.reshape(-1).copy(),
+                xx=np.zeros(n * 6)), n
+
+
+@pytest.mark.parametrize("dims,open_y", [((6, 10, 32), False), ((5, 8, 64), True),
+                                         ((4, 16, 16), False)])
+def test_block_sliding_window(gpu_ctx, dims, open_y):
+    """The sliding-window schedule (rows y - 1 .. y + 1 of a chunk in LDS) on
+    3D lattices of uneven extents, periodic and open, every tile height that
+    divides the rows per slab and every W: the same moments as the natural
+    schedule, direct (<L|psi> with L != R) and by the product identities."""
+    from gpuqt_amd import native
+    Lx, Ly, Lz = dims
+    print(f"[gpu] sliding window {dims} open_y={open_y}", flush=True)
+    arrays, n = _lattice3d(Lx, Ly, Lz, open_y)
+    H = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, 6, 12.0)
+    info = H.info()
+    assert (info.slab, info.row) == (Ly * Lz, Lz)
+    R, M = 32, 80
+    rng = np.random.default_rng(9)
+    pl, pr = rng.uniform(0, 2 * np.pi, (2, R, n))
+    bl, br = native.Block(gpu_ctx, n, R), native.Block(gpu_ctx, n, R)
+    bl.upload(np.cos(pl), np.sin(pl))
+    br.upload(np.cos(pr), np.sin(pr))
+    H.block_set_config(R, 2, 256)
+    ref = H.moments_block(bl, br, M)
+    ref_self = H.moments_block_self(br, M)
+    tol = 1e-12 * np.abs(ref).max()
+    tol_self = 1e-12 * np.abs(ref_self).max()
+    tried = 0
+    for W in (1, 2, 4):
+        chunk = 256 * W // R
+        for by in range(1, Ly + 1):
+            if Ly % by or Lz % chunk:
+                continue
+            H.block_set_config(R, W, by, blocked=3)
+            tried += 1
+            np.testing.assert_allclose(H.moments_block(bl, br, M), ref, rtol=0, atol=tol,
+                                       err_msg=f"W={W} by={by}")
+            np.testing.assert_allclose(H.moments_block_self(br, M), ref_self, rtol=0,
+                                       atol=tol_self, err_msg=f"self W={W} by={by}")
+    assert tried > 0
+    with pytest.raises(native.LsqtError):
+        H.block_set_config(R, 8, 1, blocked=3)  # W > 4
+    with pytest.raises(native.LsqtError):
+        H.block_set_config(R, 2, 3, blocked=3)  # 3 does not divide the rows per slab
+    H.block_set_config(R, 0)
+    for o in (bl, br, H):
+        o.close()
+
+
 @pytest.mark.parametrize("case,R", [("cube_peierls", 8), ("config1_honeycomb_20k", 32),
                                     ("square_vacancy_ldos", 4)])
 def test_block_self_products(gpu_ctx, case, R):
