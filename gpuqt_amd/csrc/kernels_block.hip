@@ -477,10 +477,13 @@ __global__ __launch_bounds__(kBlock) void k_block_win(
   const int lane_site = threadIdx.x / vpw;
   const int r0 = (threadIdx.x % vpw) * W;
   const int span = (P + 2) * R;          // float2 per LDS slot
-  const int x = tile % wm.slabs;
-  const int u = tile / wm.slabs;
-  const int z0 = (u % wm.chunks) * P;
-  const int y0 = (u / wm.chunks) * wm.by;
+  // tile order: chunk fastest, then slab, then row block, so the tiles in
+  // flight on one XCD cover whole lattice rows (contiguous memory) of a run
+  // of neighbouring slabs (their x-neighbour gathers hit that XCD's L2)
+  const int z0 = (tile % wm.chunks) * P;
+  const int u = tile / wm.chunks;
+  const int x = u % wm.slabs;
+  const int y0 = (u / wm.slabs) * wm.by;
   const int xbase = x * wm.slab;
   // site at position d (0 .. P + 1) of the window row y (any integer row)
   auto win_site = [&](int y, int d) {

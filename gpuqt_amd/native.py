@@ -471,7 +471,7 @@ class Hamiltonian:
 
     def block_set_config(self, R, W=0, rows=0, blocked=0, depth=1):
         """blocked: 0 natural tiles, 1 (or True) blocked, 2 blocked with the
-        column walk; depth: passes in flight per wave (1 or 2)"""
+        column walk, 3 the sliding window (rows = lattice rows per tile); depth: passes in flight per wave (1 or 2)"""
         _check(load().lsqt_block_set_config(self.h, int(R), int(W), int(rows), int(blocked),
                                             int(depth)))
 

@@ -283,14 +283,21 @@ int lsqt_block_set_graphs(lsqt_ham* h, int enable);
  * fastest, so that the tiles in flight together are the same rows of
  * neighbouring slabs and share their neighbours in cache; 2 blocked with the
  * column walk: a tile of whole rows is swept one chunk of every row at a time,
- * so that neighbours one row apart are also read close together. */
+ * so that neighbours one row apart are also read close together; 3 the
+ * sliding window, for lattices of at least 3 rows per slab whose rows are
+ * whole chunks of 256*W/R sites (W <= 4, max_neighbor 3, 4 or 6): rows =
+ * lattice rows per tile (a divisor of the rows per slab); a tile is one
+ * chunk of those rows in one slab, swept row by row with rows y-1 .. y+1 of
+ * the recursion block in LDS (neighbours there are read from LDS, all
+ * others from global memory). */
 int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right, int force,
                     int* W, int* rows, int* blocked, int* depth, double* us_per_step);
 /* fixes the shape for R on this Hamiltonian (W = 0 returns to the tuner);
- * blocked != 0 needs a detected slab that rows divides, blocked = 2 also
- * tiles of whole rows that passes of 256*W/R sites divide; depth = passes in
- * flight per wave (2: the loads of the next pass are issued before the
- * current one is computed; used for max_neighbor 3, 4, 6) */
+ * blocked = 1 or 2 needs a detected slab that rows divides, blocked = 2 also
+ * tiles of whole rows that passes of 256*W/R sites divide, blocked = 3 the
+ * conditions above with rows = lattice rows per tile and depth 1; depth =
+ * passes in flight per wave (2: the loads of the next pass are issued before
+ * the current one is computed; used for max_neighbor 3, 4, 6) */
 int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int blocked, int depth);
 
 /* the number of Chebyshev terms evolve/evolvex use for time_step_scaled */
