@@ -212,6 +212,18 @@ bool valid_W(int R, int W)
   return (W == 1 || W == 2 || W == 4 || W == 8) && R % W == 0 && R / W <= 64;
 }
 
+// whether a block-step shape can run on this Hamiltonian (the conditions
+// lsqt_block_set_config checks)
+bool shape_usable(const lsqt_ham* h, int R, const BlockShape& c)
+{
+  if (!valid_W(R, c.W) || (c.depth != 1 && c.depth != 2))
+    return false;
+  if (c.blocked == 3)
+    return c.depth == 1 && (h->d.K == 3 || h->d.K == 4 || h->d.K == 6) &&
+           window_fits(h->d.n, h->slab, h->row, c.rows, R, c.W);
+  return c.blocked >= 0 && c.blocked <= 2 && c.rows >= 64 && c.rows % 64 == 0;
+}
+
 int block_alloc(lsqt_ctx* ctx, size_t count, float2** out)
 {
   void* p = nullptr;
@@ -1001,7 +1013,10 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
   // for lattices, and their best tile size depends on the slab
   TuneKey key{R, h->d.K, h->d.layout == kPackedLds ? 1 : 0, size_class(h->d.n), h->slab};
   auto it = g_tune.find(key);
-  if (it != g_tune.end() && !force) {
+  // a cached shape is taken only if it is valid here (the cache file is
+  // plain text, and the key holds the slab, not the row length that a
+  // window shape also depends on)
+  if (it != g_tune.end() && !force && shape_usable(h, R, it->second.shape)) {
     *out = it->second.shape;
     *us_out = it->second.us;
     return LSQT_OK;
