@@ -1,7 +1,8 @@
 """ctypes binding of the oracle (liblsqt_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, always as the checker (or the CPU baseline),
+bench.py's cpu_baseline leg, always as the reference results to compare
+with (or the CPU baseline),
 never as the thing measured or shipped.  See lsqt_oracle.h for what each
 function restates in the reference.
 """

@@ -2,7 +2,7 @@
  * lsqt_oracle.h -- CPU restatement of GPUQT's linear-scaling quantum
  * transport path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This library supplies the expected results: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
  * only to check (or time beside) the HIP path.  No product code links it.
  *
