@@ -55,9 +55,8 @@ struct BlockScratch {
 };
 }  // namespace
 
-// a kernel shape: W vectors per lane (R/W lanes per site), tiles of `rows`
-// sites, natural or blocked tile schedule
-// Tile schedule of the block step.  blocked: 0 natural tiles of `rows`
+// A kernel shape: W vectors per lane (R/W lanes per site), tile size and
+// tile schedule of the block step.  blocked: 0 natural tiles of `rows`
 // sites, 1 blocked tiles (rows at one offset of every lattice slab), 2
 // blocked with the column walk, 3 the sliding window (tiles of `rows`
 // lattice rows of one chunk; see make_window_map).
@@ -460,10 +459,10 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, c
       launch_block_dot(A, L, R, tm, part, st);
     } else if (m == 1) {
       launch_block_step(h->d, R, c.W, c.depth == 2, tm, A, nullptr, B, L, part, nullptr, false,
-                        false, st);
+                        kStepLeft, st);
     } else {
-      launch_block_step(h->d, R, c.W, c.depth == 2, tm, B, A, A, L, part, nullptr, true, false,
-                        st);
+      launch_block_step(h->d, R, c.W, c.depth == 2, tm, B, A, A, L, part, nullptr, true,
+                        kStepLeft, st);
       float2* t = A;
       A = B;
       B = t;
@@ -481,23 +480,33 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, c
 // the moments follow from T_{2m} = 2 T_m^2 - T_0, T_{2m+1} = 2 T_m T_{m+1} - T_1
 // (H Hermitian): mu_0 = S_0, mu_1 = S_1, mu_k = 2 S_k - mu_{k mod 2}.  Moment k
 // >= 1 goes to ring slot (k - 1) mod kRing; M/2 steps give moments 0..M-1.
+// The product-identity recursion (M/2 steps): for R vectors (pairs false)
+// the raw products <psi_0|psi_0>, then per step m <psi_{m-1}|psi_m> and
+// <psi_m|psi_m> as entries 2m - 1 and 2m of mu; with pairs, A holds R pairs
+// of columns (left L_r, right R_r) and the products are <L|R>, then
+// <T_{m-1} L|T_m R> and <T_m L|T_m R>.  Either way mu_k = 2 S_k - mu_{k mod 2}
+// for k >= 2 (T_{2m} = 2 T_m^2 - T_0, T_{2m+1} = 2 T_m T_{m+1} - T_1, T_m
+// Hermitian).
 void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, float2*& A,
-                        float2*& B, double2* ring, double2* mu, int M, hipStream_t st)
+                        float2*& B, double2* ring, double2* mu, int M, hipStream_t st,
+                        bool pairs = false)
 {
   const int tiles = tm.tiles;
   const size_t slot = (size_t)tiles * R;
-  launch_block_dot(A, A, R, tm, ring, st);
+  const int cols = pairs ? 2 * R : R;
+  const int mode = pairs ? kStepPair : kStepSelf;
+  launch_block_dot(A, A, cols, tm, ring, st, pairs);
   launch_block_reduce(ring, tm, R, M, 0, 1, mu, st);
   const int steps = M / 2;
   for (int m = 1; m <= steps; ++m) {
     double2* odd = ring + (size_t)((2 * m - 2) % kRing) * slot;
     double2* even = ring + (size_t)((2 * m - 1) % kRing) * slot;
     if (m == 1) {
-      launch_block_step(h->d, R, c.W, c.depth == 2, tm, A, nullptr, B, nullptr, odd, even, false,
-                        true, st);
+      launch_block_step(h->d, cols, c.W, c.depth == 2, tm, A, nullptr, B, nullptr, odd, even,
+                        false, mode, st);
     } else {
-      launch_block_step(h->d, R, c.W, c.depth == 2, tm, B, A, A, nullptr, odd, even, true, true,
-                        st);
+      launch_block_step(h->d, cols, c.W, c.depth == 2, tm, B, A, A, nullptr, odd, even, true,
+                        mode, st);
       float2* t = A;
       A = B;
       B = t;
@@ -576,30 +585,41 @@ extern "C" int lsqt_block_set_config(lsqt_ham* h, int R, int W, int rows, int bl
 
 namespace {
 
-// One moment set for R vectors of every segment: <left_r|T_m|right_r> (or the
-// self products of `right` by the product identities), downloaded into
-// mu[(k*R + r)*M + m].  A and B of the scratch are overwritten.
+// How a moment set is formed: <left|T_m|right> directly, the self products
+// of `right`, or the products of a left and a right recursion run side by
+// side (both by the product identities, M/2 steps).
+enum MomentMode { kDirect = 0, kSelfProducts = 1, kPairProducts = 2 };
+
+// One moment set for R vectors of every segment: <left_r|T_m|right_r>,
+// downloaded into mu[(k*R + r)*M + m].  A and B of the scratch are
+// overwritten.  kPairProducts: c and tm are the shape and tile map of 2R
+// columns, A and B hold 2R columns, and left / right differ from A.
 int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, BlockScratch* s,
-               const float2* left, const float2* right, bool self, int M, double* mu_re,
+               const float2* left, const float2* right, int mode, int M, double* mu_re,
                double* mu_im)
 {
   hipStream_t st = h->ctx->stream;
   float2* A = s->A;
   float2* B = s->B;
-  if (A != right)
+  if (mode == kPairProducts) {
+    if (left == A || right == A)
+      return set_error(LSQT_ERR_INVALID, "pair moment set inputs overlap its scratch");
+    launch_block_interleave(left, right, h->d.n, R, A, st);
+  } else if (A != right) {
     LSQT_HIP(hipMemcpyAsync(A, right, sizeof(float2) * (size_t)h->d.n * R,
                             hipMemcpyDeviceToDevice, st));
+  }
   auto enqueue = [&]() {
     float2* a = A;
     float2* b = B;
-    if (self)
-      enqueue_self_steps(h, R, c, tm, a, b, s->ring, s->mu, M, st);
-    else
+    if (mode == kDirect)
       enqueue_steps(h, R, c, tm, left, a, b, s->ring, s->mu, M, 0, M, st);
+    else
+      enqueue_self_steps(h, R, c, tm, a, b, s->ring, s->mu, M, st, mode == kPairProducts);
   };
   if (h->blk->graphs) {
-    const GraphKey key{self ? 1 : 0, M,    R, c.W,    c.rows, c.blocked, c.depth,
-                       tm.tiles,     left, A, B,      s->ring, s->mu};
+    const GraphKey key{mode, M, R, c.W, c.rows, c.blocked, c.depth, tm.tiles,
+                       mode == kDirect ? left : nullptr, A, B, s->ring, s->mu};
     auto it = h->blk->graph.find(key);
     if (it == h->blk->graph.end()) {
       hipGraph_t g = nullptr;
@@ -627,7 +647,7 @@ int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Block
     const double2* S = s->mu_host + v * M;
     double* re = mu_re + v * M;
     double* im = mu_im + v * M;
-    if (!self) {
+    if (mode == kDirect) {
       for (int k = 0; k < M; ++k) {
         re[k] = S[k].x;
         im[k] = S[k].y;
@@ -643,6 +663,36 @@ int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Block
       im[k] = 2.0 * S[k].y - im[k % 2];
     }
   }
+  return LSQT_OK;
+}
+
+// The shape and tile map of pair moment sets for R vectors (2R columns):
+// the R-vector shape c with twice the vectors per lane, so that lanes per
+// site, tiles and schedule stay those of c (else W = 2 on natural tiles of
+// 256 sites); scratch A / B of 2R columns.  *ok false (no error) where 2R
+// columns exceed the block limit.
+int pair_setup(lsqt_ham* h, int R, const BlockShape& c, int M, BlockShape* c2, TileMap* tm2,
+               BlockScratch** s, bool* ok)
+{
+  *ok = false;
+  const int R2 = 2 * R;
+  if (!valid_R(R2) || (size_t)h->d.n * R2 > (size_t)INT_MAX)
+    return LSQT_OK;
+  BlockShape p = c;
+  p.W = 2 * c.W;
+  if (!shape_usable(h, R2, p)) {
+    p = BlockShape{};
+    p.W = 2;
+    p.rows = 256;
+    if (!valid_W(R2, p.W))
+      p.W = 4;
+    if (!valid_W(R2, p.W))
+      return LSQT_OK;
+  }
+  LSQT_TRY(tile_map(h, R2, p, tm2));
+  LSQT_TRY(scratch_for(h, R2, M, tm2->tiles, s));
+  *c2 = p;
+  *ok = true;
   return LSQT_OK;
 }
 
@@ -672,7 +722,28 @@ extern "C" int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsq
   TileMap tm;
   BlockScratch* s;
   LSQT_TRY(block_setup(h, left->R, left->d, right->d, M, &c, &tm, &s));
-  return moment_set(h, left->R, c, tm, s, left->d, right->d, false, M, mu_re, mu_im);
+  return moment_set(h, left->R, c, tm, s, left->d, right->d, kDirect, M, mu_re, mu_im);
+}
+
+extern "C" int lsqt_moments_block_pair(lsqt_ham* h, const lsqt_block* left,
+                                       const lsqt_block* right, int M, double* mu_re,
+                                       double* mu_im)
+{
+  LSQT_TRY(check_block(h, left, right));
+  if (M < 2)
+    return set_error(LSQT_ERR_INVALID, "number of moments %d must be >= 2", M);
+  if (!mu_re || !mu_im)
+    return set_error(LSQT_ERR_INVALID, "null moment output");
+  const int R = left->R;
+  BlockShape c, c2;
+  TileMap tm, tm2;
+  BlockScratch* s;
+  LSQT_TRY(block_setup(h, R, left->d, right->d, M, &c, &tm, &s));
+  bool pairs = false;
+  LSQT_TRY(pair_setup(h, R, c, M, &c2, &tm2, &s, &pairs));
+  if (!pairs)
+    return set_error(LSQT_ERR_UNSUPPORTED, "R = %d vectors: pairs need 2R <= 256", R);
+  return moment_set(h, R, c2, tm2, s, left->d, right->d, kPairProducts, M, mu_re, mu_im);
 }
 
 extern "C" int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, double* mu_re,
@@ -687,7 +758,7 @@ extern "C" int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, 
   TileMap tm;
   BlockScratch* s;
   LSQT_TRY(block_setup(h, v->R, v->d, v->d, M, &c, &tm, &s));
-  return moment_set(h, v->R, c, tm, s, nullptr, v->d, true, M, mu_re, mu_im);
+  return moment_set(h, v->R, c, tm, s, nullptr, v->d, kSelfProducts, M, mu_re, mu_im);
 }
 
 // ---------------------------------------------------------------------------
@@ -790,7 +861,6 @@ void block_evolvex(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, B
   }
 }
 
-// host vectors (vector-major [R][n]) into a site-major FP64 block
 // host vectors (vector-major [R][n]) into a site-major FP64 block, through
 // `staging` (n*R double2 of device memory, overwritten)
 int upload_states(lsqt_ham* h, int R, const double* re, const double* im, double2* S,
@@ -845,6 +915,12 @@ extern "C" int lsqt_block_vac(lsqt_ham* h, int R, const double* re, const double
   BlockShape c;
   TileMap tm;
   LSQT_TRY(block_setup(h, R, L, L, M, &c, &tm, &s));
+  // the moment sets <state_right|T_m|state_left_copy> as pairs of
+  // recursions by the product identities where 2R columns fit
+  BlockShape c2;
+  TileMap tm2;
+  bool pairs = false;
+  LSQT_TRY(pair_setup(h, R, c, M, &c2, &tm2, &s, &pairs));
   LSQT_TRY(evo_scratch(h, R, s));
   // state_right = V state_left (src/sigma.cu:293)
   launch_to_f32(left, s->B, nR, st);
@@ -854,12 +930,17 @@ extern "C" int lsqt_block_vac(lsqt_ham* h, int R, const double* re, const double
   const size_t per_step = (size_t)M * R * segments(h);
   for (int m = 0; m < n_steps; ++m) {
     // state_left_copy = V state_left; moments <state_right|T|state_left_copy>
+    float2* copy = pairs ? s->F[1] : s->A;
     launch_to_f32(left, s->B, nR, st);
-    launch_block_evo(h->d, R, c.W, tm, 3, nullptr, s->B, s->A, nullptr, nullptr, nullptr,
+    launch_block_evo(h->d, R, c.W, tm, 3, nullptr, s->B, copy, nullptr, nullptr, nullptr,
                      nullptr, 0.0, 0, st);
     launch_to_f32(right, L, nR, st);
-    LSQT_TRY(moment_set(h, R, c, tm, s, L, s->A, false, M, mu_re + m * per_step,
-                        mu_im + m * per_step));
+    if (pairs)
+      LSQT_TRY(moment_set(h, R, c2, tm2, s, L, copy, kPairProducts, M, mu_re + m * per_step,
+                          mu_im + m * per_step));
+    else
+      LSQT_TRY(moment_set(h, R, c, tm, s, L, copy, kDirect, M, mu_re + m * per_step,
+                          mu_im + m * per_step));
     if (m < n_steps - 1) {
       block_evolve(h, R, c, tm, s, -1, time_step_scaled[m], left, true);
       block_evolve(h, R, c, tm, s, -1, time_step_scaled[m], right, true);
@@ -894,7 +975,7 @@ extern "C" int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double
   const size_t per_step = (size_t)M * R * segments(h);
   for (int m = 0; m < n_steps; ++m) {
     launch_to_f32(state_x, s->A, nR, st);
-    LSQT_TRY(moment_set(h, R, c, tm, s, nullptr, s->A, true, M, mu_re + m * per_step,
+    LSQT_TRY(moment_set(h, R, c, tm, s, nullptr, s->A, kSelfProducts, M, mu_re + m * per_step,
                         mu_im + m * per_step));
     if (m < n_steps - 1) {
       const double t = time_step_scaled[m + 1];
@@ -929,14 +1010,23 @@ extern "C" int lsqt_block_spin(lsqt_ham* h, int R, const double* re, const doubl
   BlockShape c;
   TileMap tm;
   LSQT_TRY(block_setup(h, R, L, L, M, &c, &tm, &s));
+  BlockShape c2;
+  TileMap tm2;
+  bool pairs = false;
+  LSQT_TRY(pair_setup(h, R, c, M, &c2, &tm2, &s, &pairs));
   LSQT_TRY(evo_scratch(h, R, s));
   const size_t per_step = (size_t)M * R;
   for (int m = 0; m < n_steps; ++m) {
     // <state|T_m|S_z state> (src/sigma.cu:395-442)
+    float2* sz = pairs ? s->F[1] : s->A;
     launch_to_f32(state, L, nR, st);
-    launch_block_sz(L, h->d.n, R, s->A, st);
-    LSQT_TRY(moment_set(h, R, c, tm, s, L, s->A, false, M, mu_re + m * per_step,
-                        mu_im + m * per_step));
+    launch_block_sz(L, h->d.n, R, sz, st);
+    if (pairs)
+      LSQT_TRY(moment_set(h, R, c2, tm2, s, L, sz, kPairProducts, M, mu_re + m * per_step,
+                          mu_im + m * per_step));
+    else
+      LSQT_TRY(moment_set(h, R, c, tm, s, L, sz, kDirect, M, mu_re + m * per_step,
+                          mu_im + m * per_step));
     if (m < n_steps - 1)
       block_evolve(h, R, c, tm, s, 1, time_step_scaled[m], state, true);
   }

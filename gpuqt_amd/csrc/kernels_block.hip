@@ -191,7 +191,82 @@ __device__ __forceinline__ void tile_sums(double (&acc_re)[W], double (&acc_im)[
 enum BlockProducts {
   kLeft = 1,  // <L_r|out_r> into partial
   kSelf = 2,  // <psi1_r|out_r> into partial, <out_r|out_r> into partial2
+  // a block of R/2 pairs of recursions, columns 2p (from L_p) and 2p + 1
+  // (from R_p): <psi1_2p|out_2p+1> into partial, <out_2p|out_2p+1> into
+  // partial2, per pair p (W even)
+  kPair = 3,
 };
+
+// Per-lane FP64 accumulators of a tile's moment products.
+template <int W>
+struct Products {
+  double re[W], im[W], re2[W], im2[W];
+};
+
+template <int W>
+__device__ __forceinline__ void clear_products(Products<W>& a)
+{
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    a.re[w] = a.im[w] = a.re2[w] = a.im2[w] = 0.0;
+}
+
+// adds one site's products: o = the new values, p1 = psi1 at the site, l =
+// the left vector (kLeft only); conj(x) y in FP64 (exact products of FP32
+// values)
+template <int IPM, int W>
+__device__ __forceinline__ void add_products(Products<W>& a, const Vec<W>& o, const Vec<W>& p1,
+                                             const Vec<W>& l)
+{
+  if constexpr (IPM == kPair) {
+#pragma unroll
+    for (int w = 0; w + 1 < W; w += 2) {
+      const double lx = p1.v[w].x, ly = p1.v[w].y;  // psi_{m-1} of the L recursion
+      const double ax = o.v[w].x, ay = o.v[w].y;    // psi_m of the L recursion
+      const double rx = o.v[w + 1].x, ry = o.v[w + 1].y;  // psi_m of the R recursion
+      a.re[w / 2] += lx * rx + ly * ry;
+      a.im[w / 2] += lx * ry - ly * rx;
+      a.re2[w / 2] += ax * rx + ay * ry;
+      a.im2[w / 2] += ax * ry - ay * rx;
+    }
+  } else {
+    const Vec<W>& m = IPM == kSelf ? p1 : l;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const double ox = o.v[w].x, oy = o.v[w].y, lx = m.v[w].x, ly = m.v[w].y;
+      a.re[w] += ox * lx + oy * ly;
+      a.im[w] += oy * lx - ox * ly;
+      if constexpr (IPM == kSelf)
+        a.re2[w] += ox * ox + oy * oy;
+    }
+  }
+}
+
+// the tile's per-vector (or per-pair) sums into partial / partial2 [tile][.]
+template <int IPM, int W>
+__device__ __forceinline__ void store_products(Products<W>& a, int vpw, int R, int r0, int tile,
+                                               double (*s_red)[kBlock / 64][64 * 4],
+                                               double2* partial, double2* partial2)
+{
+  if constexpr (IPM == kPair) {
+    constexpr int WP = W >= 2 ? W / 2 : 1;
+    double r1[WP], i1[WP], r2[WP], i2[WP];
+#pragma unroll
+    for (int w = 0; w < WP; ++w) {
+      r1[w] = a.re[w];
+      i1[w] = a.im[w];
+      r2[w] = a.re2[w];
+      i2[w] = a.im2[w];
+    }
+    const int P = R / 2;
+    tile_sums<WP>(r1, i1, vpw, P, r0 / 2, s_red, partial + (size_t)tile * P);
+    tile_sums<WP>(r2, i2, vpw, P, r0 / 2, s_red, partial2 + (size_t)tile * P);
+  } else {
+    tile_sums<W>(a.re, a.im, vpw, R, r0, s_red, partial + (size_t)tile * R);
+    if constexpr (IPM == kSelf)
+      tile_sums<W>(a.re2, a.im2, vpw, R, r0, s_red, partial2 + (size_t)tile * R);
+  }
+}
 
 // One Chebyshev step for R vectors (RECUR: out = 2 H~ psi1 - psi0, else
 // out = H~ psi1) in FP32 arithmetic, fused with the per-vector moment
@@ -226,10 +301,8 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
   int begin, end;
   tile_range(tm, tile, begin, end);
   const float inv_scale = (float)(1.0 / h.emax);
-  double acc_re[W], acc_im[W], acc_nn[W];
-#pragma unroll
-  for (int w = 0; w < W; ++w)
-    acc_re[w] = acc_im[w] = acc_nn[w] = 0.0;
+  Products<W> acc;
+  clear_products(acc);
 
   // With a compile-time slot count the loop is software-pipelined: the words
   // of the next site are loaded while this site's gathers are in flight, so a
@@ -329,15 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
           }
         }
         stv<W, true>(out + base, o);
-        const Vec<W>& l = IPM == kSelf ? cur.p1 : cur.l;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-          const double ox = o.v[w].x, oy = o.v[w].y, lx = l.v[w].x, ly = l.v[w].y;
-          acc_re[w] += ox * lx + oy * ly;
-          acc_im[w] += oy * lx - ox * ly;
-          if constexpr (IPM == kSelf)
-            acc_nn[w] += ox * ox + oy * oy;
-        }
+        add_products<IPM, W>(acc, o, cur.p1, cur.l);
         cur = nxt;
         s = s1;
         wy = y1;
@@ -414,29 +479,14 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
       }
     }
     stv<W, true>(out + base, o);
-    if constexpr (IPM == kSelf)
-      l = p1;  // <psi1|out>: psi1 was loaded for the on-site term
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      const double ox = o.v[w].x, oy = o.v[w].y, lx = l.v[w].x, ly = l.v[w].y;
-      acc_re[w] += ox * lx + oy * ly;
-      acc_im[w] += oy * lx - ox * ly;
-      if constexpr (IPM == kSelf)
-        acc_nn[w] += ox * ox + oy * oy;
-    }
+    // kSelf / kPair use psi1, loaded for the on-site term
+    add_products<IPM, W>(acc, o, p1, l);
     s = sn;
     wy = yn;
   }
   }
 
-  tile_sums<W>(acc_re, acc_im, vpw, R, r0, s_red, partial + (size_t)tile * R);
-  if constexpr (IPM == kSelf) {
-    double zero[W];
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-      zero[w] = 0.0;
-    tile_sums<W>(acc_nn, zero, vpw, R, r0, s_red, partial2 + (size_t)tile * R);
-  }
+  store_products<IPM, W>(acc, vpw, R, r0, tile, s_red, partial, partial2);
 }
 
 // The block step on a lattice with the sliding window of WindowMap: the
@@ -533,10 +583,8 @@ __global__ __launch_bounds__(kBlock) void k_block_win(
   __syncthreads();
 
   const float inv_scale = (float)(1.0 / h.emax);
-  double acc_re[W], acc_im[W], acc_nn[W];
-#pragma unroll
-  for (int w = 0; w < W; ++w)
-    acc_re[w] = acc_im[w] = acc_nn[w] = 0.0;
+  Products<W> acc;
+  clear_products(acc);
   constexpr int C = KT;
   int s = xbase + y0 * wm.row + z0 + lane_site;
   uint32_t wnext[C];
@@ -626,30 +674,14 @@ __global__ __launch_bounds__(kBlock) void k_block_win(
       }
     }
     stv<W, true>(out + base, o);
-    if constexpr (IPM == kSelf)
-      l = p1;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      const double ox = o.v[w].x, oy = o.v[w].y, lx = l.v[w].x, ly = l.v[w].y;
-      acc_re[w] += ox * lx + oy * ly;
-      acc_im[w] += oy * lx - ox * ly;
-      if constexpr (IPM == kSelf)
-        acc_nn[w] += ox * ox + oy * oy;
-    }
+    add_products<IPM, W>(acc, o, p1, l);
     if (more)
       put((p + 3) & 3, na, nb);
     __syncthreads();
     s = sn;
   }
 
-  tile_sums<W>(acc_re, acc_im, vpw, R, r0, s_red, partial + (size_t)tile * R);
-  if constexpr (IPM == kSelf) {
-    double zero[W];
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-      zero[w] = 0.0;
-    tile_sums<W>(acc_nn, zero, vpw, R, r0, s_red, partial2 + (size_t)tile * R);
-  }
+  store_products<IPM, W>(acc, vpw, R, r0, tile, s_red, partial, partial2);
 }
 
 // mu[(k*R + r)*M + m0 + i] = sum over the tiles t of segment k of
@@ -691,6 +723,8 @@ __global__ __launch_bounds__(kBlock) void k_block_reduce(const double2* __restri
 }
 
 // partial[t][r] = sum over the tile of conj(L) psi for each vector (moment 0)
+// PAIRS: the R columns are R/2 pairs, partial[t][p] = conj(psi_2p) psi_2p+1
+template <bool PAIRS>
 __global__ __launch_bounds__(kBlock) void k_block_dot(const float2* __restrict__ psi,
                                                       const float2* __restrict__ L, int R,
                                                       TileMap tm, double2* __restrict__ partial,
@@ -702,12 +736,20 @@ __global__ __launch_bounds__(kBlock) void k_block_dot(const float2* __restrict__
     return;
   int b0, e0;
   tile_range(tm, tile, b0, e0);
-  const size_t begin = (size_t)b0 * R;
-  const size_t end = (size_t)e0 * R;
-  // thread t accumulates vector t % R over the tile, for R dividing kBlock
+  const int V = PAIRS ? R / 2 : R;  // products per site
+  const size_t begin = (size_t)b0 * V;
+  const size_t end = (size_t)e0 * V;
+  // thread t accumulates product t % V over the tile, for V dividing kBlock
   double re = 0.0, im = 0.0;
   for (size_t i = begin + threadIdx.x; i < end; i += kBlock) {
-    float2 p = psi[i], l = L[i];
+    float2 p, l;
+    if constexpr (PAIRS) {
+      l = psi[2 * i];
+      p = psi[2 * i + 1];
+    } else {
+      p = psi[i];
+      l = L[i];
+    }
     double px = p.x, py = p.y, lx = l.x, ly = l.y;
     re += px * lx + py * ly;
     im += py * lx - px * ly;
@@ -715,13 +757,13 @@ __global__ __launch_bounds__(kBlock) void k_block_dot(const float2* __restrict__
   s_re[threadIdx.x] = re;
   s_im[threadIdx.x] = im;
   __syncthreads();
-  for (int r = threadIdx.x; r < R; r += kBlock) {
+  for (int r = threadIdx.x; r < V; r += kBlock) {
     double a = 0.0, b = 0.0;
-    for (int k = r; k < kBlock; k += R) {
+    for (int k = r; k < kBlock; k += V) {
       a += s_re[k];
       b += s_im[k];
     }
-    partial[(size_t)tile * R + r] = make_double2(a, b);
+    partial[(size_t)tile * V + r] = make_double2(a, b);
   }
 }
 
@@ -1003,6 +1045,24 @@ __global__ __launch_bounds__(kBlock) void k_block_sz(const float2* __restrict__ 
   dst[idx] = ((idx / R) % 2 == 0) ? v : make_float2(-v.x, -v.y);
 }
 
+// dst[s][2r] = L[s][r], dst[s][2r + 1] = Rb[s][r]: two R-vector blocks as one
+// block of R pairs (columns 2r, 2r + 1); n * R < 2^31
+__global__ __launch_bounds__(kBlock) void k_block_interleave(const float2* __restrict__ L,
+                                                            const float2* __restrict__ Rb, int nR,
+                                                            float2* __restrict__ dst)
+{
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= nR)
+    return;
+  fvec4_t t;
+  const float2 a = L[idx], b = Rb[idx];
+  t.x = a.x;
+  t.y = a.y;
+  t.z = b.x;
+  t.w = b.y;
+  __builtin_nontemporal_store(t, reinterpret_cast<fvec4_t*>(dst) + idx);
+}
+
 // ==========================================================================
 // launchers
 // ==========================================================================
@@ -1128,9 +1188,11 @@ Grid block_grid(const TileMap& t)
 
 void launch_block_step(const HamDev& h, int R, int W, bool db, const TileMap& tm,
                        const float2* psi1, const float2* psi0, float2* out, const float2* L,
-                       double2* partial, double2* partial2, bool recur, bool self,
+                       double2* partial, double2* partial2, bool recur, int products,
                        hipStream_t st)
 {
+  static_assert(kStepLeft == kLeft && kStepSelf == kSelf && kStepPair == kPair,
+                "product modes");
   Grid g = block_grid(tm);
   if (tm.win.by > 0 && W <= 4 && (h.K == 3 || h.K == 4 || h.K == 6)) {
     // ring of four window rows; the tile sums reuse the same memory
@@ -1141,10 +1203,14 @@ void launch_block_step(const HamDev& h, int R, int W, bool db, const TileMap& tm
                                                                    L, partial, partial2,       \
                                                                    g.tiles, g.blocks)
 #define LSQTB_WIN_MODE                                                  \
-  if (self)                                                             \
+  if (products == kPair) {                                              \
+    if constexpr (WW >= 2)                                              \
+      recur ? LSQTB_WIN(true, kPair) : LSQTB_WIN(false, kPair);         \
+  } else if (products == kSelf) {                                       \
     recur ? LSQTB_WIN(true, kSelf) : LSQTB_WIN(false, kSelf);           \
-  else                                                                  \
-    recur ? LSQTB_WIN(true, kLeft) : LSQTB_WIN(false, kLeft);
+  } else {                                                              \
+    recur ? LSQTB_WIN(true, kLeft) : LSQTB_WIN(false, kLeft);           \
+  }
     LSQTB_LAYOUT_SWITCH(h, LAY, {
       switch (h.K) {
         case 3: {
@@ -1169,10 +1235,14 @@ void launch_block_step(const HamDev& h, int R, int W, bool db, const TileMap& tm
   k_block_step<LAY, KT, WW, RC, MODE, false, DBV><<<g.blocks, kBlock, 0, st>>>(   \
     h, R, tm, psi1, psi0, out, L, partial, partial2, g.tiles, g.blocks)
 #define LSQTB_STEP_DB(DBV)                                                  \
-  if (self)                                                                 \
+  if (products == kPair) {                                                  \
+    if constexpr (WW >= 2)                                                  \
+      recur ? LSQTB_STEP(true, kPair, DBV) : LSQTB_STEP(false, kPair, DBV); \
+  } else if (products == kSelf) {                                           \
     recur ? LSQTB_STEP(true, kSelf, DBV) : LSQTB_STEP(false, kSelf, DBV);   \
-  else                                                                      \
-    recur ? LSQTB_STEP(true, kLeft, DBV) : LSQTB_STEP(false, kLeft, DBV);
+  } else {                                                                  \
+    recur ? LSQTB_STEP(true, kLeft, DBV) : LSQTB_STEP(false, kLeft, DBV);   \
+  }
   LSQTB_LAYOUT_SWITCH(h, LAY, LSQTB_K_SWITCH(h.K, KT, LSQTB_W_SWITCH(W, WW, {
     if (db && KT > 0) {
       LSQTB_STEP_DB(true)
@@ -1185,10 +1255,13 @@ void launch_block_step(const HamDev& h, int R, int W, bool db, const TileMap& tm
 }
 
 void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& tm,
-                      double2* partial, hipStream_t st)
+                      double2* partial, hipStream_t st, bool pairs)
 {
   Grid g = block_grid(tm);
-  k_block_dot<<<g.blocks, kBlock, 0, st>>>(psi, L, R, tm, partial, g.tiles, g.blocks);
+  if (pairs)
+    k_block_dot<true><<<g.blocks, kBlock, 0, st>>>(psi, psi, R, tm, partial, g.tiles, g.blocks);
+  else
+    k_block_dot<false><<<g.blocks, kBlock, 0, st>>>(psi, L, R, tm, partial, g.tiles, g.blocks);
 }
 
 void launch_block_reduce(const double2* partial, const TileMap& tm, int R, int M, int m0,
@@ -1273,6 +1346,13 @@ void launch_vector_major(const float2* src, int n, int R, double* re, double* im
 {
   const int total = n * R;
   k_vector_major<<<(total + kBlock - 1) / kBlock, kBlock, 0, st>>>(src, n, R, re, im);
+}
+
+void launch_block_interleave(const float2* L, const float2* Rb, int n, int R, float2* dst,
+                             hipStream_t st)
+{
+  const int total = n * R;
+  k_block_interleave<<<(total + kBlock - 1) / kBlock, kBlock, 0, st>>>(L, Rb, total, dst);
 }
 
 void launch_block_sz(const float2* src, int n, int R, float2* dst, hipStream_t st)

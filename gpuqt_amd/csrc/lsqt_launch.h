@@ -60,15 +60,24 @@ TileMap make_window_map(int n, int slab, int row, int by, int chunk);
 // whether the window schedule applies to this lattice, tile height and shape
 bool window_fits(int n, int slab, int row, int by, int R, int W);
 Grid block_grid(const TileMap& t);
-// self: <psi1|out> into partial and <out|out> into partial2 (L unused);
-// otherwise <L|out> into partial
+// Moment products of a block step (values of the kernels' BlockProducts):
+// kStepLeft <L|out> into partial; kStepSelf <psi1|out> into partial and
+// <out|out> into partial2 (L unused); kStepPair: the R columns are R/2
+// pairs (2p from a left, 2p + 1 from a right recursion), <psi1_2p|out_2p+1>
+// into partial and <out_2p|out_2p+1> into partial2, [tile][R/2] (W >= 2)
+enum StepProducts { kStepLeft = 1, kStepSelf = 2, kStepPair = 3 };
 // db: double-buffered pass loop (compile-time slot counts only)
 void launch_block_step(const HamDev& h, int R, int W, bool db, const TileMap& tm,
                        const float2* psi1, const float2* psi0, float2* out, const float2* L,
-                       double2* partial, double2* partial2, bool recur, bool self,
+                       double2* partial, double2* partial2, bool recur, int products,
                        hipStream_t st);
+// partial[t][r] = tile sums of conj(L_r) psi_r; pairs: R columns as R/2
+// pairs, partial[t][p] = conj(psi_2p) psi_2p+1 (L unused)
 void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& tm,
-                      double2* partial, hipStream_t st);
+                      double2* partial, hipStream_t st, bool pairs = false);
+// dst ([n][2R]) = L and Rb ([n][R] each) as R pairs of columns (2r, 2r + 1)
+void launch_block_interleave(const float2* L, const float2* Rb, int n, int R, float2* dst,
+                             hipStream_t st);
 // block time evolution (op: 0 H~, 1 Chebyshev term of U, 2 [X, H~], 3 V,
 // 4 Chebyshev term of [X, U]); S is the FP64 accumulated state [n][R]
 void launch_block_evo(const HamDev& h, int R, int W, const TileMap& tm, int op,

@@ -93,6 +93,7 @@ SIGNATURES = {
     "lsqt_block_to_host": (C.c_int, [_vp, _dp, _dp]),
     "lsqt_moments_block": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp, _dp]),
     "lsqt_moments_block_self": (C.c_int, [_vp, _vp, C.c_int, _dp, _dp]),
+    "lsqt_moments_block_pair": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp, _dp]),
     "lsqt_block_tune": (C.c_int, [_vp, _vp, _vp, C.c_int, _ip, _ip, _ip, _ip, _dp]),
     "lsqt_block_set_config": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
 }
@@ -425,6 +426,17 @@ class Hamiltonian:
         S = self.segments
         re, im = np.empty(S * R * M), np.empty(S * R * M)
         _check(load().lsqt_moments_block_self(self.h, v.h, int(M), _dptr(re), _dptr(im)))
+        mu = (re + 1j * im).reshape(S, R, M)
+        return mu[0] if S == 1 else mu
+
+    def moments_block_pair(self, left, right, M):
+        """complex array (R, M) of <left_r|T_m|right_r> from the left and
+        right recursions run side by side (product identities, M/2 steps)"""
+        R = left.R
+        S = self.segments
+        re, im = np.empty(S * R * M), np.empty(S * R * M)
+        _check(load().lsqt_moments_block_pair(self.h, left.h, right.h, int(M), _dptr(re),
+                                              _dptr(im)))
         mu = (re + 1j * im).reshape(S, R, M)
         return mu[0] if S == 1 else mu
 

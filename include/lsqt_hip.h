@@ -244,6 +244,16 @@ int lsqt_moments_block(lsqt_ham* h, const lsqt_block* left, const lsqt_block* ri
  * FP32 rounding of the states. */
 int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, double* mu_re,
                             double* mu_im);
+/* mu[r*M + m] = <left_r|T_m(H~)|right_r> (as lsqt_moments_block) in M/2
+ * steps: the recursions of left and right run side by side as R pairs of
+ * columns of one 2R-vector block (2R <= 256), and <L|T_{2m}|R> =
+ * 2 <T_m L|T_m R> - <L|R>, <L|T_{2m+1}|R> = 2 <T_m L|T_{m+1} R> - <L|T_1|R>
+ * (T_m Hermitian): per vector, site and moment 24 bytes move instead of 32
+ * (two recursions of 24 bytes per step, two moments per step, no left
+ * stream).  The VAC and spin drivers below take this route whenever
+ * 2R <= 256. */
+int lsqt_moments_block_pair(lsqt_ham* h, const lsqt_block* left, const lsqt_block* right,
+                            int M, double* mu_re, double* mu_im);
 /* out = op(in) for every vector of a block: op 0 = H~ (apply), 1 = [X, H~]
  * (apply_commutator), 2 = V (apply_current); FP32 arithmetic
  * (src/hamiltonian.cu:225-237, 308-320, 387-399) */

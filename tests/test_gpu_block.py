@@ -224,15 +224,51 @@ def test_block_self_products(gpu_ctx, case, R):
 
 
 def test_block_left_right_differ(gpu_ctx):
-    """<L_r|T_m|R_r> with L != R (the VAC shape), against the FP64 path"""
+    """<L_r|T_m|R_r> with L != R (the VAC shape), directly and from the left
+    and right recursions side by side, against the FP64 path"""
     m, H = _setup(gpu_ctx, "cube_peierls")
     R = 4
     bl, lre, lim = _block_of(gpu_ctx, m, R)
     br, rre, rim = _block_of(gpu_ctx, m, R)
-    mu = H.moments_block(bl, br, 40)
+    mu = H.moments_block(bl, br, 41)
+    for M in (2, 3, 40, 41):
+        pair = H.moments_block_pair(bl, br, M)
+        assert pair.shape == (R, M)
+        np.testing.assert_allclose(pair, mu[:, :M], rtol=0, atol=2e-6 * np.abs(mu).max(),
+                                   err_msg=f"M={M}")
     for r in range(R):
-        ref = H.moments(H.vector(lre[r], lim[r]), H.vector(rre[r], rim[r]), 40)
+        ref = H.moments(H.vector(lre[r], lim[r]), H.vector(rre[r], rim[r]), 41)
         assert np.abs(mu[r] - ref).max() <= 2e-6 * np.abs(ref).max()
+        assert np.abs(pair[r] - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("case,R", [("config1_honeycomb_20k", 8), ("cube_peierls", 32),
+                                    ("honeycomb_small", 128)])
+def test_block_pairs_across_shapes(gpu_ctx, case, R):
+    """Pair moment sets (2R columns) for every block shape the pair route can
+    take, across the 64-slot partial ring, against the direct moments."""
+    print(f"[gpu] block pairs {case} R={R}", flush=True)
+    m, H = _setup(gpu_ctx, case)
+    bl, _, _ = _block_of(gpu_ctx, m, R)
+    br, _, _ = _block_of(gpu_ctx, m, R)
+    M = 150
+    H.block_set_config(R, max(1, R // 64), 256)
+    ref = H.moments_block(bl, br, M)
+    tol = 2e-6 * np.abs(ref).max()
+    for W in (1, 2, 4):
+        for rows in (256, 1024):
+            for depth in (1, 2):
+                if R % W or R // W > 64:
+                    continue
+                H.block_set_config(R, W, rows, depth=depth)
+                got = H.moments_block_pair(bl, br, M)
+                np.testing.assert_allclose(got, ref, rtol=0, atol=tol,
+                                           err_msg=f"W={W} rows={rows} depth={depth}")
+    H.block_set_config(R, 0)
+    got = H.moments_block_pair(bl, br, M)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=tol, err_msg="tuned shape")
+    for o in (bl, br, H):
+        o.close()
 
 
 def test_block_round_trip_and_errors(gpu_ctx):
