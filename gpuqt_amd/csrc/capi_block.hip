@@ -133,12 +133,13 @@ namespace {
 
 constexpr int kRing = 64;  // moment partials are reduced every kRing steps
 
+// pair: shapes of pair moment sets (R columns = R/2 pairs), tuned apart
 struct TuneKey {
-  int R, K, lds, size_class, slab;
+  int R, K, lds, size_class, slab, pair = 0;
   bool operator<(const TuneKey& o) const
   {
-    return std::tie(R, K, lds, size_class, slab) <
-           std::tie(o.R, o.K, o.lds, o.size_class, o.slab);
+    return std::tie(R, K, lds, size_class, slab, pair) <
+           std::tie(o.R, o.K, o.lds, o.size_class, o.slab, o.pair);
   }
 };
 
@@ -173,7 +174,8 @@ void tune_load()
   while (fscanf(f, "%31s %d %d %d %d %d %d %d %d %d %lf", arch, &k.R, &k.K, &k.lds,
                 &k.size_class, &k.slab, &v.shape.W, &v.shape.rows, &v.shape.blocked,
                 &v.shape.depth, &v.us) == 11) {
-    if (std::strcmp(arch, "gfx950") == 0)
+    k.pair = std::strcmp(arch, "gfx950-pair") == 0;
+    if (k.pair || std::strcmp(arch, "gfx950") == 0)
       g_tune[k] = v;
   }
   fclose(f);
@@ -188,8 +190,9 @@ void tune_store(const TuneKey& k, const TuneVal& v)
   FILE* f = fopen(path.c_str(), "a");
   if (!f)
     return;  // the cache file is an optimisation only
-  fprintf(f, "gfx950 %d %d %d %d %d %d %d %d %d %.3f\n", k.R, k.K, k.lds, k.size_class, k.slab,
-          v.shape.W, v.shape.rows, v.shape.blocked, v.shape.depth, v.us);
+  fprintf(f, "%s %d %d %d %d %d %d %d %d %d %.3f\n", k.pair ? "gfx950-pair" : "gfx950", k.R,
+          k.K, k.lds, k.size_class, k.slab, v.shape.W, v.shape.rows, v.shape.blocked,
+          v.shape.depth, v.us);
   fclose(f);
 }
 
@@ -474,19 +477,15 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, c
   }
 }
 
-// The self-product moments of A's vectors (A is overwritten):
-//   S_0 = <v|v>;  step m >= 1 makes psi_m (psi_0 = v) and adds
-//   S_{2m-1} = <psi_{m-1}|psi_m>, S_{2m} = <psi_m|psi_m>;
-// the moments follow from T_{2m} = 2 T_m^2 - T_0, T_{2m+1} = 2 T_m T_{m+1} - T_1
-// (H Hermitian): mu_0 = S_0, mu_1 = S_1, mu_k = 2 S_k - mu_{k mod 2}.  Moment k
-// >= 1 goes to ring slot (k - 1) mod kRing; M/2 steps give moments 0..M-1.
-// The product-identity recursion (M/2 steps): for R vectors (pairs false)
-// the raw products <psi_0|psi_0>, then per step m <psi_{m-1}|psi_m> and
-// <psi_m|psi_m> as entries 2m - 1 and 2m of mu; with pairs, A holds R pairs
-// of columns (left L_r, right R_r) and the products are <L|R>, then
-// <T_{m-1} L|T_m R> and <T_m L|T_m R>.  Either way mu_k = 2 S_k - mu_{k mod 2}
-// for k >= 2 (T_{2m} = 2 T_m^2 - T_0, T_{2m+1} = 2 T_m T_{m+1} - T_1, T_m
-// Hermitian).
+// The product-identity moments (A is overwritten).  R vectors (pairs
+// false): S_0 = <v|v>; step m >= 1 makes psi_m (psi_0 = v) and adds
+// S_{2m-1} = <psi_{m-1}|psi_m>, S_{2m} = <psi_m|psi_m>.  Pairs: A holds R
+// pairs of columns (a left vector L_r, a right vector R_r), both recursions
+// advance together and S_0 = <L|R>, S_{2m-1} = <T_{m-1} L|T_m R>, S_{2m} =
+// <T_m L|T_m R>.  The moments follow from T_{2m} = 2 T_m^2 - T_0 and
+// T_{2m+1} = 2 T_m T_{m+1} - T_1 with T_m Hermitian: mu_0 = S_0, mu_1 = S_1,
+// mu_k = 2 S_k - mu_{k mod 2}.  Moment k >= 1 goes to ring slot (k - 1) mod
+// kRing; M/2 steps give moments 0..M-1.
 void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, float2*& A,
                         float2*& B, double2* ring, double2* mu, int M, hipStream_t st,
                         bool pairs = false)
@@ -520,8 +519,10 @@ void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& 
   }
 }
 
+// pairs: R columns of R/2 pairs in `right` (left unused), shapes with W >= 2
+// timed as pair moment sets
 int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, BlockShape* out,
-                double* us_out, bool force);
+                double* us_out, bool force, bool pairs = false);
 
 }  // namespace
 
@@ -667,27 +668,40 @@ int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Block
 }
 
 // The shape and tile map of pair moment sets for R vectors (2R columns):
-// the R-vector shape c with twice the vectors per lane, so that lanes per
-// site, tiles and schedule stay those of c (else W = 2 on natural tiles of
-// 256 sites); scratch A / B of 2R columns.  *ok false (no error) where 2R
-// columns exceed the block limit.
-int pair_setup(lsqt_ham* h, int R, const BlockShape& c, int M, BlockShape* c2, TileMap* tm2,
-               BlockScratch** s, bool* ok)
+// autotuned on the pair block of (left, right), or, where the caller fixed
+// the R-vector shape c, c with twice the vectors per lane (the same lanes
+// per site, tiles and schedule; else W = 2 on natural tiles of 256 sites);
+// scratch A / B of 2R columns.  *ok false (no error) where 2R columns exceed
+// the block limit.
+int pair_setup(lsqt_ham* h, int R, const BlockShape& c, const float2* left,
+               const float2* right, int M, BlockShape* c2, TileMap* tm2, BlockScratch** s,
+               bool* ok)
 {
   *ok = false;
   const int R2 = 2 * R;
   if (!valid_R(R2) || (size_t)h->d.n * R2 > (size_t)INT_MAX)
     return LSQT_OK;
-  BlockShape p = c;
-  p.W = 2 * c.W;
-  if (!shape_usable(h, R2, p)) {
-    p = BlockShape{};
-    p.W = 2;
-    p.rows = 256;
-    if (!valid_W(R2, p.W))
-      p.W = 4;
-    if (!valid_W(R2, p.W))
-      return LSQT_OK;
+  BlockShape p;
+  if (h->blk && h->blk->forced.count(R)) {
+    // a shape set for R: the same lanes per site, tiles and schedule
+    p = c;
+    p.W = 2 * c.W;
+    if (!shape_usable(h, R2, p)) {
+      p = BlockShape{};
+      p.W = 2;
+      p.rows = 256;
+      if (!valid_W(R2, p.W))
+        p.W = 4;
+      if (!valid_W(R2, p.W))
+        return LSQT_OK;
+    }
+  } else {
+    // tuned on the pair block of (left, right), staged in B
+    BlockScratch* t;
+    LSQT_TRY(scratch_for(h, R2, 2, 1, &t));
+    launch_block_interleave(left, right, h->d.n, R, t->B, h->ctx->stream);
+    double us;
+    LSQT_TRY(pick_config(h, R2, nullptr, t->B, &p, &us, false, true));
   }
   LSQT_TRY(tile_map(h, R2, p, tm2));
   LSQT_TRY(scratch_for(h, R2, M, tm2->tiles, s));
@@ -740,7 +754,7 @@ extern "C" int lsqt_moments_block_pair(lsqt_ham* h, const lsqt_block* left,
   BlockScratch* s;
   LSQT_TRY(block_setup(h, R, left->d, right->d, M, &c, &tm, &s));
   bool pairs = false;
-  LSQT_TRY(pair_setup(h, R, c, M, &c2, &tm2, &s, &pairs));
+  LSQT_TRY(pair_setup(h, R, c, left->d, right->d, M, &c2, &tm2, &s, &pairs));
   if (!pairs)
     return set_error(LSQT_ERR_UNSUPPORTED, "R = %d vectors: pairs need 2R <= 256", R);
   return moment_set(h, R, c2, tm2, s, left->d, right->d, kPairProducts, M, mu_re, mu_im);
@@ -920,7 +934,7 @@ extern "C" int lsqt_block_vac(lsqt_ham* h, int R, const double* re, const double
   BlockShape c2;
   TileMap tm2;
   bool pairs = false;
-  LSQT_TRY(pair_setup(h, R, c, M, &c2, &tm2, &s, &pairs));
+  LSQT_TRY(pair_setup(h, R, c, L, L, M, &c2, &tm2, &s, &pairs));
   LSQT_TRY(evo_scratch(h, R, s));
   // state_right = V state_left (src/sigma.cu:293)
   launch_to_f32(left, s->B, nR, st);
@@ -1013,7 +1027,7 @@ extern "C" int lsqt_block_spin(lsqt_ham* h, int R, const double* re, const doubl
   BlockShape c2;
   TileMap tm2;
   bool pairs = false;
-  LSQT_TRY(pair_setup(h, R, c, M, &c2, &tm2, &s, &pairs));
+  LSQT_TRY(pair_setup(h, R, c, L, L, M, &c2, &tm2, &s, &pairs));
   LSQT_TRY(evo_scratch(h, R, s));
   const size_t per_step = (size_t)M * R;
   for (int m = 0; m < n_steps; ++m) {
@@ -1087,9 +1101,9 @@ extern "C" int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_b
 namespace {
 
 int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, BlockShape* out,
-                double* us_out, bool force)
+                double* us_out, bool force, bool pairs)
 {
-  if (h->blk) {
+  if (h->blk && !pairs) {
     auto fit = h->blk->forced.find(R);
     if (fit != h->blk->forced.end()) {
       *out = fit->second;
@@ -1101,12 +1115,14 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
   tune_load();
   // the slab of a detected lattice enters the key: blocked shapes exist only
   // for lattices, and their best tile size depends on the slab
-  TuneKey key{R, h->d.K, h->d.layout == kPackedLds ? 1 : 0, size_class(h->d.n), h->slab};
+  TuneKey key{R, h->d.K, h->d.layout == kPackedLds ? 1 : 0, size_class(h->d.n), h->slab,
+              pairs ? 1 : 0};
   auto it = g_tune.find(key);
   // a cached shape is taken only if it is valid here (the cache file is
   // plain text, and the key holds the slab, not the row length that a
   // window shape also depends on)
-  if (it != g_tune.end() && !force && shape_usable(h, R, it->second.shape)) {
+  if (it != g_tune.end() && !force && shape_usable(h, R, it->second.shape) &&
+      (!pairs || it->second.shape.W >= 2)) {
     *out = it->second.shape;
     *us_out = it->second.us;
     return LSQT_OK;
@@ -1115,7 +1131,7 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
   // blocked tiles of a whole number of rows that divide the slab
   std::vector<BlockShape> cands;
   for (int W : {1, 2, 4, 8}) {
-    if (!valid_W(R, W))
+    if (!valid_W(R, W) || (pairs && W < 2))
       continue;
     for (int rows : {256, 1024, 4096}) {
       if (rows < kBlock / (R / W))
@@ -1153,7 +1169,7 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
     if (h->slab > 0 && h->row > 0 && h->slab % h->row == 0) {
       const int ny = h->slab / h->row;
       for (int W : {1, 2, 4}) {
-        if (!valid_W(R, W))
+        if (!valid_W(R, W) || (pairs && W < 2))
           continue;
         int last = 0;
         for (int want : {8, 16, 32}) {
@@ -1195,14 +1211,21 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
     LSQT_TRY(tile_map(h, R, c, &tm));
     LSQT_HIP(hipMemcpyAsync(A, right, sizeof(float2) * (size_t)h->d.n * R,
                             hipMemcpyDeviceToDevice, st));
-    enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, steps, 0, 2, st);
-    LSQT_HIP(hipEventRecord(e0, st));
-    enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, steps, 2, steps, st);
-    LSQT_HIP(hipEventRecord(e1, st));
+    if (pairs) {
+      // a pair moment set of 2 * steps moments (steps recursion steps)
+      LSQT_HIP(hipEventRecord(e0, st));
+      enqueue_self_steps(h, R / 2, c, tm, A, B, s->ring, s->mu, 2 * steps, st, true);
+      LSQT_HIP(hipEventRecord(e1, st));
+    } else {
+      enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, steps, 0, 2, st);
+      LSQT_HIP(hipEventRecord(e0, st));
+      enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, steps, 2, steps, st);
+      LSQT_HIP(hipEventRecord(e1, st));
+    }
     LSQT_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     LSQT_HIP(hipEventElapsedTime(&ms, e0, e1));
-    double us = 1e3 * ms / (steps - 2);
+    double us = 1e3 * ms / (pairs ? steps : steps - 2);
     if (us < best.us)
       best = TuneVal{c, us};
   }
