@@ -188,6 +188,10 @@ def test_block_sliding_window(gpu_ctx, dims, open_y):
                                        err_msg=f"W={W} by={by}")
             np.testing.assert_allclose(H.moments_block_self(br, M), ref_self, rtol=0,
                                        atol=tol_self, err_msg=f"self W={W} by={by}")
+            if W <= 2:  # pairs: 2W vectors per lane on the same window
+                np.testing.assert_allclose(H.moments_block_pair(bl, br, M), ref, rtol=0,
+                                           atol=2e-6 * np.abs(ref).max(),
+                                           err_msg=f"pairs W={W} by={by}")
     assert tried > 0
     with pytest.raises(native.LsqtError):
         H.block_set_config(R, 8, 1, blocked=3)  # W > 4
