@@ -802,17 +802,6 @@ __device__ __forceinline__ void accumulate_phase(double2& s, double bm, double t
   }
 }
 
-template <int W>
-__device__ __forceinline__ void accumulate_block(double2* S, const Vec<W>& o, double bm, int label)
-{
-#pragma unroll
-  for (int w = 0; w < W; ++w) {
-    double2 v = S[w];
-    accumulate_phase(v, bm, (double)o.v[w].x, (double)o.v[w].y, label);
-    S[w] = v;
-  }
-}
-
 template <int LAYOUT, int KT, int W, int OP, bool NT>
 __global__ __launch_bounds__(kBlock) void k_block_evo(
   HamDev h, int R, TileMap tm, const float2* s0, const float2* __restrict__ s1, float2* s2,
@@ -847,6 +836,20 @@ __global__ __launch_bounds__(kBlock) void k_block_evo(
   for (int s = begin + lane_site; s < end; s += per_pass) {
     const size_t base = (size_t)s * R + r0;
     float tr[W], ti[W], xr[W], xi[W];
+    // the streams read once per pass (psi_{m-2}, the accumulated state) are
+    // loaded with the first loads of the pass, so that their latency
+    // overlaps the gathers (s0 may be s2: each site is read before written)
+    constexpr bool kRecur = OP == kCheb2 || OP == kCheb2x;
+    Vec<W> p0, p0x;
+    double2 Sv[W];
+    if constexpr (kRecur) {
+      p0 = ldv<W, NT>(s0 + base);
+      if constexpr (OP == kCheb2x)
+        p0x = ldv<W, NT>(s0x + base);
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        Sv[w] = S[base + w];
+    }
     if constexpr (kOnsite) {
       const float v = ld<NT>(h.pot32 + s);
       Vec<W> p1 = ldv<W, false>(s1 + base);
@@ -931,22 +934,25 @@ __global__ __launch_bounds__(kBlock) void k_block_evo(
         o.v[w] = make_float2(ti[w], -tr[w]);
       stv<W>(s2 + base, o);
     } else {
-      Vec<W> p0 = ldv<W, NT>(s0 + base);
 #pragma unroll
       for (int w = 0; w < W; ++w)
         o.v[w] = make_float2(__builtin_fmaf(2.0f, tr[w] * inv_scale, -p0.v[w].x),
                              __builtin_fmaf(2.0f, ti[w] * inv_scale, -p0.v[w].y));
       stv<W>(s2 + base, o);
-      if constexpr (OP == kCheb2) {
-        accumulate_block<W>(S + base, o, bm, label);
-      } else {
-        Vec<W> p0x = ldv<W, NT>(s0x + base), ox;
+      const Vec<W>* acc = &o;
+      Vec<W> ox;
+      if constexpr (OP == kCheb2x) {
 #pragma unroll
         for (int w = 0; w < W; ++w)
           ox.v[w] = make_float2(__builtin_fmaf(2.0f, xr[w] * inv_scale, -p0x.v[w].x),
                                 __builtin_fmaf(2.0f, xi[w] * inv_scale, -p0x.v[w].y));
         stv<W>(s2x + base, ox);
-        accumulate_block<W>(S + base, ox, bm, label);
+        acc = &ox;
+      }
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        accumulate_phase(Sv[w], bm, (double)acc->v[w].x, (double)acc->v[w].y, label);
+        S[base + w] = Sv[w];
       }
     }
   }
