@@ -15,6 +15,13 @@
 // moment products <L_r|psi_r> are formed and summed in FP64 (a product of two
 // FP32 numbers is exact in FP64), reduced per vector by a fixed shuffle / LDS
 // tree into per-tile partials [M][tiles][R] and then in tile order.
+//
+// Kernels: k_block_step (one recursion step with its moment products: <L|.>,
+// the self products of the product identities, or pairs of a left and a
+// right recursion), k_block_win (the same step on a lattice with rows
+// y-1..y+1 of the recursion block staged in LDS), k_block_evo (the terms of
+// U(t), [X, U(t)] and the operators, FP64 accumulated state), reductions,
+// norms and layout changes.
 #include <algorithm>
 
 #include "lsqt_device.h"
