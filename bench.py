@@ -695,8 +695,12 @@ def main():
             f"MSD max {max(errs):.2e}")
 
     result = {
-        "metric": "Chebyshev site-updates/s (FP64 KPM + Bessel evolution), HBM GB/s of the "
-                  "fused recursion kernel",
+        "metric": "Chebyshev site-updates/s + HBM GB/s; block-FP32 vs plain FP64 at stated "
+                  "error",
+        "metric_detail": "value: FP64 KPM + Bessel evolution site-updates/s of the whole job; "
+                         "roofline: HBM GB/s of the fused recursion kernel; precision_f32 and "
+                         "batching_block_f32: the FP32 and block-FP32 paths against plain FP64 "
+                         "with their errors",
         "value": value,
         "unit": "site-updates/s",
         "n_gpus": 1,
