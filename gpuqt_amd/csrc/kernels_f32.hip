@@ -142,6 +142,13 @@ __global__ __launch_bounds__(kBlock) void k_hpsi32(HamDev h, const float2* __res
   if (n < h.n) {
     fvec4_t p1 = *reinterpret_cast<const fvec4_t*>(psi1 + n);
     fvec2_t v = ld<NT>(reinterpret_cast<const fvec2_t*>(h.pot32 + n));
+    // the once-read streams are loaded with the first loads (their latency
+    // overlaps the gathers; out may be psi0: each site is read before written)
+    fvec4_t p0, l;
+    if constexpr (RECUR)
+      p0 = ld<NT>(reinterpret_cast<const fvec4_t*>(psi0 + n));
+    if constexpr (IP)
+      l = ld<NT>(reinterpret_cast<const fvec4_t*>(L + n));
     double tr0 = (double)v.x * (double)p1.x, ti0 = (double)v.x * (double)p1.y;
     double tr1 = (double)v.y * (double)p1.z, ti1 = (double)v.y * (double)p1.w;
     constexpr int C = SlotBatch<KT>::C;
@@ -189,7 +196,6 @@ __global__ __launch_bounds__(kBlock) void k_hpsi32(HamDev h, const float2* __res
     tr1 /= h.emax;
     ti1 /= h.emax;
     if constexpr (RECUR) {
-      fvec4_t p0 = ld<NT>(reinterpret_cast<const fvec4_t*>(psi0 + n));
       tr0 = 2.0 * tr0 - (double)p0.x;
       ti0 = 2.0 * ti0 - (double)p0.y;
       tr1 = 2.0 * tr1 - (double)p0.z;
@@ -202,7 +208,6 @@ __global__ __launch_bounds__(kBlock) void k_hpsi32(HamDev h, const float2* __res
     o.w = (float)ti1;
     *reinterpret_cast<fvec4_t*>(out + n) = o;
     if constexpr (IP) {
-      fvec4_t l = ld<NT>(reinterpret_cast<const fvec4_t*>(L + n));
       double ox = o.x, oy = o.y, lx = l.x, ly = l.y;
       acc_re = ox * lx + oy * ly;
       acc_im = oy * lx - ox * ly;
