@@ -141,9 +141,11 @@ __global__ __launch_bounds__(kBlock) void k_hpsi32(HamDev h, const float2* __res
   double acc_re = 0.0, acc_im = 0.0;
   if (n < h.n) {
     fvec4_t p1 = *reinterpret_cast<const fvec4_t*>(psi1 + n);
-    fvec2_t v = ld<NT>(reinterpret_cast<const fvec2_t*>(h.pot32 + n));
-    // the once-read streams are loaded with the first loads (their latency
-    // overlaps the gathers; out may be psi0: each site is read before written)
+    // slot words and potential with ordinary loads (non-temporal: 1 % slower
+    // here), the once-read streams non-temporally and with the first loads
+    // (their latency overlaps the gathers; out may be psi0: each site is
+    // read before written)
+    fvec2_t v = *reinterpret_cast<const fvec2_t*>(h.pot32 + n);
     fvec4_t p0, l;
     if constexpr (RECUR)
       p0 = ld<NT>(reinterpret_cast<const fvec4_t*>(psi0 + n));
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi32(HamDev h, const float2* __res
 #pragma unroll
       for (int q = 0; q < C; ++q)
         w[q] = (m0 + q < K)
-                 ? ld<NT>(reinterpret_cast<const uvec2_t*>(h.words + (size_t)(m0 + q) * h.stride + n))
+                 ? ld<false>(reinterpret_cast<const uvec2_t*>(h.words + (size_t)(m0 + q) * h.stride + n))
                  : uvec2_t{kEmptySlot, kEmptySlot};
       int j0[C], j1[C];
       double a0[C], b0[C], a1[C], b1[C];
