@@ -264,7 +264,8 @@ int lsqt_block_apply(lsqt_ham* h, int op, const lsqt_block* in, lsqt_block* out)
  * Model::initialize_state makes them), time_step_scaled[t] = dt_t *
  * energy_max, and mu[((t*S + k)*R + r)*M + m] receives moment m of time step
  * t, segment k (S = 1 unless the Hamiltonian is segmented), vector r.
- *   lsqt_block_vac: find_vac (src/sigma.cu:278-329): <U(-t) V phi|T_m|V U(-t) phi>
+ *   lsqt_block_vac: find_vac (src/sigma.cu:278-329): <U(-t) V phi|T_m|V U(-t) phi>,
+ *                   as pair moment sets (lsqt_moments_block_pair) when 2R <= 256
  *   lsqt_block_msd: find_msd (src/sigma.cu:333-393): <[X,U(t)] phi|T_m|[X,U(t)] phi>,
  *                   by the product identities (M/2 recursion steps) */
 int lsqt_block_vac(lsqt_ham* h, int R, const double* re, const double* im, int M, int n_steps,
@@ -272,7 +273,8 @@ int lsqt_block_vac(lsqt_ham* h, int R, const double* re, const double* im, int M
 int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double* im, int M, int n_steps,
                    const double* time_step_scaled, double* mu_re, double* mu_im);
 /*   lsqt_block_spin: find_spin_polarization (src/sigma.cu:395-442):
- *                   <U(t) phi|T_m|S_z U(t) phi> (one system only) */
+ *                   <U(t) phi|T_m|S_z U(t) phi> (one system only), as pair
+ *                   moment sets when 2R <= 256 */
 int lsqt_block_spin(lsqt_ham* h, int R, const double* re, const double* im, int M, int n_steps,
                     const double* time_step_scaled, double* mu_re, double* mu_im);
 /* Block moment sets are captured once per shape and buffer set and replayed
