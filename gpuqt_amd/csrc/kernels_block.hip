@@ -496,17 +496,123 @@ __global__ __launch_bounds__(kBlock) void k_block_step(
   store_products<IPM, W>(acc, vpw, R, r0, tile, s_red, partial, partial2);
 }
 
-// The block step on a lattice with the sliding window of WindowMap: the
-// workgroup sweeps the rows y0 .. y0 + by - 1 of one chunk of one slab; rows
-// y - 1, y, y + 1 of psi1 (chunk plus one site either side) sit in a ring
-// of four LDS slots, and row y + 2 is loaded while row y is computed.
-// Neighbours inside the window (same slab, rows y - 1 .. y + 1, periodic
-// wrap in y and z) are read from LDS, all others (the neighbouring slabs and
-// anything the lattice does not predict) from global memory, so any
-// Hamiltonian gives the same values as k_block_step; only how often a row
-// of psi1 crosses the L1 / L2 path changes (once per row instead of once per
-// neighbour).  Same arithmetic and products as k_block_step; per-tile sums
-// over the tile's by x chunk sites.
+// One tile of the sliding window of WindowMap: the tile is rows y0 .. y0 +
+// by - 1 of one chunk (z0 .. z0 + P - 1) of one slab; rows y - 1, y, y + 1
+// of a source block (the chunk plus one site either side, periodic in y and
+// z) sit in a ring of four LDS slots of (P + 2) * R float2.  Tile order:
+// chunk fastest, then slab, then row block, so the tiles in flight on one
+// XCD cover whole lattice rows (contiguous memory) of a run of neighbouring
+// slabs (their x-neighbour gathers hit that XCD's L2).
+template <int W>
+struct WindowTile {
+  int slab, row, ny, by, R, P, span, z0, y0, xbase;
+  Divisor row_div;
+  // a window row is (P + 2) * vpw items of W vectors: item i = (position
+  // i / vpw, lane group i % vpw); thread t loads item t and, for t < 2 vpw,
+  // item kBlock + t
+  int d_a, g_a, d_b, g_b;
+  bool has_b;
+
+  __device__ WindowTile(const WindowMap& m, int R_, int tile)
+  {
+    slab = m.slab;
+    row = m.row;
+    ny = m.ny;
+    by = m.by;
+    row_div = m.row_div;
+    R = R_;
+    P = m.chunk;
+    span = (P + 2) * R;
+    z0 = (tile % m.chunks) * P;
+    const int u = tile / m.chunks;
+    xbase = (u % m.slabs) * slab;
+    y0 = (u / m.slabs) * by;
+    const int vpw = R / W;
+    d_a = threadIdx.x / vpw;
+    g_a = threadIdx.x % vpw;
+    has_b = threadIdx.x < 2 * vpw;
+    d_b = (kBlock + threadIdx.x) / vpw;
+    g_b = (kBlock + threadIdx.x) % vpw;
+  }
+  // site at position d (0 .. P + 1) of window row y (any integer row)
+  __device__ int site(int y, int d) const
+  {
+    int yy = y;
+    if (yy < 0)
+      yy += ny;
+    else if (yy >= ny)
+      yy -= ny;
+    int z = z0 - 1 + d;
+    if (z < 0)
+      z += row;
+    else if (z >= row)
+      z -= row;
+    return xbase + yy * row + z;
+  }
+  __device__ void fetch(const float2* src, int y, Vec<W>& va, Vec<W>& vb) const
+  {
+    va = ldv<W, false>(src + (size_t)site(y, d_a) * R + g_a * W);
+    if (has_b)
+      vb = ldv<W, false>(src + (size_t)site(y, d_b) * R + g_b * W);
+  }
+  __device__ void put(float2* slots, int slot, const Vec<W>& va, const Vec<W>& vb) const
+  {
+    stv<W>(slots + slot * span + d_a * R + g_a * W, va);
+    if (has_b)
+      stv<W>(slots + slot * span + d_b * R + g_b * W, vb);
+  }
+  // rows y0 - 1 .. y0 + 1 into slots 0 .. 2 (callers then synchronise)
+  __device__ void prologue(const float2* src, float2* slots) const
+  {
+    Vec<W> a0, b0, a1, b1, a2, b2;
+    fetch(src, y0 - 1, a0, b0);
+    fetch(src, y0, a1, b1);
+    fetch(src, y0 + 1, a2, b2);
+    put(slots, 0, a0, b0);
+    put(slots, 1, a1, b1);
+    put(slots, 2, a2, b2);
+  }
+  // slot offset (float2) of neighbour j's values for vectors r0.. in the
+  // window while row y = y0 + p is swept, or -1 where j lies outside it
+  // (other slab, |dy| > 1, or z outside the chunk +- 1)
+  __device__ int pos(int j, int p, int r0) const
+  {
+    const int rr = j - xbase;
+    if ((unsigned)rr >= (unsigned)slab)
+      return -1;
+    const int yj = div_by(rr, row_div);
+    int dy = yj - (y0 + p);
+    if (dy == ny - 1)
+      dy = -1;
+    else if (dy == 1 - ny)
+      dy = 1;
+    int dz = rr - yj * row - z0 + 1;  // -row < dz <= row
+    if (dz < 0)
+      dz += row;
+    else if (dz >= row)
+      dz -= row;
+    if (dy < -1 || dy > 1 || dz >= P + 2)
+      return -1;
+    return ((p + 1 + dy) & 3) * span + dz * R + r0;
+  }
+  static __device__ Vec<W> read(const float2* slots, int pos)
+  {
+    Vec<W> v;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      v.v[w] = slots[pos + w];
+    return v;
+  }
+};
+
+// The block step on a lattice with the sliding window (WindowTile): the
+// workgroup sweeps the rows of its tile; row y + 2 of psi1 is loaded while
+// row y is computed.  Neighbours inside the window are read from LDS, all
+// others (the neighbouring slabs and anything the lattice does not predict)
+// from global memory, so any Hamiltonian gives the same values as
+// k_block_step; only how often a row of psi1 crosses the L1 / L2 path
+// changes (once per row instead of once per neighbour).  Same arithmetic and
+// products as k_block_step; per-tile sums over the tile's by x chunk sites.
 template <int LAYOUT, int KT, int W, bool RECUR, int IPM>
 __global__ __launch_bounds__(kBlock) void k_block_win(
   HamDev h, int R, TileMap tm, const float2* __restrict__ psi1, const float2* psi0, float2* out,
@@ -528,81 +634,26 @@ __global__ __launch_bounds__(kBlock) void k_block_win(
       s_tab[i] = make_float2((float)d.x, (float)d.y);
     }
   }
-  const WindowMap& wm = tm.win;
-  const int vpw = R / W;                 // lanes per site
-  const int P = wm.chunk;                // sites per pass (kBlock / vpw)
+  const WindowTile<W> wt(tm.win, R, tile);
+  const int vpw = R / W;  // lanes per site
   const int lane_site = threadIdx.x / vpw;
   const int r0 = (threadIdx.x % vpw) * W;
-  const int span = (P + 2) * R;          // float2 per LDS slot
-  // tile order: chunk fastest, then slab, then row block, so the tiles in
-  // flight on one XCD cover whole lattice rows (contiguous memory) of a run
-  // of neighbouring slabs (their x-neighbour gathers hit that XCD's L2)
-  const int z0 = (tile % wm.chunks) * P;
-  const int u = tile / wm.chunks;
-  const int x = u % wm.slabs;
-  const int y0 = (u / wm.slabs) * wm.by;
-  const int xbase = x * wm.slab;
-  // site at position d (0 .. P + 1) of the window row y (any integer row)
-  auto win_site = [&](int y, int d) {
-    int yy = y;
-    if (yy < 0)
-      yy += wm.ny;
-    else if (yy >= wm.ny)
-      yy -= wm.ny;
-    int z = z0 - 1 + d;
-    if (z < 0)
-      z += wm.row;
-    else if (z >= wm.row)
-      z -= wm.row;
-    return xbase + yy * wm.row + z;
-  };
-  // a window row is (P + 2) * vpw items of W vectors: item i = (d, lane
-  // group i % vpw); thread t loads item t and, for t < 2 vpw, item kBlock + t
-  const int d_a = threadIdx.x / vpw, g_a = threadIdx.x % vpw;
-  const bool has_b = threadIdx.x < 2 * vpw;
-  const int d_b = (kBlock + threadIdx.x) / vpw, g_b = (kBlock + threadIdx.x) % vpw;
-  auto fetch = [&](int y, Vec<W>& va, Vec<W>& vb) {
-    va = ldv<W, false>(psi1 + (size_t)win_site(y, d_a) * R + g_a * W);
-    if (has_b)
-      vb = ldv<W, false>(psi1 + (size_t)win_site(y, d_b) * R + g_b * W);
-  };
-  auto put = [&](int slot, const Vec<W>& va, const Vec<W>& vb) {
-    stv<W>(s_win + slot * span + d_a * R + g_a * W, va);
-    if (has_b)
-      stv<W>(s_win + slot * span + d_b * R + g_b * W, vb);
-  };
-  auto lds_row = [&](int pos) {
-    Vec<W> v;
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-      v.v[w] = s_win[pos + w];
-    return v;
-  };
-  {
-    Vec<W> a0, b0, a1, b1, a2, b2;
-    fetch(y0 - 1, a0, b0);
-    fetch(y0, a1, b1);
-    fetch(y0 + 1, a2, b2);
-    put(0, a0, b0);
-    put(1, a1, b1);
-    put(2, a2, b2);
-  }
+  wt.prologue(psi1, s_win);
   __syncthreads();
 
   const float inv_scale = (float)(1.0 / h.emax);
   Products<W> acc;
   clear_products(acc);
   constexpr int C = KT;
-  int s = xbase + y0 * wm.row + z0 + lane_site;
+  int s = wt.xbase + wt.y0 * wt.row + wt.z0 + lane_site;
   uint32_t wnext[C];
   load_words<C, false>(h, s, wnext);
-  for (int p = 0; p < wm.by; ++p) {
-    const int y = y0 + p;
-    const bool more = p + 1 < wm.by;
+  for (int p = 0; p < wt.by; ++p) {
+    const bool more = p + 1 < wt.by;
     Vec<W> na, nb;
     if (more)
-      fetch(y + 2, na, nb);
-    const int sn = more ? s + wm.row : s;
+      wt.fetch(psi1, wt.y0 + p + 2, na, nb);
+    const int sn = more ? s + wt.row : s;
     const size_t base = (size_t)s * R + r0;
     const float v = h.pot32[s];
     Vec<W> p0, l;
@@ -620,28 +671,10 @@ __global__ __launch_bounds__(kBlock) void k_block_win(
       load_words<C, false>(h, sn, wnext);
       decode_words32<LAYOUT, C>(h, s_tab, s, wcur, j, a, b);
     }
-    // window position of each neighbour (-1: outside, read from global)
     int pos[C];
 #pragma unroll
-    for (int q = 0; q < C; ++q) {
-      pos[q] = -1;
-      const int rr = j[q] - xbase;
-      if ((unsigned)rr < (unsigned)wm.slab) {
-        const int yj = div_by(rr, wm.row_div);
-        int dy = yj - y;
-        if (dy == wm.ny - 1)
-          dy = -1;
-        else if (dy == 1 - wm.ny)
-          dy = 1;
-        int dz = rr - yj * wm.row - z0 + 1;  // -row < dz <= row
-        if (dz < 0)
-          dz += wm.row;
-        else if (dz >= wm.row)
-          dz -= wm.row;
-        if (dy >= -1 && dy <= 1 && dz < P + 2)
-          pos[q] = ((p + 1 + dy) & 3) * span + dz * R + r0;
-      }
-    }
+    for (int q = 0; q < C; ++q)
+      pos[q] = wt.pos(j[q], p, r0);
     Vec<W> c[C];
 #pragma unroll
     for (int q = 0; q < C; ++q)
@@ -650,8 +683,9 @@ __global__ __launch_bounds__(kBlock) void k_block_win(
 #pragma unroll
     for (int q = 0; q < C; ++q)
       if (pos[q] >= 0)
-        c[q] = lds_row(pos[q]);
-    const Vec<W> p1 = lds_row(((p + 1) & 3) * span + (lane_site + 1) * R + r0);
+        c[q] = WindowTile<W>::read(s_win, pos[q]);
+    const Vec<W> p1 =
+      WindowTile<W>::read(s_win, ((p + 1) & 3) * wt.span + (lane_site + 1) * R + r0);
     float tr[W], ti[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -683,7 +717,7 @@ __global__ __launch_bounds__(kBlock) void k_block_win(
     stv<W, true>(out + base, o);
     add_products<IPM, W>(acc, o, p1, l);
     if (more)
-      put((p + 3) & 3, na, nb);
+      wt.put(s_win, (p + 3) & 3, na, nb);
     __syncthreads();
     s = sn;
   }
@@ -962,6 +996,198 @@ __global__ __launch_bounds__(kBlock) void k_block_evo(
         S[base + w] = Sv[w];
       }
     }
+  }
+}
+
+// k_block_evo on a lattice with the sliding window (WindowTile): the source
+// block s1 (and s1x for the [X, U] term) is staged row by row in LDS and
+// neighbours inside the window are read from there; the same arithmetic, in
+// the same order, as k_block_evo.
+template <int LAYOUT, int KT, int W, int OP>
+__global__ __launch_bounds__(kBlock) void k_block_evo_win(
+  HamDev h, int R, TileMap tm, const float2* s0, const float2* __restrict__ s1, float2* s2,
+  const float2* s0x, const float2* __restrict__ s1x, float2* s2x, double2* __restrict__ S,
+  double bm, int label, int tiles, int blocks)
+{
+  static_assert(KT > 0, "the window kernel needs a compile-time slot count");
+  constexpr bool kXX = OP == kComm || OP == kCurrent || OP == kCheb2x;
+  constexpr bool kOnsite = OP == kApplyH || OP == kCheb2 || OP == kCheb2x;
+  constexpr bool kRecur = OP == kCheb2 || OP == kCheb2x;
+  constexpr bool kTwo = OP == kCheb2x;  // windows of s1 and s1x
+  __shared__ float2 s_tab[LAYOUT == kPackedLds ? kBlockLdsTableMax : 1];
+  __shared__ float s_xx[LAYOUT == kPackedLds && kXX ? kBlockLdsTableMax : 1];
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  float2* s_win = reinterpret_cast<float2*>(s_dyn);
+  int tile = xcd_tile(blockIdx.x, blocks);
+  if (tile >= tiles)
+    return;
+  if constexpr (LAYOUT == kPackedLds) {
+    for (int i = threadIdx.x; i < h.ncls; i += kBlock) {
+      const double2 d = h.tab[i];
+      s_tab[i] = make_float2((float)d.x, (float)d.y);
+      if constexpr (kXX)
+        s_xx[i] = (float)h.tab_xx[i];
+    }
+  }
+  const WindowTile<W> wt(tm.win, R, tile);
+  float2* s_winx = s_win + 4 * wt.span;
+  const int vpw = R / W;
+  const int lane_site = threadIdx.x / vpw;
+  const int r0 = (threadIdx.x % vpw) * W;
+  wt.prologue(s1, s_win);
+  if constexpr (kTwo)
+    wt.prologue(s1x, s_winx);
+  __syncthreads();
+
+  const float inv_scale = (float)(1.0 / h.emax);
+  constexpr int C = KT;
+  int s = wt.xbase + wt.y0 * wt.row + wt.z0 + lane_site;
+  uint32_t wnext[C];
+  load_words<C, false>(h, s, wnext);
+  for (int p = 0; p < wt.by; ++p) {
+    const bool more = p + 1 < wt.by;
+    Vec<W> na, nb, nax, nbx;
+    if (more) {
+      wt.fetch(s1, wt.y0 + p + 2, na, nb);
+      if constexpr (kTwo)
+        wt.fetch(s1x, wt.y0 + p + 2, nax, nbx);
+    }
+    const int sn = more ? s + wt.row : s;
+    const size_t base = (size_t)s * R + r0;
+    Vec<W> p0, p0x;
+    double2 Sv[W];
+    if constexpr (kRecur) {
+      p0 = ldv<W, true>(s0 + base);
+      if constexpr (kTwo)
+        p0x = ldv<W, true>(s0x + base);
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        Sv[w] = S[base + w];
+    }
+    const int own = ((p + 1) & 3) * wt.span + (lane_site + 1) * R + r0;
+    float tr[W], ti[W], xr[W], xi[W];
+    if constexpr (kOnsite) {
+      const float v = h.pot32[s];
+      const Vec<W> p1 = WindowTile<W>::read(s_win, own);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        tr[w] = v * p1.v[w].x;
+        ti[w] = v * p1.v[w].y;
+      }
+      if constexpr (kTwo) {
+        const Vec<W> p1x = WindowTile<W>::read(s_winx, own);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          xr[w] = v * p1x.v[w].x;
+          xi[w] = v * p1x.v[w].y;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        tr[w] = ti[w] = 0.0f;
+    }
+    uint32_t wd[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      wd[q] = wnext[q];
+    load_words<C, false>(h, sn, wnext);
+    int j[C];
+    float a[C], b[C], x[C];
+    decode_words32<LAYOUT, C>(h, s_tab, s, wd, j, a, b);
+    if constexpr (kXX) {
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        const int c = wd[q] != kEmptySlot ? (int)(wd[q] >> h.idx_bits) : 0;
+        x[q] = (LAYOUT == kPackedLds) ? s_xx[c] : (float)h.tab_xx[c];
+      }
+    }
+    int pos[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      pos[q] = wt.pos(j[q], p, r0);
+    Vec<W> c[C], cx[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      if (pos[q] < 0) {
+        c[q] = ldv<W, false>(s1 + (size_t)j[q] * R + r0);
+        if constexpr (kTwo)
+          cx[q] = ldv<W, false>(s1x + (size_t)j[q] * R + r0);
+      }
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      if (pos[q] >= 0) {
+        c[q] = WindowTile<W>::read(s_win, pos[q]);
+        if constexpr (kTwo)
+          cx[q] = WindowTile<W>::read(s_winx, pos[q]);
+      }
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const float cr = c[q].v[w].x, ci = c[q].v[w].y;
+        const float hr = __builtin_fmaf(a[q], cr, -b[q] * ci);
+        const float hi = __builtin_fmaf(a[q], ci, b[q] * cr);
+        if constexpr (OP == kApplyH || OP == kCheb2) {
+          tr[w] += hr;
+          ti[w] += hi;
+        } else if constexpr (OP == kComm) {
+          tr[w] = __builtin_fmaf(-hr, x[q], tr[w]);
+          ti[w] = __builtin_fmaf(-hi, x[q], ti[w]);
+        } else if constexpr (OP == kCurrent) {
+          tr[w] = __builtin_fmaf(hr, x[q], tr[w]);
+          ti[w] = __builtin_fmaf(hi, x[q], ti[w]);
+        } else {  // kCheb2x
+          tr[w] += hr;
+          ti[w] += hi;
+          const float dr = cx[q].v[w].x, di = cx[q].v[w].y;
+          xr[w] += __builtin_fmaf(a[q], dr, -b[q] * di);
+          xi[w] += __builtin_fmaf(a[q], di, b[q] * dr);
+          xr[w] = __builtin_fmaf(-hr, x[q], xr[w]);
+          xi[w] = __builtin_fmaf(-hi, x[q], xi[w]);
+        }
+      }
+    }
+    Vec<W> o;
+    if constexpr (OP == kApplyH || OP == kComm) {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        o.v[w] = make_float2(tr[w] * inv_scale, ti[w] * inv_scale);
+      stv<W>(s2 + base, o);
+    } else if constexpr (OP == kCurrent) {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        o.v[w] = make_float2(ti[w], -tr[w]);
+      stv<W>(s2 + base, o);
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        o.v[w] = make_float2(__builtin_fmaf(2.0f, tr[w] * inv_scale, -p0.v[w].x),
+                             __builtin_fmaf(2.0f, ti[w] * inv_scale, -p0.v[w].y));
+      stv<W>(s2 + base, o);
+      const Vec<W>* acc = &o;
+      Vec<W> ox;
+      if constexpr (OP == kCheb2x) {
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          ox.v[w] = make_float2(__builtin_fmaf(2.0f, xr[w] * inv_scale, -p0x.v[w].x),
+                                __builtin_fmaf(2.0f, xi[w] * inv_scale, -p0x.v[w].y));
+        stv<W>(s2x + base, ox);
+        acc = &ox;
+      }
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        accumulate_phase(Sv[w], bm, (double)acc->v[w].x, (double)acc->v[w].y, label);
+        S[base + w] = Sv[w];
+      }
+    }
+    if (more) {
+      wt.put(s_win, (p + 3) & 3, na, nb);
+      if constexpr (kTwo)
+        wt.put(s_winx, (p + 3) & 3, nax, nbx);
+    }
+    __syncthreads();
+    s = sn;
   }
 }
 
@@ -1293,6 +1519,54 @@ void launch_block_evo(const HamDev& h, int R, int W, const TileMap& tm, int op,
                       hipStream_t st)
 {
   Grid g = block_grid(tm);
+  // one ring of four window rows per source block (two for kCheb2x), beside
+  // the class tables (12 KB); shapes whose rings exceed the 64 KB of LDS a
+  // workgroup may hold take the kernel without the window
+  const size_t lds = sizeof(float2) * 4 * (size_t)(tm.win.chunk + 2) * R * (op == kCheb2x ? 2 : 1);
+  if (tm.win.by > 0 && W <= 4 && (h.K == 3 || h.K == 4 || h.K == 6) &&
+      lds + sizeof(float2) * kBlockLdsTableMax + sizeof(float) * kBlockLdsTableMax <= 65536) {
+#define LSQTB_EVOW(OPV)                                                                    \
+  k_block_evo_win<LAY, KT, WW, OPV><<<g.blocks, kBlock, lds, st>>>(h, R, tm, s0, s1, s2, s0x, \
+                                                                  s1x, s2x, S, bm, label,     \
+                                                                  g.tiles, g.blocks)
+#define LSQTB_EVOW_OP                   \
+  switch (op) {                         \
+    case kApplyH:                       \
+      LSQTB_EVOW(kApplyH);              \
+      break;                            \
+    case kCheb2:                        \
+      LSQTB_EVOW(kCheb2);               \
+      break;                            \
+    case kComm:                         \
+      LSQTB_EVOW(kComm);                \
+      break;                            \
+    case kCurrent:                      \
+      LSQTB_EVOW(kCurrent);             \
+      break;                            \
+    default:                            \
+      LSQTB_EVOW(kCheb2x);              \
+      break;                            \
+  }
+    LSQTB_LAYOUT_SWITCH(h, LAY, {
+      switch (h.K) {
+        case 3: {
+          constexpr int KT = 3;
+          LSQTB_WIN_W_SWITCH(W, WW, LSQTB_EVOW_OP)
+        } break;
+        case 4: {
+          constexpr int KT = 4;
+          LSQTB_WIN_W_SWITCH(W, WW, LSQTB_EVOW_OP)
+        } break;
+        default: {
+          constexpr int KT = 6;
+          LSQTB_WIN_W_SWITCH(W, WW, LSQTB_EVOW_OP)
+        } break;
+      }
+    })
+#undef LSQTB_EVOW_OP
+#undef LSQTB_EVOW
+    return;
+  }
   const bool nt = (size_t)h.n * R >= (size_t)kStreamingMinSites;
 #define LSQTB_EVO(OPV, NTV)                                                              \
   k_block_evo<LAY, KT, WW, OPV, NTV><<<g.blocks, kBlock, 0, st>>>(                       \
