@@ -879,7 +879,8 @@ __global__ __launch_bounds__(kBlock) void k_block_evo(
     float tr[W], ti[W], xr[W], xi[W];
     // the streams read once per pass (psi_{m-2}, the accumulated state) are
     // loaded with the first loads of the pass, so that their latency
-    // overlaps the gathers (s0 may be s2: each site is read before written)
+    // overlaps the gathers (s0 may be s2: each site is read before written);
+    // slot words and potential take ordinary loads (non-temporal: 2 % slower)
     constexpr bool kRecur = OP == kCheb2 || OP == kCheb2x;
     Vec<W> p0, p0x;
     double2 Sv[W];
@@ -892,7 +893,7 @@ __global__ __launch_bounds__(kBlock) void k_block_evo(
         Sv[w] = S[base + w];
     }
     if constexpr (kOnsite) {
-      const float v = ld<NT>(h.pot32 + s);
+      const float v = ld<false>(h.pot32 + s);
       Vec<W> p1 = ldv<W, false>(s1 + base);
 #pragma unroll
       for (int w = 0; w < W; ++w) {
@@ -916,7 +917,7 @@ __global__ __launch_bounds__(kBlock) void k_block_evo(
       uint32_t wd[C];
 #pragma unroll
       for (int q = 0; q < C; ++q)
-        wd[q] = (m0 + q < K) ? ld<NT>(h.words + (size_t)(m0 + q) * h.stride + s) : kEmptySlot;
+        wd[q] = (m0 + q < K) ? ld<false>(h.words + (size_t)(m0 + q) * h.stride + s) : kEmptySlot;
       int j[C];
       float a[C], b[C], x[C];
       decode_words32<LAYOUT, C>(h, s_tab, s, wd, j, a, b);
