@@ -3,8 +3,8 @@
 // Phase timers are wall-clock (the reference times CPU time with clock()).
 #include "lsqt.h"
 
-#include <chrono>
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <iostream>
 #include <memory>
@@ -63,22 +63,26 @@ void lsqt(std::string input_directory)
   const bool block = lsqt_host::path() == lsqt_host::Path::block ||
                      lsqt_host::path() == lsqt_host::Path::batch;
   if (block && model.number_of_random_vectors > 0) {
-    // every random vector of the directory at once; the RNG draws are the
-    // same as vector by vector (nothing else draws in between)
-    const int used = model.number_of_random_vectors;
-    int R = 1;
-    while (R < used)
-      R *= 2;
-    if (R > 256)
-      throw lsqt_host::InputError("Error: the block path takes at most 256 random vectors");
+    // the random vectors of the directory in blocks of up to 256 at once;
+    // the RNG draws are the same as vector by vector (nothing else draws in
+    // between), and each .out file receives its rows in vector order
+    const int total = model.number_of_random_vectors;
     const size_t n = (size_t)model.number_of_atoms;
-    std::vector<real> re(n * R, 0.0), im(n * R, 0.0);
-    for (int i = 0; i < used; ++i)
-      model.initialize_state_host(re.data() + i * n, im.data() + i * n, -1);
-    banner("Started  simulation with " + std::to_string(used) + " random vectors (block path)");
-    std::vector<lsqt_host::BlockMember> members{{&model, 0, used}};
-    lsqt_host::find_block_quantities(members, H.handle(), R, re, im);
-    banner_end("Finished simulation with " + std::to_string(used) + " random vectors");
+    for (int first = 0; first < total; first += 256) {
+      const int used = std::min(256, total - first);
+      int R = 1;
+      while (R < used)
+        R *= 2;
+      std::vector<real> re(n * R, 0.0), im(n * R, 0.0);
+      for (int i = 0; i < used; ++i)
+        model.initialize_state_host(re.data() + i * n, im.data() + i * n, -1);
+      banner("Started  simulation with random vectors " + std::to_string(first) + " to " +
+             std::to_string(first + used - 1) + " (block path)");
+      std::vector<lsqt_host::BlockMember> members{{&model, 0, used}};
+      lsqt_host::find_block_quantities(members, H.handle(), R, re, im);
+      banner_end("Finished simulation with random vectors " + std::to_string(first) + " to " +
+                 std::to_string(first + used - 1));
+    }
   } else {
     lsqt_vectors(model, H, random_state);
   }

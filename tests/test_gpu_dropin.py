@@ -210,3 +210,33 @@ def test_batch_path_uneven_realisations(tmp_path):
             for k in range(want.shape[0]):
                 err = np.abs(got[k] - want[k]).max() / np.abs(want[k]).max()
                 assert err <= 2e-5, f"{d.name}/{name} row {k}: {err:.2e}"
+
+
+def test_block_path_more_than_256_vectors(tmp_path):
+    """LSQT_PATH=block with 260 random vectors (two blocks: 256 and 4, drawn in
+    the reference's order): every row of every .out file within 2e-5 of the
+    default FP64 program's."""
+    src = os.path.join(CASES_DIR, "honeycomb_small")
+    runs = {}
+    for path in ("f64", "block"):
+        work = tmp_path / path
+        shutil.copytree(src, work, ignore=shutil.ignore_patterns("expected", "dump.npz"))
+        text = (work / "para.in").read_text()
+        (work / "para.in").write_text(text.replace("number_of_random_vectors 2",
+                                                   "number_of_random_vectors 260")
+                                          .replace("number_of_moments 64", "number_of_moments 24"))
+        driver = tmp_path / f"input_{path}.txt"
+        driver.write_text(f"{work}\n")
+        print(f"[gpu] lsqt_gpu LSQT_PATH={path}, 260 random vectors", flush=True)
+        r = hostlib.run_cli(str(driver), digits=17, timeout=600,
+                            env=dict(os.environ, LSQT_PATH=path))
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        runs[path] = work
+    names = sorted(f for f in os.listdir(runs["f64"]) if f.endswith(".out"))
+    assert names == sorted(f for f in os.listdir(runs["block"]) if f.endswith(".out"))
+    for name in names:
+        want, got = read_out(runs["f64"] / name), read_out(runs["block"] / name)
+        assert got.shape == want.shape and want.shape[0] >= 260, name
+        for k in range(want.shape[0]):
+            err = np.abs(got[k] - want[k]).max() / np.abs(want[k]).max()
+            assert err <= 2e-5, f"{name} row {k}: {err:.2e}"
