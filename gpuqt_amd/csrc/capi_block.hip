@@ -1,12 +1,17 @@
 // capi_block.hip -- the block path of the C ABI: blocks of R vectors, the
-// block moment recursion, and the autotuner that picks the kernel shape.
+// block moment recursion (direct, self products and pairs of recursions by
+// the product identities), the block time-correlation drivers, segmented
+// Hamiltonians, hipGraph replay of moment sets, and the autotuner that picks
+// the kernel shape.
 //
 // Autotuner: for a Hamiltonian and R it times a few fused steps of every
-// candidate (W = vectors per lane in {1, 2, 4}, i.e. vectors-per-wavefront
-// VPW = R / W lanes per site; ROWS = sites per workgroup tile in {256, 1024,
-// 4096}) on the caller's data, keeps the fastest, and remembers it per
-// (R, z, table placement, size class) in memory and in a text cache file
-// ($LSQT_TUNE_CACHE, default $HOME/.cache/lsqt_hip_tune.txt).
+// candidate (W = vectors per lane in {1, 2, 4, 8}, i.e. vectors-per-wavefront
+// VPW = R / W lanes per site; natural tiles of 256, 1024 or 4096 sites; on a
+// detected lattice also blocked tiles, the column walk and the sliding
+// window; one or two passes in flight) on the caller's data, keeps the
+// fastest, and remembers it per (R, z, table placement, size class, lattice
+// slab, pair mode) in memory and in a text cache file ($LSQT_TUNE_CACHE,
+// default $HOME/.cache/lsqt_hip_tune.txt).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
