@@ -97,6 +97,29 @@ def test_reference_examples(tmp_path, example):
             _compare_text(got, want, f"{example}/{name}")
 
 
+@pytest.mark.parametrize("path,tol", [("f32", 1e-5), ("block", 2e-5)])
+@pytest.mark.parametrize("example", ["diffusive", "localized"])
+def test_reference_examples_other_paths(tmp_path, example, path, tol):
+    """The reference's own fixed-seed 4.0M-site examples through LSQT_PATH=f32
+    and LSQT_PATH=block: every row of dos.out / vac.out / msd.out within `tol`
+    (normwise) of the reference's checked-in 6-digit rows."""
+    src = os.path.join(GOLDEN, "fan2018cpc", example)
+    work = tmp_path / example
+    shutil.copytree(src, work, ignore=shutil.ignore_patterns("expected", "dump.npz"))
+    driver = tmp_path / "input.txt"
+    driver.write_text(f"{work}\n")
+    print(f"[gpu] lsqt_gpu LSQT_PATH={path} fan2018cpc/{example}", flush=True)
+    r = hostlib.run_cli(str(driver), timeout=600, env=dict(os.environ, LSQT_PATH=path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    exp = os.path.join(src, "expected")
+    for name in sorted(os.listdir(exp)):
+        got, want = read_out(work / name), read_out(os.path.join(exp, name))
+        assert got.shape == want.shape, f"{example}/{name}"
+        for k in range(want.shape[0]):
+            err = np.abs(got[k] - want[k]).max() / np.abs(want[k]).max()
+            assert err <= tol, f"{example}/{name} row {k}: {err:.2e}"
+
+
 def test_append_and_ldos(tmp_path):
     src = os.path.join(CASES_DIR, "square_vacancy_ldos")
     work = tmp_path / "twice"
