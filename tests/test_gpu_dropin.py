@@ -263,3 +263,25 @@ def test_block_path_more_than_256_vectors(tmp_path):
         for k in range(want.shape[0]):
             err = np.abs(got[k] - want[k]).max() / np.abs(want[k]).max()
             assert err <= 2e-5, f"{name} row {k}: {err:.2e}"
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_inputs_against_oracle(tmp_path, seed):
+    """The drop-in program (FP64 path) on the random lattices of
+    test_random_inputs.py against the oracle's .out files (which equal the
+    reference program's text): equal at 6 digits wherever |value| > 1e-6 of
+    the row maximum, normwise within 1e-5."""
+    import oracle
+    from test_random_inputs import _random_case
+    base = tmp_path / "case"
+    base.mkdir()
+    _random_case(str(base), seed)
+    orc = tmp_path / "orc"
+    shutil.copytree(base, orc)
+    oracle.lsqt(str(orc))
+    print(f"[gpu] lsqt_gpu random lattice {seed}", flush=True)
+    work = _run(tmp_path, str(base), name="gpu")
+    outs = sorted(f for f in os.listdir(orc) if f.endswith(".out"))
+    assert outs == sorted(f for f in os.listdir(work) if f.endswith(".out"))
+    for name in outs:
+        _compare_text(read_out(work / name), read_out(orc / name), f"seed {seed} {name}")
