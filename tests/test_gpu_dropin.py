@@ -265,17 +265,17 @@ def test_block_path_more_than_256_vectors(tmp_path):
             assert err <= 2e-5, f"{name} row {k}: {err:.2e}"
 
 
-@pytest.mark.parametrize("seed", range(16))
+@pytest.mark.parametrize("seed", list(range(16)) + [("general", s) for s in range(12)])
 def test_random_inputs_against_oracle(tmp_path, seed):
     """The drop-in program (FP64 path) on the random lattices of
     test_random_inputs.py against the oracle's .out files (which equal the
     reference program's text): equal at 6 digits wherever |value| > 1e-6 of
     the row maximum, normwise within 1e-5."""
     import oracle
-    from test_random_inputs import _random_case
+    from test_random_inputs import make_case
     base = tmp_path / "case"
     base.mkdir()
-    _random_case(str(base), seed)
+    make_case(str(base), seed)
     orc = tmp_path / "orc"
     shutil.copytree(base, orc)
     oracle.lsqt(str(orc))

@@ -1,7 +1,8 @@
 """Randomised small inputs (CPU): lattices of random shape, orbitals,
 Hermitian complex hoppings, boundaries and disorder (Anderson, charged
-impurities, vacancies), random moment counts, energies, time steps and
-quantities.  For every case the oracle's .out files equal, text for text,
+impurities, vacancies), and general-model systems (ragged neighbour graphs,
+complex or real hopping files, optional potentials, spin), with random
+moment counts, energies, time steps and quantities.  For every case the oracle's .out files equal, text for text,
 those of the reference program compiled from source (oracle/_ref), and the
 drop-in's host Model builds the oracle's arrays and random vectors bit for
 bit.  Skipped where the reference program has not been built."""
@@ -77,11 +78,60 @@ def _random_case(d, seed):
         gi.write_local_orbitals(d, sorted(set(int(v) for v in rng.integers(0, 4, 2))))
 
 
-@pytest.mark.parametrize("seed", SEEDS)
+def _random_general_case(d, seed):
+    """A random general-model directory: symmetric neighbour graph with
+    ragged degrees, Hermitian hoppings (complex or real files), positions in
+    a periodic box, an optional potential file; spin (even site counts) and
+    LDOS sometimes."""
+    rng = np.random.default_rng(5000 + seed)
+    n = 2 * int(rng.integers(2, 60))
+    K = int(rng.integers(1, 6))
+    nb = [[] for _ in range(n)]
+    hop = {}
+    real = rng.random() < 0.4
+    pairs = [(i, j) for i in range(n) for j in range(i + 1, n)]
+    rng.shuffle(pairs)
+    for i, j in pairs[: 2 * n]:
+        if len(nb[i]) < K and len(nb[j]) < K:
+            t = complex(rng.uniform(-2, 2), 0.0 if real else rng.uniform(-1, 1))
+            nb[i].append(j)
+            nb[j].append(i)
+            hop[(i, j)], hop[(j, i)] = t, t.conjugate()
+    box = float(rng.uniform(n / 2, 2 * n))
+    potential = rng.uniform(-1, 1, n) if rng.random() < 0.6 else None
+    gi.write_general(d, nb, K, rng.uniform(0, box, n), box, float(n), potential=potential,
+                     hoppings=[[hop[(i, j)] for j in nb[i]] for i in range(n)],
+                     hopping_kind="real" if real else "complex")
+    tmax = max([sum(abs(hop[(i, j)]) for j in nb[i]) for i in range(n)] + [0.0])
+    emax = 1.2 * (tmax + 1.0) + 0.5
+    ldos = rng.random() < 0.3
+    flags = dict(vac0=rng.random() < 0.5, vac=rng.random() < 0.5, msd=rng.random() < 0.5,
+                 spin=rng.random() < 0.3)
+    gi.write_para(d, model=0, ldos=ldos, random_vectors=int(rng.integers(1, 3)),
+                  moments=int(rng.integers(2, 41)), energy_max=emax, **flags)
+    gi.write_energy(d, np.sort(rng.uniform(-0.9, 0.9, int(rng.integers(1, 6))) * emax))
+    gi.write_time_steps(d, rng.uniform(0.1, 2.0, int(rng.integers(1, 4))))
+    if ldos:
+        gi.write_local_orbitals(d, sorted(set(int(v) for v in rng.integers(0, n, 2))))
+
+
+def make_case(d, case):
+    """case: seed for a random lattice, ("general", seed) for a random
+    general-model system"""
+    if isinstance(case, tuple):
+        _random_general_case(d, case[1])
+    else:
+        _random_case(d, case)
+
+
+CASES = SEEDS + [("general", s) for s in range(12)]
+
+
+@pytest.mark.parametrize("seed", CASES)
 def test_random_case(tmp_path, seed):
     base = tmp_path / "case"
     base.mkdir()
-    _random_case(str(base), seed)
+    make_case(str(base), seed)
     ref_dir, orc_dir = tmp_path / "ref", tmp_path / "orc"
     shutil.copytree(base, ref_dir)
     shutil.copytree(base, orc_dir)
