@@ -285,3 +285,38 @@ def test_random_inputs_against_oracle(tmp_path, seed):
     assert outs == sorted(f for f in os.listdir(work) if f.endswith(".out"))
     for name in outs:
         _compare_text(read_out(work / name), read_out(orc / name), f"seed {seed} {name}")
+
+
+@pytest.mark.parametrize("path,tol", [("f32", 1e-5), ("block", 2e-5)])
+@pytest.mark.parametrize("seed", list(range(16)) + [("general", s) for s in range(12)])
+def test_random_inputs_other_paths(tmp_path, seed, path, tol):
+    """LSQT_PATH=f32 and LSQT_PATH=block on the random lattices and general
+    systems: every row within `tol` of the oracle's, normwise over the row
+    (rows whose maximum is below 1e-6 of the file's maximum are compared
+    against that floor instead)."""
+    import oracle
+    from test_random_inputs import make_case
+    base = tmp_path / "case"
+    base.mkdir()
+    make_case(str(base), seed)
+    orc = tmp_path / "orc"
+    shutil.copytree(base, orc)
+    oracle.lsqt(str(orc))
+    driver = tmp_path / "input.txt"
+    driver.write_text(f"{base}\n")
+    print(f"[gpu] lsqt_gpu LSQT_PATH={path} random case {seed}", flush=True)
+    r = hostlib.run_cli(str(driver), digits=17, timeout=300, env=dict(os.environ, LSQT_PATH=path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    outs = sorted(f for f in os.listdir(orc) if f.endswith(".out"))
+    assert outs == sorted(f for f in os.listdir(base) if f.endswith(".out"))
+    for name in outs:
+        got, want = read_out(base / name), read_out(orc / name)
+        assert got.shape == want.shape, name
+        floor = 1e-6 * np.abs(want).max()
+        if floor == 0.0:  # an identically zero quantity (no bond along the transport axis)
+            assert np.abs(got).max() == 0.0, f"{seed} {name}"
+            continue
+        for k in range(want.shape[0]):
+            scale = max(np.abs(want[k]).max(), floor)
+            err = np.abs(got[k] - want[k]).max() / scale
+            assert err <= tol, f"{seed} {name} row {k}: {err:.2e}"
