@@ -1133,13 +1133,18 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
     return LSQT_OK;
   }
   // candidates: natural tiles of 256..4096 sites; on a detected lattice also
-  // blocked tiles of a whole number of rows that divide the slab
+  // blocked tiles of a whole number of rows that divide the slab.  Tiles of
+  // 64 and 128 sites (one to four passes) are tried only where the ring of
+  // moment partials they need stays within 4 GB.
   std::vector<BlockShape> cands;
   for (int W : {1, 2, 4, 8}) {
     if (!valid_W(R, W) || (pairs && W < 2))
       continue;
-    for (int rows : {256, 1024, 4096}) {
+    for (int rows : {64, 128, 256, 1024, 4096}) {
       if (rows < kBlock / (R / W))
+        continue;
+      const double ring_bytes = (double)kRing * ((double)h->d.n / rows) * R * sizeof(double2);
+      if (rows < 256 && ring_bytes > 4.0e9)
         continue;
       BlockShape c;
       c.W = W;
