@@ -138,7 +138,12 @@ namespace {
 
 constexpr int kRing = 64;  // moment partials are reduced every kRing steps
 
-// pair: shapes of pair moment sets (R columns = R/2 pairs), tuned apart
+// How candidate shapes are timed (and cached apart): direct steps
+// <left|psi_m>, pair moment sets (R columns of R/2 pairs in `right`, left
+// unused, shapes with W >= 2), or self-product steps of `right`.
+enum TuneMode { kTuneDirect = 0, kTunePairs = 1, kTuneSelf = 2 };
+
+// pair: the TuneMode of the cached shape
 struct TuneKey {
   int R, K, lds, size_class, slab, pair = 0;
   bool operator<(const TuneKey& o) const
@@ -179,8 +184,10 @@ void tune_load()
   while (fscanf(f, "%31s %d %d %d %d %d %d %d %d %d %lf", arch, &k.R, &k.K, &k.lds,
                 &k.size_class, &k.slab, &v.shape.W, &v.shape.rows, &v.shape.blocked,
                 &v.shape.depth, &v.us) == 11) {
-    k.pair = std::strcmp(arch, "gfx950-pair") == 0;
-    if (k.pair || std::strcmp(arch, "gfx950") == 0)
+    k.pair = !std::strcmp(arch, "gfx950-pair") ? kTunePairs
+             : !std::strcmp(arch, "gfx950-self") ? kTuneSelf
+                                                 : kTuneDirect;
+    if (k.pair != kTuneDirect || std::strcmp(arch, "gfx950") == 0)
       g_tune[k] = v;
   }
   fclose(f);
@@ -195,7 +202,8 @@ void tune_store(const TuneKey& k, const TuneVal& v)
   FILE* f = fopen(path.c_str(), "a");
   if (!f)
     return;  // the cache file is an optimisation only
-  fprintf(f, "%s %d %d %d %d %d %d %d %d %d %.3f\n", k.pair ? "gfx950-pair" : "gfx950", k.R,
+  static const char* kTag[3] = {"gfx950", "gfx950-pair", "gfx950-self"};
+  fprintf(f, "%s %d %d %d %d %d %d %d %d %d %.3f\n", kTag[k.pair], k.R,
           k.K, k.lds, k.size_class, k.slab, v.shape.W, v.shape.rows, v.shape.blocked,
           v.shape.depth, v.us);
   fclose(f);
@@ -524,10 +532,8 @@ void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& 
   }
 }
 
-// pairs: R columns of R/2 pairs in `right` (left unused), shapes with W >= 2
-// timed as pair moment sets
 int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, BlockShape* out,
-                double* us_out, bool force, bool pairs = false);
+                double* us_out, bool force, int mode = kTuneDirect);
 
 }  // namespace
 
@@ -706,7 +712,7 @@ int pair_setup(lsqt_ham* h, int R, const BlockShape& c, const float2* left,
     LSQT_TRY(scratch_for(h, R2, 2, 1, &t));
     launch_block_interleave(left, right, h->d.n, R, t->B, h->ctx->stream);
     double us;
-    LSQT_TRY(pick_config(h, R2, nullptr, t->B, &p, &us, false, true));
+    LSQT_TRY(pick_config(h, R2, nullptr, t->B, &p, &us, false, kTunePairs));
   }
   LSQT_TRY(tile_map(h, R2, p, tm2));
   LSQT_TRY(scratch_for(h, R2, M, tm2->tiles, s));
@@ -717,11 +723,11 @@ int pair_setup(lsqt_ham* h, int R, const BlockShape& c, const float2* left,
 
 // shape, tile map and scratch (M moments) for R vectors
 int block_setup(lsqt_ham* h, int R, const float2* left, const float2* right, int M,
-                BlockShape* c, TileMap* tm, BlockScratch** s)
+                BlockShape* c, TileMap* tm, BlockScratch** s, int mode = kTuneDirect)
 {
   LSQT_TRY(f32_potential(h));
   double us;
-  LSQT_TRY(pick_config(h, R, left, right, c, &us, false));
+  LSQT_TRY(pick_config(h, R, left, right, c, &us, false, mode));
   LSQT_TRY(tile_map(h, R, *c, tm));
   LSQT_TRY(scratch_for(h, R, M, tm->tiles, s));
   return LSQT_OK;
@@ -776,7 +782,7 @@ extern "C" int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, 
   BlockShape c;
   TileMap tm;
   BlockScratch* s;
-  LSQT_TRY(block_setup(h, v->R, v->d, v->d, M, &c, &tm, &s));
+  LSQT_TRY(block_setup(h, v->R, v->d, v->d, M, &c, &tm, &s, kTuneSelf));
   return moment_set(h, v->R, c, tm, s, nullptr, v->d, kSelfProducts, M, mu_re, mu_im);
 }
 
@@ -986,7 +992,7 @@ extern "C" int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double
   launch_to_f32(state, s->F[0], nR, st);
   BlockShape c;
   TileMap tm;
-  LSQT_TRY(block_setup(h, R, s->F[0], s->F[0], M, &c, &tm, &s));
+  LSQT_TRY(block_setup(h, R, s->F[0], s->F[0], M, &c, &tm, &s, kTuneSelf));
   LSQT_TRY(evo_scratch(h, R, s));
   launch_copy(state_x, state, nR, st);
   block_evolve(h, R, c, tm, s, 1, time_step_scaled[0], state, true);
@@ -1106,8 +1112,9 @@ extern "C" int lsqt_block_tune(lsqt_ham* h, const lsqt_block* left, const lsqt_b
 namespace {
 
 int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, BlockShape* out,
-                double* us_out, bool force, bool pairs)
+                double* us_out, bool force, int mode)
 {
+  const bool pairs = mode == kTunePairs;
   if (h->blk && !pairs) {
     auto fit = h->blk->forced.find(R);
     if (fit != h->blk->forced.end()) {
@@ -1120,8 +1127,7 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
   tune_load();
   // the slab of a detected lattice enters the key: blocked shapes exist only
   // for lattices, and their best tile size depends on the slab
-  TuneKey key{R, h->d.K, h->d.layout == kPackedLds ? 1 : 0, size_class(h->d.n), h->slab,
-              pairs ? 1 : 0};
+  TuneKey key{R, h->d.K, h->d.layout == kPackedLds ? 1 : 0, size_class(h->d.n), h->slab, mode};
   auto it = g_tune.find(key);
   // a cached shape is taken only if it is valid here (the cache file is
   // plain text, and the key holds the slab, not the row length that a
@@ -1221,10 +1227,12 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
     LSQT_TRY(tile_map(h, R, c, &tm));
     LSQT_HIP(hipMemcpyAsync(A, right, sizeof(float2) * (size_t)h->d.n * R,
                             hipMemcpyDeviceToDevice, st));
-    if (pairs) {
-      // a pair moment set of 2 * steps moments (steps recursion steps)
+    if (mode != kTuneDirect) {
+      // a pair or self-product moment set of 2 * steps moments (steps
+      // recursion steps)
       LSQT_HIP(hipEventRecord(e0, st));
-      enqueue_self_steps(h, R / 2, c, tm, A, B, s->ring, s->mu, 2 * steps, st, true);
+      enqueue_self_steps(h, pairs ? R / 2 : R, c, tm, A, B, s->ring, s->mu, 2 * steps, st,
+                         pairs);
       LSQT_HIP(hipEventRecord(e1, st));
     } else {
       enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, steps, 0, 2, st);
@@ -1235,7 +1243,7 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
     LSQT_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     LSQT_HIP(hipEventElapsedTime(&ms, e0, e1));
-    double us = 1e3 * ms / (pairs ? steps : steps - 2);
+    double us = 1e3 * ms / (mode != kTuneDirect ? steps : steps - 2);
     if (us < best.us)
       best = TuneVal{c, us};
   }
