@@ -320,3 +320,54 @@ def test_random_inputs_other_paths(tmp_path, seed, path, tol):
             scale = max(np.abs(want[k]).max(), floor)
             err = np.abs(got[k] - want[k]).max() / scale
             assert err <= tol, f"{seed} {name} row {k}: {err:.2e}"
+
+
+def test_batch_path_random_realisations(tmp_path):
+    """LSQT_PATH=batch on random lattices of different shapes, orbitals,
+    boundaries and disorder that share their settings (moments, energies,
+    time steps, energy_max, quantities, vector count): one segmented
+    Hamiltonian of ragged systems; every directory's rows within 2e-5 of the
+    oracle's (normwise, with the floor of test_random_inputs_other_paths)."""
+    import oracle
+    from test_random_inputs import make_case
+    from gpuqt_amd import inputs as gi
+    dirs, refs = [], []
+    for seed in range(6):
+        d = tmp_path / f"r{seed}"
+        d.mkdir()
+        make_case(str(d), seed)
+        # shared settings (the disorder of each lattice stays its own)
+        text = [ln for ln in (d / "para.in").read_text().splitlines()
+                if ln.split()[0] in ("model", "anderson_disorder", "charged_impurity",
+                                     "vacancy_disorder")]
+        text += ["number_of_random_vectors 3", "number_of_moments 30", "energy_max 14.0",
+                 "calculate_vac0", "calculate_vac", "calculate_msd"]
+        (d / "para.in").write_text("\n".join(text) + "\n")
+        gi.write_energy(str(d), gi.energy_grid(-6, 6, 9))
+        gi.write_time_steps(str(d), [0.5, 1.0])
+        if (d / "local_orbitals.in").exists():
+            (d / "local_orbitals.in").unlink()
+        o = tmp_path / f"o{seed}"
+        shutil.copytree(d, o)
+        oracle.lsqt(str(o))
+        dirs.append(d)
+        refs.append(o)
+    driver = tmp_path / "input.txt"
+    driver.write_text("".join(f"{d}\n" for d in dirs))
+    print("[gpu] lsqt_gpu LSQT_PATH=batch, 6 random lattices", flush=True)
+    r = hostlib.run_cli(str(driver), digits=17, timeout=300, env=dict(os.environ, LSQT_PATH="batch"))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "simulation of 6 systems together" in r.stdout, r.stdout[-2000:]
+    for d, o in zip(dirs, refs):
+        outs = sorted(f for f in os.listdir(o) if f.endswith(".out"))
+        assert outs == sorted(f for f in os.listdir(d) if f.endswith(".out")), d.name
+        for name in outs:
+            got, want = read_out(d / name), read_out(o / name)
+            assert got.shape == want.shape, f"{d.name}/{name}"
+            floor = 1e-6 * np.abs(want).max()
+            if floor == 0.0:
+                assert np.abs(got).max() == 0.0, f"{d.name}/{name}"
+                continue
+            for k in range(want.shape[0]):
+                err = np.abs(got[k] - want[k]).max() / max(np.abs(want[k]).max(), floor)
+                assert err <= 2e-5, f"{d.name}/{name} row {k}: {err:.2e}"
