@@ -139,9 +139,11 @@ namespace {
 constexpr int kRing = 64;  // moment partials are reduced every kRing steps
 
 // How candidate shapes are timed (and cached apart): direct steps
-// <left|psi_m>, pair moment sets (R columns of R/2 pairs in `right`, left
-// unused, shapes with W >= 2), or self-product steps of `right`.
-enum TuneMode { kTuneDirect = 0, kTunePairs = 1, kTuneSelf = 2 };
+// <left|psi_m> (also the shape of self-product steps: timing those on their
+// own picked shapes up to 4 % slower over a whole moment set on the 160^3
+// cube), or pair moment sets (R columns of R/2 pairs in `right`, left
+// unused, shapes with W >= 2).
+enum TuneMode { kTuneDirect = 0, kTunePairs = 1 };
 
 // pair: the TuneMode of the cached shape
 struct TuneKey {
@@ -184,9 +186,7 @@ void tune_load()
   while (fscanf(f, "%31s %d %d %d %d %d %d %d %d %d %lf", arch, &k.R, &k.K, &k.lds,
                 &k.size_class, &k.slab, &v.shape.W, &v.shape.rows, &v.shape.blocked,
                 &v.shape.depth, &v.us) == 11) {
-    k.pair = !std::strcmp(arch, "gfx950-pair") ? kTunePairs
-             : !std::strcmp(arch, "gfx950-self") ? kTuneSelf
-                                                 : kTuneDirect;
+    k.pair = !std::strcmp(arch, "gfx950-pair") ? kTunePairs : kTuneDirect;
     if (k.pair != kTuneDirect || std::strcmp(arch, "gfx950") == 0)
       g_tune[k] = v;
   }
@@ -202,7 +202,7 @@ void tune_store(const TuneKey& k, const TuneVal& v)
   FILE* f = fopen(path.c_str(), "a");
   if (!f)
     return;  // the cache file is an optimisation only
-  static const char* kTag[3] = {"gfx950", "gfx950-pair", "gfx950-self"};
+  static const char* kTag[2] = {"gfx950", "gfx950-pair"};
   fprintf(f, "%s %d %d %d %d %d %d %d %d %d %.3f\n", kTag[k.pair], k.R,
           k.K, k.lds, k.size_class, k.slab, v.shape.W, v.shape.rows, v.shape.blocked,
           v.shape.depth, v.us);
@@ -723,11 +723,11 @@ int pair_setup(lsqt_ham* h, int R, const BlockShape& c, const float2* left,
 
 // shape, tile map and scratch (M moments) for R vectors
 int block_setup(lsqt_ham* h, int R, const float2* left, const float2* right, int M,
-                BlockShape* c, TileMap* tm, BlockScratch** s, int mode = kTuneDirect)
+                BlockShape* c, TileMap* tm, BlockScratch** s)
 {
   LSQT_TRY(f32_potential(h));
   double us;
-  LSQT_TRY(pick_config(h, R, left, right, c, &us, false, mode));
+  LSQT_TRY(pick_config(h, R, left, right, c, &us, false));
   LSQT_TRY(tile_map(h, R, *c, tm));
   LSQT_TRY(scratch_for(h, R, M, tm->tiles, s));
   return LSQT_OK;
@@ -782,7 +782,7 @@ extern "C" int lsqt_moments_block_self(lsqt_ham* h, const lsqt_block* v, int M, 
   BlockShape c;
   TileMap tm;
   BlockScratch* s;
-  LSQT_TRY(block_setup(h, v->R, v->d, v->d, M, &c, &tm, &s, kTuneSelf));
+  LSQT_TRY(block_setup(h, v->R, v->d, v->d, M, &c, &tm, &s));
   return moment_set(h, v->R, c, tm, s, nullptr, v->d, kSelfProducts, M, mu_re, mu_im);
 }
 
@@ -992,7 +992,7 @@ extern "C" int lsqt_block_msd(lsqt_ham* h, int R, const double* re, const double
   launch_to_f32(state, s->F[0], nR, st);
   BlockShape c;
   TileMap tm;
-  LSQT_TRY(block_setup(h, R, s->F[0], s->F[0], M, &c, &tm, &s, kTuneSelf));
+  LSQT_TRY(block_setup(h, R, s->F[0], s->F[0], M, &c, &tm, &s));
   LSQT_TRY(evo_scratch(h, R, s));
   launch_copy(state_x, state, nR, st);
   block_evolve(h, R, c, tm, s, 1, time_step_scaled[0], state, true);
@@ -1213,37 +1213,43 @@ int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, Blo
     max_tiles = std::max(max_tiles, tm.tiles);
   }
   BlockScratch* s;
-  LSQT_TRY(scratch_for(h, R, steps, max_tiles, &s));
+  // moment outputs for the 2 * steps moments of a pair set
+  LSQT_TRY(scratch_for(h, R, 2 * steps, max_tiles, &s));
   hipStream_t st = h->ctx->stream;
   hipEvent_t e0, e1;
   LSQT_HIP(hipEventCreate(&e0));
   LSQT_HIP(hipEventCreate(&e1));
   TuneVal best{BlockShape{}, 1e300};
   for (const BlockShape& c : cands) {
-    // time a few fused steps on the caller's blocks
-    float2* A = s->A;
-    float2* B = s->B;
+    // time a few fused steps on the caller's blocks, twice (the better of
+    // the two counts), after an untimed warm-up: the first launches of a
+    // kernel shape also load its code
     TileMap tm;
     LSQT_TRY(tile_map(h, R, c, &tm));
-    LSQT_HIP(hipMemcpyAsync(A, right, sizeof(float2) * (size_t)h->d.n * R,
-                            hipMemcpyDeviceToDevice, st));
-    if (mode != kTuneDirect) {
-      // a pair or self-product moment set of 2 * steps moments (steps
-      // recursion steps)
-      LSQT_HIP(hipEventRecord(e0, st));
-      enqueue_self_steps(h, pairs ? R / 2 : R, c, tm, A, B, s->ring, s->mu, 2 * steps, st,
-                         pairs);
-      LSQT_HIP(hipEventRecord(e1, st));
-    } else {
-      enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, steps, 0, 2, st);
-      LSQT_HIP(hipEventRecord(e0, st));
-      enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, steps, 2, steps, st);
-      LSQT_HIP(hipEventRecord(e1, st));
+    double us = 1e300;
+    for (int rep = 0; rep < 3; ++rep) {
+      float2* A = s->A;
+      float2* B = s->B;
+      LSQT_HIP(hipMemcpyAsync(A, right, sizeof(float2) * (size_t)h->d.n * R,
+                              hipMemcpyDeviceToDevice, st));
+      const int n_steps = rep == 0 ? 2 : steps;
+      if (pairs) {
+        // a pair moment set of 2 n_steps moments
+        LSQT_HIP(hipEventRecord(e0, st));
+        enqueue_self_steps(h, R / 2, c, tm, A, B, s->ring, s->mu, 2 * n_steps, st, true);
+        LSQT_HIP(hipEventRecord(e1, st));
+      } else {
+        enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, n_steps, 0, 2, st);
+        LSQT_HIP(hipEventRecord(e0, st));
+        enqueue_steps(h, R, c, tm, left, A, B, s->ring, nullptr, n_steps, 2, n_steps, st);
+        LSQT_HIP(hipEventRecord(e1, st));
+      }
+      LSQT_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      LSQT_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (rep > 0)
+        us = std::min(us, 1e3 * ms / (pairs ? steps : steps - 2));
     }
-    LSQT_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    LSQT_HIP(hipEventElapsedTime(&ms, e0, e1));
-    double us = 1e3 * ms / (mode != kTuneDirect ? steps : steps - 2);
     if (us < best.us)
       best = TuneVal{c, us};
   }
