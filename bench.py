@@ -567,7 +567,7 @@ def main():
     ap.add_argument("--cubic-L", type=int, default=160)
     ap.add_argument("--block-R", type=int, default=32)
     ap.add_argument("--block-moments", type=int, default=2048)
-    ap.add_argument("--cpu-moments", type=int, default=100)
+    ap.add_argument("--cpu-moments", type=int, default=250)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the uneven-batch (VAC + MSD over many realisations) section")
