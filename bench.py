@@ -206,6 +206,18 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     t32, mu32 = timed(lambda: H.moments_f32(v0, v0, M))
     tb, mub = timed(lambda: H.moments_block(blk, blk, M))
     ts, mus = timed(lambda: H.moments_block_self(blk, M))
+    # <L|T_m|R> with L != R (the VAC shape) by pair moment sets
+    ph2 = rng.uniform(0, 2 * np.pi, (R, n))
+    blk2 = native.Block(ctx, n, R)
+    blk2.upload(np.cos(ph2), np.sin(ph2))
+    v0b = H.vector(np.cos(ph2[0]), np.sin(ph2[0]))
+    del ph2
+    tp, mup = timed(lambda: H.moments_block_pair(blk, blk2, M))
+    ref_pair = H.moments(v0, v0b, M)
+    err_pair = float(np.abs(mup[0] - ref_pair).max() / np.abs(ref_pair).max())
+    # the same error on the scale |L||R| = n of unit-modulus vectors (random
+    # pairs have |<L|T_m|R>| ~ sqrt(n), which inflates the normwise figure)
+    err_pair_n = float(np.abs(mup[0] - ref_pair).max() / n)
     e = np.linspace(-14, 14, 201)
     vol = float(L ** 3)
     errs, errs_self = [], []
@@ -228,6 +240,7 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     steps_self = M // 2
     us_ = n * R * steps_self / ts          # recursion site-updates/s
     um_ = n * R * (M - 1) / ts             # moments x sites x vectors per second
+    up_ = n * R * (M - 1) / tp             # the same for R pairs L != R
     bpu_blk = (4.0 + 4.0 * 6) / R + 32.0
     bpu64 = info.bytes_per_site_f64
     res = {
@@ -256,13 +269,22 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
             "moment_site_updates_per_s": um_,
             "time_to_moments_speedup_vs_f64": um_ / u64,
             "dos_normwise_error_vs_f64": errs_self},
+        "block_f32_pair_products": {
+            "R": R, "steps": steps_self, "moments": M,
+            "what": "<L_r|T_m|R_r> for R independent pairs L_r != R_r (the VAC / spin shape)",
+            "moment_site_updates_per_s": up_,
+            "time_to_moments_speedup_vs_f64": up_ / u64,
+            "moment_normwise_error_vs_f64": err_pair,
+            "moment_error_vs_f64_over_norm_product": err_pair_n},
     }
     log(f"block R={R}: {ub:.3e} site-updates/s = {ub / u64:.2f}x FP64 single "
         f"({u64:.3e}); DOS error {max(errs):.2e}")
     log(f"block R={R} product identities: {us_:.3e} site-updates/s in {steps_self} steps, "
         f"{um_:.3e} moment site-updates/s = {um_ / u64:.2f}x FP64 single; "
         f"DOS error {max(errs_self):.2e}")
-    for o in (H, blk, v0):
+    log(f"block R={R} pairs L!=R: {up_:.3e} moment site-updates/s = {up_ / u64:.2f}x FP64 "
+        f"single; moment error {err_pair:.2e} ({err_pair_n:.2e} of |L||R|)")
+    for o in (H, blk, blk2, v0, v0b):
         o.close()
     return res
 
