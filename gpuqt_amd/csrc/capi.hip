@@ -967,7 +967,8 @@ int lsqt::evolve_enqueue(lsqt_ham* h, int direction, double t, double2* state)
   launch_hpsi(h->d, state, nullptr, B, nullptr, nullptr, false, false, st);
   launch_cheb01(A, B, state, bt.b0, bt.b1, direction, n, st);
   for (size_t k = 0; k < bt.bm.size(); ++k) {
-    launch_cheb2(h->d, A, B, A, state, bt.bm[k], bt.label[k], st);
+    const DeferredTerm d = deferred_term(bt, k);
+    launch_cheb2(h->d, A, B, A, state, d.bm, d.label, st, d.bm_prev, d.label_prev);
     double2* tmp = A;
     A = B;
     B = tmp;
@@ -990,7 +991,9 @@ int lsqt::evolvex_enqueue(lsqt_ham* h, int direction, double t, double2* state)
   launch_xop(h->d, state, Bx, false, st);
   launch_cheb1x(Bx, state, bt.b1, n, st);
   for (size_t k = 0; k < bt.bm.size(); ++k) {
-    launch_cheb2x(h->d, A, Ax, B, Bx, A, Ax, state, bt.bm[k], bt.label[k], st);
+    const DeferredTerm d = deferred_term(bt, k);
+    launch_cheb2x(h->d, A, Ax, B, Bx, A, Ax, state, d.bm, d.label, st, d.bm_prev,
+                  d.label_prev);
     double2* tmp = A;
     A = B;
     B = tmp;

@@ -854,8 +854,9 @@ void block_evolve(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Bl
                    0, st);
   launch_cheb01_32(B, S, bt.b0, bt.b1, direction, nR, st);
   for (size_t k = 0; k < bt.bm.size(); ++k) {
-    launch_block_evo(h->d, R, c.W, tm, 1, A, B, A, nullptr, nullptr, nullptr, S, bt.bm[k],
-                     bt.label[k], st);
+    const DeferredTerm d = deferred_term(bt, k);
+    launch_block_evo(h->d, R, c.W, tm, 1, A, B, A, nullptr, nullptr, nullptr, S, d.bm, d.label,
+                     st, d.bm_prev, d.label_prev);
     std::swap(A, B);
   }
   if (renormalise) {
@@ -880,7 +881,9 @@ void block_evolvex(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, B
                    0, st);
   launch_cheb1x_32(Bx, S, bt.b1, nR, st);
   for (size_t k = 0; k < bt.bm.size(); ++k) {
-    launch_block_evo(h->d, R, c.W, tm, 4, A, B, A, Ax, Bx, Ax, S, bt.bm[k], bt.label[k], st);
+    const DeferredTerm d = deferred_term(bt, k);
+    launch_block_evo(h->d, R, c.W, tm, 4, A, B, A, Ax, Bx, Ax, S, d.bm, d.label, st, d.bm_prev,
+                     d.label_prev);
     std::swap(A, B);
     std::swap(Ax, Bx);
   }

@@ -163,7 +163,8 @@ int lsqt::evolve_f32_enqueue(lsqt_ham* h, int direction, double t, double2* stat
   launch_hpsi32(h->d, A, nullptr, B, nullptr, nullptr, false, false, st);
   launch_cheb01_32(B, state, bt.b0, bt.b1, direction, n, st);
   for (size_t k = 0; k < bt.bm.size(); ++k) {
-    launch_cheb2_32(h->d, A, B, A, state, bt.bm[k], bt.label[k], st);
+    const DeferredTerm d = deferred_term(bt, k);
+    launch_cheb2_32(h->d, A, B, A, state, d.bm, d.label, st, d.bm_prev, d.label_prev);
     float2* tmp = A;
     A = B;
     B = tmp;
@@ -189,7 +190,9 @@ int lsqt::evolvex_f32_enqueue(lsqt_ham* h, int direction, double t, double2* sta
   launch_comm32(h->d, A, Bx, st);
   launch_cheb1x_32(Bx, state, bt.b1, n, st);
   for (size_t k = 0; k < bt.bm.size(); ++k) {
-    launch_cheb2x_32(h->d, A, Ax, B, Bx, A, Ax, state, bt.bm[k], bt.label[k], st);
+    const DeferredTerm d = deferred_term(bt, k);
+    launch_cheb2x_32(h->d, A, Ax, B, Bx, A, Ax, state, d.bm, d.label, st, d.bm_prev,
+                     d.label_prev);
     float2* tmp = A;
     A = B;
     B = tmp;

@@ -843,11 +843,18 @@ __device__ __forceinline__ void accumulate_phase(double2& s, double bm, double t
   }
 }
 
+// Expansion terms into S (kCheb2, kCheb2x): label / bm for this launch's
+// term (the new s2, or s2x), label_prev / bm_prev for the term of the
+// launch before, whose value at each site is this launch's source s1 (s1x)
+// there; label 0 adds nothing, and a launch with both 0 leaves S alone.
+// Callers defer every other term to the next launch, so S is read and
+// written every second launch; the additions are the same, in the same
+// order, as one term per launch.
 template <int LAYOUT, int KT, int W, int OP, bool NT>
 __global__ __launch_bounds__(kBlock) void k_block_evo(
   HamDev h, int R, TileMap tm, const float2* s0, const float2* __restrict__ s1, float2* s2,
   const float2* s0x, const float2* __restrict__ s1x, float2* s2x, double2* __restrict__ S,
-  double bm, int label, int tiles, int blocks)
+  double bm, int label, double bm_prev, int label_prev, int tiles, int blocks)
 {
   constexpr bool kXX = OP == kComm || OP == kCurrent || OP == kCheb2x;
   constexpr bool kOnsite = OP == kApplyH || OP == kCheb2 || OP == kCheb2x;
@@ -874,6 +881,7 @@ __global__ __launch_bounds__(kBlock) void k_block_evo(
   int begin, end;
   tile_range(tm, tile, begin, end);
   const float inv_scale = (float)(1.0 / h.emax);
+  const bool touch = label != 0 || label_prev != 0;  // the same for the whole launch
   for (int s = begin + lane_site; s < end; s += per_pass) {
     const size_t base = (size_t)s * R + r0;
     float tr[W], ti[W], xr[W], xi[W];
@@ -882,26 +890,28 @@ __global__ __launch_bounds__(kBlock) void k_block_evo(
     // overlaps the gathers (s0 may be s2: each site is read before written);
     // slot words and potential take ordinary loads (non-temporal: 2 % slower)
     constexpr bool kRecur = OP == kCheb2 || OP == kCheb2x;
-    Vec<W> p0, p0x;
+    Vec<W> p0, p0x, p1, p1x;
     double2 Sv[W];
     if constexpr (kRecur) {
       p0 = ldv<W, NT>(s0 + base);
       if constexpr (OP == kCheb2x)
         p0x = ldv<W, NT>(s0x + base);
+      if (touch) {
 #pragma unroll
-      for (int w = 0; w < W; ++w)
-        Sv[w] = S[base + w];
+        for (int w = 0; w < W; ++w)
+          Sv[w] = S[base + w];
+      }
     }
     if constexpr (kOnsite) {
       const float v = ld<false>(h.pot32 + s);
-      Vec<W> p1 = ldv<W, false>(s1 + base);
+      p1 = ldv<W, false>(s1 + base);
 #pragma unroll
       for (int w = 0; w < W; ++w) {
         tr[w] = v * p1.v[w].x;
         ti[w] = v * p1.v[w].y;
       }
       if constexpr (OP == kCheb2x) {
-        Vec<W> p1x = ldv<W, false>(s1x + base);
+        p1x = ldv<W, false>(s1x + base);
 #pragma unroll
         for (int w = 0; w < W; ++w) {
           xr[w] = v * p1x.v[w].x;
@@ -982,6 +992,7 @@ __global__ __launch_bounds__(kBlock) void k_block_evo(
                              __builtin_fmaf(2.0f, ti[w] * inv_scale, -p0.v[w].y));
       stv<W>(s2 + base, o);
       const Vec<W>* acc = &o;
+      const Vec<W>* prev = &p1;
       Vec<W> ox;
       if constexpr (OP == kCheb2x) {
 #pragma unroll
@@ -990,11 +1001,16 @@ __global__ __launch_bounds__(kBlock) void k_block_evo(
                                 __builtin_fmaf(2.0f, xi[w] * inv_scale, -p0x.v[w].y));
         stv<W>(s2x + base, ox);
         acc = &ox;
+        prev = &p1x;
       }
+      if (touch) {
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        accumulate_phase(Sv[w], bm, (double)acc->v[w].x, (double)acc->v[w].y, label);
-        S[base + w] = Sv[w];
+        for (int w = 0; w < W; ++w) {
+          accumulate_phase(Sv[w], bm_prev, (double)prev->v[w].x, (double)prev->v[w].y,
+                           label_prev);
+          accumulate_phase(Sv[w], bm, (double)acc->v[w].x, (double)acc->v[w].y, label);
+          S[base + w] = Sv[w];
+        }
       }
     }
   }
@@ -1008,7 +1024,7 @@ template <int LAYOUT, int KT, int W, int OP>
 __global__ __launch_bounds__(kBlock) void k_block_evo_win(
   HamDev h, int R, TileMap tm, const float2* s0, const float2* __restrict__ s1, float2* s2,
   const float2* s0x, const float2* __restrict__ s1x, float2* s2x, double2* __restrict__ S,
-  double bm, int label, int tiles, int blocks)
+  double bm, int label, double bm_prev, int label_prev, int tiles, int blocks)
 {
   static_assert(KT > 0, "the window kernel needs a compile-time slot count");
   constexpr bool kXX = OP == kComm || OP == kCurrent || OP == kCheb2x;
@@ -1041,6 +1057,7 @@ __global__ __launch_bounds__(kBlock) void k_block_evo_win(
   __syncthreads();
 
   const float inv_scale = (float)(1.0 / h.emax);
+  const bool touch = label != 0 || label_prev != 0;  // the same for the whole launch
   constexpr int C = KT;
   int s = wt.xbase + wt.y0 * wt.row + wt.z0 + lane_site;
   uint32_t wnext[C];
@@ -1055,28 +1072,30 @@ __global__ __launch_bounds__(kBlock) void k_block_evo_win(
     }
     const int sn = more ? s + wt.row : s;
     const size_t base = (size_t)s * R + r0;
-    Vec<W> p0, p0x;
+    Vec<W> p0, p0x, p1, p1x;
     double2 Sv[W];
     if constexpr (kRecur) {
       p0 = ldv<W, true>(s0 + base);
       if constexpr (kTwo)
         p0x = ldv<W, true>(s0x + base);
+      if (touch) {
 #pragma unroll
-      for (int w = 0; w < W; ++w)
-        Sv[w] = S[base + w];
+        for (int w = 0; w < W; ++w)
+          Sv[w] = S[base + w];
+      }
     }
     const int own = ((p + 1) & 3) * wt.span + (lane_site + 1) * R + r0;
     float tr[W], ti[W], xr[W], xi[W];
     if constexpr (kOnsite) {
       const float v = h.pot32[s];
-      const Vec<W> p1 = WindowTile<W>::read(s_win, own);
+      p1 = WindowTile<W>::read(s_win, own);
 #pragma unroll
       for (int w = 0; w < W; ++w) {
         tr[w] = v * p1.v[w].x;
         ti[w] = v * p1.v[w].y;
       }
       if constexpr (kTwo) {
-        const Vec<W> p1x = WindowTile<W>::read(s_winx, own);
+        p1x = WindowTile<W>::read(s_winx, own);
 #pragma unroll
         for (int w = 0; w < W; ++w) {
           xr[w] = v * p1x.v[w].x;
@@ -1167,6 +1186,7 @@ __global__ __launch_bounds__(kBlock) void k_block_evo_win(
                              __builtin_fmaf(2.0f, ti[w] * inv_scale, -p0.v[w].y));
       stv<W>(s2 + base, o);
       const Vec<W>* acc = &o;
+      const Vec<W>* prev = &p1;
       Vec<W> ox;
       if constexpr (OP == kCheb2x) {
 #pragma unroll
@@ -1175,11 +1195,16 @@ __global__ __launch_bounds__(kBlock) void k_block_evo_win(
                                 __builtin_fmaf(2.0f, xi[w] * inv_scale, -p0x.v[w].y));
         stv<W>(s2x + base, ox);
         acc = &ox;
+        prev = &p1x;
       }
+      if (touch) {
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        accumulate_phase(Sv[w], bm, (double)acc->v[w].x, (double)acc->v[w].y, label);
-        S[base + w] = Sv[w];
+        for (int w = 0; w < W; ++w) {
+          accumulate_phase(Sv[w], bm_prev, (double)prev->v[w].x, (double)prev->v[w].y,
+                           label_prev);
+          accumulate_phase(Sv[w], bm, (double)acc->v[w].x, (double)acc->v[w].y, label);
+          S[base + w] = Sv[w];
+        }
       }
     }
     if (more) {
@@ -1431,7 +1456,8 @@ void launch_block_step(const HamDev& h, int R, int W, bool db, const TileMap& tm
                        double2* partial, double2* partial2, bool recur, int products,
                        hipStream_t st)
 {
-  static_assert(kStepLeft == kLeft && kStepSelf == kSelf && kStepPair == kPair,
+  static_assert((int)kStepLeft == (int)kLeft && (int)kStepSelf == (int)kSelf &&
+                  (int)kStepPair == (int)kPair,
                 "product modes");
   Grid g = block_grid(tm);
   if (tm.win.by > 0 && W <= 4 && (h.K == 3 || h.K == 4 || h.K == 6)) {
@@ -1517,7 +1543,7 @@ void launch_block_reduce(const double2* partial, const TileMap& tm, int R, int M
 void launch_block_evo(const HamDev& h, int R, int W, const TileMap& tm, int op,
                       const float2* s0, const float2* s1, float2* s2, const float2* s0x,
                       const float2* s1x, float2* s2x, double2* S, double bm, int label,
-                      hipStream_t st)
+                      hipStream_t st, double bm_prev, int label_prev)
 {
   Grid g = block_grid(tm);
   // one ring of four window rows per source block (two for kCheb2x), beside
@@ -1529,7 +1555,8 @@ void launch_block_evo(const HamDev& h, int R, int W, const TileMap& tm, int op,
 #define LSQTB_EVOW(OPV)                                                                    \
   k_block_evo_win<LAY, KT, WW, OPV><<<g.blocks, kBlock, lds, st>>>(h, R, tm, s0, s1, s2, s0x, \
                                                                   s1x, s2x, S, bm, label,     \
-                                                                  g.tiles, g.blocks)
+                                                                  bm_prev, label_prev, g.tiles, \
+                                                                  g.blocks)
 #define LSQTB_EVOW_OP                   \
   switch (op) {                         \
     case kApplyH:                       \
@@ -1571,7 +1598,7 @@ void launch_block_evo(const HamDev& h, int R, int W, const TileMap& tm, int op,
   const bool nt = (size_t)h.n * R >= (size_t)kStreamingMinSites;
 #define LSQTB_EVO(OPV, NTV)                                                              \
   k_block_evo<LAY, KT, WW, OPV, NTV><<<g.blocks, kBlock, 0, st>>>(                       \
-    h, R, tm, s0, s1, s2, s0x, s1x, s2x, S, bm, label, g.tiles, g.blocks)
+    h, R, tm, s0, s1, s2, s0x, s1x, s2x, S, bm, label, bm_prev, label_prev, g.tiles, g.blocks)
 #define LSQTB_EVO_OP(NTV)                  \
   switch (op) {                            \
     case kApplyH:                          \

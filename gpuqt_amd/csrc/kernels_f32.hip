@@ -324,12 +324,16 @@ __device__ __forceinline__ void accumulate_label64(double2& s, double bm, double
   }
 }
 
-// one term of U(t): FP32 recursion, FP64 accumulation into state
+// one term of U(t): FP32 recursion, FP64 accumulation into state.
+// bm_prev / label_prev: the previous launch's term, s1, added first where
+// terms are added in pairs (label 0 adds nothing; with both 0 the state is
+// neither read nor written): the same additions, in the same order
 template <int LAYOUT, int KT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_cheb2_32(HamDev h, const float2* s0,
                                                      const float2* __restrict__ s1, float2* s2,
                                                      double2* __restrict__ s, double bm,
-                                                     int label, int tiles, int blocks)
+                                                     int label, double bm_prev, int label_prev,
+                                                     int tiles, int blocks)
 {
   __shared__ double2 s_tab[LAYOUT == kPackedLds ? kLdsTableMax : 1];
   int tile = xcd_tile(blockIdx.x, blocks);
@@ -346,9 +350,15 @@ __global__ __launch_bounds__(kBlock) void k_cheb2_32(HamDev h, const float2* s0,
   ti = 2.0 * ti - (double)p0.y;
   float2 o = make_float2((float)tr, (float)ti);
   s2[n] = o;
-  double2 st = ld2<NT>(s + n);
-  accumulate_label64(st, bm, (double)o.x, (double)o.y, label);
-  s[n] = st;
+  if (label != 0 || label_prev != 0) {
+    double2 st = ld2<NT>(s + n);
+    if (label_prev != 0) {
+      const float2 p1 = s1[n];  // a cache hit: the on-site term read it
+      accumulate_label64(st, bm_prev, (double)p1.x, (double)p1.y, label_prev);
+    }
+    accumulate_label64(st, bm, (double)o.x, (double)o.y, label);
+    s[n] = st;
+  }
 }
 
 // one term of [X, U(t)]: coupled FP32 recursions, FP64 accumulation
@@ -359,7 +369,8 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x_32(HamDev h, const float2* s0
                                                       const float2* __restrict__ s1x,
                                                       float2* s2, float2* s2x,
                                                       double2* __restrict__ s, double bm,
-                                                      int label, int tiles, int blocks)
+                                                      int label, double bm_prev,
+                                                      int label_prev, int tiles, int blocks)
 {
   __shared__ double2 s_tab[LAYOUT == kPackedLds ? kLdsTableMax : 1];
   __shared__ double s_xx[LAYOUT == kPackedLds ? kLdsTableMax : 1];
@@ -409,9 +420,12 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x_32(HamDev h, const float2* s0
   s2[n] = make_float2((float)tr, (float)ti);
   float2 ox = make_float2((float)txr, (float)txi);
   s2x[n] = ox;
-  double2 st = ld2<NT>(s + n);
-  accumulate_label64(st, bm, (double)ox.x, (double)ox.y, label);
-  s[n] = st;
+  if (label != 0 || label_prev != 0) {
+    double2 st = ld2<NT>(s + n);
+    accumulate_label64(st, bm_prev, (double)p1x.x, (double)p1x.y, label_prev);
+    accumulate_label64(st, bm, (double)ox.x, (double)ox.y, label);
+    s[n] = st;
+  }
 }
 
 // element-wise: conversions and the first terms of the series
@@ -627,25 +641,27 @@ void launch_comm32(const HamDev& h, const float2* in, float2* out, hipStream_t s
 }
 
 void launch_cheb2_32(const HamDev& h, const float2* s0, const float2* s1, float2* s2,
-                     double2* s, double bm, int label, hipStream_t st)
+                     double2* s, double bm, int label, hipStream_t st, double bm_prev,
+                     int label_prev)
 {
   Grid g = make_grid(h.n, 1);
   const bool nt = h.n >= kStreamingMinSites;
   LSQT32_LAYOUT_SWITCH(h.layout, LAY, LSQT32_K_SWITCH(h.K, KT, LSQT32_NT_SWITCH(nt, NT, {
-    k_cheb2_32<LAY, KT, NT><<<g.blocks, kBlock, 0, st>>>(h, s0, s1, s2, s, bm, label, g.tiles,
-                                                          g.blocks);
+    k_cheb2_32<LAY, KT, NT><<<g.blocks, kBlock, 0, st>>>(h, s0, s1, s2, s, bm, label, bm_prev,
+                                                          label_prev, g.tiles, g.blocks);
   })))
 }
 
 void launch_cheb2x_32(const HamDev& h, const float2* s0, const float2* s0x, const float2* s1,
                       const float2* s1x, float2* s2, float2* s2x, double2* s, double bm,
-                      int label, hipStream_t st)
+                      int label, hipStream_t st, double bm_prev, int label_prev)
 {
   Grid g = make_grid(h.n, 1);
   const bool nt = h.n >= kStreamingMinSites;
   LSQT32_LAYOUT_SWITCH(h.layout, LAY, LSQT32_K_SWITCH(h.K, KT, LSQT32_NT_SWITCH(nt, NT, {
     k_cheb2x_32<LAY, KT, NT><<<g.blocks, kBlock, 0, st>>>(h, s0, s0x, s1, s1x, s2, s2x, s, bm,
-                                                           label, g.tiles, g.blocks);
+                                                           label, bm_prev, label_prev, g.tiles,
+                                                           g.blocks);
   })))
 }
 

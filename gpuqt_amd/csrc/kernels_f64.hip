@@ -210,11 +210,15 @@ __device__ __forceinline__ void accumulate_label(double2& s, double bm, double t
 }
 
 // --- one term of U(t): s2 = 2 H~ s1 - s0, s += b (phase) s2 -------------
+// bm_prev / label_prev: the previous launch's term, s1, added first where
+// terms are added in pairs (label 0 adds nothing; with both 0 the state s is
+// neither read nor written): the same additions, in the same order
 template <int LAYOUT, int KT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_cheb2(HamDev h, const double2* s0,
                                                   const double2* __restrict__ s1, double2* s2,
                                                   double2* __restrict__ s, double bm, int label,
-                                                  int tiles, int blocks)
+                                                  double bm_prev, int label_prev, int tiles,
+                                                  int blocks)
 {
   __shared__ double2 s_tab[LAYOUT == kPackedLds ? kLdsTableMax : 1];
   int tile = xcd_tile(blockIdx.x, blocks);
@@ -234,9 +238,12 @@ __global__ __launch_bounds__(kBlock) void k_cheb2(HamDev h, const double2* s0,
   double2 p0 = ld2<NT>(s0 + n);
   tr = 2.0 * tr - p0.x;
   ti = 2.0 * ti - p0.y;
-  double2 st = ld2<NT>(s + n);
-  accumulate_label(st, bm, tr, ti, label);
-  st2<false>(s + n, st);
+  if (label != 0 || label_prev != 0) {
+    double2 st = ld2<NT>(s + n);
+    accumulate_label(st, bm_prev, p1.x, p1.y, label_prev);
+    accumulate_label(st, bm, tr, ti, label);
+    st2<false>(s + n, st);
+  }
   st2<false>(s2 + n, make_double2(tr, ti));
 }
 
@@ -247,7 +254,8 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x(HamDev h, const double2* s0,
                                                    const double2* __restrict__ s1,
                                                    const double2* __restrict__ s1x, double2* s2,
                                                    double2* s2x, double2* __restrict__ s,
-                                                   double bm, int label, int tiles, int blocks)
+                                                   double bm, int label, double bm_prev,
+                                                   int label_prev, int tiles, int blocks)
 {
   __shared__ double2 s_tab[LAYOUT == kPackedLds ? kLdsTableMax : 1];
   __shared__ double s_xx[LAYOUT == kPackedLds ? kLdsTableMax : 1];
@@ -299,9 +307,12 @@ __global__ __launch_bounds__(kBlock) void k_cheb2x(HamDev h, const double2* s0,
   txr = 2.0 * txr - p0x.x;
   txi = 2.0 * txi - p0x.y;
   st2<false>(s2x + n, make_double2(txr, txi));
-  double2 st = ld2<NT>(s + n);
-  accumulate_label(st, bm, txr, txi, label);
-  st2<false>(s + n, st);
+  if (label != 0 || label_prev != 0) {
+    double2 st = ld2<NT>(s + n);
+    accumulate_label(st, bm_prev, p1x.x, p1x.y, label_prev);
+    accumulate_label(st, bm, txr, txi, label);
+    st2<false>(s + n, st);
+  }
 }
 
 // --- element-wise kernels -------------------------------------------------
@@ -507,25 +518,27 @@ void launch_xop(const HamDev& h, const double2* in, double2* out, bool current, 
 }
 
 void launch_cheb2(const HamDev& h, const double2* s0, const double2* s1, double2* s2,
-                  double2* s, double bm, int label, hipStream_t st)
+                  double2* s, double bm, int label, hipStream_t st, double bm_prev,
+                  int label_prev)
 {
   Grid g = make_grid(h.n, 1);
   const bool nt = h.n >= kStreamingMinSites;
   LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, LSQT_NT_SWITCH(nt, NT, {
-    k_cheb2<LAY, KT, NT><<<g.blocks, kBlock, 0, st>>>(h, s0, s1, s2, s, bm, label, g.tiles,
-                                                        g.blocks);
+    k_cheb2<LAY, KT, NT><<<g.blocks, kBlock, 0, st>>>(h, s0, s1, s2, s, bm, label, bm_prev,
+                                                        label_prev, g.tiles, g.blocks);
   })))
 }
 
 void launch_cheb2x(const HamDev& h, const double2* s0, const double2* s0x, const double2* s1,
                    const double2* s1x, double2* s2, double2* s2x, double2* s, double bm,
-                   int label, hipStream_t st)
+                   int label, hipStream_t st, double bm_prev, int label_prev)
 {
   Grid g = make_grid(h.n, 1);
   const bool nt = h.n >= kStreamingMinSites;
   LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, LSQT_NT_SWITCH(nt, NT, {
     k_cheb2x<LAY, KT, NT><<<g.blocks, kBlock, 0, st>>>(h, s0, s0x, s1, s1x, s2, s2x, s, bm,
-                                                         label, g.tiles, g.blocks);
+                                                         label, bm_prev, label_prev, g.tiles,
+                                                         g.blocks);
   })))
 }
 

@@ -10,11 +10,16 @@ int hpsi_tiles(int n);
 void launch_hpsi(const HamDev& h, const double2* psi1, const double2* psi0, double2* out,
                  const double2* L, double2* partial, bool recur, bool ip, hipStream_t st);
 void launch_xop(const HamDev& h, const double2* in, double2* out, bool current, hipStream_t st);
+// one expansion term of U (of [X, U]): s2 = 2 H~ s1 - s0 (and s2x) with
+// s += bm (label phase) s2 (s2x); bm_prev / label_prev add the previous
+// launch's term (s1, s1x) first, label 0 adds nothing, and with both 0 s is
+// neither read nor written (terms added in pairs: deferred_term)
 void launch_cheb2(const HamDev& h, const double2* s0, const double2* s1, double2* s2,
-                  double2* s, double bm, int label, hipStream_t st);
+                  double2* s, double bm, int label, hipStream_t st, double bm_prev = 0.0,
+                  int label_prev = 0);
 void launch_cheb2x(const HamDev& h, const double2* s0, const double2* s0x, const double2* s1,
                    const double2* s1x, double2* s2, double2* s2x, double2* s, double bm,
-                   int label, hipStream_t st);
+                   int label, hipStream_t st, double bm_prev = 0.0, int label_prev = 0);
 void launch_zero(double2* v, int n, hipStream_t st);
 void launch_copy(double2* dst, const double2* src, int n, hipStream_t st);
 void launch_add(double2* dst, const double2* src, int n, hipStream_t st);
@@ -32,11 +37,13 @@ int hpsi32_tiles(int n);
 void launch_hpsi32(const HamDev& h, const float2* psi1, const float2* psi0, float2* out,
                    const float2* L, double2* partial, bool recur, bool ip, hipStream_t st);
 void launch_comm32(const HamDev& h, const float2* in, float2* out, hipStream_t st);
+// the same with FP32 recursion vectors (FP64 state s)
 void launch_cheb2_32(const HamDev& h, const float2* s0, const float2* s1, float2* s2,
-                     double2* s, double bm, int label, hipStream_t st);
+                     double2* s, double bm, int label, hipStream_t st, double bm_prev = 0.0,
+                     int label_prev = 0);
 void launch_cheb2x_32(const HamDev& h, const float2* s0, const float2* s0x, const float2* s1,
                       const float2* s1x, float2* s2, float2* s2x, double2* s, double bm,
-                      int label, hipStream_t st);
+                      int label, hipStream_t st, double bm_prev = 0.0, int label_prev = 0);
 void launch_zero32(float2* v, int n, hipStream_t st);
 // packed slot words of a lattice model (class m*K + i for hopping i of orbital m)
 void launch_lattice_words(const LatticeDev& L, uint32_t* words, int stride, int idx_bits, int n,
@@ -79,11 +86,14 @@ void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& 
 void launch_block_interleave(const float2* L, const float2* Rb, int n, int R, float2* dst,
                              hipStream_t st);
 // block time evolution (op: 0 H~, 1 Chebyshev term of U, 2 [X, H~], 3 V,
-// 4 Chebyshev term of [X, U]); S is the FP64 accumulated state [n][R]
+// 4 Chebyshev term of [X, U]); S is the FP64 accumulated state [n][R].
+// Ops 1 and 4 add bm_prev * (label_prev phase of s1, or s1x: the previous
+// launch's term) and then bm * (label phase of the new s2, or s2x) to S;
+// label 0 adds nothing, and with both 0 S is neither read nor written.
 void launch_block_evo(const HamDev& h, int R, int W, const TileMap& tm, int op,
                       const float2* s0, const float2* s1, float2* s2, const float2* s0x,
                       const float2* s1x, float2* s2x, double2* S, double bm, int label,
-                      hipStream_t st);
+                      hipStream_t st, double bm_prev = 0.0, int label_prev = 0);
 // partial[t][r] = (|S_r|^2 over tile t, 0)
 void launch_block_norm64(const double2* S, int R, const TileMap& tm, double2* partial,
                          hipStream_t st);
