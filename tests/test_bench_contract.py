@@ -1,0 +1,30 @@
+"""bench.py's contract on the CPU: more than one GPU is refused with a
+message, and the committed HBM-traffic profile holds the kernels whose
+`traffic` the JSON line reports (the FP64 fused step of the default
+workload and the tuned block kernels of the 160^3 cube)."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
+    assert r.stdout == ""
+
+
+def test_traffic_profile_entries():
+    import bench
+    fp64 = bench.profiled_traffic("k_hpsi<1,3,1,true,true,true>")
+    # HBM bytes per launch within a few per cent of the 84 B x 8.0M algorithmic
+    assert fp64 is not None and abs(fp64 / (84.0 * 8_000_000) - 1.0) < 0.05
+    assert bench.profiled_traffic("k_hpsi32<1,3,true,true,true>") is not None
+    for mode in (1, 2):  # <L|psi> and product-identity block steps, sliding window
+        t = bench.profiled_traffic(f"k_block_win<1,6,2,true,{mode}>")
+        assert t is not None and t > 0
+    assert bench.profiled_traffic("no_such_kernel<1>") is None
