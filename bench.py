@@ -92,6 +92,28 @@ class HipEvents:
         return ms.value
 
 
+def copy_bandwidth(ctx, n=1 << 27, reps=10):
+    """GB/s of the library's plain copy kernel (k_copy: 16 B read and 16 B
+    written per element) between two FP64 vectors of n elements (2.1 GB
+    each), timed with hipEvents on the context's stream: what a pure stream
+    kernel reaches on this GPU (one element per thread, no cache hints),
+    beside which the fused step's rate reads."""
+    from gpuqt_amd import native
+    a, b = native.Vector(ctx, n), native.Vector(ctx, n)
+    b.copy_from(a)
+    ctx.sync()
+    ev = HipEvents(ctx.stream)
+    e0, e1 = ev.new(), ev.new()
+    ev.record(e0)
+    for _ in range(reps):
+        b.copy_from(a)
+    ev.record(e1)
+    ms = ev.elapsed_ms(e0, e1) / reps
+    for v in (a, b):
+        v.close()
+    return 32.0 * n / (ms * 1e-3) / 1e9
+
+
 def run_job(H, model, r_vec, energies, time_steps, emax, volume, M, msd, ev=None, acc=None,
             precision="f64"):
     """One DOS (+ MSD) job on the device.  Returns (site_updates, rows).
@@ -675,6 +697,8 @@ def main():
     default_workload = (args.nx, args.ny) == (2000, 1000) and info.layout == 2 \
         and info.max_neighbor == 3
     traffic = profiled_traffic("k_hpsi<1,3,1,true,true,true>") if default_workload else None
+    stream = copy_bandwidth(ctx)
+    log(f"plain copy: {stream:.0f} GB/s; fused step {achieved:.0f} GB/s")
     dos = rows[0]
     log(f"DOS(E=0) = {dos[len(dos) // 2]:.6g}, {len(rows)} rows")
 
@@ -759,6 +783,8 @@ def main():
             "bytes_per_launch_algorithmic": bytes_per_launch,
             "bytes_per_site_update": info.bytes_per_site_f64,
             "avg_launch_us": avg_launch_ms * 1e3,
+            "plain_copy_GBs": stream,
+            "ratio_to_plain_copy": achieved / stream,
         },
     }
     if f32 is not None:
