@@ -696,7 +696,7 @@ def main():
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     default_workload = (args.nx, args.ny) == (2000, 1000) and info.layout == 2 \
         and info.max_neighbor == 3
-    traffic = profiled_traffic("k_hpsi<1,3,1,true,true,true>") if default_workload else None
+    traffic = profiled_traffic("k_hpsi<1,3,2,true,true,true>") if default_workload else None
     stream = copy_bandwidth(ctx)
     log(f"plain copy: {stream:.0f} GB/s; fused step {achieved:.0f} GB/s")
     dos = rows[0]
@@ -771,7 +771,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_hpsi (2 H~ psi_{m-1} - psi_{m-2} fused with <L|psi_m>)",
+            "kernel": "k_hpsi (2 H~ psi_{m-1} - psi_{m-2} fused with <L|psi_m>; slot words "
+                      "computed from the lattice pattern, 2 sites per thread)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

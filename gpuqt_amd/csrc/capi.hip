@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -278,6 +279,103 @@ void detect_lattice_offsets(lsqt_ham* h, int n, int K, const int* nn, const int*
   }
 }
 
+// The slot pattern of a packed Hamiltonian.  On a periodic lattice whose
+// orbital index runs fastest, slot m of site n holds neighbour n + off and
+// class c that depend only on n mod p, except at the boundaries.  The period
+// is the smallest p <= 64 (with p x slots <= 512 table entries) whose most
+// common entries per residue describe at least 90 % of 4096 sites from the
+// middle of the system; a device pass then marks every site whose words
+// differ from the pattern in any slot, and the pattern is kept where at most
+// 10 % of the sites are so marked.  The kernels that use it compute the words
+// of the unmarked sites, which equal the stored ones bit for bit, and load
+// the marked sites' words.  LSQT_PATTERN=0 in the environment turns it off.
+// Any failure leaves the Hamiltonian without a pattern (it is an
+// optimisation only).
+void detect_slot_pattern(lsqt_ham* h)
+{
+  HamDev& d = h->d;
+  const char* env = getenv("LSQT_PATTERN");
+  if (d.layout == kFull || d.n < 4096 || (env && env[0] == '0'))
+    return;
+  const int K = d.K;
+  constexpr int kSample = 4096;
+  const int s0 = (d.n / 2 - kSample / 2) / 64 * 64;
+  std::vector<uint32_t> sample((size_t)K * kSample);
+  for (int m = 0; m < K; ++m)
+    if (hipMemcpy(sample.data() + (size_t)m * kSample, d.words + (size_t)m * d.stride + s0,
+                  sizeof(uint32_t) * kSample, hipMemcpyDeviceToHost) != hipSuccess)
+      return;
+  // (offset, class) of sample site i in slot m, as two 32-bit halves
+  auto entry = [&](int i, int m) -> unsigned long long {
+    const uint32_t w = sample[(size_t)m * kSample + i];
+    const int off = w == kEmptySlot ? kPatternEmpty : (int)(w & d.idx_mask) - (s0 + i);
+    const uint32_t cls = w == kEmptySlot ? 0u : w >> d.idx_bits;
+    return ((unsigned long long)(uint32_t)off << 32) | cls;
+  };
+  int period = 0;
+  std::vector<int2> table;
+  for (int p = 1; p <= kPatternMaxPeriod && p * K <= kPatternMaxEntries && !period; ++p) {
+    std::vector<int2> tab((size_t)p * K);
+    long long matched = 0;
+    for (int r = 0; r < p; ++r) {
+      std::map<std::vector<unsigned long long>, int> counts;
+      for (int i = (r - s0 % p + p) % p; i < kSample; i += p) {
+        std::vector<unsigned long long> key(K);
+        for (int m = 0; m < K; ++m)
+          key[m] = entry(i, m);
+        ++counts[key];
+      }
+      auto best = counts.begin();
+      for (auto it = counts.begin(); it != counts.end(); ++it)
+        if (it->second > best->second)
+          best = it;
+      if (best == counts.end())
+        continue;
+      matched += best->second;
+      for (int m = 0; m < K; ++m)
+        tab[(size_t)m * p + r] = make_int2((int)(uint32_t)(best->first[m] >> 32),
+                                           (int)(uint32_t)best->first[m]);
+    }
+    if (matched * 10 >= 9ll * kSample) {
+      period = p;
+      table.swap(tab);
+    }
+  }
+  if (!period)
+    return;
+  const int2* pat = nullptr;
+  if (upload(h, &pat, table.data(), table.size()) != LSQT_OK)
+    return;
+  const int words = pattern_mask_words(d.n);
+  void* mp = nullptr;
+  if (hipMalloc(&mp, sizeof(uint32_t) * words) != hipSuccess)
+    return;
+  h->allocations.push_back(mp);
+  uint32_t* mask = (uint32_t*)mp;
+  d.pat = pat;
+  d.pat_p = period;
+  d.pat_div = period >= 2 ? make_divisor(period) : Divisor{0u, 0};
+  launch_pattern_mask(d, mask, h->ctx->stream);
+  std::vector<uint32_t> bits(words);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->ctx->stream) != hipSuccess ||
+      hipMemcpy(bits.data(), mask, sizeof(uint32_t) * words, hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+    d.pat = nullptr;
+    d.pat_p = 0;
+    return;
+  }
+  long long irregular = 0;
+  for (int i = 0; i < d.n; ++i)
+    irregular += (bits[i >> 5] >> (i & 31)) & 1u;
+  if (irregular * 10 > d.n) {
+    d.pat = nullptr;
+    d.pat_p = 0;
+    return;
+  }
+  d.irregular = mask;
+  h->pat_irregular = irregular;
+}
+
 }  // namespace
 
 int lsqt::ham_scratch(lsqt_ham* h, int count)
@@ -424,6 +522,7 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
     return rc;
   }
   detect_lattice_offsets(h, n, K, nn, nl);
+  detect_slot_pattern(h);
   // host copies for the FP32 / block paths, built on demand
   h->host_pot.assign(pot, pot + n);
   h->host_tab = tab;
@@ -563,6 +662,7 @@ extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
     h->slab = (int)slab;
     h->row = (int)row;
   }
+  detect_slot_pattern(h);
   h->host_pot.assign(potential, potential + n);
   h->host_tab = tab;
   h->host_tab_xx = tab_xx;
@@ -628,9 +728,16 @@ extern "C" int lsqt_ham_get_info(const lsqt_ham* h, lsqt_ham_info* info)
   info->index_bits = h->d.idx_bits;
   info->energy_max = h->d.emax;
   // potential 8 + per slot (word 4 | index 4 + re 8 + im 8) + psi_{m-1} 16 +
-  // psi_{m-2} 16 + psi_m 16 + left vector 16
+  // psi_{m-2} 16 + psi_m 16 + left vector 16; with a slot pattern the words
+  // of the irregular sites only, and the mask (1 bit per site)
   double per_slot = h->d.layout == kFull ? 20.0 : 4.0;
-  info->bytes_per_site_f64 = 8.0 + per_slot * h->d.K + 64.0;
+  info->pattern_period = h->d.pat ? h->d.pat_p : 0;
+  info->pattern_irregular = h->d.pat ? (double)h->pat_irregular / h->d.n : 0.0;
+  if (h->d.pat)
+    info->bytes_per_site_f64 =
+      8.0 + per_slot * h->d.K * info->pattern_irregular + 1.0 / 8.0 + 64.0;
+  else
+    info->bytes_per_site_f64 = 8.0 + per_slot * h->d.K + 64.0;
   info->slab = h->slab;
   info->row = h->row;
   return LSQT_OK;
@@ -920,7 +1027,7 @@ int lsqt::moments_enqueue(lsqt_ham* h, const double2* left, const double2* right
   lsqt_ctx* ctx = h->ctx;
   hipStream_t st = ctx->stream;
   const int n = h->d.n;
-  const int tiles = hpsi_tiles(n);
+  const int tiles = hpsi_tiles(n, h->d.K);
   LSQT_TRY(ctx_reserve_moments(ctx, M, tiles));
   LSQT_TRY(ham_scratch(h, 2));
   double2* A = h->scratch[0];
@@ -928,7 +1035,7 @@ int lsqt::moments_enqueue(lsqt_ham* h, const double2* left, const double2* right
   const size_t stride = (size_t)tiles;
   // psi_0 = right; mu_0 = <left|psi_0>
   launch_copy(A, right, n, st);
-  launch_dot(A, left, ctx->partial, n, st);
+  launch_dot(A, left, ctx->partial, n, st, hpsi_spt(h->d.K));
   // psi_1 = H~ psi_0; mu_1
   launch_hpsi(h->d, A, nullptr, B, left, ctx->partial + stride, false, true, st);
   // psi_m = 2 H~ psi_{m-1} - psi_{m-2}, written over psi_{m-2}

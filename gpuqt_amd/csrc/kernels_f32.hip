@@ -132,14 +132,30 @@ __global__ __launch_bounds__(kBlock) void k_hpsi32(HamDev h, const float2* __res
                                                    int blocks)
 {
   __shared__ double2 s_tab[LAYOUT == kPackedLds ? kLdsTableMax : 1];
+  __shared__ int2 s_pat[kPatternMaxEntries];
   int tile = xcd_tile(blockIdx.x, blocks);
   if (tile >= tiles)
     return;
   stage32<LAYOUT, false>(h, s_tab, nullptr);
+  // a slot pattern: regular sites compute their words (the same words)
+  const bool pat = h.pat != nullptr;  // the same for the whole launch
+  if (pat) {
+    for (int i = threadIdx.x; i < h.pat_p * h.K; i += kBlock)
+      s_pat[i] = h.pat[i];
+    __syncthreads();
+  }
   const int K = KT > 0 ? KT : h.K;
   const int n = 2 * (tile * kBlock + threadIdx.x);
   double acc_re = 0.0, acc_im = 0.0;
   if (n < h.n) {
+    int r0 = 0, r1 = 0;
+    bool irr = false;
+    if (pat) {
+      r0 = pattern_residue(h, n);
+      r1 = r0 + 1 == h.pat_p ? 0 : r0 + 1;
+      // n is even: both bits sit in one mask word (padding past h.n is marked)
+      irr = pattern_irregular(h, n) || pattern_irregular(h, n + 1);
+    }
     fvec4_t p1 = *reinterpret_cast<const fvec4_t*>(psi1 + n);
     // slot words and potential with ordinary loads (non-temporal: 1 % slower
     // here), the once-read streams non-temporally and with the first loads
@@ -156,11 +172,27 @@ __global__ __launch_bounds__(kBlock) void k_hpsi32(HamDev h, const float2* __res
     constexpr int C = SlotBatch<KT>::C;
     for (int m0 = 0; m0 < K; m0 += C) {
       uvec2_t w[C];
+      if (pat) {
 #pragma unroll
-      for (int q = 0; q < C; ++q)
-        w[q] = (m0 + q < K)
-                 ? ld<false>(reinterpret_cast<const uvec2_t*>(h.words + (size_t)(m0 + q) * h.stride + n))
-                 : uvec2_t{kEmptySlot, kEmptySlot};
+        for (int q = 0; q < C; ++q)
+          w[q] = (m0 + q < K)
+                   ? uvec2_t{pattern_word(h, s_pat[(m0 + q) * h.pat_p + r0], n),
+                             pattern_word(h, s_pat[(m0 + q) * h.pat_p + r1], n + 1)}
+                   : uvec2_t{kEmptySlot, kEmptySlot};
+        if (irr) {
+#pragma unroll
+          for (int q = 0; q < C; ++q)
+            if (m0 + q < K)
+              w[q] = ld<false>(
+                reinterpret_cast<const uvec2_t*>(h.words + (size_t)(m0 + q) * h.stride + n));
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < C; ++q)
+          w[q] = (m0 + q < K)
+                   ? ld<false>(reinterpret_cast<const uvec2_t*>(h.words + (size_t)(m0 + q) * h.stride + n))
+                   : uvec2_t{kEmptySlot, kEmptySlot};
+      }
       int j0[C], j1[C];
       double a0[C], b0[C], a1[C], b1[C];
       bool ok0[C], ok1[C];

@@ -42,10 +42,10 @@ __device__ __forceinline__ void stage_table(const HamDev& h, double2* s_tab, dou
 // Every slot's loads are issued before any is used; the adds are predicated
 // so the skipped slots leave the sums bit-identical to a loop that stops at
 // neighbor_number[n] (src/hamiltonian.cu:206-215).
-template <int LAYOUT, int KT, bool NT>
+template <int LAYOUT, int KT, bool NT, bool PAT = false>
 __device__ __forceinline__ void hsum(const HamDev& h, const double2* s_tab,
                                      const double2* __restrict__ psi, int n, double& tr,
-                                     double& ti)
+                                     double& ti, const int2* s_pat = nullptr)
 {
   constexpr int C = SlotBatch<KT>::C;
   const int K = KT > 0 ? KT : h.K;
@@ -53,7 +53,7 @@ __device__ __forceinline__ void hsum(const HamDev& h, const double2* s_tab,
     int j[C];
     double a[C], b[C], x[C];
     bool ok[C];
-    load_slots<LAYOUT, C, NT, false>(h, s_tab, nullptr, n, m0, K, j, a, b, x, ok);
+    load_slots<LAYOUT, C, NT, false, PAT>(h, s_tab, nullptr, n, m0, K, j, a, b, x, ok, s_pat);
     double2 c[C];
 #pragma unroll
     for (int q = 0; q < C; ++q)
@@ -98,6 +98,8 @@ __device__ __forceinline__ void block_sum_store(double re, double im, double2* o
 // RECUR: out = 2 H~ psi1 - psi0 (kernel_polynomial) else out = H~ psi1 (apply)
 // IP:    partial[tile] = sum over the tile of conj(L) out
 // out may alias psi0 (each site reads its own psi0 before writing out).
+// With a slot pattern (packed layouts) the pattern table is staged in LDS and
+// regular sites compute their slot words instead of loading them.
 template <int LAYOUT, int KT, int SPT, bool RECUR, bool IP, bool NT>
 __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __restrict__ psi1,
                                                  const double2* psi0, double2* out,
@@ -106,11 +108,79 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
                                                  int blocks)
 {
   __shared__ double2 s_tab[LAYOUT == kPackedLds ? kLdsTableMax : 1];
+  __shared__ int2 s_pat[LAYOUT == kFull ? 1 : kPatternMaxEntries];
   int tile = xcd_tile(blockIdx.x, blocks);
   if (tile >= tiles)
     return;
   stage_table<LAYOUT, false>(h, s_tab, nullptr);
+  const bool pat = LAYOUT != kFull && h.pat != nullptr;  // the same for the whole launch
+  if (pat) {
+    for (int i = threadIdx.x; i < h.pat_p * h.K; i += kBlock)
+      s_pat[i] = h.pat[i];
+    __syncthreads();
+  }
   double acc_re = 0.0, acc_im = 0.0;
+  if constexpr (KT > 0) {
+    // all loads of the thread's SPT sites first (out may alias psi0, so the
+    // compiler would not move a later site's loads above an earlier store),
+    // then the same per-site arithmetic as below
+    constexpr int C = KT;
+    double2 p1[SPT], p0[SPT], l[SPT], c[SPT][C];
+    double v[SPT], a[SPT][C], b[SPT][C], x[SPT][C];
+    int j[SPT][C];
+    bool ok[SPT][C];
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      const int nk = (tile * SPT + k) * kBlock + threadIdx.x;
+      const int n = nk < h.n ? nk : h.n - 1;  // an in-bounds site; its results are dropped
+      p1[k] = psi1[n];
+      v[k] = ld<NT>(h.pot + n);
+      if constexpr (RECUR)
+        p0[k] = ld2<NT>(psi0 + n);
+      if constexpr (IP)
+        l[k] = ld2<NT>(L + n);
+      if (pat)
+        load_slots<LAYOUT, C, NT, false, true>(h, s_tab, nullptr, n, 0, C, j[k], a[k], b[k], x[k],
+                                               ok[k], s_pat);
+      else
+        load_slots<LAYOUT, C, NT, false>(h, s_tab, nullptr, n, 0, C, j[k], a[k], b[k], x[k],
+                                         ok[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < SPT; ++k)
+#pragma unroll
+      for (int q = 0; q < C; ++q)
+        c[k][q] = psi1[j[k][q]];
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      const int n = (tile * SPT + k) * kBlock + threadIdx.x;
+      if (n < h.n) {
+        double tr = v[k] * p1[k].x;
+        double ti = v[k] * p1[k].y;
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+          if (ok[k][q]) {
+            tr += a[k][q] * c[k][q].x - b[k][q] * c[k][q].y;
+            ti += a[k][q] * c[k][q].y + b[k][q] * c[k][q].x;
+          }
+        }
+        tr /= h.emax;
+        ti /= h.emax;
+        if constexpr (RECUR) {
+          tr = 2.0 * tr - p0[k].x;
+          ti = 2.0 * ti - p0[k].y;
+        }
+        st2<false>(out + n, make_double2(tr, ti));
+        if constexpr (IP) {
+          acc_re += tr * l[k].x + ti * l[k].y;
+          acc_im += ti * l[k].x - tr * l[k].y;
+        }
+      }
+    }
+    if constexpr (IP)
+      block_sum_store(acc_re, acc_im, partial + tile);
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < SPT; ++k) {
     int n = (tile * SPT + k) * kBlock + threadIdx.x;
@@ -119,7 +189,10 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
       double v = ld<NT>(h.pot + n);
       double tr = v * p1.x;
       double ti = v * p1.y;
-      hsum<LAYOUT, KT, NT>(h, s_tab, psi1, n, tr, ti);
+      if (pat)
+        hsum<LAYOUT, KT, NT, true>(h, s_tab, psi1, n, tr, ti, s_pat);
+      else
+        hsum<LAYOUT, KT, NT>(h, s_tab, psi1, n, tr, ti);
       tr /= h.emax;
       ti /= h.emax;
       if constexpr (RECUR) {
@@ -379,20 +452,24 @@ __global__ __launch_bounds__(kBlock) void k_cheb1x(const double2* __restrict__ s
 }
 
 // partial[tile] = sum over the tile of conj(L) psi
+// partial[tile] = sum over tile of conj(L) psi, tiles of spt * kBlock sites
+// (the tiling of the fused step whose partials it joins)
 __global__ __launch_bounds__(kBlock) void k_dot(const double2* __restrict__ psi,
                                                 const double2* __restrict__ L,
-                                                double2* __restrict__ partial, int n, int tiles,
-                                                int blocks)
+                                                double2* __restrict__ partial, int n, int spt,
+                                                int tiles, int blocks)
 {
   int tile = xcd_tile(blockIdx.x, blocks);
   if (tile >= tiles)
     return;
   double re = 0.0, im = 0.0;
-  int i = tile * kBlock + threadIdx.x;
-  if (i < n) {
-    double2 p = psi[i], l = L[i];
-    re = p.x * l.x + p.y * l.y;
-    im = p.y * l.x - p.x * l.y;
+  for (int k = 0; k < spt; ++k) {
+    const int i = (tile * spt + k) * kBlock + threadIdx.x;
+    if (i < n) {
+      double2 p = psi[i], l = L[i];
+      re += p.x * l.x + p.y * l.y;
+      im += p.y * l.x - p.x * l.y;
+    }
   }
   block_sum_store(re, im, partial + tile);
 }
@@ -410,6 +487,28 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double2* __res
     im += v.y;
   }
   block_sum_store(re, im, mu + blockIdx.x);
+}
+
+// --- the irregular-site mask of a slot pattern -----------------------------
+// bit n of mask: site n's words differ from the pattern in some slot; one
+// wave64 writes the two mask words of its 64 sites.  Sites past h.n (the
+// padding that two-site kernels may touch) are marked irregular, so their
+// words (all empty) are loaded, never computed.
+__global__ __launch_bounds__(kBlock) void k_pattern_mask(HamDev h, uint32_t* __restrict__ mask)
+{
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  bool irr = n >= h.n;
+  if (n < h.n) {
+    const int r = pattern_residue(h, n);
+    for (int m = 0; m < h.K; ++m)
+      irr |= pattern_word(h, h.pat[m * h.pat_p + r], n) != h.words[(size_t)m * h.stride + n];
+  }
+  const unsigned long long bits = __ballot(irr);
+  if ((threadIdx.x & 63) == 0) {
+    const int w = (blockIdx.x * kBlock + threadIdx.x) >> 5;
+    mask[w] = (uint32_t)bits;
+    mask[w + 1] = (uint32_t)(bits >> 32);
+  }
 }
 
 // ==========================================================================
@@ -473,36 +572,46 @@ inline int blocks_1d(int n)
     __VA_ARGS__;                       \
   }
 
-// sites per thread of the fused step (a build-time tuning knob; 1 measured best)
-#ifndef LSQT_SPT_HPSI
-#define LSQT_SPT_HPSI 1
-#endif
-constexpr int kSptHpsi = LSQT_SPT_HPSI;
+// sites per thread of the fused step, by slot count.  Measured on the
+// 8.0M-site honeycomb (3 slots) in one process: 1 site 102.6 us per step, 2
+// sites 102.2 us; with the slot pattern 105.2 and 100.4 us; 4 sites slower.
+// On the 160^3 cube (6 slots) 2 sites were 9 % slower than 1.
+constexpr int hpsi_spt_for(int K)
+{
+  return K == 3 ? 2 : 1;
+}
+
+int hpsi_spt(int K)
+{
+  return hpsi_spt_for(K);
+}
 
 void launch_hpsi(const HamDev& h, const double2* psi1, const double2* psi0, double2* out,
                  const double2* L, double2* partial, bool recur, bool ip, hipStream_t st)
 {
-  Grid g = make_grid(h.n, kSptHpsi);
   const bool nt = h.n >= kStreamingMinSites;
   LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, LSQT_NT_SWITCH(nt, NT, {
+    constexpr int SPT = hpsi_spt_for(KT);
+    static_assert(KT == 0 || SPT == 1 || KT == 3, "two sites per thread for 3 slots only");
+    const Grid g = make_grid(h.n, SPT);
     if (recur && ip)
-      k_hpsi<LAY, KT, kSptHpsi, true, true, NT>
+      k_hpsi<LAY, KT, SPT, true, true, NT>
         <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
     else if (recur)
-      k_hpsi<LAY, KT, kSptHpsi, true, false, NT>
+      k_hpsi<LAY, KT, SPT, true, false, NT>
         <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
     else if (ip)
-      k_hpsi<LAY, KT, kSptHpsi, false, true, NT>
+      k_hpsi<LAY, KT, SPT, false, true, NT>
         <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
     else
-      k_hpsi<LAY, KT, kSptHpsi, false, false, NT>
+      k_hpsi<LAY, KT, SPT, false, false, NT>
         <<<g.blocks, kBlock, 0, st>>>(h, psi1, psi0, out, L, partial, g.tiles, g.blocks);
   })))
 }
 
-int hpsi_tiles(int n)
+int hpsi_tiles(int n, int K)
 {
-  return make_grid(n, kSptHpsi).tiles;
+  return make_grid(n, hpsi_spt_for(K)).tiles;
 }
 
 void launch_xop(const HamDev& h, const double2* in, double2* out, bool current, hipStream_t st)
@@ -540,6 +649,16 @@ void launch_cheb2x(const HamDev& h, const double2* s0, const double2* s0x, const
                                                          label, bm_prev, label_prev, g.tiles,
                                                          g.blocks);
   })))
+}
+
+int pattern_mask_words(int n)
+{
+  return blocks_1d(n) * (kBlock / 32);
+}
+
+void launch_pattern_mask(const HamDev& h, uint32_t* mask, hipStream_t st)
+{
+  k_pattern_mask<<<blocks_1d(h.n), kBlock, 0, st>>>(h, mask);
 }
 
 void launch_zero(double2* v, int n, hipStream_t st)
@@ -584,10 +703,11 @@ int dot_tiles(int n)
   return make_grid(n, 1).tiles;
 }
 
-void launch_dot(const double2* psi, const double2* L, double2* partial, int n, hipStream_t st)
+void launch_dot(const double2* psi, const double2* L, double2* partial, int n, hipStream_t st,
+                int spt)
 {
-  Grid g = make_grid(n, 1);
-  k_dot<<<g.blocks, kBlock, 0, st>>>(psi, L, partial, n, g.tiles, g.blocks);
+  Grid g = make_grid(n, spt);
+  k_dot<<<g.blocks, kBlock, 0, st>>>(psi, L, partial, n, spt, g.tiles, g.blocks);
 }
 
 void launch_reduce_partials(const double2* partial, size_t stride, int tiles, int count,

@@ -44,6 +44,8 @@ struct lsqt_ham {
   // most one slab away, and slabs in rows of `row` sites (slab % row == 0)
   int slab = 0;
   int row = 0;
+  // sites whose slot words differ from the slot pattern (d.pat set)
+  long long pat_irregular = 0;
   // segmented Hamiltonian: independent systems on sites [seg_starts[k],
   // seg_starts[k+1]); empty for one system
   std::vector<int> seg_starts;

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstddef>
 #include <cstdint>
 
@@ -15,6 +16,38 @@ constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;  // packed word of an unused ELL sl
 constexpr int kLdsTableMax = 512;    // packed tables up to this size are staged in LDS
 
 enum Layout { kFull = 0, kPackedLds = 1, kPackedGlobal = 2 };
+
+// Slot patterns of packed words (periodic lattices): the longest period and
+// the most entries (period x slots) a pattern may have; the entry of a slot
+// that the pattern leaves unused
+constexpr int kPatternMaxPeriod = 64;
+constexpr int kPatternMaxEntries = 512;
+constexpr int kPatternEmpty = INT_MIN;
+
+// Division of 0 <= v < 2^31 by an invariant divisor 2 <= d < 2^31 with one
+// high multiply and a shift (Granlund and Montgomery, "Division by invariant
+// integers using multiplication", PLDI 1994, section 4): with
+// l = ceil(log2 d) and m = ceil(2^(31+l) / d) < 2^32, v / d equals
+// umulhi(v, m) >> (l - 1), since m d - 2^(31+l) < d keeps the error of
+// v m / 2^(31+l) below 1/d.
+struct Divisor {
+  uint32_t m;
+  int shift;
+};
+
+inline Divisor make_divisor(int d)
+{
+  int l = 0;
+  while ((1ll << l) < d)
+    ++l;
+  const unsigned long long p = 1ull << (31 + l);
+  return Divisor{(uint32_t)((p + (unsigned long long)d - 1) / (unsigned long long)d), l - 1};
+}
+
+__device__ __forceinline__ int div_by(int v, const Divisor& q)
+{
+  return (int)(__umulhi((uint32_t)v, q.m) >> q.shift);
+}
 
 // Device view of a Hamiltonian.  ELL slots are column-major: slot m of site
 // n is at m*n_sites + n.  Unused slots are marked (kEmptySlot / index -1)
@@ -41,7 +74,33 @@ struct HamDev {
   int layout;
   // FP32 copy of the potential (the precision path; null until first used)
   const float* pot32;
+  // periodic slot pattern of the packed words (pat null: none).  Site n
+  // whose bit in `irregular` (32 sites per word) is clear holds in slot m
+  // exactly pattern_word(pat[m * pat_p + n mod pat_p], n); the moment
+  // kernels compute those words instead of loading them
+  const int2* pat;
+  const uint32_t* irregular;
+  int pat_p;
+  Divisor pat_div;  // for pat_p >= 2
 };
+
+// the packed word of a slot that follows pattern entry e at site n
+__device__ __forceinline__ uint32_t pattern_word(const HamDev& h, int2 e, int n)
+{
+  return e.x == kPatternEmpty ? kEmptySlot
+                              : (((uint32_t)e.y << h.idx_bits) | (uint32_t)(n + e.x));
+}
+
+__device__ __forceinline__ int pattern_residue(const HamDev& h, int n)
+{
+  return h.pat_p == 1 ? 0 : n - div_by(n, h.pat_div) * h.pat_p;
+}
+
+// whether site n's words differ from the pattern (they are then loaded)
+__device__ __forceinline__ bool pattern_irregular(const HamDev& h, int n)
+{
+  return (h.irregular[n >> 5] >> (n & 31)) & 1u;
+}
 
 constexpr int kPairStrideAlign = 64;
 
@@ -102,11 +161,16 @@ __device__ __forceinline__ void st2(double2* p, double2 v)
 // unused slots get j = n (an in-bounds address) and ok = false; callers skip
 // their terms.  The pair arrays are padded to `stride`, so every slot load
 // below `count` is in bounds.
-template <int LAYOUT, int C, bool NT, bool WITH_XX>
+//
+// PAT (packed layouts with a slot pattern, s_pat = the pattern table): the
+// words of a regular site are computed from the pattern and only those of
+// irregular sites are loaded; both give the same words.
+template <int LAYOUT, int C, bool NT, bool WITH_XX, bool PAT = false>
 __device__ __forceinline__ void load_slots(const HamDev& h, const double2* s_tab,
                                            const double* s_xx, int n, int m0, int count,
                                            int (&j)[C], double (&a)[C], double (&b)[C],
-                                           double (&x)[C], bool (&ok)[C])
+                                           double (&x)[C], bool (&ok)[C],
+                                           const int2* s_pat = nullptr)
 {
   if constexpr (LAYOUT == 0) {  // full layout: index, re, im (and xx) per slot
     int jj[C];
@@ -128,9 +192,22 @@ __device__ __forceinline__ void load_slots(const HamDev& h, const double2* s_tab
     }
   } else {  // packed: one word per slot, (re, im, xx) from the class table
     uint32_t w[C];
+    if constexpr (PAT) {
+      const int r = pattern_residue(h, n);
 #pragma unroll
-    for (int q = 0; q < C; ++q)
-      w[q] = (m0 + q < count) ? ld<NT>(h.words + (size_t)(m0 + q) * h.stride + n) : kEmptySlot;
+      for (int q = 0; q < C; ++q)
+        w[q] = (m0 + q < count) ? pattern_word(h, s_pat[(m0 + q) * h.pat_p + r], n) : kEmptySlot;
+      if (pattern_irregular(h, n)) {
+#pragma unroll
+        for (int q = 0; q < C; ++q)
+          if (m0 + q < count)
+            w[q] = ld<NT>(h.words + (size_t)(m0 + q) * h.stride + n);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < C; ++q)
+        w[q] = (m0 + q < count) ? ld<NT>(h.words + (size_t)(m0 + q) * h.stride + n) : kEmptySlot;
+    }
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       ok[q] = w[q] != kEmptySlot;
@@ -167,31 +244,6 @@ struct Grid {
 // table (segmented Hamiltonians: independent systems side by side) tile t is
 // sites [table[t].x, table[t].y) of segment tile_seg[t], and the tiles of
 // segment k are seg_tiles[k] .. seg_tiles[k+1]-1.
-// Division of 0 <= v < 2^31 by an invariant divisor 2 <= d < 2^31 with one
-// high multiply and a shift (Granlund and Montgomery, "Division by invariant
-// integers using multiplication", PLDI 1994, section 4): with
-// l = ceil(log2 d) and m = ceil(2^(31+l) / d) < 2^32, v / d equals
-// umulhi(v, m) >> (l - 1), since m d - 2^(31+l) < d keeps the error of
-// v m / 2^(31+l) below 1/d.
-struct Divisor {
-  uint32_t m;
-  int shift;
-};
-
-inline Divisor make_divisor(int d)
-{
-  int l = 0;
-  while ((1ll << l) < d)
-    ++l;
-  const unsigned long long p = 1ull << (31 + l);
-  return Divisor{(uint32_t)((p + (unsigned long long)d - 1) / (unsigned long long)d), l - 1};
-}
-
-__device__ __forceinline__ int div_by(int v, const Divisor& q)
-{
-  return (int)(__umulhi((uint32_t)v, q.m) >> q.shift);
-}
-
 // Sliding-window geometry of a lattice (sites x * slab + y * row + z): a tile
 // is `by` consecutive rows y of one slab x, restricted to one chunk of
 // `chunk` consecutive z, swept row by row with rows y-1, y, y+1 of psi_{m-1}

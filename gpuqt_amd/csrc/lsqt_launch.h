@@ -6,7 +6,10 @@
 namespace lsqt {
 
 // FP64 drop-in path (kernels_f64.hip)
-int hpsi_tiles(int n);
+// tiles (moment partials per step) of the fused step for n sites and K
+// slots, and its sites per thread
+int hpsi_tiles(int n, int K);
+int hpsi_spt(int K);
 void launch_hpsi(const HamDev& h, const double2* psi1, const double2* psi0, double2* out,
                  const double2* L, double2* partial, bool recur, bool ip, hipStream_t st);
 void launch_xop(const HamDev& h, const double2* in, double2* out, bool current, hipStream_t st);
@@ -20,6 +23,10 @@ void launch_cheb2(const HamDev& h, const double2* s0, const double2* s1, double2
 void launch_cheb2x(const HamDev& h, const double2* s0, const double2* s0x, const double2* s1,
                    const double2* s1x, double2* s2, double2* s2x, double2* s, double bm,
                    int label, hipStream_t st, double bm_prev = 0.0, int label_prev = 0);
+// the irregular-site mask of h's slot pattern (h.pat, h.pat_p set): the
+// mask holds pattern_mask_words(h.n) words
+int pattern_mask_words(int n);
+void launch_pattern_mask(const HamDev& h, uint32_t* mask, hipStream_t st);
 void launch_zero(double2* v, int n, hipStream_t st);
 void launch_copy(double2* dst, const double2* src, int n, hipStream_t st);
 void launch_add(double2* dst, const double2* src, int n, hipStream_t st);
@@ -27,8 +34,12 @@ void launch_sz(double2* dst, const double2* src, int n, hipStream_t st);
 void launch_cheb01(const double2* s0, const double2* s1, double2* s, double b0, double b1,
                    int direction, int n, hipStream_t st);
 void launch_cheb1x(const double2* s1x, double2* s, double b1, int n, hipStream_t st);
+// partials of conj(L) psi over tiles of spt * 256 sites: spt 1 gives
+// dot_tiles(n) tiles, hpsi_spt(K) the tiles of the fused step (the moment
+// mu_0 of a moment set)
 int dot_tiles(int n);
-void launch_dot(const double2* psi, const double2* L, double2* partial, int n, hipStream_t st);
+void launch_dot(const double2* psi, const double2* L, double2* partial, int n, hipStream_t st,
+                int spt = 1);
 void launch_reduce_partials(const double2* partial, size_t stride, int tiles, int count,
                             double2* mu, hipStream_t st);
 

@@ -26,7 +26,8 @@ _vp = C.c_void_p
 class HamInfo(C.Structure):
     _fields_ = [("n", C.c_int), ("max_neighbor", C.c_int), ("layout", C.c_int),
                 ("classes", C.c_int), ("index_bits", C.c_int), ("energy_max", C.c_double),
-                ("bytes_per_site_f64", C.c_double), ("slab", C.c_int), ("row", C.c_int)]
+                ("bytes_per_site_f64", C.c_double), ("slab", C.c_int), ("row", C.c_int),
+                ("pattern_period", C.c_int), ("pattern_irregular", C.c_double)]
 
 
 class Lattice(C.Structure):

@@ -72,6 +72,12 @@ typedef struct {
                                 product included) */
   int slab;           /* detected lattice slab in sites (0: none), see the */
   int row;            /* block tile schedules below; row: the next level    */
+  int pattern_period; /* slot-pattern period (0: none): packed words that
+                         depend only on the site index modulo the period
+                         (up to a shift of the neighbour index) are computed
+                         by the moment kernels instead of loaded, except on
+                         the irregular sites; LSQT_PATTERN=0 turns it off */
+  double pattern_irregular; /* fraction of the sites with irregular words */
 } lsqt_ham_info;
 
 const char* lsqt_strerror(int status);
