@@ -139,10 +139,24 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
         p0[k] = ld2<NT>(psi0 + n);
       if constexpr (IP)
         l[k] = ld2<NT>(L + n);
-      if (pat)
-        load_slots<LAYOUT, C, NT, false, true>(h, s_tab, nullptr, n, 0, C, j[k], a[k], b[k], x[k],
-                                               ok[k], s_pat);
-      else
+      if (pat) {
+        // the pattern's words for every site, so that the gathers issue
+        // without waiting for the mask (indices clamped into range); the
+        // irregular sites load their words and gather again below (measured:
+        // 93.5 us per step against 99.9 us with the mask read first)
+        const int r = pattern_residue(h, n);
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+          const uint32_t w = pattern_word(h, s_pat[q * h.pat_p + r], n);
+          ok[k][q] = w != kEmptySlot;
+          const int jj = ok[k][q] ? (int)(w & h.idx_mask) : n;
+          j[k][q] = (unsigned)jj < (unsigned)h.n ? jj : n;
+          const int cc = ok[k][q] ? (int)(w >> h.idx_bits) : 0;
+          const double2 t = (LAYOUT == 1) ? s_tab[cc] : h.tab[cc];
+          a[k][q] = t.x;
+          b[k][q] = t.y;
+        }
+      } else
         load_slots<LAYOUT, C, NT, false>(h, s_tab, nullptr, n, 0, C, j[k], a[k], b[k], x[k],
                                          ok[k]);
     }
@@ -151,6 +165,20 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
 #pragma unroll
       for (int q = 0; q < C; ++q)
         c[k][q] = psi1[j[k][q]];
+    if (pat) {
+#pragma unroll
+      for (int k = 0; k < SPT; ++k) {
+        const int nk = (tile * SPT + k) * kBlock + threadIdx.x;
+        const int n = nk < h.n ? nk : h.n - 1;
+        if (pattern_irregular(h, n)) {
+          load_slots<LAYOUT, C, NT, false>(h, s_tab, nullptr, n, 0, C, j[k], a[k], b[k], x[k],
+                                           ok[k]);
+#pragma unroll
+          for (int q = 0; q < C; ++q)
+            c[k][q] = psi1[j[k][q]];
+        }
+      }
+    }
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
       const int n = (tile * SPT + k) * kBlock + threadIdx.x;
