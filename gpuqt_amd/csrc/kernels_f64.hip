@@ -602,11 +602,16 @@ inline int blocks_1d(int n)
 
 // sites per thread of the fused step, by slot count.  Measured on the
 // 8.0M-site honeycomb (3 slots) in one process: 1 site 102.6 us per step, 2
-// sites 102.2 us; with the slot pattern 105.2 and 100.4 us; 4 sites slower.
-// On the 160^3 cube (6 slots) 2 sites were 9 % slower than 1.
+// sites 102.2 us; with the slot pattern 105.2 and 100.4 us, and with the
+// pattern's gathers issued before the mask read 2 sites 93.8 us, 3 sites
+// 100.0 us, 4 sites 108.2 us.  On the 160^3 cube (6 slots) 2 sites were 9 %
+// slower than 1.  LSQT_SPT_HPSI3 overrides the 3-slot count at build time.
+#ifndef LSQT_SPT_HPSI3
+#define LSQT_SPT_HPSI3 2
+#endif
 constexpr int hpsi_spt_for(int K)
 {
-  return K == 3 ? 2 : 1;
+  return K == 3 ? LSQT_SPT_HPSI3 : 1;
 }
 
 int hpsi_spt(int K)
@@ -620,7 +625,6 @@ void launch_hpsi(const HamDev& h, const double2* psi1, const double2* psi0, doub
   const bool nt = h.n >= kStreamingMinSites;
   LSQT_LAYOUT_SWITCH(h.layout, LAY, LSQT_K_SWITCH(h.K, KT, LSQT_NT_SWITCH(nt, NT, {
     constexpr int SPT = hpsi_spt_for(KT);
-    static_assert(KT == 0 || SPT == 1 || KT == 3, "two sites per thread for 3 slots only");
     const Grid g = make_grid(h.n, SPT);
     if (recur && ip)
       k_hpsi<LAY, KT, SPT, true, true, NT>
