@@ -102,6 +102,31 @@ def test_pattern_interior_irregular_sites(gpu_ctx, tmp_path, monkeypatch):
     H.close()
 
 
+def test_pattern_odd_site_count(gpu_ctx, tmp_path, monkeypatch):
+    """A 67 x 63 square lattice (4221 sites, one orbital: period 1): the
+    two-site FP32 step's last pair ends in a padding site, which the mask
+    marks irregular so that its (empty) words are loaded."""
+    import oracle
+    from gpuqt_amd import inputs as gi
+    print("[gpu] slot pattern, square 67x63 (odd site count)", flush=True)
+    d = str(tmp_path)
+    gi.write_square(d, 67, 63)
+    gi.write_para(d, anderson=1.0, moments=64, energy_max=6.0)
+    gi.write_energy(d, [0.0])
+    gi.write_time_steps(d, [1.0])
+    m = oracle.OracleModel(d)
+    assert m.n == 67 * 63 and m.n % 2 == 1
+    r = m.random_state()
+    H, info = _compare(gpu_ctx, monkeypatch, m.arrays(), m.n, m.max_neighbor, m.energy_max,
+                       [r, m.random_state()], True)
+    assert info.pattern_period == 1
+    v = H.vector(*r)
+    ref = m.moments(r, r, 64)
+    assert_moments(H.moments(v, v, 64), ref[0] + 1j * ref[1], "moments")
+    for o in (v, H):
+        o.close()
+
+
 def test_pattern_vacancies(gpu_ctx, tmp_path, monkeypatch):
     """Vacancy disorder removes sites and renumbers the rest, which shifts
     the orbital residues; whatever the detection decides, results equal the
