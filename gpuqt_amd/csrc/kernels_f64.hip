@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
           const int jj = ok[k][q] ? (int)(w & h.idx_mask) : n;
           j[k][q] = (unsigned)jj < (unsigned)h.n ? jj : n;
           const int cc = ok[k][q] ? (int)(w >> h.idx_bits) : 0;
-          const double2 t = (LAYOUT == 1) ? s_tab[cc] : h.tab[cc];
+          const double2 t = (LAYOUT == kPackedLds) ? s_tab[cc] : h.tab[cc];
           a[k][q] = t.x;
           b[k][q] = t.y;
         }
