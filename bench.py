@@ -34,6 +34,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from gpuqt_amd import workloads as wl  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                "hbm_traffic_probe.md")
@@ -56,15 +58,6 @@ def profiled_traffic(kernel):
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def write_config2(d, nx=2000, ny=1000, moments=3000, time_steps=20, dt=2.0, energies=201,
-                  msd=True):
-    from gpuqt_amd import inputs as gi
-    gi.write_honeycomb(d, nx, ny)
-    gi.write_para(d, anderson=2.0, msd=msd, moments=moments, energy_max=10.0)
-    gi.write_energy(d, gi.energy_grid(-8, 8, energies))
-    gi.write_time_steps(d, [dt] * time_steps)
 
 
 class HipEvents:
@@ -341,14 +334,6 @@ def fp64_vac_msd(H, re, im, M, ts):
     return np.array(vac), np.array(msd)
 
 
-def batch_cells(count, lo=200, hi=1500, seed=5):
-    """Cell counts of the uneven batch: (Nc1, Nc2) uniform in [lo, hi]^2 from
-    a fixed seed; a four-orbital cell holds two two-site cells, so Nx = Nc1,
-    Ny = Nc2 / 2 (Nc2 rounded up to even)."""
-    c = np.random.default_rng(seed).integers(lo, hi + 1, size=(count, 2))
-    return [(int(a), int((b + 1) // 2)) for a, b in c]
-
-
 def bench_batch(ctx, realisations=8, R=8, M=1500, n_steps=16, dt=2.0, emax=12.0,
                 threads=16, check=True):
     """Uneven batch: honeycomb realisations of uneven size (a_cc = 1.42 A,
@@ -359,19 +344,14 @@ def bench_batch(ctx, realisations=8, R=8, M=1500, n_steps=16, dt=2.0, emax=12.0,
     import concurrent.futures as cf
     from gpuqt_amd import hostlib, native
     from gpuqt_amd import inputs as gi
-    cells = batch_cells(64)[:realisations]
+    cells = wl.batch_cells(64)[:realisations]
     t0 = time.perf_counter()
     with tempfile.TemporaryDirectory() as tmp:
         dirs = []
         for i, (nx, ny) in enumerate(cells):
             d = os.path.join(tmp, f"r{i}")
-            os.makedirs(d)
-            n = 4 * nx * ny
-            gi.write_honeycomb(d, nx, ny, ax=2.4595, ay=4.26)
-            gi.write_para(d, charged=(int(round(0.001 * n)), 5.6, 4.26), moments=M,
-                          energy_max=emax, vac=True, msd=True, random_vectors=R)
-            gi.write_energy(d, gi.energy_grid(-8, 8, 201))
-            gi.write_time_steps(d, [dt] * n_steps)
+            wl.write_batch_realisation(d, nx, ny, R=R, moments=M, n_steps=n_steps, dt=dt,
+                                       emax=emax)
             dirs.append(d)
 
         def build(d):
@@ -529,7 +509,7 @@ def cpu_baseline(nx, ny, moments_sample, threads):
     with tempfile.TemporaryDirectory() as tmp:
         d = os.path.join(tmp, "cpu")
         os.makedirs(d)
-        write_config2(d, nx=nx, ny=ny, moments=moments_sample, msd=False)
+        wl.write_config2(d, nx=nx, ny=ny, moments=moments_sample, msd=False)
         n_sites = 4 * nx * ny
         updates = n_sites * (moments_sample - 1)
         ref = oracle.reference_binary()
@@ -573,7 +553,7 @@ def parallel_reference(ref, tmp, nx, ny, moments_sample, procs, updates):
         for p in range(procs):
             d = os.path.join(tmp, f"par{moments}_{p}")
             os.makedirs(d)
-            write_config2(d, nx=nx, ny=ny, moments=moments, msd=False)
+            wl.write_config2(d, nx=nx, ny=ny, moments=moments, msd=False)
             drv = os.path.join(tmp, f"par{moments}_{p}.txt")
             with open(drv, "w") as f:
                 f.write(d + "\n")
@@ -632,7 +612,7 @@ def main():
     with tempfile.TemporaryDirectory() as tmp:
         d = os.path.join(tmp, "config2")
         os.makedirs(d)
-        write_config2(d, nx=args.nx, ny=args.ny, moments=args.moments,
+        wl.write_config2(d, nx=args.nx, ny=args.ny, moments=args.moments,
                       time_steps=args.time_steps)
         t0 = time.perf_counter()
         # the drop-in's Model (host C++): lattice, Anderson draws, random phases

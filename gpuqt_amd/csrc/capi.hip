@@ -918,6 +918,62 @@ extern "C" int lsqt_vec_dot(const lsqt_vec* left, const lsqt_vec* psi, double* r
   return ctx_download_moments(ctx, 1, re, im);
 }
 
+// sites per partial of Vector::inner_product_1: the reference's BLOCK_SIZE
+// (src/common.h), one k_dot tile of two sites per thread
+constexpr int kInnerProductBlock = 512;
+
+extern "C" int lsqt_vec_inner_product_1(const lsqt_vec* v, int number_of_atoms,
+                                        const lsqt_vec* other, lsqt_vec* target, int offset)
+{
+  LSQT_TRY(same_shape(v, other));
+  if (!target || target->ctx != v->ctx)
+    return set_error(LSQT_ERR_INVALID, "inner_product_1: null target or another context");
+  if (number_of_atoms <= 0 || number_of_atoms > v->n)
+    return set_error(LSQT_ERR_INVALID, "inner_product_1: %d atoms for vectors of length %d",
+                     number_of_atoms, v->n);
+  const long long blocks = (number_of_atoms - 1) / kInnerProductBlock + 1;
+  if (offset < 0 || offset + blocks > target->n)
+    return set_error(LSQT_ERR_INVALID, "inner_product_1: %lld partials at offset %d do not fit "
+                     "a target of length %d", blocks, offset, target->n);
+  if (target->d == v->d || target->d == other->d)
+    return set_error(LSQT_ERR_INVALID, "inner_product_1: the target must be another vector");
+  LSQT_TRY(ctx_activate(v->ctx));
+  launch_dot(v->d, other->d, target->d + offset, number_of_atoms, v->ctx->stream,
+             kInnerProductBlock / 256);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_inner_product_2(const lsqt_vec* v, int number_of_atoms,
+                                        int number_of_moments, lsqt_vec* target)
+{
+  if (!v || !target || target->ctx != v->ctx)
+    return set_error(LSQT_ERR_INVALID, "inner_product_2: null vector or another context");
+  if (number_of_atoms <= 0 || number_of_moments <= 0)
+    return set_error(LSQT_ERR_INVALID, "inner_product_2: %d atoms, %d moments",
+                     number_of_atoms, number_of_moments);
+  const long long blocks = (number_of_atoms - 1) / kInnerProductBlock + 1;
+  if (blocks * number_of_moments > v->n || number_of_moments > target->n)
+    return set_error(LSQT_ERR_INVALID, "inner_product_2: %lld x %d partials in a vector of "
+                     "length %d, %d moments into a target of length %d", blocks,
+                     number_of_moments, v->n, number_of_moments, target->n);
+  if (target->d == v->d)
+    return set_error(LSQT_ERR_INVALID, "inner_product_2: the target must be another vector");
+  LSQT_TRY(ctx_activate(v->ctx));
+  launch_reduce_partials(v->d, (size_t)blocks, (int)blocks, number_of_moments, target->d,
+                         v->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
+extern "C" int lsqt_vec_device_data(const lsqt_vec* v, void** data)
+{
+  if (!v || !data)
+    return set_error(LSQT_ERR_INVALID, "null vector or output");
+  *data = (void*)v->d;
+  return LSQT_OK;
+}
+
 // ---------------------------------------------------------------------------
 // operators
 // ---------------------------------------------------------------------------
@@ -1060,6 +1116,21 @@ extern "C" int lsqt_moments(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* r
     return set_error(LSQT_ERR_INVALID, "null moment output");
   LSQT_TRY(moments_enqueue(h, left->d, right->d, M));
   return ctx_download_moments(h->ctx, M, mu_re, mu_im);
+}
+
+extern "C" int lsqt_moments_to_vec(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right,
+                                   int M, lsqt_vec* out)
+{
+  LSQT_TRY(check_ham_vecs(h, {left, right}));
+  if (M < 2)
+    return set_error(LSQT_ERR_INVALID, "number of moments %d must be >= 2", M);
+  if (!out || out->ctx != h->ctx || out->n < M)
+    return set_error(LSQT_ERR_INVALID, "moment output: a vector of at least %d entries on the "
+                     "Hamiltonian's context", M);
+  LSQT_TRY(moments_enqueue(h, left->d, right->d, M));
+  launch_copy(out->d, h->ctx->mu_dev, M, h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
 }
 
 int lsqt::evolve_enqueue(lsqt_ham* h, int direction, double t, double2* state)

@@ -143,7 +143,10 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
         // the pattern's words for every site, so that the gathers issue
         // without waiting for the mask (indices clamped into range); the
         // irregular sites load their words and gather again below (measured:
-        // 93.5 us per step against 99.9 us with the mask read first)
+        // 93.5 us per step against 99.9 us with the mask read first).  At an
+        // irregular site n + offset may leave [0, 2^idx_bits) and carry into
+        // the class bits, so the class index is clamped into the table too
+        // (the value read there is replaced before it is used)
         const int r = pattern_residue(h, n);
 #pragma unroll
         for (int q = 0; q < C; ++q) {
@@ -151,7 +154,8 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HamDev h, const double2* __rest
           ok[k][q] = w != kEmptySlot;
           const int jj = ok[k][q] ? (int)(w & h.idx_mask) : n;
           j[k][q] = (unsigned)jj < (unsigned)h.n ? jj : n;
-          const int cc = ok[k][q] ? (int)(w >> h.idx_bits) : 0;
+          const uint32_t cw = ok[k][q] ? (w >> h.idx_bits) : 0u;
+          const int cc = cw < (uint32_t)h.ncls ? (int)cw : 0;
           const double2 t = (LAYOUT == kPackedLds) ? s_tab[cc] : h.tab[cc];
           a[k][q] = t.x;
           b[k][q] = t.y;

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "general_io.h"
+#include "hamiltonian.h"
 #include "lsqt.h"
 #include "lsqt_host.h"
 #include "model.h"
@@ -17,6 +18,7 @@
 
 struct lsqt_host_model {
   Model* model;
+  Hamiltonian* ham = nullptr;  // built on first use (releases the model's arrays)
 };
 
 namespace {
@@ -248,6 +250,7 @@ extern "C" int lsqt_host_model_load(const char* input_dir, lsqt_host_model** out
 extern "C" void lsqt_host_model_free(lsqt_host_model* m)
 {
   if (m) {
+    delete m->ham;
     delete m->model;
     delete m;
   }
@@ -315,5 +318,66 @@ extern "C" int lsqt_host_model_random_state(lsqt_host_model* h, int orbital, dou
   if (!h || !re || !im || orbital >= h->model->number_of_atoms)
     return 1;
   h->model->initialize_state_host(re, im, orbital);
+  return 0;
+}
+
+namespace {
+Hamiltonian& model_hamiltonian(lsqt_host_model* h)
+{
+  if (!h->ham)
+    h->ham = new Hamiltonian(*h->model);
+  return *h->ham;
+}
+}  // namespace
+
+extern "C" int lsqt_host_model_evolve(lsqt_host_model* h, int kind, int direction,
+                                      double time_step_scaled, int by_terms, double* re,
+                                      double* im)
+{
+  if (!h || !re || !im || (kind != 0 && kind != 1) || (direction != 1 && direction != -1))
+    return 1;
+  Model& m = *h->model;
+  Hamiltonian& H = model_hamiltonian(h);
+  Vector state(m.number_of_atoms);
+  state.copy_from_host(re, im);
+  if (kind == 0) {
+    if (by_terms)
+      lsqt_host::evolve_by_terms(m, direction, time_step_scaled, H, state);
+    else
+      evolve(m, direction, time_step_scaled, H, state);
+  } else {
+    if (by_terms)
+      lsqt_host::evolvex_by_terms(m, direction, time_step_scaled, H, state);
+    else
+      evolvex(m, direction, time_step_scaled, H, state);
+  }
+  state.copy_to_host(re, im);
+  return 0;
+}
+
+extern "C" int lsqt_host_model_moments(lsqt_host_model* h, const double* l_re,
+                                       const double* l_im, const double* r_re,
+                                       const double* r_im, int by_terms, double* mu_re,
+                                       double* mu_im)
+{
+  if (!h || !l_re || !l_im || !r_re || !r_im || !mu_re || !mu_im)
+    return 1;
+  Model& m = *h->model;
+  if (m.number_of_moments < 2)
+    return 1;
+  Hamiltonian& H = model_hamiltonian(h);
+  const int n = m.number_of_atoms, M = m.number_of_moments;
+  Vector left(n), right(n), output(M);
+  // copy_from_host takes non-const arrays, as the reference's does
+  std::vector<real> buf(l_re, l_re + n), buf_im(l_im, l_im + n);
+  left.copy_from_host(buf.data(), buf_im.data());
+  buf.assign(r_re, r_re + n);
+  buf_im.assign(r_im, r_im + n);
+  right.copy_from_host(buf.data(), buf_im.data());
+  if (by_terms)
+    lsqt_host::find_moments_by_terms(m, H, left, right, output);
+  else
+    find_moments_chebyshev(m, H, left, right, output);
+  output.copy_to_host(mu_re, mu_im);
   return 0;
 }

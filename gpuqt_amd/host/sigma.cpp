@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <fstream>
+#include <math.h>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -67,7 +68,107 @@ void find_moments_chebyshev(Model& model, Hamiltonian& H, Vector& left, Vector& 
                                 model.number_of_moments, mu_re, mu_im));
 }
 
+void find_moments_chebyshev(Model& model, Hamiltonian& H, Vector& left, Vector& right,
+                            Vector& output)
+{
+  if (lsqt_host::path() == lsqt_host::Path::f64) {
+    LSQT_CHECK(lsqt_moments_to_vec(H.handle(), left.handle(), right.handle(),
+                                   model.number_of_moments, output.handle()));
+    return;
+  }
+  const int M = model.number_of_moments;
+  std::vector<real> re((size_t)output.size(), 0.0), im((size_t)output.size(), 0.0);
+  LSQT_CHECK(lsqt_moments_f32(H.handle(), left.handle(), right.handle(), M, re.data(),
+                              im.data()));
+  output.copy_from_host(re.data(), im.data());
+}
+
 namespace lsqt_host {
+
+namespace {
+// (-i)^m-type phase case of Bessel term m (m >= 2) of U(direction t) and of
+// [X, U(t)] / [U(-t), X] (src/sigma.cu:136-146, 180-190)
+int evolve_label(int m, int direction)
+{
+  switch (m % 4) {
+    case 0: return 1;
+    case 2: return 2;
+    case 1: return direction == 1 ? 3 : 4;
+    default: return direction == 1 ? 4 : 3;
+  }
+}
+
+int evolvex_label(int m, int direction)
+{
+  switch (m % 4) {
+    case 1: return 3;
+    case 3: return 4;
+    case 0: return direction == 1 ? 1 : 2;
+    default: return direction == 1 ? 2 : 1;
+  }
+}
+
+// 2 J_m(t) while |J_m(t)| >= 1e-15 (src/sigma.cu:126-130, 171-175)
+bool bessel_term(int m, real t, real& coefficient)
+{
+  const real b = jn(m, t);
+  if (b < 1.0e-15 && b > -1.0e-15)
+    return false;
+  coefficient = 2.0 * b;
+  return true;
+}
+}  // namespace
+
+void find_moments_by_terms(Model& model, Hamiltonian& H, Vector& left, Vector& right,
+                           Vector& output)
+{
+  const int n = model.number_of_atoms, M = model.number_of_moments;
+  const int blocks = (n - 1) / 512 + 1;
+  Vector partials(blocks * M);
+  Vector psi_prev(right), psi(n), psi_next(n);
+  psi_prev.inner_product_1(n, left, partials, 0);
+  H.apply(psi_prev, psi);
+  psi.inner_product_1(n, left, partials, blocks);
+  for (int m = 2; m < M; ++m) {
+    H.kernel_polynomial(psi_prev, psi, psi_next);
+    psi_next.inner_product_1(n, left, partials, m * blocks);
+    psi_prev.swap(psi);
+    psi.swap(psi_next);
+  }
+  partials.inner_product_2(n, M, output);
+}
+
+void evolve_by_terms(Model& model, int direction, real t, Hamiltonian& H, Vector& state)
+{
+  const int n = model.number_of_atoms;
+  Vector t_prev(state), t_cur(n), t_next(n);
+  H.apply(state, t_cur);
+  H.chebyshev_01(t_prev, t_cur, state, j0(t), 2.0 * j1(t), direction);
+  real bm;
+  for (int m = 2; bessel_term(m, t, bm); ++m) {
+    H.chebyshev_2(t_prev, t_cur, t_next, state, bm, evolve_label(m, direction));
+    t_prev.swap(t_cur);
+    t_cur.swap(t_next);
+  }
+}
+
+void evolvex_by_terms(Model& model, int direction, real t, Hamiltonian& H, Vector& state)
+{
+  const int n = model.number_of_atoms;
+  Vector t_prev(state), x_prev(n), t_cur(n), x_cur(n), t_next(n), x_next(n);
+  H.apply(state, t_cur);
+  H.apply_commutator(state, x_cur);
+  H.chebyshev_1x(x_cur, state, 2.0 * j1(t));
+  real bm;
+  for (int m = 2; bessel_term(m, t, bm); ++m) {
+    H.chebyshev_2x(t_prev, x_prev, t_cur, x_cur, t_next, x_next, state, bm,
+                   evolvex_label(m, direction));
+    t_prev.swap(t_cur);
+    t_cur.swap(t_next);
+    x_prev.swap(x_cur);
+    x_cur.swap(x_next);
+  }
+}
 
 // Jackson damping, alpha = 1/(M+1) (doc/gpuqt.tex:266-267), then
 // rho(E) = 2/(pi Omega) (mu_0 + 2 sum_k mu_k T_k(E~)) / sqrt(1 - E~^2) / E_max

@@ -22,12 +22,26 @@ void find_spin_polarization(Model&, Hamiltonian&, Vector&);
 // (src/sigma.cu:33-61; the whole recursion runs on the GPU)
 void find_moments_chebyshev(Model&, Hamiltonian&, Vector& left, Vector& right, real* mu_re,
                             real* mu_im);
+// the reference's signature: output[m] = mu_m, left on the device
+void find_moments_chebyshev(Model&, Hamiltonian&, Vector& left, Vector& right, Vector& output);
 void apply_damping(Model&, real* mu_re, real* mu_im);                    // src/sigma.cu:76-86
 void perform_chebyshev_summation(Model&, real*, real*, real* out);     // src/sigma.cu:89-110
 void evolve(Model&, int direction, real time_step_scaled, Hamiltonian&, Vector&);   // :116-151
 void evolvex(Model&, int direction, real time_step_scaled, Hamiltonian&, Vector&);  // :157-200
 
 namespace lsqt_host {
+// The same algorithms sequenced method by method on Hamiltonian and Vector,
+// as the reference's own functions drive them: the recursion one
+// kernel_polynomial and one Vector::inner_product_1 per moment and
+// inner_product_2 at the end (src/sigma.cu:33-61); the evolutions one
+// chebyshev_2 / chebyshev_2x per Bessel term with the states rotated by
+// Vector::swap (src/sigma.cu:116-200).  Their states are bit-identical to the
+// fused entry points' above (the moments differ by reduction order only);
+// they serve callers that drive the methods themselves.
+void find_moments_by_terms(Model&, Hamiltonian&, Vector& left, Vector& right, Vector& output);
+void evolve_by_terms(Model&, int direction, real time_step_scaled, Hamiltonian&, Vector& state);
+void evolvex_by_terms(Model&, int direction, real time_step_scaled, Hamiltonian&, Vector& state);
+
 // apply_damping + perform_chebyshev_summation on plain arrays: mu_re is
 // damped in place (mu_im too, when non-null), out gets n_energy values
 void damped_chebyshev_sum(int M, real* mu_re, real* mu_im, int n_energy, const real* energy,

@@ -90,3 +90,20 @@ void Vector::swap(Vector& other)
 {
   LSQT_CHECK(lsqt_vec_swap(v, other.v));
 }
+
+void Vector::inner_product_1(int number_of_atoms, Vector& other, Vector& target, int offset)
+{
+  LSQT_CHECK(lsqt_vec_inner_product_1(v, number_of_atoms, other.v, target.v, offset));
+}
+
+void Vector::inner_product_2(int number_of_atoms, int number_of_moments, Vector& target)
+{
+  LSQT_CHECK(lsqt_vec_inner_product_2(v, number_of_atoms, number_of_moments, target.v));
+}
+
+real* Vector::device_data() const
+{
+  void* p = nullptr;
+  LSQT_CHECK(lsqt_vec_device_data(v, &p));
+  return static_cast<real*>(p);
+}

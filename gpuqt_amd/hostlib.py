@@ -29,6 +29,8 @@ SIGNATURES = {
     "lsqt_host_model_arrays": (C.c_int, [_vp, _ip, _ip, _dp, _dp, _dp, _dp]),
     "lsqt_host_model_inputs": (C.c_int, [_vp, _dp, _dp, _ip]),
     "lsqt_host_model_random_state": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "lsqt_host_model_evolve": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]),
+    "lsqt_host_model_moments": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]),
     "lsqt_host_conductivity": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int,
                                          _dp, C.c_int, _dp, _dp, _dp, _dp, C.c_char_p,
                                          C.c_int]),
@@ -110,6 +112,30 @@ class HostModel:
                                                im.ctypes.data_as(_dp)):
             raise InputError("bad orbital")
         return re, im
+
+    def evolve(self, kind, direction, t_scaled, re, im, by_terms=True):
+        """The host classes' evolve (kind 0) or evolvex (kind 1) of (re, im)
+        on the model's Hamiltonian (built on first use, which releases the
+        model's arrays); by_terms sequences it one chebyshev_2(x) per term."""
+        re = np.array(re, dtype=np.float64)
+        im = np.array(im, dtype=np.float64)
+        if load().lsqt_host_model_evolve(self.h, int(kind), int(direction), float(t_scaled),
+                                         1 if by_terms else 0, re.ctypes.data_as(_dp),
+                                         im.ctypes.data_as(_dp)):
+            raise InputError("lsqt_host_model_evolve failed")
+        return re, im
+
+    def moments(self, left, right, by_terms=True):
+        """find_moments_chebyshev of the host classes into a device Vector;
+        by_terms: kernel_polynomial + Vector::inner_product_1/2 per moment."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (*left, *right)]
+        M = self.number_of_moments
+        mr, mi = np.zeros(M), np.zeros(M)
+        if load().lsqt_host_model_moments(self.h, *(x.ctypes.data_as(_dp) for x in a),
+                                          1 if by_terms else 0, mr.ctypes.data_as(_dp),
+                                          mi.ctypes.data_as(_dp)):
+            raise InputError("lsqt_host_model_moments failed")
+        return mr + 1j * mi
 
     def close(self):
         if self.h:

@@ -72,6 +72,10 @@ SIGNATURES = {
     "lsqt_vec_apply_sz": (C.c_int, [_vp, _vp]),
     "lsqt_vec_swap": (C.c_int, [_vp, _vp]),
     "lsqt_vec_dot": (C.c_int, [_vp, _vp, _dp, _dp]),
+    "lsqt_vec_inner_product_1": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "lsqt_vec_inner_product_2": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+    "lsqt_vec_device_data": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "lsqt_moments_to_vec": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "lsqt_apply": (C.c_int, [_vp, _vp, _vp]),
     "lsqt_apply_commutator": (C.c_int, [_vp, _vp, _vp]),
     "lsqt_apply_current": (C.c_int, [_vp, _vp, _vp]),
@@ -236,6 +240,21 @@ class Vector:
         _check(load().lsqt_vec_dot(self.h, psi.h, C.byref(re), C.byref(im)))
         return complex(re.value, im.value)
 
+    def inner_product_1(self, number_of_atoms, other, target, offset):
+        """target[offset + b] = sum over 512-site block b of self conj(other)"""
+        _check(load().lsqt_vec_inner_product_1(self.h, int(number_of_atoms), other.h, target.h,
+                                               int(offset)))
+
+    def inner_product_2(self, number_of_atoms, number_of_moments, target):
+        """target[m] = sum of the block partials of moment m"""
+        _check(load().lsqt_vec_inner_product_2(self.h, int(number_of_atoms),
+                                               int(number_of_moments), target.h))
+
+    def device_data(self):
+        p = _vp()
+        _check(load().lsqt_vec_device_data(self.h, C.byref(p)))
+        return p.value
+
     def close(self):
         if self.h:
             load().lsqt_vec_destroy(self.h)
@@ -391,6 +410,10 @@ class Hamiltonian:
         re, im = np.empty(M), np.empty(M)
         _check(load().lsqt_moments(self.h, left.h, right.h, int(M), _dptr(re), _dptr(im)))
         return re + 1j * im
+
+    def moments_to_vec(self, left, right, M, out):
+        """find_moments_chebyshev into a device vector: out[m] = mu_m"""
+        _check(load().lsqt_moments_to_vec(self.h, left.h, right.h, int(M), out.h))
 
     def evolve(self, direction, t_scaled, state):
         _check(load().lsqt_evolve(self.h, int(direction), float(t_scaled), state.h))

@@ -1,0 +1,66 @@
+"""The benchmarked workloads as input directories and seeded vectors.
+
+Shared by bench.py, the full-size parity tests and the script that writes
+their fixtures, so that every one of them builds the same systems:
+
+* the FP64 / precision job: an 8.0M-site honeycomb (2000 x 1000 four-orbital
+  cells, a_cc = 1, t = -2.7 eV, periodic), Anderson disorder U[-1, 1] eV,
+  3000 moments, E_max = 10 eV, 201 energies in [-8, 8] eV, MSD over time
+  steps of 2 hbar/eV;
+* the block job: the 160^3 simple-cubic Anderson model with a Peierls phase
+  (inputs.cubic_peierls_arrays), E_max = 15, vectors of seeded random phases;
+* the uneven batch: honeycomb realisations with (Nc1, Nc2) cells drawn
+  uniformly in [200, 1500]^2, a_cc = 1.42 A, charged impurities at 0.1 %
+  (amplitudes U[-2.8, 2.8] eV, xi = 4.26 A), R random vectors each, VAC and
+  MSD over time steps of 2 hbar/eV, E_max = 12 eV.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from . import inputs as gi
+
+CONFIG2_CELLS = (2000, 1000)
+CUBE_L = 160
+CUBE_EMAX = 15.0
+CUBE_ENERGIES = np.linspace(-14, 14, 201)
+BATCH_EMAX = 12.0
+BATCH_DT = 2.0
+
+
+def write_config2(d, nx=2000, ny=1000, moments=3000, time_steps=20, dt=2.0, energies=201,
+                  msd=True):
+    """The honeycomb DOS (+ MSD) job's input directory."""
+    gi.write_honeycomb(d, nx, ny)
+    gi.write_para(d, anderson=2.0, msd=msd, moments=moments, energy_max=10.0)
+    gi.write_energy(d, gi.energy_grid(-8, 8, energies))
+    gi.write_time_steps(d, [dt] * time_steps)
+
+
+def cube_phases(R, n, seed=7):
+    """R vectors of unit-modulus entries with seeded random phases
+    (re, im), each of shape (R, n)."""
+    ph = np.random.default_rng(seed).uniform(0, 2 * np.pi, (R, n))
+    return np.cos(ph), np.sin(ph)
+
+
+def batch_cells(count, lo=200, hi=1500, seed=5):
+    """Cell counts of the uneven batch: (Nc1, Nc2) uniform in [lo, hi]^2 from
+    a fixed seed; a four-orbital cell holds two two-site cells, so Nx = Nc1,
+    Ny = Nc2 / 2 (Nc2 rounded up to even)."""
+    c = np.random.default_rng(seed).integers(lo, hi + 1, size=(count, 2))
+    return [(int(a), int((b + 1) // 2)) for a, b in c]
+
+
+def write_batch_realisation(d, nx, ny, R=8, moments=1500, n_steps=16, dt=BATCH_DT,
+                            emax=BATCH_EMAX):
+    """One realisation of the uneven batch as an input directory."""
+    os.makedirs(d, exist_ok=True)
+    n = 4 * nx * ny
+    gi.write_honeycomb(d, nx, ny, ax=2.4595, ay=4.26)
+    gi.write_para(d, charged=(int(round(0.001 * n)), 5.6, 4.26), moments=moments,
+                  energy_max=emax, vac=True, msd=True, random_vectors=R)
+    gi.write_energy(d, gi.energy_grid(-8, 8, 201))
+    gi.write_time_steps(d, [dt] * n_steps)

@@ -174,6 +174,24 @@ int lsqt_vec_swap(lsqt_vec* a, lsqt_vec* b);
  * src/vector.cu:331-344,425-436, for one vector pair); synchronises */
 int lsqt_vec_dot(const lsqt_vec* left, const lsqt_vec* psi, double* re, double* im);
 
+/* Vector::inner_product_1 (src/vector.cu:331-344): for every block b of 512
+ * sites (n in [512 b, 512 b + 512), n < number_of_atoms),
+ * target[offset + b] = sum_n v_n conj(other_n).  target must hold offset +
+ * ceil(number_of_atoms / 512) entries.  Asynchronous. */
+int lsqt_vec_inner_product_1(const lsqt_vec* v, int number_of_atoms, const lsqt_vec* other,
+                             lsqt_vec* target, int offset);
+/* Vector::inner_product_2 (src/vector.cu:425-436): target[m] = sum_b
+ * v[m G + b] over the G = ceil(number_of_atoms / 512) partials of moment m,
+ * m < number_of_moments.  Asynchronous. */
+int lsqt_vec_inner_product_2(const lsqt_vec* v, int number_of_atoms, int number_of_moments,
+                             lsqt_vec* target);
+/* The device address of the vector's storage: n interleaved (re, im) FP64
+ * pairs.  (The reference exposes its split device arrays as the public
+ * members real_part / imag_part, src/vector.h:39-40; this build stores the
+ * two parts interleaved, so one address replaces both.)  Valid until the
+ * vector is destroyed or swapped. */
+int lsqt_vec_device_data(const lsqt_vec* v, void** data);
+
 /* ---- operators ------------------------------------------------------------
  * Hamiltonian::apply: out = H~ in, H~ = H / energy_max (src/hamiltonian.cu:225-237) */
 int lsqt_apply(lsqt_ham* h, const lsqt_vec* in, lsqt_vec* out);
@@ -205,6 +223,11 @@ int lsqt_chebyshev_2x(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s0x, cons
  * end.  mu_re/mu_im are host arrays of M doubles. */
 int lsqt_moments(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right, int M,
                  double* mu_re, double* mu_im);
+/* find_moments_chebyshev with its Vector output (src/sigma.cu:33-61):
+ * out[m] = mu_m, m < M, left on the device (out holds at least M entries);
+ * asynchronous */
+int lsqt_moments_to_vec(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right, int M,
+                        lsqt_vec* out);
 /* evolve (src/sigma.cu:116-151): state <- U(direction * t) state, where
  * time_step_scaled = dt * energy_max.  Bessel coefficients from the host's
  * libm j0/j1/jn, stopping at the first |J_m| < 1e-15 (m >= 2). */

@@ -81,6 +81,26 @@ int lsqt_host_model_inputs(const lsqt_host_model* m, double* energy, double* tim
 /* Model::initialize_state into host arrays (consumes random draws) */
 int lsqt_host_model_random_state(lsqt_host_model* m, int orbital, double* re, double* im);
 
+/* The model's Hamiltonian (built on the first of these calls; like
+ * Hamiltonian(Model&), src/hamiltonian.cu:27-89, that releases the model's
+ * arrays, so lsqt_host_model_arrays fails afterwards) applied by the host
+ * classes:
+ *   lsqt_host_model_evolve: kind 0 evolve (src/sigma.cu:116-151), kind 1
+ *     evolvex (src/sigma.cu:157-200) of the state re/im, replaced by the
+ *     result; by_terms != 0 sequences it as the reference does, one
+ *     Hamiltonian::chebyshev_2 / chebyshev_2x call per Bessel term with the
+ *     states rotated by Vector::swap, by_terms == 0 with the fused entry
+ *     points (the same bits);
+ *   lsqt_host_model_moments: find_moments_chebyshev (src/sigma.cu:33-61) of
+ *     the model's number_of_moments into a device Vector, downloaded to
+ *     mu_re / mu_im; by_terms != 0 through Hamiltonian::kernel_polynomial and
+ *     Vector::inner_product_1 / inner_product_2, by_terms == 0 fused. */
+int lsqt_host_model_evolve(lsqt_host_model* m, int kind, int direction, double time_step_scaled,
+                           int by_terms, double* re, double* im);
+int lsqt_host_model_moments(lsqt_host_model* m, const double* left_re, const double* left_im,
+                            const double* right_re, const double* right_im, int by_terms,
+                            double* mu_re, double* mu_im);
+
 #ifdef __cplusplus
 }
 #endif

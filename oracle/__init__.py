@@ -242,6 +242,44 @@ class OracleModel:
         return self._driver("orc_find_spin", r, self._m.number_of_steps_correlation)
 
 
+class OracleHam:
+    """The oracle's operators on Hamiltonian arrays given directly (row-major
+    ELL, as a Model builds them), for systems that no input directory
+    describes: the same functions as OracleModel's numerics."""
+
+    def __init__(self, arrays, n, max_neighbor, energy_max):
+        self._keep = {
+            "neighbor_number": np.ascontiguousarray(arrays["neighbor_number"], dtype=np.int32),
+            "neighbor_list": np.ascontiguousarray(arrays["neighbor_list"], dtype=np.int32),
+        }
+        for k in ("potential", "hopping_real", "hopping_imag", "xx"):
+            self._keep[k] = np.ascontiguousarray(arrays[k], dtype=np.float64)
+        assert self._keep["neighbor_number"].size == n
+        assert self._keep["neighbor_list"].size == n * max_neighbor
+        h = Ham()
+        h.n, h.max_neighbor, h.energy_max = n, max_neighbor, float(energy_max)
+        h.neighbor_number = _ipp(self._keep["neighbor_number"])
+        h.neighbor_list = _ipp(self._keep["neighbor_list"])
+        for k in ("potential", "hopping_real", "hopping_imag", "xx"):
+            setattr(h, k, _p(self._keep[k]))
+        self._h = h
+        self.n = n
+
+    apply = OracleModel.apply
+    apply_commutator = OracleModel.apply_commutator
+    apply_current = OracleModel.apply_current
+    kernel_polynomial = OracleModel.kernel_polynomial
+    _op = OracleModel._op
+    evolve = OracleModel.evolve
+    evolvex = OracleModel.evolvex
+
+    def moments(self, L, R, M):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (*L, *R)]
+        mr, mi = np.zeros(M), np.zeros(M)
+        lib().orc_find_moments(C.byref(self._h), M, *(_p(x) for x in a), _p(mr), _p(mi))
+        return mr, mi
+
+
 def damping(mu_re, mu_im=None, lorentz=False):
     mu_re = np.array(mu_re, dtype=np.float64)
     mu_im = np.zeros_like(mu_re) if mu_im is None else np.array(mu_im, dtype=np.float64)

@@ -37,7 +37,7 @@ def assert_moments(mu, ref, what="", rtol=1e-10, scale_tol=1e-12):
                              f"bound {bound[i]:.3e})")
 
 
-def assert_rows(a, b, what="", rtol=ROW_RTOL):
+def assert_rows(a, b, what="", rtol=ROW_RTOL, floor=1e-6):
     a = np.atleast_2d(np.asarray(a, dtype=np.float64))
     b = np.atleast_2d(np.asarray(b, dtype=np.float64))
     assert a.shape == b.shape, f"{what}: shape {a.shape} vs {b.shape}"
@@ -48,7 +48,7 @@ def assert_rows(a, b, what="", rtol=ROW_RTOL):
             continue
         normwise = np.abs(a[r] - b[r]).max() / bmax
         assert normwise <= rtol, f"{what} row {r}: normwise error {normwise:.3e}"
-        mask = np.abs(b[r]) > 1e-6 * bmax
+        mask = np.abs(b[r]) > floor * bmax
         elem = (np.abs(a[r] - b[r])[mask] / np.abs(b[r])[mask]).max() if mask.any() else 0.0
         assert elem <= rtol, f"{what} row {r}: element-wise error {elem:.3e}"
 

@@ -153,3 +153,73 @@ def test_no_pattern_without_periodicity(gpu_ctx):
     H = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, K, 10.0)
     assert H.info().pattern_period == 0
     H.close()
+
+
+def _oracle_checks(H, arrays, n, K, emax, vectors):
+    """apply bit for bit and moments against the oracle on the same arrays."""
+    import oracle
+    oh = oracle.OracleHam(arrays, n, K, emax)
+    r = vectors[0]
+    v, out = H.vector(*r), H.vector()
+    H.apply(v, out)
+    got, want = out.download(), oh.apply(*r)
+    assert_bits(got[0], want[0], "apply re")
+    assert_bits(got[1], want[1], "apply im")
+    ref = oh.moments(r, r, 64)
+    assert_moments(H.moments(v, v, 64), ref[0] + 1j * ref[1], "moments")
+    for o in (v, out):
+        o.close()
+
+
+def test_pattern_with_global_class_table(gpu_ctx, tmp_path, monkeypatch):
+    """A periodic honeycomb whose hoppings take 600 extra distinct values on
+    200 sites (under 1 % of them): more than 512 classes, so the class table
+    stays in global memory, and the slot pattern is still kept.  At the
+    wrapped boundary sites the pattern's computed words carry the neighbour
+    index into the class bits before the mask is read; the class index must
+    stay inside the table.  Results equal the pattern-off ones and the
+    oracle's."""
+    import oracle
+    print("[gpu] slot pattern, honeycomb 100x60 with a global class table", flush=True)
+    d = str(tmp_path)
+    _honeycomb(d, 100, 60, None)
+    m = oracle.OracleModel(d)
+    arrays = {k: np.array(v, copy=True) for k, v in m.arrays().items()}
+    K = m.max_neighbor
+    rng = np.random.default_rng(11)
+    sites = rng.choice(m.n, 200, replace=False)
+    for s in sites:
+        for q in range(int(arrays["neighbor_number"][s])):
+            arrays["hopping_real"][s * K + q] = -2.7 + rng.uniform(-0.5, 0.5)
+    H, info = _compare(gpu_ctx, monkeypatch, arrays, m.n, K, m.energy_max,
+                       [m.random_state(), m.random_state()], True)
+    assert info.layout == 2 and info.classes > 512, (info.layout, info.classes)
+    # the index bits leave room for class indices far beyond the table
+    assert (1 << (32 - info.index_bits)) > 64 * info.classes
+    _oracle_checks(H, arrays, m.n, K, m.energy_max, [m.random_state()])
+    H.close()
+
+
+def test_pattern_eight_slots(gpu_ctx, tmp_path, monkeypatch):
+    """A square lattice with nearest and next-nearest neighbours (8 slots,
+    outside the slot counts with their own kernels): the generic slot loop
+    with the pattern, bit-identical to the pattern-off build and to the
+    oracle."""
+    import oracle
+    from gpuqt_amd import inputs as gi
+    print("[gpu] slot pattern, square 70x64 with 8 neighbours", flush=True)
+    d = str(tmp_path)
+    hops = [[(-1, 0, 0, 0, -1.0, 0.0), (1, 0, 0, 0, -1.0, 0.0), (0, 1, 0, 0, -1.0, 0.0),
+             (0, -1, 0, 0, -1.0, 0.0), (1, 1, 0, 0, 0.25, 0.1), (-1, -1, 0, 0, 0.25, -0.1),
+             (1, -1, 0, 0, 0.3, 0.0), (-1, 1, 0, 0, 0.3, 0.0)]]
+    gi.write_lattice(d, (70, 64, 1), (1, 1, 0), 0, (1.0, 1.0, 1.0), [(0.0, 0.0, 0.0)], hops)
+    gi.write_para(d, anderson=1.0, moments=64, energy_max=8.0)
+    gi.write_energy(d, [0.0])
+    gi.write_time_steps(d, [1.0])
+    m = oracle.OracleModel(d)
+    assert m.max_neighbor == 8
+    H, info = _compare(gpu_ctx, monkeypatch, m.arrays(), m.n, m.max_neighbor, m.energy_max,
+                       [m.random_state(), m.random_state()], True)
+    assert info.pattern_period == 1
+    _oracle_checks(H, m.arrays(), m.n, m.max_neighbor, m.energy_max, [m.random_state()])
+    H.close()
