@@ -138,6 +138,7 @@ extern "C" int lsqt_ctx_create(int device, lsqt_ctx** out)
     return set_error(LSQT_ERR_NODEVICE, "device %d is %s; this library is built for gfx950",
                      device, prop.gcnArchName);
   LSQT_HIP(hipSetDevice(device));
+  noalloc_setup(device);
   lsqt_ctx* c = new lsqt_ctx();
   c->device = device;
   c->cu_count = prop.multiProcessorCount;

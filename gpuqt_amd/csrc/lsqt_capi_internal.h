@@ -77,6 +77,8 @@ int evolvex_enqueue(lsqt_ham* h, int direction, double t, double2* state);
 int moments_f32_enqueue(lsqt_ham* h, const double2* left, const double2* right, int M);
 int evolve_f32_enqueue(lsqt_ham* h, int direction, double t, double2* state, int renormalise);
 int evolvex_f32_enqueue(lsqt_ham* h, int direction, double t, double2* state);
+// resolves the fixed scratch of lsqt_moments_noalloc on `device` (current)
+void noalloc_setup(int device);
 
 }  // namespace lsqt
 

@@ -76,6 +76,7 @@ SIGNATURES = {
     "lsqt_vec_inner_product_2": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     "lsqt_vec_device_data": (C.c_int, [_vp, C.POINTER(_vp)]),
     "lsqt_moments_to_vec": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "lsqt_moments_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp, _dp]),
     "lsqt_apply": (C.c_int, [_vp, _vp, _vp]),
     "lsqt_apply_commutator": (C.c_int, [_vp, _vp, _vp]),
     "lsqt_apply_current": (C.c_int, [_vp, _vp, _vp]),
@@ -409,6 +410,14 @@ class Hamiltonian:
     def moments(self, left, right, M):
         re, im = np.empty(M), np.empty(M)
         _check(load().lsqt_moments(self.h, left.h, right.h, int(M), _dptr(re), _dptr(im)))
+        return re + 1j * im
+
+    def moments_noalloc(self, left, right, M):
+        """lsqt_moments_noalloc: the moments of lsqt_moments from the fixed
+        per-GPU scratch (no device allocation)"""
+        re, im = np.empty(M), np.empty(M)
+        _check(load().lsqt_moments_noalloc(self.h, left.h, right.h, int(M), _dptr(re),
+                                           _dptr(im)))
         return re + 1j * im
 
     def moments_to_vec(self, left, right, M, out):

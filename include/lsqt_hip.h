@@ -228,6 +228,22 @@ int lsqt_moments(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right, int M
  * asynchronous */
 int lsqt_moments_to_vec(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right, int M,
                         lsqt_vec* out);
+/* (no counterpart in the reference)  lsqt_moments for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The same kernels in the same order, so
+ * the same moments bit for bit, but every buffer beyond the inputs and
+ * outputs (two recursion vectors, the moment partials, the moments) lives in
+ * one fixed scratch array per GPU, held from the library's first context on
+ * that GPU until the process ends: 38 MiB, for up to LSQT_NOALLOC_MAX_SITES
+ * sites and LSQT_NOALLOC_MAX_MOMENTS moments; larger inputs are rejected
+ * with LSQT_ERR_INVALID.  No hipMalloc / hipFree or other allocating call.
+ * All calls on one GPU share its scratch: calls that overlap on one GPU (two
+ * contexts on different streams, or threads) race and must be serialised by
+ * the caller.  Synchronises the context's stream, like lsqt_moments. */
+#define LSQT_NOALLOC_MAX_SITES (1 << 20)
+#define LSQT_NOALLOC_MAX_MOMENTS (1 << 16)
+int lsqt_moments_noalloc(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right, int M,
+                         double* mu_re, double* mu_im);
 /* evolve (src/sigma.cu:116-151): state <- U(direction * t) state, where
  * time_step_scaled = dt * energy_max.  Bessel coefficients from the host's
  * libm j0/j1/jn, stopping at the first |J_m| < 1e-15 (m >= 2). */
