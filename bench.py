@@ -334,7 +334,7 @@ def fp64_vac_msd(H, re, im, M, ts):
     return np.array(vac), np.array(msd)
 
 
-def bench_batch(ctx, realisations=8, R=8, M=1500, n_steps=16, dt=2.0, emax=12.0,
+def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=12.0,
                 threads=16, check=True):
     """Uneven batch: honeycomb realisations of uneven size (a_cc = 1.42 A,
     charged impurities at 0.1 %, amplitudes U[-2.8, 2.8] eV, xi = 4.26 A),
@@ -398,12 +398,15 @@ def bench_batch(ctx, realisations=8, R=8, M=1500, n_steps=16, dt=2.0, emax=12.0,
     H.block_vac(re, im, M, ts[:1])
     H.block_msd(re, im, M, ts[:1])
     ctx.sync()
+    log(f"batch: warm-up (one time step of VAC and MSD) done; timing {n_steps} steps")
     t = time.perf_counter()
     vac = H.block_vac(re, im, M, ts)
     t_vac = time.perf_counter() - t
+    log(f"batch: VAC over {n_steps} steps in {t_vac:.1f} s")
     t = time.perf_counter()
     msd = H.block_msd(re, im, M, ts)
     t_msd = time.perf_counter() - t
+    log(f"batch: MSD over {n_steps} steps in {t_msd:.1f} s")
     elapsed = t_vac + t_msd
     K_t = terms(dt * emax)
     # recursion site-updates per vector and site: VAC (M - 1) per step, MSD
@@ -427,27 +430,41 @@ def bench_batch(ctx, realisations=8, R=8, M=1500, n_steps=16, dt=2.0, emax=12.0,
                     "moment sets replayed from captured hipGraphs",
     }
     if check:
-        # the smallest realisation, vector 0, against the FP64 single-vector path
-        k = int(np.argmin(sizes))
-        m, a, vecs = built[k]
-        H1 = native.Hamiltonian.from_arrays(ctx, a, m.n, K, emax)
-        v64, m64 = fp64_vac_msd(H1, vecs[0][0], vecs[0][1], M, ts)
-        energies, _, _ = m.inputs()
-        row = lambda mu: hostlib.correlation_row(mu, energies, emax, m.volume)
-        errs = {}
-        for name, got, want in (("vac", vac[:, k, 0], v64), ("msd", msd[:, k, 0], m64)):
-            g = np.array([row(x) for x in got])
-            w = np.array([row(x) for x in want])
-            errs[f"{name}_rows_normwise_max"] = float(
-                max(np.abs(a_ - b_).max() / np.abs(b_).max() for a_, b_ in zip(g, w)))
-            sg = hostlib.conductivity(ts / emax, **{name: g})[f"sigma_{name}"]
-            sw = hostlib.conductivity(ts / emax, **{name: w})[f"sigma_{name}"]
-            errs[f"sigma_{name}_normwise_max"] = float(
-                max(np.abs(a_ - b_).max() / np.abs(b_).max()
-                    for a_, b_ in zip(sg, sw) if np.abs(b_).max() > 0))
-        res["error_vs_f64_smallest_realisation"] = errs
-        log(f"batch errors vs FP64 (realisation {k}, {m.n} sites, vector 0): {errs}")
-        H1.close()
+        # vector 0 of every realisation against the FP64 single-vector path
+        t_chk = time.perf_counter()
+        per = []
+        for k, (m, a, vecs) in enumerate(built):
+            H1 = native.Hamiltonian.from_arrays(ctx, a, m.n, K, emax)
+            v64, m64 = fp64_vac_msd(H1, vecs[0][0], vecs[0][1], M, ts)
+            H1.close()
+            energies, _, _ = m.inputs()
+            row = lambda mu: hostlib.correlation_row(mu, energies, emax, m.volume)
+            e = {"realisation": k, "sites": m.n}
+            for name, got, want in (("vac", vac[:, k, 0], v64), ("msd", msd[:, k, 0], m64)):
+                g = np.array([row(x) for x in got])
+                w = np.array([row(x) for x in want])
+                e[f"{name}_rows"] = float(
+                    max(np.abs(a_ - b_).max() / np.abs(b_).max() for a_, b_ in zip(g, w)))
+                sg = hostlib.conductivity(ts / emax, **{name: g})[f"sigma_{name}"]
+                sw = hostlib.conductivity(ts / emax, **{name: w})[f"sigma_{name}"]
+                e[f"sigma_{name}"] = float(
+                    max(np.abs(a_ - b_).max() / np.abs(b_).max()
+                        for a_, b_ in zip(sg, sw) if np.abs(b_).max() > 0))
+            per.append(e)
+            if k % 8 == 7 or k == len(built) - 1:
+                log(f"batch FP64 check: {k + 1} of {len(built)} realisations")
+        keys = ("vac_rows", "msd_rows", "sigma_vac", "sigma_msd")
+        res["error_vs_f64"] = {
+            "what": "normwise error of every VAC / MSD row and sigma_VAC / sigma_MSD row of "
+                    "random vector 0 of every realisation against the FP64 single-vector "
+                    "drivers on the same realisation",
+            **{f"{key}_normwise_max": max(e[key] for e in per) for key in keys},
+            "per_realisation": per,
+        }
+        log(f"batch errors vs FP64 (vector 0 of all {len(per)} realisations, "
+            f"{time.perf_counter() - t_chk:.1f} s): "
+            + ", ".join(f"{key} {res['error_vs_f64'][key + '_normwise_max']:.2e}"
+                        for key in keys))
     H.close()
     return res
 
@@ -500,11 +517,48 @@ def probe(ctx, H, rv, args, moments=10):
         log(f"probe: block R={R} moments on {n} sites")
 
 
+def host_cpu():
+    """The host CPU model and the threads this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"model": model, "logical_cpus_usable": usable}
+
+
+def reference_provenance(binary):
+    """Where the reference program came from: the build record written next
+    to it by the oracle's recipe, and the binary's sha256."""
+    import hashlib
+    info = {"binary": os.path.relpath(binary, ROOT)}
+    with open(binary, "rb") as f:
+        info["sha256"] = hashlib.sha256(f.read()).hexdigest()
+    rec = os.path.join(os.path.dirname(binary), "buildinfo.txt")
+    try:
+        with open(rec) as f:
+            for line in f:
+                key, _, val = line.strip().partition(" ")
+                if key:
+                    info[key] = val
+    except OSError:
+        info["build_record"] = "absent"
+    return info
+
+
 def cpu_baseline(nx, ny, moments_sample, threads):
-    """Times the reference CPU program (compiled from source into
-    oracle/_ref) or, where it is absent, the oracle, on a bounded sample of
-    the same workload: the DOS moment set of the same lattice with fewer
-    moments.  Returns the cpu_baseline dict."""
+    """Times the reference CPU program (compiled from the reference's sources
+    into oracle/_ref by `make -C oracle ref`) on a bounded sample of the same
+    workload: the DOS moment set of the same lattice with fewer moments, on
+    one core and as `threads` concurrent processes.  Where that program is
+    absent (a tree built without the reference sources) the oracle's C
+    restatement is timed instead, with kind "port" and a note saying so.
+    Returns the cpu_baseline dict."""
     import oracle
     with tempfile.TemporaryDirectory() as tmp:
         d = os.path.join(tmp, "cpu")
@@ -513,6 +567,7 @@ def cpu_baseline(nx, ny, moments_sample, threads):
         n_sites = 4 * nx * ny
         updates = n_sites * (moments_sample - 1)
         ref = oracle.reference_binary()
+        parallel = None
         if ref is not None:
             driver = os.path.join(tmp, "input.txt")
             with open(driver, "w") as f:
@@ -534,9 +589,17 @@ def cpu_baseline(nx, ny, moments_sample, threads):
             kind, cores = "port", oracle.num_threads()
     out = {"value": updates / secs, "unit": "site-updates/s", "cores": cores, "kind": kind,
            "sample": f"DOS moment set, {moments_sample} moments, {n_sites}-site honeycomb "
-                     f"(same lattice as the workload), {secs:.2f} s"}
-    if kind == "reference" and parallel:
-        out["parallel_processes"] = parallel
+                     f"(same lattice as the workload), {secs:.2f} s",
+           "host_cpu": host_cpu()}
+    if kind == "reference":
+        out["program"] = reference_provenance(ref)
+        out["timer"] = "the program's own DOS phase timer (model build and I/O excluded)"
+        if parallel:
+            out["parallel_processes"] = parallel
+    else:
+        out["note"] = ("the reference program is not built in this tree (oracle/_ref/lsqt_cpu; "
+                       "make -C oracle ref needs the reference sources): the oracle's C "
+                       f"restatement of the same recursion was timed on {cores} OpenMP threads")
     return out
 
 
@@ -586,6 +649,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", dest="precision_f32", action="store_false",
                     help="skip the precision-path (FP32 states) job")
+    ap.add_argument("--f32-steps", type=int, default=5,
+                    help="timed steps of the precision-path job (its own count)")
+    ap.add_argument("--f32-warmup", type=int, default=1)
     ap.add_argument("--no-block", dest="block", action="store_false",
                     help="skip the block-path (R vectors per sweep) measurement")
     ap.add_argument("--cubic-L", type=int, default=160)
@@ -595,7 +661,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the uneven-batch (VAC + MSD over many realisations) section")
-    ap.add_argument("--batch-realisations", type=int, default=8)
+    ap.add_argument("--batch-realisations", type=int, default=64)
     ap.add_argument("--batch-moments", type=int, default=1500)
     ap.add_argument("--batch-steps", type=int, default=16)
     ap.add_argument("--probe", action="store_true",
@@ -648,8 +714,8 @@ def main():
     job = dict(energies=energies, time_steps=time_steps, emax=model.energy_max,
                volume=model.volume, M=M, msd=args.time_steps > 0)
 
-    def timed(precision):
-        for w in range(args.warmup):
+    def timed(precision, steps, warmup):
+        for w in range(warmup):
             run_job(H, model, rv, precision=precision, **job)
             log(f"{precision} warmup {w} done")
         ev = HipEvents(ctx.stream)
@@ -658,7 +724,7 @@ def main():
         t0 = time.perf_counter()
         total = 0
         rows = None
-        for s in range(args.steps):
+        for s in range(steps):
             u, rows = run_job(H, model, rv, ev=ev, acc=acc, precision=precision, **job)
             total += u
             log(f"{precision} step {s} done")
@@ -667,7 +733,7 @@ def main():
         avg_launch_ms = acc["moment_ms"] / max(acc["moment_sets"] * (M - 1), 1)
         return total, elapsed, avg_launch_ms, rows
 
-    total_updates, elapsed, avg_launch_ms, rows = timed("f64")
+    total_updates, elapsed, avg_launch_ms, rows = timed("f64", args.steps, args.warmup)
     rows = np.array(rows)
     value = total_updates / elapsed
     # dominant kernel: the fused recursion + moment-product step, one site
@@ -684,7 +750,7 @@ def main():
 
     f32 = None
     if args.precision_f32:
-        u32, e32, l32, rows32 = timed("f32")
+        u32, e32, l32, rows32 = timed("f32", args.f32_steps, args.f32_warmup)
         # FP32 path: potential 4 + psi_{m-1}, psi_{m-2}, psi_m, left 4 x 8 + the
         # same pair bytes as FP64 (which has potential 8 + 4 x 16)
         bpu32 = 36.0 + (info.bytes_per_site_f64 - 72.0)
@@ -703,7 +769,9 @@ def main():
         f32 = {
             "value": u32 / e32,
             "unit": "site-updates/s",
-            "ms_per_step": e32 / args.steps * 1e3,
+            "ms_per_step": e32 / args.f32_steps * 1e3,
+            "steps": args.f32_steps,
+            "warmup": args.f32_warmup,
             "speedup_vs_f64": (u32 / e32) / value,
             "dtype": "f32 states, f64 arithmetic and accumulation",
             "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS,

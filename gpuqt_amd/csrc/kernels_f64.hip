@@ -609,13 +609,11 @@ inline int blocks_1d(int n)
 // sites 102.2 us; with the slot pattern 105.2 and 100.4 us, and with the
 // pattern's gathers issued before the mask read 2 sites 93.8 us, 3 sites
 // 100.0 us, 4 sites 108.2 us.  On the 160^3 cube (6 slots) 2 sites were 9 %
-// slower than 1.  LSQT_SPT_HPSI3 overrides the 3-slot count at build time.
-#ifndef LSQT_SPT_HPSI3
-#define LSQT_SPT_HPSI3 2
-#endif
+// slower than 1.  (bench.py names the profiled kernel k_hpsi<1, 3, 2, ...>
+// after this choice.)
 constexpr int hpsi_spt_for(int K)
 {
-  return K == 3 ? LSQT_SPT_HPSI3 : 1;
+  return K == 3 ? 2 : 1;
 }
 
 int hpsi_spt(int K)
