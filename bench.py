@@ -334,7 +334,7 @@ def fp64_vac_msd(H, re, im, M, ts):
     return np.array(vac), np.array(msd)
 
 
-def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=12.0,
+def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.BATCH_EMAX,
                 threads=16, check=True):
     """Uneven batch: honeycomb realisations of uneven size (a_cc = 1.42 A,
     charged impurities at 0.1 %, amplitudes U[-2.8, 2.8] eV, xi = 4.26 A),
@@ -370,6 +370,10 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=12.0
             os.close(devnull)
             os.close(saved)
     sizes = [m.n for m, _, _ in built]
+    bound = max(wl.gershgorin_bound(a, m.max_neighbor) for m, a, _ in built)
+    if not bound < emax:
+        raise RuntimeError(f"energy_max {emax} does not exceed the spectral bound {bound:.3f} "
+                           f"of the batch")
     starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     n = int(starts[-1])
     K = max(m.max_neighbor for m, _, _ in built)
@@ -389,7 +393,7 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=12.0
     im = np.concatenate([np.array([v[1] for v in vecs]) for _, _, vecs in built], axis=1)
     build_s = time.perf_counter() - t0
     log(f"batch: {realisations} realisations, {min(sizes)}..{max(sizes)} sites, {n} in all, "
-        f"built in {build_s:.1f} s")
+        f"built in {build_s:.1f} s; spectral bound {bound:.2f} < E_max {emax}")
     H = native.Hamiltonian.from_arrays(ctx, arrays, n, K, emax,
                                        segment_starts=starts.astype(np.int32))
     del arrays
