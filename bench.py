@@ -192,6 +192,7 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
     t0 = time.perf_counter()
     arrays, n = gi.cubic_peierls_arrays(L)
     H = native.Hamiltonian.from_arrays(ctx, arrays, n, 6, 15.0)
+    arrays_full = arrays
     del arrays
     info = H.info()
     log(f"cubic {L}^3: {n} sites, layout {'packed' if info.layout == 2 else 'full'}, "
@@ -219,6 +220,16 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
 
     t64, mu64 = timed(lambda: H.moments(v0, v0, M))
     t32, mu32 = timed(lambda: H.moments_f32(v0, v0, M))
+    # the FP64 kernel on the reference's data model (int32 index and FP64
+    # re / im / xx per pair, no packing): 76 + 20 z bytes per site-update
+    Hf = native.Hamiltonian.from_arrays(ctx, arrays_full, n, 6, 15.0, layout=native.LAYOUT_FULL)
+    del arrays_full
+    vf = Hf.vector(np.cos(ph[0]), np.sin(ph[0]))
+    t64f, mu64f = timed(lambda: Hf.moments(vf, vf, M))
+    assert np.abs(mu64f - mu64).max() <= 1e-10 * abs(mu64[0])
+    bpu64f = Hf.info().bytes_per_site_f64
+    vf.close()
+    Hf.close()
     tb, mub = timed(lambda: H.moments_block(blk, blk, M))
     ts, mus = timed(lambda: H.moments_block_self(blk, M))
     # <L|T_m|R> with L != R (the VAC shape) by pair moment sets
@@ -243,6 +254,7 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
             a = hostlib.correlation_row(got[r], e, 15.0, vol)
             out.append(float(np.abs(a - b).max() / np.abs(b).max()))
     u64 = n * (M - 1) / t64
+    u64f = n * (M - 1) / t64f
     u32 = n * (M - 1) / t32
     ub = n * R * (M - 1) / tb
     # measured HBM bytes per launch, where the profile covers this workload
@@ -262,7 +274,15 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
         "workload": f"simple cubic {L}^3 ({n} sites), hopping -1, Peierls phase 2pi*0.01*x on "
                     f"y-bonds, on-site U[-8.25,8.25], E_max=15, {M} moments, DOS",
         "f64_single": {"value": u64, "unit": "site-updates/s", "bytes_per_site_update": bpu64,
-                       "achieved_GBs": u64 * bpu64 / 1e9},
+                       "achieved_GBs": u64 * bpu64 / 1e9,
+                       "what": "the drop-in's FP64 single-vector kernel (packed layout): the "
+                               "comparator of every speedup_vs_f64 below"},
+        "f64_single_reference_layout": {
+            "value": u64f, "unit": "site-updates/s", "bytes_per_site_update": bpu64f,
+            "achieved_GBs": u64f * bpu64f / 1e9,
+            "what": "the same FP64 kernel on the reference's data model (int32 index + FP64 "
+                    "re/im/xx per pair, LSQT_LAYOUT_FULL)",
+            "block_raw_speedup": None, "block_self_time_to_moments_speedup": None},
         "f32_single": {"value": u32, "unit": "site-updates/s", "speedup_vs_f64": u32 / u64},
         "block_f32": {"R": R, "W": W, "vectors_per_wavefront": R // W, "rows_per_tile": rows,
                       "tile_schedule": SCHEDULES[blocked],
@@ -292,6 +312,12 @@ def bench_block(ctx, L=160, R=32, M=2048, reps=2, check_vectors=4):
             "moment_normwise_error_vs_f64": err_pair,
             "moment_error_vs_f64_over_norm_product": err_pair_n},
     }
+    ref_layout = res["f64_single_reference_layout"]
+    ref_layout["block_raw_speedup"] = us_ / u64f
+    ref_layout["block_self_time_to_moments_speedup"] = um_ / u64f
+    ref_layout["block_direct_speedup"] = ub / u64f
+    log(f"FP64 single on the reference data model: {u64f:.3e} site-updates/s; block product "
+        f"identities {us_ / u64f:.2f}x raw, {um_ / u64f:.2f}x time to moments")
     log(f"block R={R}: {ub:.3e} site-updates/s = {ub / u64:.2f}x FP64 single "
         f"({u64:.3e}); DOS error {max(errs):.2e}")
     log(f"block R={R} product identities: {us_:.3e} site-updates/s in {steps_self} steps, "
@@ -334,16 +360,14 @@ def fp64_vac_msd(H, re, im, M, ts):
     return np.array(vac), np.array(msd)
 
 
-def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.BATCH_EMAX,
-                threads=16, check=True):
-    """Uneven batch: honeycomb realisations of uneven size (a_cc = 1.42 A,
-    charged impurities at 0.1 %, amplitudes U[-2.8, 2.8] eV, xi = 4.26 A),
-    R random vectors each, VAC + MSD over n_steps steps of dt, all
-    realisations side by side in one segmented Hamiltonian: one launch per
-    recursion step for the whole batch, moment sets replayed as hipGraphs."""
+def build_batch(realisations, R, M, n_steps, dt, emax, threads=16):
+    """The first `realisations` honeycomb realisations of the uneven batch
+    (host Models, built in parallel), side by side: returns (built, arrays,
+    starts, n, K, re, im, build seconds) with built[k] = (Model, its arrays,
+    its R random vectors), the segmented Hamiltonian's arrays and the
+    vector-major random vectors of all realisations."""
     import concurrent.futures as cf
-    from gpuqt_amd import hostlib, native
-    from gpuqt_amd import inputs as gi
+    from gpuqt_amd import hostlib
     cells = wl.batch_cells(64)[:realisations]
     t0 = time.perf_counter()
     with tempfile.TemporaryDirectory() as tmp:
@@ -394,6 +418,20 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.B
     build_s = time.perf_counter() - t0
     log(f"batch: {realisations} realisations, {min(sizes)}..{max(sizes)} sites, {n} in all, "
         f"built in {build_s:.1f} s; spectral bound {bound:.2f} < E_max {emax}")
+    return built, arrays, starts, n, K, re, im, build_s
+
+
+def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.BATCH_EMAX,
+                threads=16, check=True):
+    """Uneven batch: honeycomb realisations of uneven size (a_cc = 1.42 A,
+    charged impurities at 0.1 %, amplitudes U[-2.8, 2.8] eV, xi = 4.26 A),
+    R random vectors each, VAC + MSD over n_steps steps of dt, all
+    realisations side by side in one segmented Hamiltonian: one launch per
+    recursion step for the whole batch, moment sets replayed as hipGraphs."""
+    from gpuqt_amd import hostlib, native
+    built, arrays, starts, n, K, re, im, build_s = build_batch(realisations, R, M, n_steps, dt,
+                                                               emax, threads)
+    sizes = [m.n for m, _, _ in built]
     H = native.Hamiltonian.from_arrays(ctx, arrays, n, K, emax,
                                        segment_starts=starts.astype(np.int32))
     del arrays
@@ -433,6 +471,19 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.B
         "launches": "one per recursion step for all realisations (segmented Hamiltonian); "
                     "moment sets replayed from captured hipGraphs",
     }
+    # counted HBM traffic of the batch's kernels over their compulsory bytes
+    # (profiled on the first 8 realisations, bench.py --probe)
+    n8 = sum(m.n for m, _, _ in built[:8]) if len(built) >= 8 else None
+    if n8 == 8973112 and R == 8:
+        ratios = {}
+        for kname, per_site in (("k_block_step<1,3,2,true,3,false,false>", 400.0),
+                                ("k_block_step<1,3,2,true,2,false,false>", 208.0),
+                                ("k_block_evo<1,3,2,1,true>", 336.0),
+                                ("k_block_evo<1,3,2,4,true>", 528.0)):
+            t = profiled_traffic(kname)
+            if t:
+                ratios[kname] = t / (per_site * n8)
+        res["traffic_ratio_to_algorithmic"] = ratios
     if check:
         # vector 0 of every realisation against the FP64 single-vector path
         t_chk = time.perf_counter()
@@ -519,6 +570,20 @@ def probe(ctx, H, rv, args, moments=10):
         Hc.block_set_config(R, 0)
         ctx.sync()
         log(f"probe: block R={R} moments on {n} sites")
+    if args.batch:
+        # the uneven batch's kernels (pair and self block steps, evolution
+        # terms) on its first 8 realisations, one time step of VAC and MSD
+        built, arrays, starts, n, K, re, im, _ = build_batch(8, 8, moments, 1, wl.BATCH_DT,
+                                                             wl.BATCH_EMAX)
+        Hb = native.Hamiltonian.from_arrays(ctx, arrays, n, K, wl.BATCH_EMAX,
+                                            segment_starts=starts.astype(np.int32))
+        del arrays
+        ts = np.full(1, wl.BATCH_DT * wl.BATCH_EMAX)
+        Hb.block_vac(re, im, moments, ts)
+        Hb.block_msd(re, im, moments, ts)
+        ctx.sync()
+        log(f"probe: uneven batch VAC + MSD ({moments} moments, one step) on {n} sites")
+        Hb.close()
 
 
 def host_cpu():

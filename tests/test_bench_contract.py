@@ -1,7 +1,8 @@
 """bench.py's contract on the CPU: more than one GPU is refused with a
 message, and the committed HBM-traffic profile holds the kernels whose
 `traffic` the JSON line reports (the FP64 fused step of the default
-workload and the tuned block kernels of the 160^3 cube)."""
+workload, the tuned block kernels of the 160^3 cube and the uneven batch's
+block steps)."""
 import os
 import subprocess
 import sys
@@ -28,4 +29,10 @@ def test_traffic_profile_entries():
     for mode in (1, 2):  # <L|psi> and product-identity block steps, sliding window
         t = bench.profiled_traffic(f"k_block_win<1,6,2,true,{mode}>")
         assert t is not None and t > 0
+    # the uneven batch's pair and self-product steps (8 realisations, 8.97M
+    # sites, 400 and 208 B per site) at their compulsory traffic within 5 %
+    for name, per_site in (("k_block_step<1,3,2,true,3,false,false>", 400.0),
+                           ("k_block_step<1,3,2,true,2,false,false>", 208.0)):
+        t = bench.profiled_traffic(name)
+        assert t is not None and abs(t / (per_site * 8_973_112) - 1.0) < 0.05, name
     assert bench.profiled_traffic("no_such_kernel<1>") is None
