@@ -360,14 +360,30 @@ def fp64_vac_msd(H, re, im, M, ts):
     return np.array(vac), np.array(msd)
 
 
+class BatchMember:
+    """One realisation of the uneven batch: its host Model (lattice kept for
+    the device), lattice description, vacancy map, potential and R random
+    vectors."""
+
+    def __init__(self, d, R):
+        from gpuqt_amd import hostlib
+        self.model = hostlib.HostModel(d, lattice_on_device=True)
+        self.n = self.model.n
+        self.lattice, self.site_index = self.model.lattice()
+        self.potential = self.model.potential()
+        self.vecs = [self.model.random_state() for _ in range(R)]
+        energies, _, _ = self.model.inputs()
+        self.energies = energies
+        self.volume = self.model.volume
+
+
 def build_batch(realisations, R, M, n_steps, dt, emax, threads=16):
-    """The first `realisations` honeycomb realisations of the uneven batch
-    (host Models, built in parallel), side by side: returns (built, arrays,
-    starts, n, K, re, im, build seconds) with built[k] = (Model, its arrays,
-    its R random vectors), the segmented Hamiltonian's arrays and the
-    vector-major random vectors of all realisations."""
+    """The first `realisations` honeycomb realisations of the uneven batch:
+    host Models built in parallel with their lattices kept for the device
+    (the neighbour lists are expanded on the GPU), side by side.  Returns
+    (members, potential, starts, n, re, im, build seconds): the segmented
+    Hamiltonian's potential and the vector-major random vectors of all."""
     import concurrent.futures as cf
-    from gpuqt_amd import hostlib
     cells = wl.batch_cells(64)[:realisations]
     t0 = time.perf_counter()
     with tempfile.TemporaryDirectory() as tmp:
@@ -377,48 +393,37 @@ def build_batch(realisations, R, M, n_steps, dt, emax, threads=16):
             wl.write_batch_realisation(d, nx, ny, R=R, moments=M, n_steps=n_steps, dt=dt,
                                        emax=emax)
             dirs.append(d)
-
-        def build(d):
-            m = hostlib.HostModel(d)
-            vecs = [m.random_state() for _ in range(R)]
-            return m, m.arrays(), vecs
-
         devnull = os.open(os.devnull, os.O_WRONLY)
         saved = os.dup(1)
         os.dup2(devnull, 1)
         try:
             with cf.ThreadPoolExecutor(max(1, min(threads, len(dirs)))) as pool:
-                built = list(pool.map(build, dirs))
+                members = list(pool.map(lambda d: BatchMember(d, R), dirs))
         finally:
             os.dup2(saved, 1)
             os.close(devnull)
             os.close(saved)
-    sizes = [m.n for m, _, _ in built]
-    bound = max(wl.gershgorin_bound(a, m.max_neighbor) for m, a, _ in built)
+    sizes = [b.n for b in members]
+    bound = max(wl.lattice_gershgorin_bound(b.lattice, b.potential) for b in members)
     if not bound < emax:
         raise RuntimeError(f"energy_max {emax} does not exceed the spectral bound {bound:.3f} "
                            f"of the batch")
     starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     n = int(starts[-1])
-    K = max(m.max_neighbor for m, _, _ in built)
-    arrays = {"neighbor_number": np.concatenate([a["neighbor_number"] for _, a, _ in built]),
-              "potential": np.concatenate([a["potential"] for _, a, _ in built])}
-    nl, rest = [], {k: [] for k in ("hopping_real", "hopping_imag", "xx")}
-    for k, (m, a, _) in enumerate(built):
-        assert m.max_neighbor == K
-        valid = np.arange(K)[None, :] < a["neighbor_number"][:, None]
-        nl.append(np.where(valid, a["neighbor_list"].reshape(m.n, K) + starts[k], 0).reshape(-1))
-        for key in rest:
-            rest[key].append(a[key])
-    arrays["neighbor_list"] = np.concatenate(nl).astype(np.int32)
-    for key in rest:
-        arrays[key] = np.concatenate(rest[key])
-    re = np.concatenate([np.array([v[0] for v in vecs]) for _, _, vecs in built], axis=1)
-    im = np.concatenate([np.array([v[1] for v in vecs]) for _, _, vecs in built], axis=1)
+    potential = np.concatenate([b.potential for b in members])
+    re = np.concatenate([np.array([v[0] for v in b.vecs]) for b in members], axis=1)
+    im = np.concatenate([np.array([v[1] for v in b.vecs]) for b in members], axis=1)
     build_s = time.perf_counter() - t0
     log(f"batch: {realisations} realisations, {min(sizes)}..{max(sizes)} sites, {n} in all, "
-        f"built in {build_s:.1f} s; spectral bound {bound:.2f} < E_max {emax}")
-    return built, arrays, starts, n, K, re, im, build_s
+        f"host models in {build_s:.1f} s; spectral bound {bound:.2f} < E_max {emax}")
+    return members, potential, starts, n, re, im, build_s
+
+
+def batch_hamiltonian(ctx, members, potential, emax):
+    """The segmented Hamiltonian of the batch, expanded on the device."""
+    from gpuqt_amd import native
+    return native.Hamiltonian.from_lattices(ctx, [b.lattice for b in members],
+                                            [b.site_index for b in members], potential, emax)
 
 
 def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.BATCH_EMAX,
@@ -429,12 +434,15 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.B
     realisations side by side in one segmented Hamiltonian: one launch per
     recursion step for the whole batch, moment sets replayed as hipGraphs."""
     from gpuqt_amd import hostlib, native
-    built, arrays, starts, n, K, re, im, build_s = build_batch(realisations, R, M, n_steps, dt,
-                                                               emax, threads)
-    sizes = [m.n for m, _, _ in built]
-    H = native.Hamiltonian.from_arrays(ctx, arrays, n, K, emax,
-                                       segment_starts=starts.astype(np.int32))
-    del arrays
+    members, potential, starts, n, re, im, build_s = build_batch(realisations, R, M, n_steps,
+                                                                 dt, emax, threads)
+    sizes = [b.n for b in members]
+    t0 = time.perf_counter()
+    H = batch_hamiltonian(ctx, members, potential, emax)
+    ctx.sync()
+    ham_s = time.perf_counter() - t0
+    log(f"batch: segmented Hamiltonian expanded on the device in {ham_s:.2f} s")
+    del potential
     ts = np.full(n_steps, dt * emax)
     # untimed: tuning and graph capture on a short run of the same shape
     H.block_vac(re, im, M, ts[:1])
@@ -467,13 +475,14 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.B
         "realisations": realisations, "sites": n, "R": R,
         "value": updates / elapsed, "unit": "site-updates/s",
         "moment_site_updates_per_s": moment_updates / elapsed,
-        "seconds": {"vac": t_vac, "msd": t_msd, "build": build_s},
+        "seconds": {"vac": t_vac, "msd": t_msd, "host_models": build_s,
+                    "hamiltonian_on_device": ham_s},
         "launches": "one per recursion step for all realisations (segmented Hamiltonian); "
                     "moment sets replayed from captured hipGraphs",
     }
     # counted HBM traffic of the batch's kernels over their compulsory bytes
     # (profiled on the first 8 realisations, bench.py --probe)
-    n8 = sum(m.n for m, _, _ in built[:8]) if len(built) >= 8 else None
+    n8 = sum(b.n for b in members[:8]) if len(members) >= 8 else None
     if n8 == 8973112 and R == 8:
         ratios = {}
         for kname, per_site in (("k_block_step<1,3,2,true,3,false,false>", 400.0),
@@ -488,13 +497,13 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.B
         # vector 0 of every realisation against the FP64 single-vector path
         t_chk = time.perf_counter()
         per = []
-        for k, (m, a, vecs) in enumerate(built):
-            H1 = native.Hamiltonian.from_arrays(ctx, a, m.n, K, emax)
-            v64, m64 = fp64_vac_msd(H1, vecs[0][0], vecs[0][1], M, ts)
+        for k, b in enumerate(members):
+            H1 = native.Hamiltonian.from_lattices(ctx, [b.lattice], [b.site_index], b.potential,
+                                                  emax)
+            v64, m64 = fp64_vac_msd(H1, b.vecs[0][0], b.vecs[0][1], M, ts)
             H1.close()
-            energies, _, _ = m.inputs()
-            row = lambda mu: hostlib.correlation_row(mu, energies, emax, m.volume)
-            e = {"realisation": k, "sites": m.n}
+            row = lambda mu: hostlib.correlation_row(mu, b.energies, emax, b.volume)
+            e = {"realisation": k, "sites": b.n}
             for name, got, want in (("vac", vac[:, k, 0], v64), ("msd", msd[:, k, 0], m64)):
                 g = np.array([row(x) for x in got])
                 w = np.array([row(x) for x in want])
@@ -506,8 +515,8 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.B
                     max(np.abs(a_ - b_).max() / np.abs(b_).max()
                         for a_, b_ in zip(sg, sw) if np.abs(b_).max() > 0))
             per.append(e)
-            if k % 8 == 7 or k == len(built) - 1:
-                log(f"batch FP64 check: {k + 1} of {len(built)} realisations")
+            if k % 8 == 7 or k == len(members) - 1:
+                log(f"batch FP64 check: {k + 1} of {len(members)} realisations")
         keys = ("vac_rows", "msd_rows", "sigma_vac", "sigma_msd")
         res["error_vs_f64"] = {
             "what": "normwise error of every VAC / MSD row and sigma_VAC / sigma_MSD row of "
@@ -573,11 +582,9 @@ def probe(ctx, H, rv, args, moments=10):
     if args.batch:
         # the uneven batch's kernels (pair and self block steps, evolution
         # terms) on its first 8 realisations, one time step of VAC and MSD
-        built, arrays, starts, n, K, re, im, _ = build_batch(8, 8, moments, 1, wl.BATCH_DT,
-                                                             wl.BATCH_EMAX)
-        Hb = native.Hamiltonian.from_arrays(ctx, arrays, n, K, wl.BATCH_EMAX,
-                                            segment_starts=starts.astype(np.int32))
-        del arrays
+        members, potential, starts, n, re, im, _ = build_batch(8, 8, moments, 1, wl.BATCH_DT,
+                                                               wl.BATCH_EMAX)
+        Hb = batch_hamiltonian(ctx, members, potential, wl.BATCH_EMAX)
         ts = np.full(1, wl.BATCH_DT * wl.BATCH_EMAX)
         Hb.block_vac(re, im, moments, ts)
         Hb.block_msd(re, im, moments, ts)

@@ -532,18 +532,15 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
   return LSQT_OK;
 }
 
-extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
-                                       const double* potential, double energy_max,
-                                       lsqt_ham** out)
+namespace {
+
+// checks one lattice description and its vacancy map; n_pristine and
+// n_sites (after vacancies) on success
+int check_lattice(const lsqt_lattice* L, const int* site_index, int* n_pristine, int* n_sites)
 {
-  if (!ctx || !L || !potential || !out)
-    return set_error(LSQT_ERR_INVALID, "null argument");
-  *out = nullptr;
   if (!L->orbital_x || !L->hop_count || !L->hop_cell || !L->hop_orbital || !L->hop_re ||
       !L->hop_im)
     return set_error(LSQT_ERR_INVALID, "null lattice table");
-  if (!(energy_max > 0.0))
-    return set_error(LSQT_ERR_INVALID, "energy_max must be > 0");
   const int No = L->orbitals, K = L->max_neighbor;
   if (No <= 0 || K <= 0 || L->transport < 0 || L->transport > 2)
     return set_error(LSQT_ERR_INVALID, "orbitals, max_neighbor or transport direction invalid");
@@ -555,7 +552,6 @@ extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
   }
   if (nll > INT_MAX)
     return set_error(LSQT_ERR_INVALID, "%lld sites exceed the int32 site index", nll);
-  const int n = (int)nll;
   for (int m = 0; m < No; ++m) {
     if (L->hop_count[m] < 0 || L->hop_count[m] > K)
       return set_error(LSQT_ERR_INVALID, "orbital %d has %d hoppings (max %d)", m,
@@ -565,33 +561,116 @@ extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
       if (L->hop_orbital[k] < 0 || L->hop_orbital[k] >= No)
         return set_error(LSQT_ERR_INVALID, "hopping %d of orbital %d: orbital %d out of range",
                          i, m, L->hop_orbital[k]);
-      for (int d = 0; d < 3; ++d)
-        if (L->pbc[d] && std::abs(L->hop_cell[3 * k + d]) > L->cells[d])
-          return set_error(LSQT_ERR_INVALID, "hopping %d of orbital %d reaches beyond one "
-                           "period in direction %d", i, m, d);
+      // a hopping that some site keeps (within the cell count of every open
+      // direction) may wrap once across a periodic one
+      bool kept = true;
+      int beyond = -1;
+      for (int d = 0; d < 3; ++d) {
+        const int c = std::abs(L->hop_cell[3 * k + d]);
+        if (!L->pbc[d] && c >= L->cells[d])
+          kept = false;
+        if (L->pbc[d] && c > L->cells[d])
+          beyond = d;
+      }
+      if (kept && beyond >= 0)
+        return set_error(LSQT_ERR_INVALID, "hopping %d of orbital %d reaches beyond one "
+                         "period in direction %d", i, m, beyond);
     }
   }
-  const int ib = index_bits_for(n);
-  const long long classes = (long long)No * K;
-  if (ib >= 32 || classes >= (1ll << (32 - ib)) || classes > kPackedMaxClasses)
-    return set_error(LSQT_ERR_UNSUPPORTED, "%lld hopping classes do not fit the packed words",
-                     classes);
-  LSQT_TRY(ctx_activate(ctx));
-  // class table: value and transport offset of every (orbital, hopping)
-  std::vector<double2> tab((size_t)classes, make_double2(0.0, 0.0));
-  std::vector<double> tab_xx((size_t)classes, 0.0);
-  std::vector<int4> hop((size_t)classes, make_int4(0, 0, 0, 0));
-  const int t = L->transport;
-  for (int m = 0; m < No; ++m)
-    for (int i = 0; i < L->hop_count[m]; ++i) {
-      const int k = m * K + i;
-      tab[k] = make_double2(L->hop_re[k], L->hop_im[k]);
-      double x12 = L->lattice_constant[t] * L->hop_cell[3 * k + t];
-      x12 += L->orbital_x[L->hop_orbital[k]] - L->orbital_x[m];
-      tab_xx[k] = x12;
-      hop[k] = make_int4(L->hop_cell[3 * k], L->hop_cell[3 * k + 1], L->hop_cell[3 * k + 2],
-                         L->hop_orbital[k]);
+  const int np = (int)nll;
+  int kept = np;
+  if (site_index) {
+    // a map onto 0 .. kept - 1, each index once, -1 for the vacancies
+    std::vector<char> seen((size_t)np, 0);
+    kept = 0;
+    for (int p = 0; p < np; ++p) {
+      const int v = site_index[p];
+      if (v < 0)
+        continue;
+      if (v >= np || seen[(size_t)v])
+        return set_error(LSQT_ERR_INVALID, "site index %d of pristine site %d is out of range "
+                         "or repeated", v, p);
+      seen[(size_t)v] = 1;
+      ++kept;
     }
+    for (int v = 0; v < kept; ++v)
+      if (!seen[(size_t)v])
+        return set_error(LSQT_ERR_INVALID, "the site index leaves site %d unused", v);
+  }
+  if (kept == 0)
+    return set_error(LSQT_ERR_INVALID, "a lattice without sites");
+  *n_pristine = np;
+  *n_sites = kept;
+  return LSQT_OK;
+}
+
+}  // namespace
+
+extern "C" int lsqt_ham_create_lattices(lsqt_ctx* ctx, int segments, const lsqt_lattice* lats,
+                                        const int* const* site_index, const double* potential,
+                                        double energy_max, lsqt_ham** out)
+{
+  if (!ctx || !lats || !potential || !out || segments < 1)
+    return set_error(LSQT_ERR_INVALID, "null argument or no lattice");
+  *out = nullptr;
+  if (!(energy_max > 0.0))
+    return set_error(LSQT_ERR_INVALID, "energy_max must be > 0");
+  std::vector<int> n_pristine((size_t)segments), starts((size_t)segments + 1, 0);
+  int K = 0;
+  long long total = 0;
+  for (int k = 0; k < segments; ++k) {
+    int np = 0, nk = 0;
+    LSQT_TRY(check_lattice(&lats[k], site_index ? site_index[k] : nullptr, &np, &nk));
+    n_pristine[k] = np;
+    starts[k] = (int)total;
+    total += nk;
+    if (total > INT_MAX)
+      return set_error(LSQT_ERR_INVALID, "%lld sites exceed the int32 site index", total);
+    K = std::max(K, lats[k].max_neighbor);
+  }
+  const int n = (int)total;
+  starts[segments] = n;
+  const int ib = index_bits_for(n);
+  // classes: the distinct (re, im, xx) of every (lattice, orbital, hopping),
+  // xx = a[transport] * offset[transport] + x[orbital'] - x[m] in the Model's
+  // operation order (src/model_lattice.cu:373-375)
+  std::unordered_map<TripleKey, int, TripleHash> ids;
+  std::vector<double2> tab;
+  std::vector<double> tab_xx;
+  std::vector<std::vector<int>> cls((size_t)segments);
+  std::vector<std::vector<int4>> hop((size_t)segments);
+  const long long max_classes =
+    std::min<long long>(ib >= 32 ? 0 : (1ll << (32 - ib)) - 1, (long long)kPackedMaxClasses);
+  for (int s = 0; s < segments; ++s) {
+    const lsqt_lattice* L = &lats[s];
+    const int No = L->orbitals, Ks = L->max_neighbor, t = L->transport;
+    cls[s].assign((size_t)No * Ks, 0);
+    hop[s].assign((size_t)No * Ks, make_int4(0, 0, 0, 0));
+    for (int m = 0; m < No; ++m)
+      for (int i = 0; i < L->hop_count[m]; ++i) {
+        const int k = m * Ks + i;
+        double x12 = L->lattice_constant[t] * L->hop_cell[3 * k + t];
+        x12 += L->orbital_x[L->hop_orbital[k]] - L->orbital_x[m];
+        TripleKey key{bits_of(L->hop_re[k]), bits_of(L->hop_im[k]), bits_of(x12)};
+        auto it = ids.find(key);
+        if (it == ids.end()) {
+          if ((long long)tab.size() >= max_classes)
+            return set_error(LSQT_ERR_UNSUPPORTED, "%zu hopping classes do not fit the packed "
+                             "words of %d sites", tab.size() + 1, n);
+          it = ids.emplace(key, (int)tab.size()).first;
+          tab.push_back(make_double2(L->hop_re[k], L->hop_im[k]));
+          tab_xx.push_back(x12);
+        }
+        cls[s][k] = it->second;
+        hop[s][k] = make_int4(L->hop_cell[3 * k], L->hop_cell[3 * k + 1], L->hop_cell[3 * k + 2],
+                              L->hop_orbital[k]);
+      }
+  }
+  if (tab.empty()) {  // no hoppings at all: a table of one unused class
+    tab.push_back(make_double2(0.0, 0.0));
+    tab_xx.push_back(0.0);
+  }
+  LSQT_TRY(ctx_activate(ctx));
   const int stride = pair_stride(n);
   lsqt_ham* h = new lsqt_ham();
   h->ctx = ctx;
@@ -605,19 +684,12 @@ extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
   d.layout = tab.size() <= (size_t)kLdsTableMax ? kPackedLds : kPackedGlobal;
   d.idx_bits = ib;
   d.idx_mask = (1u << ib) - 1u;
-  d.ncls = (int)classes;
-  const int* dev_count = nullptr;
-  const int4* dev_hop = nullptr;
-  std::vector<int> counts(L->hop_count, L->hop_count + No);
+  d.ncls = (int)tab.size();
   int rc = upload(h, &d.pot, potential, (size_t)n);
   if (rc == LSQT_OK)
     rc = upload(h, &d.tab, tab.data(), tab.size());
   if (rc == LSQT_OK)
     rc = upload(h, &d.tab_xx, tab_xx.data(), tab_xx.size());
-  if (rc == LSQT_OK)
-    rc = upload(h, &dev_count, counts.data(), counts.size());
-  if (rc == LSQT_OK)
-    rc = upload(h, &dev_hop, hop.data(), hop.size());
   uint32_t* words = nullptr;
   if (rc == LSQT_OK) {
     void* p = nullptr;
@@ -630,24 +702,46 @@ extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
     }
   }
   if (rc == LSQT_OK) {
+    // the padding columns past n stay marked empty
+    hipError_t e = hipMemsetD32Async((hipDeviceptr_t)words, (int)kEmptySlot,
+                                     (size_t)stride * K, ctx->stream);
+    if (e != hipSuccess)
+      rc = set_error(LSQT_ERR_HIP, "clearing the lattice words: %s", hipGetErrorString(e));
+  }
+  for (int s = 0; s < segments && rc == LSQT_OK; ++s) {
+    const lsqt_lattice* L = &lats[s];
+    const int* dev_count = nullptr;
+    const int4* dev_hop = nullptr;
+    const int* dev_cls = nullptr;
+    const int* dev_index = nullptr;
+    rc = upload(h, &dev_count, L->hop_count, (size_t)L->orbitals);
+    if (rc == LSQT_OK)
+      rc = upload(h, &dev_hop, hop[s].data(), hop[s].size());
+    if (rc == LSQT_OK)
+      rc = upload(h, &dev_cls, cls[s].data(), cls[s].size());
+    if (rc == LSQT_OK && site_index && site_index[s])
+      rc = upload(h, &dev_index, site_index[s], (size_t)n_pristine[s]);
+    if (rc != LSQT_OK)
+      break;
     LatticeDev ld{};
     for (int q = 0; q < 3; ++q) {
       ld.cells[q] = L->cells[q];
       ld.pbc[q] = L->pbc[q];
     }
-    ld.orbitals = No;
-    ld.K = K;
+    ld.orbitals = L->orbitals;
+    ld.K = L->max_neighbor;
     ld.hop_count = dev_count;
     ld.hop = dev_hop;
-    // the padding columns past n stay marked empty
-    hipError_t e = hipMemsetD32Async((hipDeviceptr_t)words, (int)kEmptySlot,
-                                     (size_t)stride * K, ctx->stream);
-    if (e == hipSuccess) {
-      launch_lattice_words(ld, words, stride, ib, n, ctx->stream);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess)
-      e = hipStreamSynchronize(ctx->stream);
+    ld.cls = dev_cls;
+    ld.site_index = dev_index;
+    ld.offset = starts[s];
+    launch_lattice_words(ld, words, stride, ib, K, n_pristine[s], ctx->stream);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      rc = set_error(LSQT_ERR_HIP, "building the lattice words: %s", hipGetErrorString(e));
+  }
+  if (rc == LSQT_OK) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess)
       rc = set_error(LSQT_ERR_HIP, "building the lattice words: %s", hipGetErrorString(e));
     d.words = words;
@@ -656,12 +750,18 @@ extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
     lsqt_ham_destroy(h);
     return rc;
   }
-  // block tile schedules: slabs of one x cell, rows of one y cell
-  const long long slab = (long long)L->cells[1] * L->cells[2] * No;
-  const long long row = (long long)L->cells[2] * No;
-  if (L->cells[0] > 1 && slab >= 64 && row >= 8 && slab / row >= 4) {
-    h->slab = (int)slab;
-    h->row = (int)row;
+  if (segments > 1) {
+    h->seg_starts = starts;  // lattice tile schedules do not cross segments
+  } else if (!site_index || !site_index[0]) {
+    // block tile schedules of a pristine lattice: slabs of one x cell, rows
+    // of one y cell
+    const lsqt_lattice* L = &lats[0];
+    const long long slab = (long long)L->cells[1] * L->cells[2] * L->orbitals;
+    const long long row = (long long)L->cells[2] * L->orbitals;
+    if (L->cells[0] > 1 && slab >= 64 && row >= 8 && slab / row >= 4) {
+      h->slab = (int)slab;
+      h->row = (int)row;
+    }
   }
   detect_slot_pattern(h);
   h->host_pot.assign(potential, potential + n);
@@ -669,6 +769,15 @@ extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
   h->host_tab_xx = tab_xx;
   *out = h;
   return LSQT_OK;
+}
+
+extern "C" int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* L,
+                                       const double* potential, double energy_max,
+                                       lsqt_ham** out)
+{
+  if (!ctx || !L || !potential || !out)
+    return set_error(LSQT_ERR_INVALID, "null argument");
+  return lsqt_ham_create_lattices(ctx, 1, L, nullptr, potential, energy_max, out);
 }
 
 extern "C" int lsqt_ham_create_segmented(lsqt_ctx* ctx, int segments, const int* starts, int n,

@@ -270,9 +270,13 @@ struct TileMap {
   WindowMap win;
 };
 
-// A lattice model for device-side construction: cells[3] x orbitals sites,
-// K hopping slots per site; hop[m*K + i] = (dx, dy, dz, target orbital) of
-// hopping i of orbital m, hop_count[m] of them.
+// A lattice model for device-side construction: cells[3] x orbitals
+// pristine sites, K hopping slots per orbital in the tables; hop[m*K + i] =
+// (dx, dy, dz, target orbital) of hopping i of orbital m (hop_count[m] of
+// them), cls[m*K + i] its class in the Hamiltonian's table.  site_index
+// (null: the identity) maps a pristine site to its index after vacancies
+// were removed, -1 for a vacancy; offset is the lattice's first site in the
+// Hamiltonian (segmented Hamiltonians hold several lattices side by side).
 struct LatticeDev {
   int cells[3];
   int pbc[3];
@@ -280,6 +284,9 @@ struct LatticeDev {
   int K;
   const int* hop_count;
   const int4* hop;
+  const int* cls;
+  const int* site_index;
+  int offset;
 };
 
 inline Grid make_grid(int n, int spt)

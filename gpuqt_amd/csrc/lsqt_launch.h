@@ -57,8 +57,8 @@ void launch_cheb2x_32(const HamDev& h, const float2* s0, const float2* s0x, cons
                       int label, hipStream_t st, double bm_prev = 0.0, int label_prev = 0);
 void launch_zero32(float2* v, int n, hipStream_t st);
 // packed slot words of a lattice model (class m*K + i for hopping i of orbital m)
-void launch_lattice_words(const LatticeDev& L, uint32_t* words, int stride, int idx_bits, int n,
-                          hipStream_t st);
+void launch_lattice_words(const LatticeDev& L, uint32_t* words, int stride, int idx_bits,
+                          int Kmax, int n_pristine, hipStream_t st);
 void launch_to_f32(const double2* src, float2* dst, int n, hipStream_t st);
 void launch_to_f64(const float2* src, double2* dst, int n, hipStream_t st);
 void launch_cheb01_32(const float2* s1, double2* s, double b0, double b1, int direction, int n,

@@ -5,12 +5,46 @@
 #include "model.h"
 #include "vector.h"
 
+namespace lsqt_host {
+
+lsqt_lattice abi_lattice(const LatticeSpec& L)
+{
+  lsqt_lattice a{};
+  for (int d = 0; d < 3; ++d) {
+    a.cells[d] = L.N_cell[d];
+    a.pbc[d] = L.pbc[d];
+    a.lattice_constant[d] = L.lattice_constant[d];
+  }
+  a.transport = L.transport_direction;
+  a.orbitals = L.N_orbital;
+  a.orbital_x = L.x_cell.data();
+  a.max_neighbor = L.max_neighbor;
+  a.hop_count = L.number_of_hoppings.data();
+  a.hop_cell = L.hop_cell.data();
+  a.hop_orbital = L.hop_orbital.data();
+  a.hop_re = L.hop_re.data();
+  a.hop_im = L.hop_im.data();
+  return a;
+}
+
+}  // namespace lsqt_host
+
 Hamiltonian::Hamiltonian(Model& model)
 {
-  LSQT_CHECK(lsqt_ham_create(lsqt_host::default_context(), model.number_of_atoms,
-                             model.max_neighbor, model.neighbor_number, model.neighbor_list,
-                             model.potential, model.hopping_real, model.hopping_imag, model.xx,
-                             model.energy_max, LSQT_LAYOUT_AUTO, &h));
+  if (const LatticeSpec* L = model.lattice()) {
+    // a lattice model whose neighbour lists were not expanded on the host:
+    // expanded on the GPU (the same arrays, so the same results bit for bit)
+    const lsqt_lattice lat = lsqt_host::abi_lattice(*L);
+    const int* index = L->site_index.empty() ? nullptr : L->site_index.data();
+    LSQT_CHECK(lsqt_ham_create_lattices(lsqt_host::default_context(), 1, &lat,
+                                        index ? &index : nullptr, model.potential,
+                                        model.energy_max, &h));
+  } else {
+    LSQT_CHECK(lsqt_ham_create(lsqt_host::default_context(), model.number_of_atoms,
+                               model.max_neighbor, model.neighbor_number, model.neighbor_list,
+                               model.potential, model.hopping_real, model.hopping_imag,
+                               model.xx, model.energy_max, LSQT_LAYOUT_AUTO, &h));
+  }
   model.release_hamiltonian_arrays();
 }
 

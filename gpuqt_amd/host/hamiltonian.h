@@ -7,6 +7,7 @@
 
 class Vector;
 class Model;
+struct LatticeSpec;
 struct lsqt_ham;
 
 class Hamiltonian

@@ -9,12 +9,17 @@
 
 #include "general_io.h"
 #include "hamiltonian.h"
+#include "lsqt_hip.h"
 #include "lsqt.h"
 #include "lsqt_host.h"
 #include "model.h"
 #include "postprocess.h"
 #include "sigma.h"
 #include "vector.h"
+
+namespace lsqt_host {
+lsqt_lattice abi_lattice(const LatticeSpec& L);
+}
 
 struct lsqt_host_model {
   Model* model;
@@ -235,11 +240,17 @@ extern "C" void lsqt_host_set_output_precision(int digits)
 extern "C" int lsqt_host_model_load(const char* input_dir, lsqt_host_model** out, char* err,
                                     int err_len)
 {
+  return lsqt_host_model_load_ex(input_dir, 0, out, err, err_len);
+}
+
+extern "C" int lsqt_host_model_load_ex(const char* input_dir, int flags, lsqt_host_model** out,
+                                       char* err, int err_len)
+{
   if (!out)
     return 1;
   *out = nullptr;
   try {
-    Model* m = new Model(input_dir);
+    Model* m = new Model(input_dir, (flags & LSQT_HOST_LATTICE_ON_DEVICE) != 0);
     *out = new lsqt_host_model{m};
   } catch (const lsqt_host::InputError& e) {
     return report(e, err, err_len);
@@ -379,5 +390,37 @@ extern "C" int lsqt_host_model_moments(lsqt_host_model* h, const double* l_re,
   else
     find_moments_chebyshev(m, H, left, right, output);
   output.copy_to_host(mu_re, mu_im);
+  return 0;
+}
+
+extern "C" int lsqt_host_model_lattice(const lsqt_host_model* h, lsqt_lattice* lattice,
+                                       const int** site_index, int* n_pristine)
+{
+  if (!h || !lattice || !site_index || !n_pristine || !h->model->lattice())
+    return 1;
+  const LatticeSpec& L = *h->model->lattice();
+  *lattice = lsqt_host::abi_lattice(L);
+  *site_index = L.site_index.empty() ? nullptr : L.site_index.data();
+  *n_pristine = L.N_orbital * L.N_cell[0] * L.N_cell[1] * L.N_cell[2];
+  return 0;
+}
+
+extern "C" int lsqt_host_model_potential(const lsqt_host_model* h, double* potential)
+{
+  if (!h || !potential || !h->model->potential)
+    return 1;
+  std::memcpy(potential, h->model->potential, sizeof(double) * (size_t)h->model->number_of_atoms);
+  return 0;
+}
+
+extern "C" int lsqt_host_model_expand_lattice(lsqt_host_model* h)
+{
+  if (!h)
+    return 1;
+  try {
+    h->model->expand_lattice_arrays();
+  } catch (const lsqt_host::InputError&) {
+    return 1;
+  }
   return 0;
 }

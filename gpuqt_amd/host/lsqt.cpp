@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cstdlib>
 #include <iostream>
 #include <memory>
 #include <vector>
@@ -54,10 +55,22 @@ void timed(const char* what, F&& f)
 // one random vector at a time (src/lsqt.cu:140-150)
 void lsqt_vectors(Model& model, Hamiltonian& H, Vector& random_state);
 
+namespace lsqt_host {
+lsqt_lattice abi_lattice(const LatticeSpec& L);
+
+// Lattice models are expanded on the GPU unless LSQT_DEVICE_LATTICE=0 (the
+// same Hamiltonian either way)
+bool lattice_on_device()
+{
+  const char* e = getenv("LSQT_DEVICE_LATTICE");
+  return !(e && e[0] == '0');
+}
+}  // namespace lsqt_host
+
 void lsqt(std::string input_directory)
 {
   lsqt_host::set_path(lsqt_host::path_from_environment());
-  Model model(input_directory);
+  Model model(input_directory, lsqt_host::lattice_on_device());
   Hamiltonian H(model);
   Vector random_state(model.number_of_atoms);
   const bool block = lsqt_host::path() == lsqt_host::Path::block ||
@@ -143,9 +156,37 @@ std::vector<double> batch_key(const Model& m)
   return k;
 }
 
-// one segmented Hamiltonian for the members' models (arrays released after)
+// one segmented Hamiltonian for the members' models (arrays released after):
+// expanded on the GPU when every member is a lattice model that kept its
+// description, else from the host arrays (expanded for those that kept it)
 lsqt_ham* segmented_hamiltonian(std::vector<lsqt_host::BlockMember>& members, int n_total)
 {
+  bool all_lattices = true;
+  for (auto& mb : members)
+    all_lattices = all_lattices && mb.model->lattice() != nullptr;
+  if (all_lattices) {
+    std::vector<lsqt_lattice> lats;
+    std::vector<const int*> index;
+    std::vector<real> pot((size_t)n_total);
+    bool any_index = false;
+    for (auto& mb : members) {
+      const LatticeSpec& L = *mb.model->lattice();
+      lats.push_back(lsqt_host::abi_lattice(L));
+      index.push_back(L.site_index.empty() ? nullptr : L.site_index.data());
+      any_index = any_index || !L.site_index.empty();
+      std::copy(mb.model->potential, mb.model->potential + mb.model->number_of_atoms,
+                pot.begin() + mb.start);
+    }
+    lsqt_ham* h = nullptr;
+    LSQT_CHECK(lsqt_ham_create_lattices(lsqt_host::default_context(), (int)members.size(),
+                                        lats.data(), any_index ? index.data() : nullptr,
+                                        pot.data(), members[0].model->energy_max, &h));
+    for (auto& mb : members)
+      mb.model->release_hamiltonian_arrays();
+    return h;
+  }
+  for (auto& mb : members)
+    mb.model->expand_lattice_arrays();
   int K = 0;
   for (auto& mb : members)
     K = std::max(K, mb.model->max_neighbor);
@@ -203,7 +244,7 @@ void lsqt_batch(const std::vector<std::string>& directories)
   std::vector<std::unique_ptr<Model>> models;
   std::vector<std::string> alone;
   for (const std::string& d : directories) {
-    std::unique_ptr<Model> m(new Model(d));
+    std::unique_ptr<Model> m(new Model(d, lsqt_host::lattice_on_device()));
     if (m->calculate_spin || m->calculate_ldos || m->number_of_random_vectors <= 0 ||
         m->number_of_random_vectors > 256)
       alone.push_back(d);  // run by itself below

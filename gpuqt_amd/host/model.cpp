@@ -64,7 +64,10 @@ T* new_zeroed(size_t n)
 
 }  // namespace
 
-Model::Model(std::string input_dir) : generator(12345678u)
+Model::Model(std::string input_dir) : Model(input_dir, false) {}
+
+Model::Model(std::string input_dir, bool lattice_on_device)
+  : generator(12345678u), defer_lattice(lattice_on_device)
 {
   this->input_dir = input_dir;
   try {
@@ -96,8 +99,14 @@ Model::~Model()
   delete[] time_step;
 }
 
+const LatticeSpec* Model::lattice() const
+{
+  return spec.get();
+}
+
 void Model::release_hamiltonian_arrays()
 {
+  spec.reset();
   delete[] neighbor_number;
   delete[] neighbor_list;
   delete[] potential;
@@ -646,78 +655,153 @@ void Model::initialize_lattice_model()
   number_of_atoms = (int)total;
   std::cout << "- total number of orbitals = " << number_of_atoms << std::endl;
   number_of_pairs = number_of_atoms * max_neighbor;
-  neighbor_number = new_zeroed<int>((size_t)number_of_atoms);
-  neighbor_list = new_zeroed<int>((size_t)number_of_pairs);
-  hopping_real = new_zeroed<real>((size_t)number_of_pairs);
-  hopping_imag = new_zeroed<real>((size_t)number_of_pairs);
-  xx = new_zeroed<real>((size_t)number_of_pairs);
   potential = new_zeroed<real>((size_t)number_of_atoms);
 
+  std::unique_ptr<LatticeSpec> L(new LatticeSpec());
+  for (int d = 0; d < 3; ++d) {
+    L->N_cell[d] = N_cell[d];
+    L->pbc[d] = pbc[d];
+    L->lattice_constant[d] = lattice_constant[d];
+  }
+  L->transport_direction = transport_direction;
+  L->N_orbital = N_orbital;
+  L->max_neighbor = max_neighbor;
+  L->x_cell.assign((size_t)N_orbital, 0.0);
+  std::vector<real> y_cell(N_orbital), z_cell(N_orbital);
+  const int per_cell = N_orbital * max_neighbor;
+  L->hop_cell.assign((size_t)per_cell * 3, 0);
+  L->hop_orbital.assign((size_t)per_cell, 0);
+  L->hop_re.assign((size_t)per_cell, 0.0);
+  L->hop_im.assign((size_t)per_cell, 0.0);
+  std::cout << std::endl << "\torbital\tx\ty\tz" << std::endl;
+  for (int n = 0; n < N_orbital; ++n) {
+    input >> L->x_cell[n] >> y_cell[n] >> z_cell[n];
+    std::cout << "\t" << n << "\t" << L->x_cell[n] << "\t" << y_cell[n] << "\t" << z_cell[n]
+              << std::endl;
+  }
+  L->number_of_hoppings.assign((size_t)N_orbital, 0);
+  for (int m = 0; m < N_orbital; m++) {
+    int& count = L->number_of_hoppings[m];
+    input >> count;
+    std::cout << std::endl
+              << "- number of hoppings for orbital " << m << " = " << count << std::endl;
+    if (count < 0 || count > max_neighbor)
+      throw InputError("Error: orbital " + std::to_string(m) +
+                       " has more hoppings than the maximum");
+    for (int n = 0; n < count; ++n) {
+      int q = m * max_neighbor + n;
+      int* c = &L->hop_cell[3 * (size_t)q];
+      input >> c[0] >> c[1] >> c[2] >> L->hop_orbital[q] >> L->hop_re[q] >> L->hop_im[q];
+      std::cout << "\tH(0,0,0," << m << "; " << c[0] << "," << c[1] << "," << c[2] << ","
+                << L->hop_orbital[q] << ") = " << L->hop_re[q] << " + i " << L->hop_im[q]
+                << std::endl;
+      if (L->hop_orbital[q] < 0 || L->hop_orbital[q] >= N_orbital)
+        throw InputError("Error: hopping to a non-existent orbital");
+    }
+  }
+
+  // impurity positions, indexed by the pristine lattice
   std::vector<real> x, y, z;
   if (has_charge) {
     x.resize((size_t)number_of_atoms);
     y.resize((size_t)number_of_atoms);
     z.resize((size_t)number_of_atoms);
-  }
-  std::vector<real> x_cell(N_orbital), y_cell(N_orbital), z_cell(N_orbital);
-  const int per_cell = N_orbital * max_neighbor;
-  std::vector<int> hop_nx(per_cell, 0), hop_ny(per_cell, 0), hop_nz(per_cell, 0),
-    hop_orb(per_cell, 0);
-  std::vector<real> hop_re(per_cell, 0), hop_im(per_cell, 0);
-  std::cout << std::endl << "\torbital\tx\ty\tz" << std::endl;
-  for (int n = 0; n < N_orbital; ++n) {
-    input >> x_cell[n] >> y_cell[n] >> z_cell[n];
-    std::cout << "\t" << n << "\t" << x_cell[n] << "\t" << y_cell[n] << "\t" << z_cell[n]
-              << std::endl;
-  }
-  std::vector<int> number_of_hoppings(N_orbital, 0);
-  for (int m = 0; m < N_orbital; m++) {
-    input >> number_of_hoppings[m];
-    std::cout << std::endl
-              << "- number of hoppings for orbital " << m << " = " << number_of_hoppings[m]
-              << std::endl;
-    if (number_of_hoppings[m] < 0 || number_of_hoppings[m] > max_neighbor)
-      throw InputError("Error: orbital " + std::to_string(m) +
-                       " has more hoppings than the maximum");
-    for (int n = 0; n < number_of_hoppings[m]; ++n) {
-      int q = m * max_neighbor + n;
-      input >> hop_nx[q] >> hop_ny[q] >> hop_nz[q] >> hop_orb[q] >> hop_re[q] >> hop_im[q];
-      std::cout << "\tH(0,0,0," << m << "; " << hop_nx[q] << "," << hop_ny[q] << ","
-                << hop_nz[q] << "," << hop_orb[q] << ") = " << hop_re[q] << " + i " << hop_im[q]
-                << std::endl;
-      if (hop_orb[q] < 0 || hop_orb[q] >= N_orbital)
-        throw InputError("Error: hopping to a non-existent orbital");
-    }
+    for (int nx1 = 0; nx1 < N_cell[0]; ++nx1)
+      for (int ny1 = 0; ny1 < N_cell[1]; ++ny1)
+        for (int nz1 = 0; nz1 < N_cell[2]; ++nz1)
+          for (int m = 0; m < N_orbital; ++m) {
+            int n1 = find_index(nx1, ny1, nz1, N_cell[0], N_cell[1], N_cell[2], m, N_orbital);
+            x[n1] = L->x_cell[m] + lattice_constant[0] * nx1;
+            y[n1] = y_cell[m] + lattice_constant[1] * ny1;
+            z[n1] = z_cell[m] + lattice_constant[2] * nz1;
+          }
   }
 
-  const int* hop_cell[3] = {hop_nx.data(), hop_ny.data(), hop_nz.data()};
+  if (!defer_lattice) {
+    expand_lattice(*L);
+  } else {
+    // the condition under which the expansion finds a neighbour index out
+    // of range: a hopping that some site keeps (every open direction's
+    // offset below the cell count) reaching further than one period of a
+    // periodic direction (one wrap, src/model_lattice.cu:148-163)
+    for (int m = 0; m < N_orbital; ++m)
+      for (int i = 0; i < L->number_of_hoppings[m]; ++i) {
+        const int* c = &L->hop_cell[3 * (size_t)(m * max_neighbor + i)];
+        bool kept = true, beyond = false;
+        for (int d = 0; d < 3; ++d) {
+          if (!pbc[d] && std::abs(c[d]) >= N_cell[d])
+            kept = false;
+          if (pbc[d] && std::abs(c[d]) > N_cell[d])
+            beyond = true;
+        }
+        if (kept && beyond)
+          throw InputError("Error: a hopping leaves the lattice by more than one cell");
+      }
+  }
+
+  if (has_vacancy_disorder) {
+    std::vector<int> new_index;
+    draw_vacancies(new_index);
+    if (!defer_lattice) {
+      remove_vacancies(new_index);
+    } else {
+      L->site_index = new_index;
+      number_of_atoms -= number_of_vacancies;
+      number_of_pairs = number_of_atoms * max_neighbor;
+    }
+  }
+  if (has_anderson)
+    add_anderson_disorder();
+  if (has_charge)
+    add_charged_impurities(pbc, box_length, x, y, z);
+  if (defer_lattice)
+    spec = std::move(L);
+  print_finished_reading(filename);
+}
+
+// the neighbour arrays of the pristine lattice (src/model_lattice.cu:342-386):
+// hoppings in file order, those leaving an open boundary skipped
+void Model::expand_lattice(const LatticeSpec& L)
+{
+  const int* N_cell = L.N_cell;
+  const int N_orbital = L.N_orbital, K = L.max_neighbor;
+  const int N0 = N_orbital * N_cell[0] * N_cell[1] * N_cell[2];
+  number_of_atoms = N0;
+  number_of_pairs = N0 * K;
+  delete[] neighbor_number;
+  delete[] neighbor_list;
+  delete[] hopping_real;
+  delete[] hopping_imag;
+  delete[] xx;
+  neighbor_number = new_zeroed<int>((size_t)N0);
+  neighbor_list = new_zeroed<int>((size_t)number_of_pairs);
+  hopping_real = new_zeroed<real>((size_t)number_of_pairs);
+  hopping_imag = new_zeroed<real>((size_t)number_of_pairs);
+  xx = new_zeroed<real>((size_t)number_of_pairs);
+  const int t = L.transport_direction;
   for (int nx1 = 0; nx1 < N_cell[0]; ++nx1)
     for (int ny1 = 0; ny1 < N_cell[1]; ++ny1)
       for (int nz1 = 0; nz1 < N_cell[2]; ++nz1)
         for (int m = 0; m < N_orbital; ++m) {
           int n1 = find_index(nx1, ny1, nz1, N_cell[0], N_cell[1], N_cell[2], m, N_orbital);
-          if (has_charge) {
-            x[n1] = x_cell[m] + lattice_constant[0] * nx1;
-            y[n1] = y_cell[m] + lattice_constant[1] * ny1;
-            z[n1] = z_cell[m] + lattice_constant[2] * nz1;
-          }
           int count = 0;
-          for (int i = 0; i < number_of_hoppings[m]; ++i) {
-            size_t slot = (size_t)n1 * max_neighbor + count;
-            int k = m * max_neighbor + i;
-            int nx2 = hop_nx[k] + nx1, ny2 = hop_ny[k] + ny1, nz2 = hop_nz[k] + nz1;
-            bool skip_x = !pbc[0] && (nx2 < 0 || nx2 >= N_cell[0]);
-            bool skip_y = !pbc[1] && (ny2 < 0 || ny2 >= N_cell[1]);
-            bool skip_z = !pbc[2] && (nz2 < 0 || nz2 >= N_cell[2]);
+          for (int i = 0; i < L.number_of_hoppings[m]; ++i) {
+            size_t slot = (size_t)n1 * K + count;
+            int k = m * K + i;
+            const int* c = &L.hop_cell[3 * (size_t)k];
+            int nx2 = c[0] + nx1, ny2 = c[1] + ny1, nz2 = c[2] + nz1;
+            bool skip_x = !L.pbc[0] && (nx2 < 0 || nx2 >= N_cell[0]);
+            bool skip_y = !L.pbc[1] && (ny2 < 0 || ny2 >= N_cell[1]);
+            bool skip_z = !L.pbc[2] && (nz2 < 0 || nz2 >= N_cell[2]);
             if (skip_x || skip_y || skip_z)
               continue;
-            neighbor_list[slot] =
-              find_index(nx2, ny2, nz2, N_cell[0], N_cell[1], N_cell[2], hop_orb[k], N_orbital);
-            real x12 = lattice_constant[transport_direction] * hop_cell[transport_direction][k];
-            x12 += x_cell[hop_orb[k]] - x_cell[m];
+            neighbor_list[slot] = find_index(nx2, ny2, nz2, N_cell[0], N_cell[1], N_cell[2],
+                                             L.hop_orbital[k], N_orbital);
+            real x12 = L.lattice_constant[t] * c[t];
+            x12 += L.x_cell[L.hop_orbital[k]] - L.x_cell[m];
             xx[slot] = x12;
-            hopping_real[slot] = hop_re[k];
-            hopping_imag[slot] = hop_im[k];
+            hopping_real[slot] = L.hop_re[k];
+            hopping_imag[slot] = L.hop_im[k];
             ++count;
           }
           neighbor_number[n1] = count;
@@ -725,32 +809,48 @@ void Model::initialize_lattice_model()
   for (size_t q = 0; q < (size_t)number_of_pairs; ++q)
     if (neighbor_list[q] < 0 || neighbor_list[q] >= number_of_atoms)
       throw InputError("Error: a hopping leaves the lattice by more than one cell");
-
-  if (has_vacancy_disorder)
-    add_vacancies();
-  if (has_anderson)
-    add_anderson_disorder();
-  if (has_charge)
-    add_charged_impurities(pbc, box_length, x, y, z);
-  print_finished_reading(filename);
 }
 
-// src/model_lattice.cu:70-146
-void Model::add_vacancies()
+void Model::expand_lattice_arrays()
 {
-  const int N0 = number_of_atoms, K = max_neighbor;
+  if (!spec)
+    return;
+  std::unique_ptr<LatticeSpec> L = std::move(spec);
+  const int N1 = number_of_atoms;
+  expand_lattice(*L);
+  if (!L->site_index.empty())
+    remove_vacancies(L->site_index);
+  if (number_of_atoms != N1)
+    throw InputError("Error: the lattice and its vacancies do not match the model");
+}
+
+// src/model_lattice.cu:26-68: the vacancies are the first
+// number_of_vacancies entries of a full shuffle of the pristine sites;
+// new_index[n] = index of pristine site n after their removal, -1 for a
+// vacancy
+void Model::draw_vacancies(std::vector<int>& new_index)
+{
+  const int N0 = number_of_atoms;
   if (number_of_vacancies > N0)
     throw InputError("Error: more vacancies than sites");
-  std::vector<int> is_vacancy((size_t)N0, 0), vacancy((size_t)number_of_vacancies);
+  std::vector<int> vacancy((size_t)number_of_vacancies);
   random_permutation_prefix(N0, number_of_vacancies, vacancy.data());
+  new_index.assign((size_t)N0, 0);
   for (int v : vacancy)
-    is_vacancy[v] = 1;
-  std::vector<int> new_index((size_t)N0, 0);
+    new_index[v] = -1;
   for (int n = 0, c = 0; n < N0; ++n)
-    if (!is_vacancy[n])
+    if (new_index[n] == 0)
       new_index[n] = c++;
+}
 
-  const int N1 = N0 - number_of_vacancies;
+// src/model_lattice.cu:70-146: the arrays without the vacancies; neighbours
+// that are vacancies are dropped and the later slots move up
+void Model::remove_vacancies(const std::vector<int>& new_index)
+{
+  const int N0 = number_of_atoms, K = max_neighbor;
+  int N1 = 0;
+  for (int v : new_index)
+    N1 += v >= 0;
   const size_t P1 = (size_t)N1 * K;
   int* nn = new_zeroed<int>((size_t)N1);
   int* nl = new_zeroed<int>(P1);
@@ -759,13 +859,13 @@ void Model::add_vacancies()
   real* x = new_zeroed<real>(P1);
   int count_atom = 0;
   for (int n = 0; n < N0; ++n) {
-    if (is_vacancy[n])
+    if (new_index[n] < 0)
       continue;
     int count_neighbor = 0;
     for (int m = 0; m < neighbor_number[n]; ++m) {
       size_t old = (size_t)n * K + m;
       int k = neighbor_list[old];
-      if (!is_vacancy[k]) {
+      if (new_index[k] >= 0) {
         size_t nw = (size_t)count_atom * K + count_neighbor;
         nl[nw] = new_index[k];
         hr[nw] = hopping_real[old];

@@ -24,6 +24,11 @@ SIGNATURES = {
     "lsqt_host_damped_sum": (C.c_int, [C.c_int, _dp, _dp, C.c_int, _dp, C.c_double,
                                        C.c_double, _dp]),
     "lsqt_host_model_load": (C.c_int, [C.c_char_p, C.POINTER(_vp), C.c_char_p, C.c_int]),
+    "lsqt_host_model_load_ex": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_vp), C.c_char_p,
+                                          C.c_int]),
+    "lsqt_host_model_lattice": (C.c_int, [_vp, C.c_void_p, C.POINTER(_ip), _ip]),
+    "lsqt_host_model_potential": (C.c_int, [_vp, _dp]),
+    "lsqt_host_model_expand_lattice": (C.c_int, [_vp]),
     "lsqt_host_model_free": (None, [_vp]),
     "lsqt_host_model_info": (C.c_int, [_vp, _ip, _dp]),
     "lsqt_host_model_arrays": (C.c_int, [_vp, _ip, _ip, _dp, _dp, _dp, _dp]),
@@ -61,13 +66,20 @@ def load():
     return _lib
 
 
-class HostModel:
-    """The drop-in's Model built from an input directory."""
+LATTICE_ON_DEVICE = 1
 
-    def __init__(self, dirname: str):
+
+class HostModel:
+    """The drop-in's Model built from an input directory.  lattice_on_device:
+    a lattice model keeps its description (lattice()) instead of expanding
+    its neighbour arrays, as the drop-in does before building the
+    Hamiltonian on the GPU."""
+
+    def __init__(self, dirname: str, lattice_on_device: bool = False):
         self.h = _vp()
         err = C.create_string_buffer(4096)
-        if load().lsqt_host_model_load(dirname.encode(), C.byref(self.h), err, 4096):
+        flags = LATTICE_ON_DEVICE if lattice_on_device else 0
+        if load().lsqt_host_model_load_ex(dirname.encode(), flags, C.byref(self.h), err, 4096):
             raise InputError(err.value.decode())
         ints = (C.c_int * 8)()
         reals = (C.c_double * 2)()
@@ -87,10 +99,46 @@ class HostModel:
         nn = np.zeros(N, np.int32)
         nl = np.zeros(P, np.int32)
         pot, hr, hi, xx = np.zeros(N), np.zeros(P), np.zeros(P), np.zeros(P)
-        load().lsqt_host_model_arrays(self.h, nn.ctypes.data_as(_ip), nl.ctypes.data_as(_ip),
-                                      *(a.ctypes.data_as(_dp) for a in (pot, hr, hi, xx)))
+        if load().lsqt_host_model_arrays(self.h, nn.ctypes.data_as(_ip), nl.ctypes.data_as(_ip),
+                                         *(a.ctypes.data_as(_dp) for a in (pot, hr, hi, xx))):
+            raise InputError("the model holds no neighbour arrays (lattice kept for the "
+                             "device: expand_lattice() first)")
         return dict(neighbor_number=nn, neighbor_list=nl, potential=pot, hopping_real=hr,
                     hopping_imag=hi, xx=xx)
+
+    def lattice(self):
+        """(lattice dict as gpuqt_amd.inputs.read_lattice gives it, site index
+        after vacancies or None) of a model loaded with lattice_on_device;
+        None for a model without a lattice description."""
+        from .native import Lattice
+        L = Lattice()
+        idx = _ip()
+        npr = C.c_int()
+        if load().lsqt_host_model_lattice(self.h, C.byref(L), C.byref(idx), C.byref(npr)):
+            return None
+        No, K = L.orbitals, L.max_neighbor
+        arr = lambda p, n: np.ctypeslib.as_array(p, (n,)).copy()
+        lat = {"cells": list(L.cells), "pbc": list(L.pbc), "transport": L.transport,
+               "lattice_constant": list(L.lattice_constant), "orbitals": No,
+               "max_neighbor": K, "orbital_x": arr(L.orbital_x, No),
+               "hop_count": arr(L.hop_count, No),
+               "hop_cell": arr(L.hop_cell, No * K * 3).reshape(No, K, 3),
+               "hop_orbital": arr(L.hop_orbital, No * K).reshape(No, K),
+               "hop_re": arr(L.hop_re, No * K).reshape(No, K),
+               "hop_im": arr(L.hop_im, No * K).reshape(No, K)}
+        site_index = arr(idx, npr.value) if idx else None
+        return lat, site_index
+
+    def expand_lattice(self):
+        """Expands a lattice_on_device model's neighbour arrays on the host."""
+        if load().lsqt_host_model_expand_lattice(self.h):
+            raise InputError("expanding the lattice failed")
+
+    def potential(self):
+        pot = np.zeros(self.n)
+        if load().lsqt_host_model_potential(self.h, pot.ctypes.data_as(_dp)):
+            raise InputError("the model holds no potential")
+        return pot
 
     def valid_pairs(self):
         a = self.arrays()

@@ -53,6 +53,8 @@ SIGNATURES = {
     "lsqt_ham_destroy": (C.c_int, [_vp]),
     "lsqt_ham_create_lattice": (C.c_int, [_vp, C.POINTER(Lattice), _dp, C.c_double,
                                           C.POINTER(_vp)]),
+    "lsqt_ham_create_lattices": (C.c_int, [_vp, C.c_int, C.POINTER(Lattice),
+                                           C.POINTER(_ip), _dp, C.c_double, C.POINTER(_vp)]),
     "lsqt_ham_create_segmented": (C.c_int, [_vp, C.c_int, _ip, C.c_int, C.c_int, _ip, _ip, _dp,
                                             _dp, _dp, _dp, C.c_double, C.POINTER(_vp)]),
     "lsqt_block_set_graphs": (C.c_int, [_vp, C.c_int]),
@@ -329,12 +331,12 @@ class Hamiltonian:
                                                     float(energy_max), C.byref(self.h)))
         _live["ham"].add(self)
 
-    @classmethod
-    def from_lattice(cls, ctx, lattice: dict, potential, energy_max):
-        """Builds a pristine lattice model on the device (lsqt_ham_create_lattice);
-        `lattice` as returned by gpuqt_amd.inputs.read_lattice."""
+    @staticmethod
+    def _lattice_struct(lattice: dict, keep: list):
+        """lsqt_lattice for a dict as returned by gpuqt_amd.inputs.read_lattice;
+        the arrays it points to are appended to keep."""
         No, K = lattice["orbitals"], lattice["max_neighbor"]
-        keep = {
+        arr = {
             "x": np.ascontiguousarray(lattice["orbital_x"], dtype=np.float64),
             "count": np.ascontiguousarray(lattice["hop_count"], dtype=np.int32),
             "cell": np.ascontiguousarray(lattice["hop_cell"], dtype=np.int32).reshape(-1),
@@ -342,25 +344,61 @@ class Hamiltonian:
             "re": np.ascontiguousarray(lattice["hop_re"], dtype=np.float64).reshape(-1),
             "im": np.ascontiguousarray(lattice["hop_im"], dtype=np.float64).reshape(-1),
         }
-        assert keep["count"].shape == (No,) and keep["orb"].shape == (No * K,)
+        assert arr["count"].shape == (No,) and arr["orb"].shape == (No * K,)
+        keep.append(arr)
         L = Lattice()
         L.cells[:] = [int(c) for c in lattice["cells"]]
         L.pbc[:] = [int(c) for c in lattice["pbc"]]
         L.transport = int(lattice["transport"])
         L.lattice_constant[:] = [float(a) for a in lattice["lattice_constant"]]
         L.orbitals, L.max_neighbor = int(No), int(K)
-        L.orbital_x, L.hop_count = _dptr(keep["x"]), _iptr(keep["count"])
-        L.hop_cell, L.hop_orbital = _iptr(keep["cell"]), _iptr(keep["orb"])
-        L.hop_re, L.hop_im = _dptr(keep["re"]), _dptr(keep["im"])
+        L.orbital_x, L.hop_count = _dptr(arr["x"]), _iptr(arr["count"])
+        L.hop_cell, L.hop_orbital = _iptr(arr["cell"]), _iptr(arr["orb"])
+        L.hop_re, L.hop_im = _dptr(arr["re"]), _dptr(arr["im"])
+        return L
+
+    @classmethod
+    def from_lattice(cls, ctx, lattice: dict, potential, energy_max):
+        """Builds a pristine lattice model on the device (lsqt_ham_create_lattice);
+        `lattice` as returned by gpuqt_amd.inputs.read_lattice."""
+        keep = []
+        L = cls._lattice_struct(lattice, keep)
         pot = np.ascontiguousarray(potential, dtype=np.float64)
         self = cls.__new__(cls)
         self.ctx = ctx
-        self.n = int(np.prod(lattice["cells"]) * No)
+        self.n = int(np.prod(lattice["cells"]) * lattice["orbitals"])
         assert pot.shape == (self.n,)
         self.segments = 1
         self.h = _vp()
         _check(load().lsqt_ham_create_lattice(ctx.h, C.byref(L), _dptr(pot), float(energy_max),
                                               C.byref(self.h)))
+        _live["ham"].add(self)
+        return self
+
+    @classmethod
+    def from_lattices(cls, ctx, lattices, site_indices, potential, energy_max):
+        """Lattice models (with vacancies: site_indices[k] maps pristine sites
+        to their index, -1 for vacancies; None for a pristine lattice) built
+        on the device, side by side as a segmented Hamiltonian when there are
+        several (lsqt_ham_create_lattices)."""
+        keep = []
+        S = len(lattices)
+        arr = (Lattice * S)(*[cls._lattice_struct(l, keep) for l in lattices])
+        idx = [None if si is None else np.ascontiguousarray(si, dtype=np.int32)
+               for si in (site_indices or [None] * S)]
+        ptrs = (_ip * S)(*[(_iptr(i) if i is not None else _ip()) for i in idx])
+        n = 0
+        for l, i in zip(lattices, idx):
+            n += int(np.prod(l["cells"]) * l["orbitals"]) if i is None else int((i >= 0).sum())
+        pot = np.ascontiguousarray(potential, dtype=np.float64)
+        assert pot.shape == (n,)
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.n = n
+        self.segments = S
+        self.h = _vp()
+        _check(load().lsqt_ham_create_lattices(ctx.h, S, arr, ptrs, _dptr(pot),
+                                               float(energy_max), C.byref(self.h)))
         _live["ham"].add(self)
         return self
 

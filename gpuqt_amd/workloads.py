@@ -79,3 +79,12 @@ def gershgorin_bound(arrays, max_neighbor):
     t = np.hypot(np.asarray(arrays["hopping_real"]).reshape(-1, K),
                  np.asarray(arrays["hopping_imag"]).reshape(-1, K))
     return float((np.abs(arrays["potential"]) + np.where(valid, t, 0.0).sum(axis=1)).max())
+
+
+def lattice_gershgorin_bound(lattice, potential):
+    """The same bound for a lattice model: max |V| plus the largest sum of
+    |t| over the hoppings of one orbital (a site keeps at most those)."""
+    hop = np.hypot(np.asarray(lattice["hop_re"]), np.asarray(lattice["hop_im"]))
+    counts = np.asarray(lattice["hop_count"])
+    per_orbital = [hop[m, :counts[m]].sum() for m in range(lattice["orbitals"])]
+    return float(np.abs(potential).max() + max(per_orbital))

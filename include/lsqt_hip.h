@@ -136,6 +136,24 @@ typedef struct {
  * draws of the Model).  Packed layout, one class per (orbital, hopping). */
 int lsqt_ham_create_lattice(lsqt_ctx* ctx, const lsqt_lattice* lattice, const double* potential,
                             double energy_max, lsqt_ham** out);
+/* The same for lattice models with vacancies and for several lattices side
+ * by side: the neighbour expansion and the vacancy re-indexing of the Model
+ * (src/model_lattice.cu:58-146, 342-386) on the device.  site_index (NULL,
+ * or an array with an entry per lattice that may itself be NULL) maps each
+ * pristine site ((nx*Ny + ny)*Nz + nz)*orbitals + m of lattice k to its
+ * index after the vacancies were removed, -1 for a vacancy
+ * (Model::find_new_atom_index): vacancies get no site, neighbours that are
+ * vacancies are dropped and the later slots move up, as in
+ * Model::add_vacancies.  Lattice k then holds sites [s_k, s_k + n_k) with
+ * s_0 = 0 and n_k its sites after vacancies; potential holds all of them in
+ * that order (the Model's disorder draws).  segments > 1 makes a segmented
+ * Hamiltonian (as lsqt_ham_create_segmented).  Packed layout; the classes
+ * are the distinct (re, im, x12) of all (lattice, orbital, hopping) entries.
+ * Every result equals that of lsqt_ham_create / lsqt_ham_create_segmented on
+ * the Model's arrays bit for bit. */
+int lsqt_ham_create_lattices(lsqt_ctx* ctx, int segments, const lsqt_lattice* lattices,
+                             const int* const* site_index, const double* potential,
+                             double energy_max, lsqt_ham** out);
 /* (no counterpart in the reference, which runs one system per input
  * directory)  A block-diagonal Hamiltonian of `segments` independent systems
  * side by side: system k on sites [starts[k], starts[k+1]), starts[0] = 0,

@@ -11,6 +11,8 @@
 #ifndef LSQT_HOST_H
 #define LSQT_HOST_H
 
+#include "lsqt_hip.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -65,6 +67,14 @@ void lsqt_host_set_output_precision(int digits);
  * model-phase random draw, the row-major ELL arrays, the random vectors */
 typedef struct lsqt_host_model lsqt_host_model;
 int lsqt_host_model_load(const char* input_dir, lsqt_host_model** out, char* err, int err_len);
+/* the same with flags: LSQT_HOST_LATTICE_ON_DEVICE keeps a lattice model's
+ * description instead of expanding its neighbour arrays (what the drop-in
+ * does before building the Hamiltonian on the GPU); lsqt_host_model_arrays
+ * then fails, lsqt_host_model_lattice and lsqt_host_model_potential give
+ * what lsqt_ham_create_lattices takes */
+#define LSQT_HOST_LATTICE_ON_DEVICE 1
+int lsqt_host_model_load_ex(const char* input_dir, int flags, lsqt_host_model** out, char* err,
+                            int err_len);
 void lsqt_host_model_free(lsqt_host_model* m);
 /* ints: number_of_atoms, max_neighbor, number_of_moments,
  *       number_of_random_vectors, number_of_energy_points,
@@ -78,6 +88,17 @@ int lsqt_host_model_arrays(const lsqt_host_model* m, int* neighbor_number, int* 
                            double* xx);
 int lsqt_host_model_inputs(const lsqt_host_model* m, double* energy, double* time_step,
                            int* local_orbitals);
+/* the description of a model loaded with LSQT_HOST_LATTICE_ON_DEVICE (its
+ * arrays stay owned by the model): the lattice, the index of every pristine
+ * site after the vacancies (NULL: none) and the pristine site count; 1 when
+ * the model has no such description (a general model, or arrays expanded) */
+int lsqt_host_model_lattice(const lsqt_host_model* m, lsqt_lattice* lattice,
+                            const int** site_index, int* n_pristine);
+/* expands the neighbour arrays of such a model on the host after all (the
+ * arrays lsqt_host_model_load gives, bit for bit) */
+int lsqt_host_model_expand_lattice(lsqt_host_model* m);
+/* the on-site energies (number_of_atoms values) */
+int lsqt_host_model_potential(const lsqt_host_model* m, double* potential);
 /* Model::initialize_state into host arrays (consumes random draws) */
 int lsqt_host_model_random_state(lsqt_host_model* m, int orbital, double* re, double* im);
 

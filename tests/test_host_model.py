@@ -53,10 +53,33 @@ def test_failure_messages(tmp_path):
     for name, exp in expected.items():
         work = tmp_path / name
         shutil.copytree(os.path.join(fdir, name), work)
-        with pytest.raises(hostlib.InputError) as e:
-            hostlib.HostModel(str(work))
-        msg = str(e.value).replace(str(work), "<dir>")
-        assert msg.split("\n")[0] == exp["message"], name
+        for on_device in (False, True):  # the lattice kept for the GPU as well
+            with pytest.raises(hostlib.InputError) as e:
+                hostlib.HostModel(str(work), lattice_on_device=on_device)
+            msg = str(e.value).replace(str(work), "<dir>")
+            assert msg.split("\n")[0] == exp["message"], name
+
+
+@pytest.mark.parametrize("offset,open_y,fails", [(4, False, False), (5, False, True),
+                                                 (5, True, False), (-6, False, True)])
+def test_hopping_beyond_one_period(tmp_path, offset, open_y, fails):
+    """A hopping reaching further than one period of a periodic direction
+    leaves the lattice (an error) -- unless an open direction drops it at
+    every site -- whether the lattice is expanded on the host or kept."""
+    from gpuqt_amd import inputs as gi
+    d = str(tmp_path)
+    dy = 9 if open_y else 0  # beyond the 3 cells of the open y direction
+    hops = [[(1, 0, 0, 0, -1.0, 0.0), (offset, dy, 0, 0, -0.5, 0.0)]]
+    gi.write_lattice(d, (4, 3, 1), (1, 0 if open_y else 1, 0), 0, (1.0, 1.0, 1.0),
+                     [(0.0, 0.0, 0.0)], hops)
+    gi.write_para(d, moments=8, energy_max=4.0)
+    gi.write_energy(d, [0.0])
+    for on_device in (False, True):
+        if fails:
+            with pytest.raises(hostlib.InputError, match="more than one cell"):
+                hostlib.HostModel(d, lattice_on_device=on_device)
+        else:
+            hostlib.HostModel(d, lattice_on_device=on_device)
 
 
 def test_correlation_row_matches_reference():
