@@ -533,6 +533,9 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.B
     return res
 
 
+probe_arrays = None
+
+
 def probe(ctx, H, rv, args, moments=10):
     """Short runs of the hot kernels for counter profiles: a plain copy of
     one vector (k_copy: n x 16 B read and written, the counters' scale), the
@@ -548,6 +551,20 @@ def probe(ctx, H, rv, args, moments=10):
     H.moments_f32(rv, rv, moments)
     ctx.sync()
     log(f"probe: {moments} FP64 and FP32 moments on {H.n} sites")
+    if H.info().sweep_slab:
+        # the one-step kernels on the same system, for comparison
+        os.environ["LSQT_SWEEP"] = "0"
+        H1 = native.Hamiltonian.from_arrays(ctx, probe_arrays, H.n, H.info().max_neighbor,
+                                            H.info().energy_max)
+        del os.environ["LSQT_SWEEP"]
+        v1 = H1.vector()
+        v1.copy_from(rv)
+        H1.moments(v1, v1, moments)
+        H1.moments_f32(v1, v1, moments)
+        ctx.sync()
+        log(f"probe: the same with one step per launch")
+        v1.close()
+        H1.close()
     if args.block:
         arrays, n = gi.cubic_peierls_arrays(args.cubic_L)
         Hc = native.Hamiltonian.from_arrays(ctx, arrays, n, 6, 15.0)
@@ -776,6 +793,9 @@ def main():
     ctx = native.Context(0)
     H = native.Hamiltonian.from_arrays(ctx, arrays, model.n, model.max_neighbor,
                                        model.energy_max)
+    if args.probe:
+        global probe_arrays
+        probe_arrays = arrays
     del arrays
     info = H.info()
     rv = H.vector(*r)
@@ -789,6 +809,8 @@ def main():
     M = args.moments
     job = dict(energies=energies, time_steps=time_steps, emax=model.energy_max,
                volume=model.volume, M=M, msd=args.time_steps > 0)
+
+    sweep = info.sweep_slab > 0
 
     def timed(precision, steps, warmup):
         for w in range(warmup):
@@ -806,19 +828,34 @@ def main():
             log(f"{precision} step {s} done")
         ctx.sync()
         elapsed = time.perf_counter() - t0
-        avg_launch_ms = acc["moment_ms"] / max(acc["moment_sets"] * (M - 1), 1)
+        # the moment set's launches of its dominant kernel: M - 1 one-step
+        # launches, or (M - 2) // 2 two-step sweep launches (the set's other
+        # launches -- mu_0, mu_1, a last odd step, the reduction -- are
+        # counted into their time, which overstates it by under 0.2 %)
+        per_set = (M - 2) // 2 if sweep else M - 1
+        avg_launch_ms = acc["moment_ms"] / max(acc["moment_sets"] * per_set, 1)
         return total, elapsed, avg_launch_ms, rows
 
     total_updates, elapsed, avg_launch_ms, rows = timed("f64", args.steps, args.warmup)
     rows = np.array(rows)
     value = total_updates / elapsed
-    # dominant kernel: the fused recursion + moment-product step, one site
-    # update per site per launch
-    bytes_per_launch = info.bytes_per_site_f64 * model.n
+    # dominant kernel: the two-step sweep (two site-updates per site per
+    # launch) where the lattice has one, else the fused one-step kernel
+    updates_per_site = 2 if sweep else 1
+    bytes_per_site = info.bytes_per_site_sweep_f64 if sweep else info.bytes_per_site_f64
+    bytes_per_launch = bytes_per_site * model.n
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     default_workload = (args.nx, args.ny) == (2000, 1000) and info.layout == 2 \
         and info.max_neighbor == 3
-    traffic = profiled_traffic("k_hpsi<1,3,2,true,true,true>") if default_workload else None
+    kname = "k_sweep2<double2,3,4,false>" if sweep else "k_hpsi<1,3,2,true,true,true>"
+    traffic = profiled_traffic(kname) if default_workload else None
+    # the same launch time priced on other byte counts: the one-step kernel's
+    # (what it moves for the same site-updates on this layout) and the
+    # reference's data model (int32 index + FP64 re / im per pair, separate
+    # inner-product pass: 76 + 20 z + 16 bytes per site-update)
+    one_step_GBs = info.bytes_per_site_f64 * updates_per_site * model.n / (avg_launch_ms * 1e-3) / 1e9
+    ref_model_bpu = 76.0 + 20.0 * info.max_neighbor + 16.0
+    ref_model_GBs = ref_model_bpu * updates_per_site * model.n / (avg_launch_ms * 1e-3) / 1e9
     stream = copy_bandwidth(ctx)
     log(f"plain copy: {stream:.0f} GB/s; fused step {achieved:.0f} GB/s")
     dos = rows[0]
@@ -827,9 +864,11 @@ def main():
     f32 = None
     if args.precision_f32:
         u32, e32, l32, rows32 = timed("f32", args.f32_steps, args.f32_warmup)
-        # FP32 path: potential 4 + psi_{m-1}, psi_{m-2}, psi_m, left 4 x 8 + the
-        # same pair bytes as FP64 (which has potential 8 + 4 x 16)
-        bpu32 = 36.0 + (info.bytes_per_site_f64 - 72.0)
+        # FP32 path: the sweep moves potential 4 + five vector streams x 8 per
+        # site and launch; the one-step kernel potential 4 + 4 x 8 + the same
+        # pair bytes as FP64 (which has potential 8 + 4 x 16)
+        bpu32 = (info.bytes_per_site_sweep_f32 if sweep
+                 else 36.0 + (info.bytes_per_site_f64 - 72.0))
         ach32 = bpu32 * model.n / (l32 * 1e-3) / 1e9
         errs = [float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(rows32, rows)]
         elem = []
@@ -850,9 +889,17 @@ def main():
             "warmup": args.f32_warmup,
             "speedup_vs_f64": (u32 / e32) / value,
             "dtype": "f32 states, f64 arithmetic and accumulation",
-            "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS,
+            "roofline": {"bound": "hbm",
+                         "kernel": ("k_sweep2<float2,3,4,true> (two steps per launch)" if sweep
+                                    else "k_hpsi32"),
+                         "achieved": ach32, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": ach32 / HBM_PEAK_GBS,
-                         "bytes_per_site_update": bpu32, "avg_launch_us": l32 * 1e3},
+                         "traffic": (profiled_traffic("k_sweep2<float2,3,4,true>")
+                                     if sweep and default_workload else None),
+                         "bytes_per_site_per_launch": bpu32,
+                         "site_updates_per_site_per_launch": updates_per_site,
+                         "bytes_per_launch_algorithmic": bpu32 * model.n,
+                         "avg_launch_us": l32 * 1e3},
             "error_vs_f64": {
                 "dos_normwise": errs[0],
                 "msd_normwise_max": max(errs[1:]) if len(errs) > 1 else None,
@@ -895,8 +942,11 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_hpsi (2 H~ psi_{m-1} - psi_{m-2} fused with <L|psi_m>; slot words "
-                      "computed from the lattice pattern, 2 sites per thread)",
+            "kernel": ("k_sweep2<double2,3,4,false> (two Chebyshev steps per launch: psi_m "
+                       "and psi_{m+1} from psi_{m-2}, psi_{m-1}, four slabs at a time from "
+                       "LDS, fused with <L|psi_m> and <L|psi_{m+1}>)" if sweep else
+                       "k_hpsi (2 H~ psi_{m-1} - psi_{m-2} fused with <L|psi_m>; slot words "
+                       "computed from the lattice pattern, 2 sites per thread)"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -906,10 +956,20 @@ def main():
                               "calibrated on a plain copy (profiles/hbm_traffic_probe.md)"
                               if traffic else None,
             "bytes_per_launch_algorithmic": bytes_per_launch,
-            "bytes_per_site_update": info.bytes_per_site_f64,
+            "bytes_per_site_per_launch": bytes_per_site,
+            "site_updates_per_site_per_launch": updates_per_site,
             "avg_launch_us": avg_launch_ms * 1e3,
             "plain_copy_GBs": stream,
             "ratio_to_plain_copy": achieved / stream,
+            "one_step_equivalent_GBs": one_step_GBs,
+            "one_step_equivalent_note": "the same launch time priced at the one-step kernel's "
+                                        f"{info.bytes_per_site_f64:.2f} B per site-update: "
+                                        "what a one-step kernel would have to stream",
+            "reference_data_model_GBs": ref_model_GBs,
+            "reference_data_model_note": f"priced at {ref_model_bpu:.0f} B per site-update "
+                                         "(int32 index + FP64 re / im per pair and a separate "
+                                         "inner-product pass); not a roofline: above the peak "
+                                         "where the layout moves fewer bytes",
         },
     }
     if f32 is not None:

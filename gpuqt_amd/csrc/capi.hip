@@ -244,7 +244,10 @@ int upload(lsqt_ham* h, const T** dst, const T* src, size_t count)
 // excluded) that divides n, the row the largest smaller regular offset that
 // divides the slab.  Lattices numbered x-slowest (src/model_lattice.cu:162)
 // and the simple-cubic general inputs have this form; anything else gets 0.
-void detect_lattice_offsets(lsqt_ham* h, int n, int K, const int* nn, const int* nl)
+// Returns the regular offsets (ascending), the candidates of the two-step
+// sweep's slab.
+std::vector<long long> detect_lattice_offsets(lsqt_ham* h, int n, int K, const int* nn,
+                                              const int* nl)
 {
   std::unordered_map<long long, long long> count;
   long long pairs = 0;
@@ -256,10 +259,10 @@ void detect_lattice_offsets(lsqt_ham* h, int n, int K, const int* nn, const int*
       ++count[d];
       ++pairs;
       if (count.size() > 4096)
-        return;  // irregular structure
+        return {};  // irregular structure
     }
   if (pairs == 0)
-    return;
+    return {};
   std::vector<long long> regular;
   for (auto& kv : count)
     if (kv.second * 8 * K >= pairs)
@@ -274,10 +277,145 @@ void detect_lattice_offsets(lsqt_ham* h, int n, int K, const int* nn, const int*
       if (D % d == 0 && d >= 8 && D / d >= 4) {
         h->slab = (int)D;
         h->row = (int)d;
-        return;
+        return regular;
       }
     }
   }
+  return regular;
+}
+
+// Launch shape of the two-step sweep for one storage precision: runs of
+// slabs per chunk so that the tiles fill the device once at the occupancy
+// the LDS rows allow (LSQT_SWEEP_TILES=<count> in the environment sets the
+// tile count instead), at least two slabs per run.  The shape depends on the
+// geometry only, not on the class table or slot pattern, so a Hamiltonian
+// built with and without its pattern sums the same tiles.
+SweepMap sweep_shape(SweepMap g, bool f32)
+{
+  const size_t vb = f32 ? sizeof(float2) : sizeof(double2);
+  const size_t lds = (vb * 2 * (kSweepBlock + 2) + 8 + vb) * kSweepThreads;
+  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / lds));
+  long long target = (long long)kComputeUnits * per_cu;
+  if (const char* env = getenv("LSQT_SWEEP_TILES"))
+    if (atoll(env) > 0)
+      target = atoll(env);
+  long long runs = (target + g.nch - 1) / g.nch;
+  runs = std::max(1ll, std::min(runs, (long long)(g.nslab / 2)));
+  g.nruns = (int)runs;
+  g.tiles = g.nch * g.nruns;
+  g.blocks = (g.tiles + kXcds - 1) / kXcds * kXcds;
+  return g;
+}
+
+bool find_slot_pattern(const HamDev& d, int* period_out, std::vector<int2>* table_out);
+
+// The two-step sweep (k_sweep2) of a packed Hamiltonian with compile-time
+// slot count: from the per-residue slot pattern (period p), for slab sizes S
+// near each candidate offset that are multiples of p and divide n into at
+// least 4 slabs, every pattern offset is split into a slab offset (-1, 0 or
+// 1) and a position offset (at most kSweepMaxHalo, unless a whole slab fits
+// one workgroup), and a device pass checks every packed word against the
+// periodic table so obtained; the first S that passes is kept.
+// LSQT_SWEEP=0 in the environment turns the sweep off.  Any failure leaves
+// the Hamiltonian without it (it is an optimisation only).
+void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
+{
+  h->sweep64 = SweepMap{};
+  h->sweep32 = SweepMap{};
+  const HamDev& d = h->d;
+  const char* env = getenv("LSQT_SWEEP");
+  if ((env && env[0] == '0') || d.layout == kFull || d.n < kSweepMinSites)
+    return;
+  if (d.K != 2 && d.K != 3 && d.K != 4 && d.K != 6)
+    return;
+  int period = 0;
+  std::vector<int2> pattern;
+  if (!find_slot_pattern(d, &period, &pattern))
+    return;
+  const int n = d.n;
+  const int blocks = sweep_check_blocks(n);
+  int* dev_bad = nullptr;
+  int4* dev_tab = nullptr;
+  if (hipMalloc(&dev_bad, sizeof(int) * (size_t)blocks) != hipSuccess)
+    return;
+  if (hipMalloc(&dev_tab, sizeof(int4) * pattern.size()) != hipSuccess) {
+    (void)hipFree(dev_bad);
+    return;
+  }
+  std::vector<int> bad(blocks);
+  std::vector<double2> classes(d.ncls);
+  if (hipMemcpy(classes.data(), d.tab, sizeof(double2) * classes.size(),
+                hipMemcpyDeviceToHost) != hipSuccess)
+    classes.clear();
+  std::vector<long long> tried;
+  bool found = false;
+  for (auto it = candidates.rbegin(); it != candidates.rend() && !found; ++it) {
+    for (long long S = *it - kSweepMaxHalo - 1; S <= *it + kSweepMaxHalo + 1 && !found; ++S) {
+      if (S < 128 || S % period != 0 || n % S != 0 || n / S < 4)
+        continue;
+      if (std::find(tried.begin(), tried.end(), S) != tried.end())
+        continue;
+      tried.push_back(S);
+      const bool whole = S <= kSweepThreads;
+      std::vector<int4> tab(pattern.size());
+      int halo = 0;
+      bool ok = true;
+      for (size_t e = 0; e < pattern.size() && ok; ++e) {
+        const int off = pattern[e].x;
+        if (off == kPatternEmpty) {
+          tab[e] = make_int4(0, 0, 0, 0);
+          continue;
+        }
+        // off = ds * S + dp with |dp| <= S / 2
+        long long ds = off >= 0 ? (off + S / 2) / S : -((-off + S / 2) / S);
+        const long long dp = off - ds * S;
+        if (ds < -1 || ds > 1 || (!whole && std::llabs(dp) > kSweepMaxHalo))
+          ok = false;
+        halo = std::max(halo, (int)std::llabs(dp));
+        tab[e] = make_int4((int)ds, (int)dp, pattern[e].y, 1);
+      }
+      if (!ok)
+        continue;
+      SweepMap g{};
+      g.S = (int)S;
+      g.nslab = (int)(n / S);
+      g.sdiv = make_divisor((int)S);
+      g.p = period;
+      g.tab = dev_tab;
+      if (whole) {
+        g.hh = 0;
+        g.nch = 1;
+        g.cs = (int)S;
+      } else {
+        g.hh = halo;
+        g.cs = kSweepThreads - 4 * halo;  // the held positions fit one workgroup
+        g.nch = (int)((S + g.cs - 1) / g.cs);
+      }
+      g.wmax = g.cs + 4 * g.hh;
+      g.real = 1;
+      for (size_t e = 0; e < tab.size(); ++e)
+        if (tab[e].w && (tab[e].z >= (int)classes.size() || classes[tab[e].z].y != 0.0))
+          g.real = 0;
+      if (hipMemcpy(dev_tab, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice) !=
+          hipSuccess)
+        break;
+      launch_sweep_check(d, g, dev_bad, h->ctx->stream);
+      if (hipMemcpyAsync(bad.data(), dev_bad, sizeof(int) * bad.size(), hipMemcpyDeviceToHost,
+                         h->ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(h->ctx->stream) != hipSuccess)
+        break;
+      if (std::any_of(bad.begin(), bad.end(), [](int b) { return b != 0; }))
+        continue;
+      h->allocations.push_back(dev_tab);  // kept with the Hamiltonian
+      h->sweep64 = sweep_shape(g, false);
+      h->sweep32 = sweep_shape(g, true);
+      found = true;
+    }
+  }
+  (void)hipGetLastError();
+  (void)hipFree(dev_bad);
+  if (!found)
+    (void)hipFree(dev_tab);
 }
 
 // The slot pattern of a packed Hamiltonian.  On a periodic lattice whose
@@ -292,12 +430,16 @@ void detect_lattice_offsets(lsqt_ham* h, int n, int K, const int* nn, const int*
 // the marked sites' words.  LSQT_PATTERN=0 in the environment turns it off.
 // Any failure leaves the Hamiltonian without a pattern (it is an
 // optimisation only).
-void detect_slot_pattern(lsqt_ham* h)
+// The per-residue slot pattern of a packed Hamiltonian's words, sampled on
+// 4096 sites from the middle of the system: the smallest period p <= 64
+// (p x slots <= 512 entries) whose most common (offset, class) entries per
+// residue describe at least 90 % of the sample; table[m * p + r] = (offset,
+// class) of slot m at residue r (offset kPatternEmpty: unused).  False where
+// there is none.
+bool find_slot_pattern(const HamDev& d, int* period_out, std::vector<int2>* table_out)
 {
-  HamDev& d = h->d;
-  const char* env = getenv("LSQT_PATTERN");
-  if (d.layout == kFull || d.n < 4096 || (env && env[0] == '0'))
-    return;
+  if (d.layout == kFull || d.n < 4096)
+    return false;
   const int K = d.K;
   constexpr int kSample = 4096;
   const int s0 = (d.n / 2 - kSample / 2) / 64 * 64;
@@ -305,7 +447,7 @@ void detect_slot_pattern(lsqt_ham* h)
   for (int m = 0; m < K; ++m)
     if (hipMemcpy(sample.data() + (size_t)m * kSample, d.words + (size_t)m * d.stride + s0,
                   sizeof(uint32_t) * kSample, hipMemcpyDeviceToHost) != hipSuccess)
-      return;
+      return false;
   // (offset, class) of sample site i in slot m, as two 32-bit halves
   auto entry = [&](int i, int m) -> unsigned long long {
     const uint32_t w = sample[(size_t)m * kSample + i];
@@ -343,6 +485,21 @@ void detect_slot_pattern(lsqt_ham* h)
     }
   }
   if (!period)
+    return false;
+  *period_out = period;
+  table_out->swap(table);
+  return true;
+}
+
+void detect_slot_pattern(lsqt_ham* h)
+{
+  HamDev& d = h->d;
+  const char* env = getenv("LSQT_PATTERN");
+  if (env && env[0] == '0')
+    return;
+  int period = 0;
+  std::vector<int2> table;
+  if (!find_slot_pattern(d, &period, &table))
     return;
   const int2* pat = nullptr;
   if (upload(h, &pat, table.data(), table.size()) != LSQT_OK)
@@ -522,8 +679,9 @@ extern "C" int lsqt_ham_create(lsqt_ctx* ctx, int n, int K, const int* nn, const
     lsqt_ham_destroy(h);
     return rc;
   }
-  detect_lattice_offsets(h, n, K, nn, nl);
+  const std::vector<long long> offsets = detect_lattice_offsets(h, n, K, nn, nl);
   detect_slot_pattern(h);
+  detect_sweep(h, offsets);
   // host copies for the FP32 / block paths, built on demand
   h->host_pot.assign(pot, pot + n);
   h->host_tab = tab;
@@ -764,6 +922,8 @@ extern "C" int lsqt_ham_create_lattices(lsqt_ctx* ctx, int segments, const lsqt_
     }
   }
   detect_slot_pattern(h);
+  if (segments == 1 && (!site_index || !site_index[0]))
+    detect_sweep(h, {(long long)lats[0].cells[1] * lats[0].cells[2] * lats[0].orbitals});
   h->host_pot.assign(potential, potential + n);
   h->host_tab = tab;
   h->host_tab_xx = tab_xx;
@@ -850,6 +1010,20 @@ extern "C" int lsqt_ham_get_info(const lsqt_ham* h, lsqt_ham_info* info)
     info->bytes_per_site_f64 = 8.0 + per_slot * h->d.K + 64.0;
   info->slab = h->slab;
   info->row = h->row;
+  info->sweep_slab = h->sweep64.S;
+  info->sweep_halo = h->sweep64.hh;
+  info->sweep_tiles = h->sweep64.tiles;
+  info->sweep_tiles_f32 = h->sweep32.tiles;
+  // one two-step launch per site: psi_{m-2} and psi_{m-1} read, psi_m and
+  // psi_{m+1} written, the left vector and the potential read (the slots
+  // come from the per-orbital table: no words)
+  if (h->sweep64.S) {
+    info->bytes_per_site_sweep_f64 = 5 * 16.0 + 8.0;
+    info->bytes_per_site_sweep_f32 = 5 * 8.0 + 4.0;
+  } else {
+    info->bytes_per_site_sweep_f64 = 0.0;
+    info->bytes_per_site_sweep_f32 = 0.0;
+  }
   return LSQT_OK;
 }
 
@@ -1142,6 +1316,25 @@ extern "C" int lsqt_kernel_polynomial(lsqt_ham* h, const lsqt_vec* s0, const lsq
   return LSQT_OK;
 }
 
+extern "C" int lsqt_kernel_polynomial_pair(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1,
+                                           lsqt_vec* s2, lsqt_vec* s3)
+{
+  LSQT_TRY(check_ham_vecs(h, {s0, s1, s2, s3}));
+  if (h->sweep64.S == 0)
+    return set_error(LSQT_ERR_UNSUPPORTED, "kernel_polynomial_pair: the Hamiltonian has no "
+                     "two-step sweep (lsqt_ham_info.sweep_slab is 0)");
+  const double2* p[4] = {s0->d, s1->d, s2->d, s3->d};
+  for (int a = 0; a < 4; ++a)
+    for (int b = a + 1; b < 4; ++b)
+      if (p[a] == p[b])
+        return set_error(LSQT_ERR_INVALID, "kernel_polynomial_pair: s0, s1, s2 and s3 must "
+                         "be distinct");
+  launch_sweep2(h->d, h->sweep64, s0->d, s1->d, s2->d, s3->d, nullptr, nullptr, nullptr,
+                h->ctx->stream);
+  LSQT_HIP(hipGetLastError());
+  return LSQT_OK;
+}
+
 extern "C" int lsqt_chebyshev_01(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1,
                                  lsqt_vec* s, double b0, double b1, int direction)
 {
@@ -1194,24 +1387,44 @@ int lsqt::moments_enqueue(lsqt_ham* h, const double2* left, const double2* right
   hipStream_t st = ctx->stream;
   const int n = h->d.n;
   const int tiles = hpsi_tiles(n, h->d.K);
-  LSQT_TRY(ctx_reserve_moments(ctx, M, tiles));
-  LSQT_TRY(ham_scratch(h, 2));
+  // two steps per launch where the lattice allows (the same per-site values;
+  // one partial per sweep tile)
+  const SweepMap& sw = h->sweep64;
+  const bool sweep = sw.S > 0 && M >= 4;
+  const int stride_tiles = sweep ? std::max(tiles, sw.tiles) : tiles;
+  LSQT_TRY(ctx_reserve_moments(ctx, M, stride_tiles));
+  LSQT_TRY(ham_scratch(h, sweep ? 4 : 2));
   double2* A = h->scratch[0];
   double2* B = h->scratch[1];
-  const size_t stride = (size_t)tiles;
+  const size_t stride = (size_t)stride_tiles;
   // psi_0 = right; mu_0 = <left|psi_0>
   launch_copy(A, right, n, st);
   launch_dot(A, left, ctx->partial, n, st, hpsi_spt(h->d.K));
   // psi_1 = H~ psi_0; mu_1
   launch_hpsi(h->d, A, nullptr, B, left, ctx->partial + stride, false, true, st);
-  // psi_m = 2 H~ psi_{m-1} - psi_{m-2}, written over psi_{m-2}
-  for (int m = 2; m < M; ++m) {
-    launch_hpsi(h->d, B, A, A, left, ctx->partial + (size_t)m * stride, true, true, st);
-    double2* t = A;
-    A = B;
-    B = t;
+  int m = 2;
+  if (sweep) {
+    // psi_m, psi_{m+1} from psi_{m-2} (A), psi_{m-1} (B) into C, D
+    double2* C = h->scratch[2];
+    double2* D = h->scratch[3];
+    for (; m + 1 < M; m += 2) {
+      launch_sweep2(h->d, sw, A, B, C, D, left, ctx->partial + (size_t)m * stride,
+                    ctx->partial + (size_t)(m + 1) * stride, st);
+      std::swap(A, C);
+      std::swap(B, D);
+    }
   }
-  launch_reduce_partials(ctx->partial, stride, tiles, M, ctx->mu_dev, st);
+  const int m_sweep = m;
+  // psi_m = 2 H~ psi_{m-1} - psi_{m-2}, written over psi_{m-2}
+  for (; m < M; ++m) {
+    launch_hpsi(h->d, B, A, A, left, ctx->partial + (size_t)m * stride, true, true, st);
+    std::swap(A, B);
+  }
+  launch_reduce_partials(ctx->partial, stride, tiles, 2, ctx->mu_dev, st);
+  launch_reduce_partials(ctx->partial + 2 * stride, stride, sweep ? sw.tiles : tiles,
+                         m_sweep - 2, ctx->mu_dev + 2, st);
+  launch_reduce_partials(ctx->partial + (size_t)m_sweep * stride, stride, tiles, M - m_sweep,
+                         ctx->mu_dev + m_sweep, st);
   LSQT_HIP(hipGetLastError());
   return LSQT_OK;
 }

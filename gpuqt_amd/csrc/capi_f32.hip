@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "bessel.h"
@@ -102,23 +103,43 @@ int lsqt::moments_f32_enqueue(lsqt_ham* h, const double2* left, const double2* r
   hipStream_t st = ctx->stream;
   const int n = h->d.n;
   const int tiles = hpsi32_tiles(n);
-  LSQT_TRY(ctx_reserve_moments(ctx, M, tiles));
+  // two steps per launch where the lattice allows
+  const SweepMap& sw = h->sweep32;
+  const bool sweep = sw.S > 0 && M >= 4;
+  const int stride_tiles = sweep ? std::max(tiles, sw.tiles) : tiles;
+  LSQT_TRY(ctx_reserve_moments(ctx, M, stride_tiles));
   LSQT_TRY(f32_prepare(h, 5));
   float2* A = h->f32->v[0];
   float2* B = h->f32->v[1];
   float2* L = h->f32->v[4];
-  const size_t stride = (size_t)tiles;
+  const size_t stride = (size_t)stride_tiles;
   launch_to_f32(left, L, n, st);
   launch_to_f32(right, A, n, st);
   launch_dot32(A, L, ctx->partial, n, st);
   launch_hpsi32(h->d, A, nullptr, B, L, ctx->partial + stride, false, true, st);
-  for (int m = 2; m < M; ++m) {
-    launch_hpsi32(h->d, B, A, A, L, ctx->partial + (size_t)m * stride, true, true, st);
-    float2* t = A;
-    A = B;
-    B = t;
+  int m = 2;
+  if (sweep) {
+    // psi_m, psi_{m+1} from psi_{m-2} (A), psi_{m-1} (B) into the buffers of
+    // the [X, U] recursion (free during a moment set)
+    float2* C = h->f32->v[2];
+    float2* D = h->f32->v[3];
+    for (; m + 1 < M; m += 2) {
+      launch_sweep2_32(h->d, sw, A, B, C, D, L, ctx->partial + (size_t)m * stride,
+                       ctx->partial + (size_t)(m + 1) * stride, st);
+      std::swap(A, C);
+      std::swap(B, D);
+    }
   }
-  launch_reduce_partials(ctx->partial, stride, tiles, M, ctx->mu_dev, st);
+  const int m_sweep = m;
+  for (; m < M; ++m) {
+    launch_hpsi32(h->d, B, A, A, L, ctx->partial + (size_t)m * stride, true, true, st);
+    std::swap(A, B);
+  }
+  launch_reduce_partials(ctx->partial, stride, tiles, 2, ctx->mu_dev, st);
+  launch_reduce_partials(ctx->partial + 2 * stride, stride, sweep ? sw.tiles : tiles,
+                         m_sweep - 2, ctx->mu_dev + 2, st);
+  launch_reduce_partials(ctx->partial + (size_t)m_sweep * stride, stride, tiles, M - m_sweep,
+                         ctx->mu_dev + m_sweep, st);
   LSQT_HIP(hipGetLastError());
   return LSQT_OK;
 }

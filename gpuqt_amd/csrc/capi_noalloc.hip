@@ -5,12 +5,15 @@
 // lsqt_moments keeps its recursion vectors on the Hamiltonian and its moment
 // partials on the context, both allocated on first use.  Here every buffer
 // beyond the caller's inputs and outputs lives in `scratch`, one fixed array
-// per GPU: the two recursion vectors, a ring of moment partials reduced every
-// kRing moments, the moments, and their split real / imaginary parts for the
-// download.  The kernels and their order are those of lsqt_moments, so the
-// moments are bit-identical to it.  Nothing in the scratch must start at
-// zero: every partial a reduction reads is written by the step before it.
+// per GPU: the recursion vectors (four where lsqt_moments advances two steps
+// per launch), a ring of moment partials reduced every kRing moments, the
+// moments, and their split real / imaginary parts for the download.  The
+// kernels and their order are those of lsqt_moments, so the moments are
+// bit-identical to it.  Nothing in the scratch must start at zero: every
+// partial a reduction reads is written by the step before it.
+#include <algorithm>
 #include <mutex>
+#include <utility>
 
 #include "lsqt_capi_internal.h"
 #include "lsqt_launch.h"
@@ -25,11 +28,14 @@ constexpr size_t kMaxTiles = (kNoAllocMaxSites + kBlock - 1) / kBlock;  // at 1 
 // offsets in double2 units; every region starts on a 256-byte boundary
 constexpr size_t kVecA = 0;
 constexpr size_t kVecB = kVecA + (size_t)kNoAllocMaxSites;
-constexpr size_t kRingAt = kVecB + (size_t)kNoAllocMaxSites;
+constexpr size_t kVecC = kVecB + (size_t)kNoAllocMaxSites;
+constexpr size_t kVecD = kVecC + (size_t)kNoAllocMaxSites;
+constexpr size_t kRingAt = kVecD + (size_t)kNoAllocMaxSites;
 constexpr size_t kMuAt = kRingAt + (size_t)kRing * kMaxTiles;
 constexpr size_t kSplitAt = kMuAt + (size_t)kNoAllocMaxMoments;  // re then im, as doubles
 constexpr size_t kScratchLen = kSplitAt + (size_t)kNoAllocMaxMoments;
-static_assert(kVecB % 16 == 0 && kRingAt % 16 == 0 && kMuAt % 16 == 0 && kSplitAt % 16 == 0,
+static_assert(kVecB % 16 == 0 && kVecC % 16 == 0 && kVecD % 16 == 0 && kRingAt % 16 == 0 &&
+                kMuAt % 16 == 0 && kSplitAt % 16 == 0,
               "scratch regions must stay 256-byte aligned");
 
 // The scratch of every GPU the library runs on (the loader gives each device
@@ -109,31 +115,54 @@ extern "C" int lsqt_moments_noalloc(lsqt_ham* h, const lsqt_vec* left, const lsq
   hipStream_t st = ctx->stream;
   const int n = h->d.n;
   const int tiles = hpsi_tiles(n, h->d.K);
-  if ((size_t)tiles > kMaxTiles)
-    return set_error(LSQT_ERR_INVALID, "%d moment partials per step exceed the ring", tiles);
+  const SweepMap& sw = h->sweep64;
+  const bool sweep = sw.S > 0 && M >= 4;  // as lsqt_moments
+  const int stride_tiles = sweep ? std::max(tiles, sw.tiles) : tiles;
+  if ((size_t)stride_tiles > kMaxTiles)
+    return set_error(LSQT_ERR_INVALID, "%d moment partials per step exceed the ring",
+                     stride_tiles);
   double2* A = base + kVecA;
   double2* B = base + kVecB;
+  double2* C = base + kVecC;
+  double2* D = base + kVecD;
   double2* ring = base + kRingAt;
   double2* mu = base + kMuAt;
   double* split = reinterpret_cast<double*>(base + kSplitAt);
-  const size_t stride = (size_t)tiles;
+  const size_t stride = (size_t)stride_tiles;
+  // moments 2 .. m_sweep - 1 come from two-step launches (sw.tiles partials)
+  const int m_sweep = sweep ? 2 + (M - 2) / 2 * 2 : 2;
   // the moment set of lsqt_moments, with moment m's partials in ring slot
-  // m % kRing and each full ring (and the last, partial one) reduced in turn
+  // m % kRing and each full ring (and the last, partial one) reduced in turn,
+  // in pieces of one partial count
   auto partial = [&](int m) { return ring + (size_t)(m % kRing) * stride; };
   auto reduce = [&](int m_end) {
     const int m0 = (m_end - 1) / kRing * kRing;
-    launch_reduce_partials(ring, stride, tiles, m_end - m0, mu + m0, st);
+    const int cuts[4] = {m0, std::min(std::max(2, m0), m_end),
+                         std::min(std::max(m_sweep, m0), m_end), m_end};
+    for (int p = 0; p < 3; ++p) {
+      const int a = cuts[p], b = cuts[p + 1];
+      const int count = p == 1 ? (sweep ? sw.tiles : tiles) : tiles;
+      launch_reduce_partials(ring + (size_t)(a - m0) * stride, stride, count, b - a, mu + a,
+                             st);
+    }
   };
   launch_copy(A, right->d, n, st);
   launch_dot(A, left->d, partial(0), n, st, hpsi_spt(h->d.K));
   launch_hpsi(h->d, A, nullptr, B, left->d, partial(1), false, true, st);
-  for (int m = 2; m < M; ++m) {
+  int m = 2;
+  for (; m < m_sweep; m += 2) {
+    // a pair never straddles a ring boundary: m is even and kRing is even
+    if (m % kRing == 0)
+      reduce(m);
+    launch_sweep2(h->d, sw, A, B, C, D, left->d, partial(m), partial(m + 1), st);
+    std::swap(A, C);
+    std::swap(B, D);
+  }
+  for (; m < M; ++m) {
     if (m % kRing == 0)
       reduce(m);
     launch_hpsi(h->d, B, A, A, left->d, partial(m), true, true, st);
-    double2* t = A;
-    A = B;
-    B = t;
+    std::swap(A, B);
   }
   reduce(M);
   k_split_moments<<<(M + kBlock - 1) / kBlock, kBlock, 0, st>>>(mu, split, split + M, M);

@@ -16,6 +16,7 @@
 // neighbour gather reads R*8 contiguous bytes and the Hamiltonian is read
 // once for all R vectors; VPW lanes share a site, each lane carries R/VPW
 // adjacent vectors.
+#include "kernels_sweep.h"
 #include "lsqt_device.h"
 #include "lsqt_launch.h"
 
@@ -616,6 +617,13 @@ __global__ __launch_bounds__(kBlock) void k_renormalise(double2* __restrict__ s,
 static inline int blocks_1d32(int n)
 {
   return (n + kBlock - 1) / kBlock;
+}
+
+void launch_sweep2_32(const HamDev& h, const SweepMap& sm, const float2* psi2,
+                      const float2* psi1, float2* out0, float2* out1, const float2* L,
+                      double2* part0, double2* part1, hipStream_t st)
+{
+  sweep_launch<float2>(h, sm, psi2, psi1, out0, out1, L, part0, part1, st);
 }
 
 int hpsi32_tiles(int n)

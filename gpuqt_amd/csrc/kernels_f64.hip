@@ -17,6 +17,7 @@
 // step: per-tile partial sums, reduced by a fixed wave64 shuffle tree, are
 // summed per moment in tile order at the end.  Nothing uses atomics, so
 // every result is run-to-run deterministic.
+#include "kernels_sweep.h"
 #include "lsqt_device.h"
 #include "lsqt_launch.h"
 
@@ -543,6 +544,45 @@ __global__ __launch_bounds__(kBlock) void k_pattern_mask(HamDev h, uint32_t* __r
   }
 }
 
+// Check of the two-step sweep's periodic slot table (sm.tab): per workgroup,
+// out[b] = 1 where the packed word of some slot of one of its sites differs
+// from the word the table gives (neighbour at the periodic slab and
+// position offsets, class, or an unused slot).
+__global__ __launch_bounds__(kBlock) void k_sweep_check(HamDev h, SweepMap sm, int* __restrict__ out)
+{
+  __shared__ int s_bad[kBlock / 64];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  int bad = 0;
+  if (i < h.n) {
+    const int x = div_by(i, sm.sdiv), pos = i - x * sm.S, o = pos % sm.p;
+    for (int q = 0; q < h.K; ++q) {
+      const uint32_t w = h.words[(size_t)q * h.stride + i];
+      const int4 e = sm.tab[q * sm.p + o];
+      if (!e.w) {
+        bad |= w != kEmptySlot;
+        continue;
+      }
+      int xs = x + e.x;
+      xs = xs < 0 ? xs + sm.nslab : (xs >= sm.nslab ? xs - sm.nslab : xs);
+      int ps = pos + e.y;
+      ps = ps < 0 ? ps + sm.S : (ps >= sm.S ? ps - sm.S : ps);
+      const uint32_t want = ((uint32_t)e.z << h.idx_bits) | (uint32_t)(xs * sm.S + ps);
+      bad |= w != want;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    bad |= __shfl_xor(bad, o, 64);
+  if ((threadIdx.x & 63) == 0)
+    s_bad[threadIdx.x >> 6] = bad;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w)
+      bad |= s_bad[w];
+    out[blockIdx.x] = bad;
+  }
+}
+
 // ==========================================================================
 // launchers
 // ==========================================================================
@@ -730,6 +770,23 @@ void launch_cheb1x(const double2* s1x, double2* s, double b1, int n, hipStream_t
 {
   if (n > 0)
     k_cheb1x<<<blocks_1d(n), kBlock, 0, st>>>(s1x, s, b1, n);
+}
+
+int sweep_check_blocks(int n)
+{
+  return blocks_1d(n);
+}
+
+void launch_sweep_check(const HamDev& h, const SweepMap& sm, int* out, hipStream_t st)
+{
+  k_sweep_check<<<blocks_1d(h.n), kBlock, 0, st>>>(h, sm, out);
+}
+
+void launch_sweep2(const HamDev& h, const SweepMap& sm, const double2* psi2,
+                   const double2* psi1, double2* out0, double2* out1, const double2* L,
+                   double2* part0, double2* part1, hipStream_t st)
+{
+  sweep_launch<double2>(h, sm, psi2, psi1, out0, out1, L, part0, part1, st);
 }
 
 int dot_tiles(int n)

@@ -46,6 +46,10 @@ struct lsqt_ham {
   int row = 0;
   // sites whose slot words differ from the slot pattern (d.pat set)
   long long pat_irregular = 0;
+  // two Chebyshev steps per launch (k_sweep2) with FP64 and with FP32
+  // storage: the slab geometry and the launch shape of each (S = 0: none)
+  lsqt::SweepMap sweep64{};
+  lsqt::SweepMap sweep32{};
   // segmented Hamiltonian: independent systems on sites [seg_starts[k],
   // seg_starts[k+1]); empty for one system
   std::vector<int> seg_starts;

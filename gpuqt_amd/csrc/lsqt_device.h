@@ -254,6 +254,40 @@ struct WindowMap {
   Divisor row_div;
 };
 
+// Two Chebyshev steps per launch on periodic slab lattices (k_sweep2):
+// slabs of S sites (n = S * nslab); slot q of a site at position pos of
+// slab x holds, for its orbital o = pos mod p (p divides S), the neighbour
+// at slab x + tab[q p + o].x (mod nslab) and position pos + tab[..].y
+// (mod S) with class tab[..].z, or nothing where tab[..].w is 0.  hh is the
+// largest |position offset|.  A tile is a chunk of cs positions (the last
+// chunk of a slab may be shorter; one chunk of the whole slab with hh = 0
+// when it fits a workgroup) of a run of slabs; tiles are numbered chunks
+// fastest (run * nch + chunk), runs split the slabs evenly.
+constexpr int kSweepThreads = 256;
+// slabs per loop iteration of k_sweep2 (measured on the 8.0M-site honeycomb:
+// 2, 4 and 8 gave 73.0-75.4, 73.7 and 102.8 us per FP64 step, 38.9-40.6,
+// 38.8 and 53.7 us per FP32 step)
+constexpr int kSweepBlock = 4;
+constexpr int kSweepMaxHalo = 8;
+constexpr int kSweepMinSites = 1 << 14;
+constexpr int kComputeUnits = 256;     // MI355X: 8 XCDs x 32 CUs
+constexpr size_t kLdsPerCu = 160 * 1024;
+struct SweepMap {
+  int S;       // sites per slab (0: no two-step sweep)
+  int nslab;
+  int hh;      // in-slab halo
+  int cs;      // positions per chunk
+  int nch;     // chunks per slab
+  int wmax;    // psi_{m-1} entries held per slab: cs + 4 hh
+  int nruns;   // runs of slabs per chunk (set per launch shape)
+  int tiles;   // nch * nruns
+  int blocks;  // tiles padded to a multiple of kXcds
+  Divisor sdiv;
+  int real;          // every used hopping has a zero imaginary part
+  int p;             // orbitals per pattern period
+  const int4* tab;   // [slot][orbital]: slab offset, position offset, class, used
+};
+
 struct TileMap {
   int n;
   int rows;

@@ -43,6 +43,21 @@ void launch_dot(const double2* psi, const double2* L, double2* partial, int n, h
 void launch_reduce_partials(const double2* partial, size_t stride, int tiles, int count,
                             double2* mu, hipStream_t st);
 
+// Two Chebyshev steps per launch on slab lattices (kernels_sweep.h): from
+// psi2 = psi_{m-2} and psi1 = psi_{m-1}, out0 = psi_m and out1 = psi_{m+1}
+// (four distinct buffers), with the tile partials of <L|psi_m> in part0 and
+// of <L|psi_{m+1}> in part1 (sm.tiles each)
+void launch_sweep2(const HamDev& h, const SweepMap& sm, const double2* psi2,
+                   const double2* psi1, double2* out0, double2* out1, const double2* L,
+                   double2* part0, double2* part1, hipStream_t st);
+void launch_sweep2_32(const HamDev& h, const SweepMap& sm, const float2* psi2,
+                      const float2* psi1, float2* out0, float2* out1, const float2* L,
+                      double2* part0, double2* part1, hipStream_t st);
+// the check of a sweep table against every packed word: one int per
+// workgroup of sweep_check_blocks(n), nonzero where a word differs
+int sweep_check_blocks(int n);
+void launch_sweep_check(const HamDev& h, const SweepMap& sm, int* out, hipStream_t st);
+
 // FP32-storage path (kernels_f32.hip)
 int hpsi32_tiles(int n);
 void launch_hpsi32(const HamDev& h, const float2* psi1, const float2* psi0, float2* out,

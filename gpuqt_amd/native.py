@@ -27,7 +27,10 @@ class HamInfo(C.Structure):
     _fields_ = [("n", C.c_int), ("max_neighbor", C.c_int), ("layout", C.c_int),
                 ("classes", C.c_int), ("index_bits", C.c_int), ("energy_max", C.c_double),
                 ("bytes_per_site_f64", C.c_double), ("slab", C.c_int), ("row", C.c_int),
-                ("pattern_period", C.c_int), ("pattern_irregular", C.c_double)]
+                ("pattern_period", C.c_int), ("pattern_irregular", C.c_double),
+                ("sweep_slab", C.c_int), ("sweep_halo", C.c_int), ("sweep_tiles", C.c_int),
+                ("sweep_tiles_f32", C.c_int), ("bytes_per_site_sweep_f64", C.c_double),
+                ("bytes_per_site_sweep_f32", C.c_double)]
 
 
 class Lattice(C.Structure):
@@ -83,6 +86,7 @@ SIGNATURES = {
     "lsqt_apply_commutator": (C.c_int, [_vp, _vp, _vp]),
     "lsqt_apply_current": (C.c_int, [_vp, _vp, _vp]),
     "lsqt_kernel_polynomial": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "lsqt_kernel_polynomial_pair": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "lsqt_chebyshev_01": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int]),
     "lsqt_chebyshev_2": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int]),
     "lsqt_chebyshev_1x": (C.c_int, [_vp, _vp, _vp, C.c_double]),
@@ -431,6 +435,9 @@ class Hamiltonian:
 
     def kernel_polynomial(self, s0, s1, s2):
         _check(load().lsqt_kernel_polynomial(self.h, s0.h, s1.h, s2.h))
+
+    def kernel_polynomial_pair(self, s0, s1, s2, s3):
+        _check(load().lsqt_kernel_polynomial_pair(self.h, s0.h, s1.h, s2.h, s3.h))
 
     def chebyshev_01(self, s0, s1, s, b0, b1, direction):
         _check(load().lsqt_chebyshev_01(self.h, s0.h, s1.h, s.h, b0, b1, direction))

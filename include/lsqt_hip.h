@@ -78,6 +78,19 @@ typedef struct {
                          by the moment kernels instead of loaded, except on
                          the irregular sites; LSQT_PATTERN=0 turns it off */
   double pattern_irregular; /* fraction of the sites with irregular words */
+  int sweep_slab;     /* slab in sites of the two-step moment sweep (0: the
+                         moment sets take one Chebyshev step per launch):
+                         every neighbour lies within one slab and within
+                         sweep_halo positions of its site, periodically, so
+                         two steps advance per launch from LDS; LSQT_SWEEP=0
+                         turns it off */
+  int sweep_halo;
+  int sweep_tiles;    /* workgroups (moment partials) per two-step launch,
+                         FP64 states */
+  int sweep_tiles_f32;  /* the same with FP32 states */
+  double bytes_per_site_sweep_f64; /* compulsory HBM bytes per site of one
+                                      two-step launch (two site-updates) */
+  double bytes_per_site_sweep_f32;
 } lsqt_ham_info;
 
 const char* lsqt_strerror(int status);
@@ -220,6 +233,14 @@ int lsqt_apply_current(lsqt_ham* h, const lsqt_vec* in, lsqt_vec* out);
 /* Hamiltonian::kernel_polynomial: s2 = 2 H~ s1 - s0 (src/hamiltonian.cu:970-984);
  * s2 may be s0 */
 int lsqt_kernel_polynomial(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1, lsqt_vec* s2);
+/* (no counterpart in the reference: two calls of Hamiltonian::kernel_polynomial,
+ * src/hamiltonian.cu:970-984, in one launch)  s2 = 2 H~ s1 - s0, then
+ * s3 = 2 H~ s2 - s1, bit-identical to the two calls, on a Hamiltonian with a
+ * two-step sweep (lsqt_ham_info.sweep_slab > 0; LSQT_ERR_UNSUPPORTED
+ * otherwise); s0, s1, s2 and s3 must be distinct.  The moment sets use the
+ * same launch. */
+int lsqt_kernel_polynomial_pair(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1,
+                                lsqt_vec* s2, lsqt_vec* s3);
 /* Hamiltonian::chebyshev_01 (src/hamiltonian.cu:447-460) */
 int lsqt_chebyshev_01(lsqt_ham* h, const lsqt_vec* s0, const lsqt_vec* s1, lsqt_vec* s,
                       double bessel_0, double bessel_1, int direction);
@@ -250,9 +271,9 @@ int lsqt_moments_to_vec(lsqt_ham* h, const lsqt_vec* left, const lsqt_vec* right
  * allocate device memory: code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc.  The same kernels in the same order, so
  * the same moments bit for bit, but every buffer beyond the inputs and
- * outputs (two recursion vectors, the moment partials, the moments) lives in
- * one fixed scratch array per GPU, held from the library's first context on
- * that GPU until the process ends: 38 MiB, for up to LSQT_NOALLOC_MAX_SITES
+ * outputs (up to four recursion vectors, the moment partials, the moments)
+ * lives in one fixed scratch array per GPU, held from the library's first
+ * context on that GPU until the process ends: 70 MiB, for up to LSQT_NOALLOC_MAX_SITES
  * sites and LSQT_NOALLOC_MAX_MOMENTS moments; larger inputs are rejected
  * with LSQT_ERR_INVALID.  No hipMalloc / hipFree or other allocating call.
  * All calls on one GPU share its scratch: calls that overlap on one GPU (two

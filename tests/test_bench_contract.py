@@ -1,8 +1,8 @@
 """bench.py's contract on the CPU: more than one GPU is refused with a
 message, and the committed HBM-traffic profile holds the kernels whose
-`traffic` the JSON line reports (the FP64 fused step of the default
-workload, the tuned block kernels of the 160^3 cube and the uneven batch's
-block steps)."""
+`traffic` the JSON line reports (the FP64 and FP32 two-step sweeps and
+one-step kernels of the default workload, the tuned block kernels of the
+160^3 cube and the uneven batch's block steps)."""
 import os
 import subprocess
 import sys
@@ -26,6 +26,11 @@ def test_traffic_profile_entries():
     # 8.0M (slot pattern: words of the 0.1 % irregular sites, 1 mask bit)
     assert fp64 is not None and abs(fp64 / (72.137 * 8_000_000) - 1.0) < 0.05
     assert bench.profiled_traffic("k_hpsi32<1,3,true,true,true>") is not None
+    # the two-step sweeps: 88 B (FP64) and 44 B (FP32) x 8.0M sites per launch
+    for name, per_site in (("k_sweep2<double2,3,4,false>", 88.0),
+                           ("k_sweep2<float2,3,4,true>", 44.0)):
+        t = bench.profiled_traffic(name)
+        assert t is not None and abs(t / (per_site * 8_000_000) - 1.0) < 0.05, name
     for mode in (1, 2):  # <L|psi> and product-identity block steps, sliding window
         t = bench.profiled_traffic(f"k_block_win<1,6,2,true,{mode}>")
         assert t is not None and t > 0
