@@ -89,6 +89,9 @@ def test_device_build_large_honeycomb(gpu_ctx, tmp_path):
     Hd = native.Hamiltonian.from_lattice(gpu_ctx, lat, a["potential"], m.energy_max)
     td = time.perf_counter() - t
     print(f"\n  upload of host arrays {th:.2f} s, device build {td:.3f} s", flush=True)
+    # both get the same two-step sweep (slabs of one x cell), so the same
+    # partial sums
+    assert Hh.info().sweep_slab == Hd.info().sweep_slab == 4000
     re, im = m.random_state()
     mus = [H.moments(H.vector(re, im), H.vector(re, im), 16) for H in (Hh, Hd)]
     _bits(mus[0].real, mus[1].real, "moments")

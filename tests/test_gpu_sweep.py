@@ -50,6 +50,7 @@ def _lattice(nx, ny, offsets, rng, phase=None, disorder=1.0):
     return arrays, n, K
 
 
+CHAINS = [(-1, 0), (1, 0)]
 SQUARE = [(-1, 0), (1, 0), (0, -1), (0, 1)]
 TRIANGULAR = [(-1, 0), (1, 0), (0, -1), (0, 1), (1, -1), (-1, 1)]
 
@@ -77,6 +78,11 @@ def _systems(tmp_path):
     out.append(("square 130x300", a, n, K, 5.0, 300, 1, None))
     a, n, K = _lattice(70, 777, TRIANGULAR, rng)              # 777: 4 chunks, uneven last
     out.append(("triangular 70x777", a, n, K, 7.0, 777, 1, None))
+    a, n, K = _lattice(160, 256, CHAINS, rng)                 # 2 slots, no in-slab bonds
+    out.append(("chains 160x256", a, n, K, 4.0, 256, 0, None))
+    a, n, K = _lattice(90, 400, SQUARE, rng,                  # complex, the same in every slab
+                       phase=lambda x: 0.7 + 0 * x)
+    out.append(("square 90x400 with a uniform complex phase", a, n, K, 5.0, 400, 1, None))
     a, n, K = _lattice(640, 300, SQUARE, rng,                 # phases change per slab
                        phase=lambda x: 2 * np.pi * 0.001 * x)
     out.append(("peierls square 640x300", a, n, K, 5.0, 0, 0, None))
