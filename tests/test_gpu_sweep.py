@@ -107,6 +107,11 @@ def test_sweep_geometry(gpu_ctx, tmp_path, monkeypatch):
     arrays, n = gi.cubic_peierls_arrays(32)
     H = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, 6, 15.0)
     assert H.info().sweep_slab == 0
+    vs = [H.vector() for _ in range(4)]
+    with pytest.raises(native.LsqtError):  # no two-step launch without a sweep
+        H.kernel_polynomial_pair(*vs)
+    for v in vs:
+        v.close()
     H.close()
     # vacancies renumber the sites
     m = _honeycomb_model(tmp_path, "hc_vac", 100, 60, vacancies=40)
