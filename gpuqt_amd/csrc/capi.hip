@@ -322,6 +322,8 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
 {
   h->sweep64 = SweepMap{};
   h->sweep32 = SweepMap{};
+  h->bsweep.clear();
+  h->bsweep_tab = nullptr;
   const HamDev& d = h->d;
   const char* env = getenv("LSQT_SWEEP");
   if ((env && env[0] == '0') || d.layout == kFull || d.n < kSweepMinSites)
@@ -382,6 +384,7 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
       g.sdiv = make_divisor((int)S);
       g.p = period;
       g.tab = dev_tab;
+      g.halo = halo;
       if (whole) {
         g.hh = 0;
         g.nch = 1;
@@ -409,6 +412,14 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
       h->allocations.push_back(dev_tab);  // kept with the Hamiltonian
       h->sweep64 = sweep_shape(g, false);
       h->sweep32 = sweep_shape(g, true);
+      // the same geometry serves the block path's two-step sweep
+      BlockSweepSeg b{};
+      b.S = g.S;
+      b.nslab = g.nslab;
+      b.halo = halo;
+      b.p = period;
+      h->bsweep.assign(1, b);
+      h->bsweep_tab = dev_tab;
       found = true;
     }
   }
@@ -416,6 +427,92 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
   (void)hipFree(dev_bad);
   if (!found)
     (void)hipFree(dev_tab);
+}
+
+// The block path's two-step sweep of several pristine lattices side by side
+// (a segmented Hamiltonian built by lsqt_ham_create_lattices): lattice k's
+// slabs are its x cells (S = Ny Nz orbitals sites), its table period the
+// orbital count, and hopping i of orbital m goes to slab offset dx and
+// position offset (dy Nz + dz) orbitals + m' - m, taken periodically within
+// the slab.  A device pass checks every packed word of every lattice against
+// its table; the sweep is kept only if all lattices pass (LSQT_SWEEP=0 in
+// the environment turns it off).  Any failure leaves the Hamiltonian without
+// it.
+void detect_block_sweep(lsqt_ham* h, int segments, const lsqt_lattice* lats,
+                        const int* const* site_index, const std::vector<int>& starts,
+                        const std::vector<std::vector<int>>& cls)
+{
+  h->bsweep.clear();
+  h->bsweep_tab = nullptr;
+  const HamDev& d = h->d;
+  const char* env = getenv("LSQT_SWEEP");
+  if ((env && env[0] == '0') || d.layout == kFull)
+    return;
+  const int K = d.K;
+  std::vector<BlockSweepSeg> segs;
+  std::vector<int4> tab;
+  for (int s = 0; s < segments; ++s) {
+    const lsqt_lattice* L = &lats[s];
+    if (site_index && site_index[s])
+      return;  // vacancies: no periodic table
+    const long long S = (long long)L->cells[1] * L->cells[2] * L->orbitals;
+    const int No = L->orbitals, Ks = L->max_neighbor, Nz = L->cells[2];
+    if (L->cells[0] < 4 || S < 8 || S > INT_MAX / 4)
+      return;
+    BlockSweepSeg g{};
+    g.base = starts[s];
+    g.S = (int)S;
+    g.nslab = L->cells[0];
+    g.p = No;
+    g.tab = (int)tab.size();
+    tab.resize(tab.size() + (size_t)K * No, make_int4(0, 0, 0, 0));
+    for (int m = 0; m < No; ++m)
+      for (int i = 0; i < L->hop_count[m]; ++i) {
+        const int* c = L->hop_cell + 3 * (m * Ks + i);
+        if (c[0] < -1 || c[0] > 1)
+          return;
+        long long dp = ((long long)c[1] * Nz + c[2]) * No + L->hop_orbital[m * Ks + i] - m;
+        dp %= S;
+        if (dp < 0)
+          dp += S;
+        if (2 * dp > S)
+          dp -= S;
+        g.halo = std::max(g.halo, (int)std::llabs(dp));
+        tab[(size_t)g.tab + (size_t)i * No + m] = make_int4(c[0], (int)dp, cls[s][m * Ks + i], 1);
+      }
+    segs.push_back(g);
+  }
+  int4* dev_tab = nullptr;
+  int* dev_bad = nullptr;
+  if (hipMalloc(&dev_tab, sizeof(int4) * tab.size()) != hipSuccess)
+    return;
+  bool ok = hipMemcpy(dev_tab, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice) ==
+            hipSuccess;
+  int most = 0;
+  for (const BlockSweepSeg& g : segs)
+    most = std::max(most, block_sweep_check_blocks(g));
+  ok = ok && hipMalloc(&dev_bad, sizeof(int) * (size_t)most * segs.size()) == hipSuccess;
+  std::vector<int> bad((size_t)most * segs.size(), 0);
+  if (ok) {
+    for (size_t s = 0; s < segs.size(); ++s)
+      launch_block_sweep_check(d, segs[s], dev_tab, dev_bad + s * most, h->ctx->stream);
+    for (size_t s = 0; s < segs.size(); ++s)
+      ok = ok && hipMemcpyAsync(bad.data() + s * most, dev_bad + s * most,
+                                sizeof(int) * (size_t)block_sweep_check_blocks(segs[s]),
+                                hipMemcpyDeviceToHost, h->ctx->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(h->ctx->stream) == hipSuccess;
+    ok = ok && std::none_of(bad.begin(), bad.end(), [](int b) { return b != 0; });
+  }
+  (void)hipGetLastError();
+  if (dev_bad)
+    (void)hipFree(dev_bad);
+  if (!ok) {
+    (void)hipFree(dev_tab);
+    return;
+  }
+  h->allocations.push_back(dev_tab);
+  h->bsweep = segs;
+  h->bsweep_tab = dev_tab;
 }
 
 // The slot pattern of a packed Hamiltonian.  On a periodic lattice whose
@@ -924,6 +1021,8 @@ extern "C" int lsqt_ham_create_lattices(lsqt_ctx* ctx, int segments, const lsqt_
   detect_slot_pattern(h);
   if (segments == 1 && (!site_index || !site_index[0]))
     detect_sweep(h, {(long long)lats[0].cells[1] * lats[0].cells[2] * lats[0].orbitals});
+  else if (segments > 1)
+    detect_block_sweep(h, segments, lats, site_index, starts, cls);
   h->host_pot.assign(potential, potential + n);
   h->host_tab = tab;
   h->host_tab_xx = tab_xx;
@@ -966,6 +1065,8 @@ extern "C" int lsqt_ham_create_segmented(lsqt_ctx* ctx, int segments, const int*
                            LSQT_LAYOUT_PACKED, &h));
   h->seg_starts.assign(starts, starts + segments + 1);
   h->slab = h->row = 0;  // lattice tile schedules do not cross segments
+  if (segments > 1)
+    h->bsweep.clear();  // a geometry per system is known only for lattices
   *out = h;
   return LSQT_OK;
 }
@@ -1024,6 +1125,7 @@ extern "C" int lsqt_ham_get_info(const lsqt_ham* h, lsqt_ham_info* info)
     info->bytes_per_site_sweep_f64 = 0.0;
     info->bytes_per_site_sweep_f32 = 0.0;
   }
+  info->block_sweep_systems = (int)h->bsweep.size();
   return LSQT_OK;
 }
 

@@ -45,6 +45,9 @@ struct BlockScratch {
   float2* A = nullptr;  // moment recursion ping-pong, evolution temporaries
   float2* B = nullptr;
   size_t elems = 0;
+  float2* C = nullptr;  // the two outputs of two-step launches
+  float2* D = nullptr;
+  size_t cd_elems = 0;
   float2* F[3] = {nullptr, nullptr, nullptr};  // left vector; [X, U] temporaries
   size_t f_elems = 0;
   double2* S[3] = {nullptr, nullptr, nullptr};  // FP64 states of the time drivers
@@ -84,19 +87,32 @@ struct SegTable {
 // a captured moment set: every argument its kernels were recorded with
 struct GraphKey {
   int self, M, R, W, rows, blocked, depth, tiles;
-  const void *left, *A, *B, *ring, *mu;
+  const void *left, *A, *B, *ring, *mu, *C, *D, *sweep;
   bool operator<(const GraphKey& o) const
   {
-    return std::tie(self, M, R, W, rows, blocked, depth, tiles, left, A, B, ring, mu) <
-           std::tie(o.self, o.M, o.R, o.W, o.rows, o.blocked, o.depth, o.tiles, o.left, o.A,
-                    o.B, o.ring, o.mu);
+    return std::tie(self, M, R, W, rows, blocked, depth, tiles, left, A, B, ring, mu, C, D,
+                    sweep) < std::tie(o.self, o.M, o.R, o.W, o.rows, o.blocked, o.depth,
+                                      o.tiles, o.left, o.A, o.B, o.ring, o.mu, o.C, o.D,
+                                      o.sweep);
   }
+};
+
+// The launch shape of the block path's two-step sweep for one column count:
+// its tile map (on the device) and the tile map that reduces its partials
+// per system.
+struct BlockSweepShape {
+  bool ok = false;
+  int W = 0;
+  BlockSweepMap map{};
+  TileMap red{};
+  std::vector<void*> mem;
 };
 
 struct lsqt_block_state {
   BlockScratch s;
   std::map<int, BlockShape> forced;  // R -> shape set by the caller
   std::map<int, SegTable> seg;       // rows -> tile table
+  std::map<int, BlockSweepShape> bsw;  // columns -> two-step sweep shape
   // moment sets are captured once and replayed as hipGraphs (LSQT_GRAPHS=0
   // or lsqt_block_set_graphs(h, 0) launches them one by one instead)
   bool graphs = true;
@@ -118,10 +134,13 @@ void lsqt::block_release(lsqt_ham* h)
     (void)hipGraphExecDestroy(kv.second);
   h->blk->graph.clear();
   BlockScratch& s = h->blk->s;
-  for (void* p : {(void*)s.A, (void*)s.B, (void*)s.ring, (void*)s.mu, (void*)s.F[0],
-                  (void*)s.F[1], (void*)s.F[2], (void*)s.S[0], (void*)s.S[1], (void*)s.S[2],
-                  (void*)s.norms})
+  for (void* p : {(void*)s.A, (void*)s.B, (void*)s.C, (void*)s.D, (void*)s.ring, (void*)s.mu,
+                  (void*)s.F[0], (void*)s.F[1], (void*)s.F[2], (void*)s.S[0], (void*)s.S[1],
+                  (void*)s.S[2], (void*)s.norms})
     if (p)
+      (void)hipFree(p);
+  for (auto& kv : h->blk->bsw)
+    for (void* p : kv.second.mem)
       (void)hipFree(p);
   for (auto& kv : h->blk->seg)
     for (void* p : {(void*)kv.second.table, (void*)kv.second.tile_seg,
@@ -463,6 +482,142 @@ int tile_map(lsqt_ham* h, int R, const BlockShape& c, TileMap* tm)
   return LSQT_OK;
 }
 
+// The two-step sweep's launch shape for blocks of `cols` columns (*out null
+// where the Hamiltonian has no sweep geometry for each of its systems or
+// the columns do not fit): W = 2 columns per lane up to 16 columns, else 4,
+// so cols / W lanes per site and kBlock W / cols positions per slab row; a
+// chunk holds them less 4 halo positions (the whole slab where it fits).
+// Runs of slabs per chunk: about twice as many tiles as resident
+// workgroups, and the dispatch order puts the longest runs first, in groups
+// of 8 neighbouring chunks dealt to the XCDs in turn, so that the systems of
+// an uneven batch finish together.
+int block_sweep_shape(lsqt_ham* h, int cols, BlockSweepShape** out)
+{
+  *out = nullptr;
+  if (!h->blk)
+    h->blk = new lsqt_block_state();
+  if (h->bsweep.empty() || !h->bsweep_tab || cols < 2 || cols > 64 || !valid_R(cols) ||
+      (int)h->bsweep.size() != segments(h))
+    return LSQT_OK;
+  if (!(h->d.K == 2 || h->d.K == 3 || h->d.K == 4 || h->d.K == 6))
+    return LSQT_OK;
+  auto it = h->blk->bsw.find(cols);
+  if (it != h->blk->bsw.end()) {
+    *out = it->second.ok ? &it->second : nullptr;
+    return LSQT_OK;
+  }
+  BlockSweepShape& sh = h->blk->bsw[cols];
+  const int W = cols <= 16 ? 2 : 4;
+  const int P = kBlock * W / cols;
+  std::vector<BlockSweepSeg> segs = h->bsweep;
+  long long units = 0;
+  for (size_t k = 0; k < segs.size(); ++k) {
+    BlockSweepSeg& g = segs[k];
+    const long long lo = h->seg_starts.empty() ? 0 : h->seg_starts[k];
+    const long long hi = h->seg_starts.empty() ? h->d.n : h->seg_starts[k + 1];
+    if (g.base != lo || (long long)g.S * g.nslab != hi - lo || g.nslab < 4)
+      return LSQT_OK;
+    if (g.S <= P) {
+      g.hh = 0;
+      g.cs = g.S;
+      g.nch = 1;
+    } else if (8 * g.halo <= P) {
+      g.hh = g.halo;
+      g.cs = P - 4 * g.hh;
+      g.nch = (g.S + g.cs - 1) / g.cs;
+    } else {
+      return LSQT_OK;
+    }
+    units += (long long)g.nch * g.nslab;
+  }
+  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / block_sweep_lds(W)));
+  const long long run_len = std::max(16ll, units / (2ll * kComputeUnits * per_cu));
+  int tiles = 0;
+  struct Entry {
+    int tile, seg, chunk, run, len;
+  };
+  std::vector<Entry> list;
+  for (size_t k = 0; k < segs.size(); ++k) {
+    BlockSweepSeg& g = segs[k];
+    long long runs = (g.nslab + run_len / 2) / run_len;
+    g.nruns = (int)std::max(1ll, std::min(runs, (long long)g.nslab / 2));
+    g.tile0 = tiles;
+    if ((long long)tiles + (long long)g.nch * g.nruns > INT_MAX / 8)
+      return LSQT_OK;
+    tiles += g.nch * g.nruns;
+    for (int r = 0; r < g.nruns; ++r) {
+      const int len = (int)((long long)(r + 1) * g.nslab / g.nruns) -
+                      (int)((long long)r * g.nslab / g.nruns);
+      for (int c = 0; c < g.nch; ++c)
+        list.push_back(Entry{g.tile0 + r * g.nch + c, (int)k, c, r, len});
+    }
+  }
+  std::stable_sort(list.begin(), list.end(),
+                   [](const Entry& a, const Entry& b) { return a.len > b.len; });
+  constexpr size_t kGroup = 8;
+  std::vector<std::vector<int4>> per_xcd(kXcds);
+  for (size_t i = 0; i < list.size(); ++i)
+    per_xcd[(i / kGroup) % kXcds].push_back(
+      make_int4(list[i].tile, list[i].seg, list[i].chunk, list[i].run));
+  size_t per = 0;
+  for (const auto& v : per_xcd)
+    per = std::max(per, v.size());
+  std::vector<int4> order(per * kXcds, make_int4(-1, 0, 0, 0));
+  for (int x = 0; x < kXcds; ++x)
+    std::copy(per_xcd[x].begin(), per_xcd[x].end(), order.begin() + (size_t)x * per);
+  std::vector<int> seg_tiles;
+  for (const BlockSweepSeg& g : segs)
+    seg_tiles.push_back(g.tile0);
+  seg_tiles.push_back(tiles);
+  auto up = [&](const void* src, size_t bytes, void** dst) -> int {
+    LSQT_HIP(hipMalloc(dst, bytes));
+    sh.mem.push_back(*dst);
+    LSQT_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return LSQT_OK;
+  };
+  void *d_seg, *d_order, *d_tiles;
+  LSQT_TRY(up(segs.data(), sizeof(BlockSweepSeg) * segs.size(), &d_seg));
+  LSQT_TRY(up(order.data(), sizeof(int4) * order.size(), &d_order));
+  LSQT_TRY(up(seg_tiles.data(), sizeof(int) * seg_tiles.size(), &d_tiles));
+  sh.W = W;
+  sh.map.tiles = tiles;
+  sh.map.blocks = (int)per * kXcds;
+  sh.map.segs = (int)segs.size();
+  sh.map.seg = (const BlockSweepSeg*)d_seg;
+  sh.map.order = (const int4*)d_order;
+  sh.map.tab = h->bsweep_tab;
+  sh.map.seg_tiles = (const int*)d_tiles;
+  sh.red = TileMap{};
+  sh.red.n = h->d.n;
+  sh.red.tiles = tiles;
+  sh.red.seg_tiles = (const int*)d_tiles;
+  sh.red.segs = (int)segs.size();
+  sh.ok = true;
+  *out = &sh;
+  return LSQT_OK;
+}
+
+// scratch of a two-step moment set of `cols` columns: the outputs C and D,
+// and a ring of kRing slots of stride_tiles partials per column
+int block_sweep_scratch(lsqt_ham* h, int cols, int stride_tiles, BlockScratch* s)
+{
+  const size_t elems = (size_t)h->d.n * cols;
+  if (s->cd_elems < elems) {
+    if (s->C)
+      (void)hipFree(s->C);
+    if (s->D)
+      (void)hipFree(s->D);
+    s->C = s->D = nullptr;
+    s->cd_elems = 0;
+    LSQT_TRY(block_alloc(h->ctx, elems, &s->C));
+    LSQT_TRY(block_alloc(h->ctx, elems, &s->D));
+    s->cd_elems = elems;
+  }
+  LSQT_TRY(ensure_device((void**)&s->ring, &s->ring_cap,
+                         sizeof(double2) * (size_t)kRing * stride_tiles * cols));
+  return LSQT_OK;
+}
+
 void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, const float2* L,
                    float2*& A, float2*& B, double2* ring, double2* mu, int M, int from, int to,
                    hipStream_t st)
@@ -497,39 +652,64 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, c
 // advance together and S_0 = <L|R>, S_{2m-1} = <T_{m-1} L|T_m R>, S_{2m} =
 // <T_m L|T_m R>.  The moments follow from T_{2m} = 2 T_m^2 - T_0 and
 // T_{2m+1} = 2 T_m T_{m+1} - T_1 with T_m Hermitian: mu_0 = S_0, mu_1 = S_1,
-// mu_k = 2 S_k - mu_{k mod 2}.  Moment k >= 1 goes to ring slot (k - 1) mod
-// kRing; M/2 steps give moments 0..M-1.
+// mu_k = 2 S_k - mu_{k mod 2}.  M/2 steps give moments 0..M-1.
+// With a two-step sweep shape (sw, outputs C and D), steps 2 .. are taken
+// two per launch (a last odd one and step 1 by the one-step kernels).
+// Moments k >= 1 fill ring slots [0, used) from moment k_first, slots
+// stride tiles apart; a ring is reduced when the next launch's slots would
+// not fit or the launches change their tile map, and at the end.
 void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, float2*& A,
                         float2*& B, double2* ring, double2* mu, int M, hipStream_t st,
-                        bool pairs = false)
+                        bool pairs = false, const BlockSweepShape* sw = nullptr,
+                        float2* C = nullptr, float2* D = nullptr)
 {
-  const int tiles = tm.tiles;
-  const size_t slot = (size_t)tiles * R;
   const int cols = pairs ? 2 * R : R;
   const int mode = pairs ? kStepPair : kStepSelf;
+  const int stride = sw ? std::max(tm.tiles, sw->map.tiles) : tm.tiles;
+  const size_t slot = (size_t)stride * R;
   launch_block_dot(A, A, cols, tm, ring, st, pairs);
-  launch_block_reduce(ring, tm, R, M, 0, 1, mu, st);
+  launch_block_reduce(ring, tm, R, M, 0, 1, mu, st, stride);
+  int used = 0, k_first = 1;
+  const TileMap* cur = &tm;
+  auto flush = [&]() {
+    launch_block_reduce(ring, *cur, R, M, k_first, std::min(used, M - k_first), mu, st, stride);
+    k_first += used;
+    used = 0;
+  };
+  auto take = [&](int count, const TileMap* map) {
+    if (used > 0 && (map != cur || used + count > kRing))
+      flush();
+    cur = map;
+    double2* p = ring + (size_t)used * slot;
+    used += count;
+    return p;
+  };
   const int steps = M / 2;
-  for (int m = 1; m <= steps; ++m) {
-    double2* odd = ring + (size_t)((2 * m - 2) % kRing) * slot;
-    double2* even = ring + (size_t)((2 * m - 1) % kRing) * slot;
+  for (int m = 1; m <= steps;) {
+    if (sw && m >= 2 && m + 1 <= steps) {
+      double2* p = take(4, &sw->red);
+      double2* const part[4] = {p, p + slot, p + 2 * slot, p + 3 * slot};
+      launch_block_sweep2(h->d, sw->map, cols, sw->W, pairs, A, B, C, D, part, st);
+      std::swap(A, C);
+      std::swap(B, D);
+      m += 2;
+      continue;
+    }
+    double2* p = take(2, &tm);
     if (m == 1) {
-      launch_block_step(h->d, cols, c.W, c.depth == 2, tm, A, nullptr, B, nullptr, odd, even,
+      launch_block_step(h->d, cols, c.W, c.depth == 2, tm, A, nullptr, B, nullptr, p, p + slot,
                         false, mode, st);
     } else {
-      launch_block_step(h->d, cols, c.W, c.depth == 2, tm, B, A, A, nullptr, odd, even, true,
+      launch_block_step(h->d, cols, c.W, c.depth == 2, tm, B, A, A, nullptr, p, p + slot, true,
                         mode, st);
       float2* t = A;
       A = B;
       B = t;
     }
-    const int j = 2 * m - 1;  // ring slot of moment 2m
-    if (j % kRing == kRing - 1 || m == steps) {
-      const int k0 = j - j % kRing + 1;
-      const int count = std::min(j - (k0 - 1) + 1, M - k0);
-      launch_block_reduce(ring, tm, R, M, k0, count, mu, st);
-    }
+    ++m;
   }
+  if (used > 0)
+    flush();
 }
 
 int pick_config(lsqt_ham* h, int R, const float2* left, const float2* right, BlockShape* out,
@@ -621,17 +801,28 @@ int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Block
     LSQT_HIP(hipMemcpyAsync(A, right, sizeof(float2) * (size_t)h->d.n * R,
                             hipMemcpyDeviceToDevice, st));
   }
+  // two steps per launch where the Hamiltonian's systems allow (product
+  // identities only: their moments need no left vector)
+  BlockSweepShape* sw = nullptr;
+  if (mode != kDirect && M / 2 >= 3) {
+    const int cols = mode == kPairProducts ? 2 * R : R;
+    LSQT_TRY(block_sweep_shape(h, cols, &sw));
+    if (sw)
+      LSQT_TRY(block_sweep_scratch(h, cols, std::max(tm.tiles, sw->map.tiles), s));
+  }
   auto enqueue = [&]() {
     float2* a = A;
     float2* b = B;
     if (mode == kDirect)
       enqueue_steps(h, R, c, tm, left, a, b, s->ring, s->mu, M, 0, M, st);
     else
-      enqueue_self_steps(h, R, c, tm, a, b, s->ring, s->mu, M, st, mode == kPairProducts);
+      enqueue_self_steps(h, R, c, tm, a, b, s->ring, s->mu, M, st, mode == kPairProducts, sw,
+                         s->C, s->D);
   };
   if (h->blk->graphs) {
     const GraphKey key{mode, M, R, c.W, c.rows, c.blocked, c.depth, tm.tiles,
-                       mode == kDirect ? left : nullptr, A, B, s->ring, s->mu};
+                       mode == kDirect ? left : nullptr, A, B, s->ring, s->mu,
+                       sw ? s->C : nullptr, sw ? s->D : nullptr, sw};
     auto it = h->blk->graph.find(key);
     if (it == h->blk->graph.end()) {
       hipGraph_t g = nullptr;

@@ -725,11 +725,19 @@ __global__ __launch_bounds__(kBlock) void k_block_win(
   store_products<IPM, W>(acc, vpw, R, r0, tile, s_red, partial, partial2);
 }
 
+}  // namespace lsqt
+
+#include "kernels_block_sweep.h"
+
+namespace lsqt {
+
 // mu[(k*R + r)*M + m0 + i] = sum over the tiles t of segment k of
-// partial[i][t][r], in tile order, i < gridDim.x, k = blockIdx.z (one
-// segment of all tiles without a segment table)
+// partial[i][t][r] (slots `stride` tiles apart), in tile order, i <
+// gridDim.x, k = blockIdx.z (one segment of all tiles without a segment
+// table)
 __global__ __launch_bounds__(kBlock) void k_block_reduce(const double2* __restrict__ partial,
-                                                         int tiles, int R, int M, int m0,
+                                                         int tiles, int stride, int R, int M,
+                                                         int m0,
                                                          const int* __restrict__ seg_tiles,
                                                          double2* __restrict__ mu)
 {
@@ -738,7 +746,7 @@ __global__ __launch_bounds__(kBlock) void k_block_reduce(const double2* __restri
   const int k = blockIdx.z;
   const int t0 = seg_tiles ? seg_tiles[k] : 0;
   const int t1 = seg_tiles ? seg_tiles[k + 1] : tiles;
-  const double2* p = partial + (size_t)blockIdx.x * tiles * R;
+  const double2* p = partial + (size_t)blockIdx.x * stride * R;
   mu += (size_t)k * R * M;
   for (int r = blockIdx.y; r < R; r += gridDim.y) {
     double re = 0.0, im = 0.0;
@@ -1520,6 +1528,65 @@ void launch_block_step(const HamDev& h, int R, int W, bool db, const TileMap& tm
 #undef LSQTB_STEP
 }
 
+size_t block_sweep_lds(int W)
+{
+  return W == 2 ? block_sweep_lds_bytes<2, 4>() : block_sweep_lds_bytes<4, 1>();
+}
+
+// W = 2 (blocks of 4 slabs) or 4 (one slab at a time: the rings of wider
+// entries in the same 49 KB of LDS), slot counts 2, 3, 4 and 6
+void launch_block_sweep2(const HamDev& h, const BlockSweepMap& sm, int cols, int W, bool pairs,
+                         const float2* p2, const float2* p1, float2* o0, float2* o1,
+                         double2* const (&part)[4], hipStream_t st)
+{
+  const size_t lds = block_sweep_lds(W);
+#define LSQTB_SW(KK, WW, BB)                                                                \
+  do {                                                                                      \
+    if (pairs)                                                                              \
+      k_block_sweep2<KK, WW, kPair, BB><<<sm.blocks, kBlock, lds, st>>>(                    \
+        h, sm, cols, p2, p1, o0, o1, part[0], part[1], part[2], part[3]);                   \
+    else                                                                                    \
+      k_block_sweep2<KK, WW, kSelf, BB><<<sm.blocks, kBlock, lds, st>>>(                    \
+        h, sm, cols, p2, p1, o0, o1, part[0], part[1], part[2], part[3]);                   \
+  } while (0)
+#define LSQTB_SW_K(KK)        \
+  do {                        \
+    if (W == 2)               \
+      LSQTB_SW(KK, 2, 4);     \
+    else                      \
+      LSQTB_SW(KK, 4, 1);     \
+  } while (0)
+  switch (h.K) {
+    case 2:
+      LSQTB_SW_K(2);
+      break;
+    case 3:
+      LSQTB_SW_K(3);
+      break;
+    case 4:
+      LSQTB_SW_K(4);
+      break;
+    case 6:
+      LSQTB_SW_K(6);
+      break;
+    default:
+      break;
+  }
+#undef LSQTB_SW_K
+#undef LSQTB_SW
+}
+
+int block_sweep_check_blocks(const BlockSweepSeg& g)
+{
+  return (int)(((long long)g.S * g.nslab + kBlock - 1) / kBlock);
+}
+
+void launch_block_sweep_check(const HamDev& h, const BlockSweepSeg& g, const int4* tab, int* out,
+                              hipStream_t st)
+{
+  k_block_sweep_check<<<block_sweep_check_blocks(g), kBlock, 0, st>>>(h, g, tab, out);
+}
+
 void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& tm,
                       double2* partial, hipStream_t st, bool pairs)
 {
@@ -1531,13 +1598,15 @@ void launch_block_dot(const float2* psi, const float2* L, int R, const TileMap& 
 }
 
 void launch_block_reduce(const double2* partial, const TileMap& tm, int R, int M, int m0,
-                         int count, double2* mu, hipStream_t st)
+                         int count, double2* mu, hipStream_t st, int stride_tiles)
 {
   if (count <= 0)
     return;
   const int segs = tm.seg_tiles ? tm.segs : 1;
   dim3 grid(count, R < 8 ? R : 8, segs);
-  k_block_reduce<<<grid, kBlock, 0, st>>>(partial, tm.tiles, R, M, m0, tm.seg_tiles, mu);
+  k_block_reduce<<<grid, kBlock, 0, st>>>(partial, tm.tiles,
+                                          stride_tiles > 0 ? stride_tiles : tm.tiles, R, M, m0,
+                                          tm.seg_tiles, mu);
 }
 
 void launch_block_evo(const HamDev& h, int R, int W, const TileMap& tm, int op,

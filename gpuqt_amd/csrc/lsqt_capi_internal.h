@@ -50,6 +50,11 @@ struct lsqt_ham {
   // storage: the slab geometry and the launch shape of each (S = 0: none)
   lsqt::SweepMap sweep64{};
   lsqt::SweepMap sweep32{};
+  // two steps per launch on the block path (k_block_sweep2): the slab
+  // geometry of every system (empty: none) and their slot tables on the
+  // device (owned by `allocations`); launch shapes are made per block width
+  std::vector<lsqt::BlockSweepSeg> bsweep;
+  const int4* bsweep_tab = nullptr;
   // segmented Hamiltonian: independent systems on sites [seg_starts[k],
   // seg_starts[k+1]); empty for one system
   std::vector<int> seg_starts;

@@ -286,6 +286,35 @@ struct SweepMap {
   int real;          // every used hopping has a zero imaginary part
   int p;             // orbitals per pattern period
   const int4* tab;   // [slot][orbital]: slab offset, position offset, class, used
+  int halo;          // largest |position offset| of the table
+};
+
+// Two steps per launch for blocks of vectors (k_block_sweep2): the slab
+// geometry of one system of a (possibly segmented) Hamiltonian, and its
+// chunks and runs for one launch shape.
+struct BlockSweepSeg {
+  int base;   // first site
+  int S;      // sites per slab
+  int nslab;  // slabs (periodic)
+  int halo;   // largest |position offset| of its table
+  int p;      // table period (orbitals)
+  int tab;    // offset of its [slot][orbital] table in BlockSweepMap::tab
+  // launch shape
+  int hh;     // in-slab halo held per chunk (0: the whole slab is one chunk)
+  int cs;     // positions per chunk
+  int nch;    // chunks per slab
+  int nruns;  // runs of slabs per chunk
+  int tile0;  // first tile (moment partial) of the system
+};
+
+struct BlockSweepMap {
+  int tiles;             // moment partials per launch (tiles of all systems)
+  int blocks;            // workgroups: dispatch slots, a multiple of kXcds
+  int segs;
+  const BlockSweepSeg* seg;
+  const int4* order;     // per dispatch slot: tile, system, chunk, run (tile < 0: idle)
+  const int4* tab;       // the systems' slot tables, as SweepMap::tab
+  const int* seg_tiles;  // first tile of each system, then the tile count
 };
 
 struct TileMap {

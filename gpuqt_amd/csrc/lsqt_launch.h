@@ -137,8 +137,23 @@ void launch_vector_major(const float2* src, int n, int R, double* re, double* im
 // dst = S_z src (sign of the odd sites flipped) for every vector of a block
 void launch_block_sz(const float2* src, int n, int R, float2* dst, hipStream_t st);
 // mu[(k*R + r)*M + m0 + i] for segment k, i < count (partials of moment
-// m0 + i at slot i)
+// m0 + i at slot i; slots stride_tiles * R apart, tm.tiles * R where 0)
 void launch_block_reduce(const double2* partial, const TileMap& tm, int R, int M, int m0,
-                         int count, double2* mu, hipStream_t st);
+                         int count, double2* mu, hipStream_t st, int stride_tiles = 0);
+// Two steps per launch on the block path (kernels_block_sweep.h): from p2 =
+// psi_{m-2} and p1 = psi_{m-1} of `cols` columns, o0 = psi_m and o1 =
+// psi_{m+1} (four distinct buffers) with the self products (pairs: the pair
+// products of cols / 2 pairs) of both steps: part[0], part[1] of step 1,
+// part[2], part[3] of step 2 (sm.tiles partials each).  W = 2 for cols <= 16,
+// 4 for cols 32 and 64; cols / W lanes per site.
+size_t block_sweep_lds(int W);
+void launch_block_sweep2(const HamDev& h, const BlockSweepMap& sm, int cols, int W, bool pairs,
+                         const float2* p2, const float2* p1, float2* o0, float2* o1,
+                         double2* const (&part)[4], hipStream_t st);
+// the check of one system's sweep table against its packed words: one int
+// per workgroup of block_sweep_check_blocks(g), nonzero where a word differs
+int block_sweep_check_blocks(const BlockSweepSeg& g);
+void launch_block_sweep_check(const HamDev& h, const BlockSweepSeg& g, const int4* tab, int* out,
+                              hipStream_t st);
 
 }  // namespace lsqt

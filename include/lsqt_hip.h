@@ -91,6 +91,10 @@ typedef struct {
   double bytes_per_site_sweep_f64; /* compulsory HBM bytes per site of one
                                       two-step launch (two site-updates) */
   double bytes_per_site_sweep_f32;
+  int block_sweep_systems; /* systems (segments) whose block moment sets of
+                              self and pair products take two steps per
+                              launch (0: none; either every system of the
+                              Hamiltonian or none) */
 } lsqt_ham_info;
 
 const char* lsqt_strerror(int status);
