@@ -35,9 +35,12 @@ def test_traffic_profile_entries():
         t = bench.profiled_traffic(f"k_block_win<1,6,2,true,{mode}>")
         assert t is not None and t > 0
     # the uneven batch's pair and self-product steps (8 realisations, 8.97M
-    # sites, 400 and 208 B per site) at their compulsory traffic within 5 %
+    # sites, 400 and 208 B per site) and their two-step sweeps (516 and 260 B
+    # per site and launch) at their compulsory traffic within 5 %
     for name, per_site in (("k_block_step<1,3,2,true,3,false,false>", 400.0),
-                           ("k_block_step<1,3,2,true,2,false,false>", 208.0)):
+                           ("k_block_step<1,3,2,true,2,false,false>", 208.0),
+                           ("k_block_sweep2<3,2,3,4>", 516.0),
+                           ("k_block_sweep2<3,2,2,4>", 260.0)):
         t = bench.profiled_traffic(name)
         assert t is not None and abs(t / (per_site * 8_973_112) - 1.0) < 0.05, name
     assert bench.profiled_traffic("no_such_kernel<1>") is None
