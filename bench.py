@@ -816,6 +816,22 @@ def main():
                volume=model.volume, M=M, msd=args.time_steps > 0)
 
     sweep = info.sweep_slab > 0
+    # steps per launch of the moment sets' dominant kernel: four (k_sweepn)
+    # or two (k_sweep2) on a periodic lattice, else one (k_hpsi)
+    steps64 = max(info.sweep_steps, 1)
+    steps32 = max(info.sweep_steps_f32, 1)
+
+    def launches_per_set(steps):
+        # the moment set's launches of its dominant kernel: M - 1 one-step
+        # launches, (M - 2) // 2 two-step launches, or (M - 4) // 4 four-step
+        # launches after the two-step one at moment 2 (the set's other
+        # launches -- mu_0, mu_1, the remainder, the reduction -- are counted
+        # into their time, which overstates it by under 0.3 %)
+        if steps == 4 and M >= 8:
+            return (M - 4) // 4
+        if steps >= 2 and M >= 4:
+            return (M - 2) // 2
+        return M - 1
 
     def timed(precision, steps, warmup):
         for w in range(warmup):
@@ -833,26 +849,25 @@ def main():
             log(f"{precision} step {s} done")
         ctx.sync()
         elapsed = time.perf_counter() - t0
-        # the moment set's launches of its dominant kernel: M - 1 one-step
-        # launches, or (M - 2) // 2 two-step sweep launches (the set's other
-        # launches -- mu_0, mu_1, a last odd step, the reduction -- are
-        # counted into their time, which overstates it by under 0.2 %)
-        per_set = (M - 2) // 2 if sweep else M - 1
+        per_set = launches_per_set(steps32 if precision == "f32" else steps64)
         avg_launch_ms = acc["moment_ms"] / max(acc["moment_sets"] * per_set, 1)
         return total, elapsed, avg_launch_ms, rows
 
     total_updates, elapsed, avg_launch_ms, rows = timed("f64", args.steps, args.warmup)
     rows = np.array(rows)
     value = total_updates / elapsed
-    # dominant kernel: the two-step sweep (two site-updates per site per
+    # dominant kernel: the four-step sweep (four site-updates per site per
     # launch) where the lattice has one, else the fused one-step kernel
-    updates_per_site = 2 if sweep else 1
+    updates_per_site = steps64
     bytes_per_site = info.bytes_per_site_sweep_f64 if sweep else info.bytes_per_site_f64
     bytes_per_launch = bytes_per_site * model.n
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     default_workload = (args.nx, args.ny) == (2000, 1000) and info.layout == 2 \
         and info.max_neighbor == 3
-    kname = "k_sweep2<double2,3,4,false>" if sweep else "k_hpsi<1,3,2,true,true,true>"
+    kname = {4: "k_sweepn<double2,3,4,1,false>", 2: "k_sweep2<double2,3,4,false>"}.get(
+        steps64, "k_hpsi<1,3,2,true,true,true>")
+    kname32 = {4: "k_sweepn<float2,3,4,2,true>", 2: "k_sweep2<float2,3,4,true>"}.get(
+        steps32, "k_hpsi32")
     traffic = profiled_traffic(kname) if default_workload else None
     # the same launch time priced on other byte counts: the one-step kernel's
     # (what it moves for the same site-updates on this layout) and the
@@ -895,14 +910,13 @@ def main():
             "speedup_vs_f64": (u32 / e32) / value,
             "dtype": "f32 states, f64 arithmetic and accumulation",
             "roofline": {"bound": "hbm",
-                         "kernel": ("k_sweep2<float2,3,4,true> (two steps per launch)" if sweep
-                                    else "k_hpsi32"),
+                         "kernel": kname32 + (f" ({steps32} steps per launch)" if sweep else ""),
                          "achieved": ach32, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": ach32 / HBM_PEAK_GBS,
-                         "traffic": (profiled_traffic("k_sweep2<float2,3,4,true>")
+                         "traffic": (profiled_traffic(kname32)
                                      if sweep and default_workload else None),
                          "bytes_per_site_per_launch": bpu32,
-                         "site_updates_per_site_per_launch": updates_per_site,
+                         "site_updates_per_site_per_launch": steps32,
                          "bytes_per_launch_algorithmic": bpu32 * model.n,
                          "avg_launch_us": l32 * 1e3},
             "error_vs_f64": {
@@ -947,9 +961,13 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("k_sweep2<double2,3,4,false> (two Chebyshev steps per launch: psi_m "
-                       "and psi_{m+1} from psi_{m-2}, psi_{m-1}, four slabs at a time from "
-                       "LDS, fused with <L|psi_m> and <L|psi_{m+1}>)" if sweep else
+            "kernel": (kname + " (four Chebyshev steps per launch: psi_m .. psi_{m+3} from "
+                       "psi_{m-2}, psi_{m-1}, each level one slab behind the one before in LDS "
+                       "rings, the last two written, fused with <L|psi> of all four)"
+                       if steps64 == 4 else
+                       kname + " (two Chebyshev steps per launch: psi_m and psi_{m+1} from "
+                       "psi_{m-2}, psi_{m-1}, four slabs at a time from LDS, fused with "
+                       "<L|psi_m> and <L|psi_{m+1}>)" if sweep else
                        "k_hpsi (2 H~ psi_{m-1} - psi_{m-2} fused with <L|psi_m>; slot words "
                        "computed from the lattice pattern, 2 sites per thread)"),
             "achieved": achieved,

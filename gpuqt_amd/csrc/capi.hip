@@ -299,7 +299,42 @@ SweepMap sweep_shape(SweepMap g, bool f32)
   if (const char* env = getenv("LSQT_SWEEP_TILES"))
     if (atoll(env) > 0)
       target = atoll(env);
-  long long runs = (target + g.nch - 1) / g.nch;
+  // at most one tile per resident workgroup: a second, partial wave of
+  // tiles would take as long as the first
+  long long runs = target / g.nch;
+  runs = std::max(1ll, std::min(runs, (long long)(g.nslab / 2)));
+  g.nruns = (int)runs;
+  g.tiles = g.nch * g.nruns;
+  g.blocks = (g.tiles + kXcds - 1) / kXcds * kXcds;
+  return g;
+}
+
+// The deeper sweep's shape for the same geometry: ns steps per launch in
+// blocks of nb slabs; a chunk holds 2 ns h fewer positions than a workgroup
+// (none where that leaves fewer than half), runs as sweep_shape.
+SweepMap sweep_shape_n(SweepMap g, bool f32, int ns, int nb)
+{
+  const bool whole = g.hh == 0 && g.nch == 1 && g.cs == g.S;
+  if (!whole) {
+    g.cs = kSweepThreads - 2 * ns * g.hh;
+    if (g.cs < kSweepThreads / 2) {
+      g.S = 0;
+      return g;
+    }
+    g.nch = (g.S + g.cs - 1) / g.cs;
+  }
+  g.wmax = g.cs + 2 * ns * g.hh;
+  g.ns = ns;
+  g.nb = nb;
+  const int per_cu =
+    (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / sweepn_lds(f32, ns, nb)));
+  long long target = (long long)kComputeUnits * per_cu;
+  if (const char* env = getenv("LSQT_SWEEP_TILES"))
+    if (atoll(env) > 0)
+      target = atoll(env);
+  // at most one tile per resident workgroup: a second, partial wave of
+  // tiles would take as long as the first
+  long long runs = target / g.nch;
   runs = std::max(1ll, std::min(runs, (long long)(g.nslab / 2)));
   g.nruns = (int)runs;
   g.tiles = g.nch * g.nruns;
@@ -322,6 +357,8 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
 {
   h->sweep64 = SweepMap{};
   h->sweep32 = SweepMap{};
+  h->sweepn64 = SweepMap{};
+  h->sweepn32 = SweepMap{};
   h->bsweep.clear();
   h->bsweep_tab = nullptr;
   const HamDev& d = h->d;
@@ -412,6 +449,16 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
       h->allocations.push_back(dev_tab);  // kept with the Hamiltonian
       h->sweep64 = sweep_shape(g, false);
       h->sweep32 = sweep_shape(g, true);
+      // four steps per launch on the same geometry (LSQT_SWEEP_STEPS=2 in
+      // the environment keeps two): FP64 one slab per loop iteration, FP32
+      // blocks of two (measured on the 8.0M-site honeycomb: 46.7 and 27.3 us
+      // per step against 74.4 and 38.3 with two steps per launch; three, five,
+      // six and eight steps and the other block sizes were slower)
+      const char* steps = getenv("LSQT_SWEEP_STEPS");
+      if (!(steps && atoi(steps) == 2)) {
+        h->sweepn64 = sweep_shape_n(h->sweep64, false, 4, 1);
+        h->sweepn32 = sweep_shape_n(h->sweep32, true, 4, 2);
+      }
       // the same geometry serves the block path's two-step sweep
       BlockSweepSeg b{};
       b.S = g.S;
@@ -1126,6 +1173,9 @@ extern "C" int lsqt_ham_get_info(const lsqt_ham* h, lsqt_ham_info* info)
     info->bytes_per_site_sweep_f32 = 0.0;
   }
   info->block_sweep_systems = (int)h->bsweep.size();
+  info->sweep_steps = h->sweepn64.S ? h->sweepn64.ns : (h->sweep64.S ? 2 : 0);
+  info->sweep_steps_f32 = h->sweepn32.S ? h->sweepn32.ns : (h->sweep32.S ? 2 : 0);
+  info->sweep_tiles_deep = h->sweepn64.S ? h->sweepn64.tiles : 0;
   return LSQT_OK;
 }
 
@@ -1492,8 +1542,11 @@ int lsqt::moments_enqueue(lsqt_ham* h, const double2* left, const double2* right
   // two steps per launch where the lattice allows (the same per-site values;
   // one partial per sweep tile)
   const SweepMap& sw = h->sweep64;
+  const SweepMap& sn = h->sweepn64;
   const bool sweep = sw.S > 0 && M >= 4;
-  const int stride_tiles = sweep ? std::max(tiles, sw.tiles) : tiles;
+  const bool deep = sweep && sn.S > 0 && M >= 4 + sn.ns;
+  const int stride_tiles =
+    std::max(tiles, std::max(sweep ? sw.tiles : 0, deep ? sn.tiles : 0));
   LSQT_TRY(ctx_reserve_moments(ctx, M, stride_tiles));
   LSQT_TRY(ham_scratch(h, sweep ? 4 : 2));
   double2* A = h->scratch[0];
@@ -1504,17 +1557,33 @@ int lsqt::moments_enqueue(lsqt_ham* h, const double2* left, const double2* right
   launch_dot(A, left, ctx->partial, n, st, hpsi_spt(h->d.K));
   // psi_1 = H~ psi_0; mu_1
   launch_hpsi(h->d, A, nullptr, B, left, ctx->partial + stride, false, true, st);
-  int m = 2;
+  // several steps per launch where the lattice allows (the same per-site
+  // values; one partial per sweep tile): moments 2, 3 by a two-step launch,
+  // then NS at a time from 4, then two at a time, the rest one at a time
+  int m = 2, m_deep0 = 2, m_deep = 2;
   if (sweep) {
-    // psi_m, psi_{m+1} from psi_{m-2} (A), psi_{m-1} (B) into C, D
     double2* C = h->scratch[2];
     double2* D = h->scratch[3];
-    for (; m + 1 < M; m += 2) {
+    auto two = [&]() {
       launch_sweep2(h->d, sw, A, B, C, D, left, ctx->partial + (size_t)m * stride,
                     ctx->partial + (size_t)(m + 1) * stride, st);
       std::swap(A, C);
       std::swap(B, D);
+      m += 2;
+    };
+    if (deep) {
+      two();
+      m_deep0 = m;
+      for (; m + sn.ns - 1 < M; m += sn.ns) {
+        launch_sweepn(h->d, sn, A, B, C, D, left, ctx->partial + (size_t)m * stride, stride,
+                      st);
+        std::swap(A, C);
+        std::swap(B, D);
+      }
     }
+    m_deep = m;
+    while (m + 1 < M)
+      two();
   }
   const int m_sweep = m;
   // psi_m = 2 H~ psi_{m-1} - psi_{m-2}, written over psi_{m-2}
@@ -1523,8 +1592,12 @@ int lsqt::moments_enqueue(lsqt_ham* h, const double2* left, const double2* right
     std::swap(A, B);
   }
   launch_reduce_partials(ctx->partial, stride, tiles, 2, ctx->mu_dev, st);
-  launch_reduce_partials(ctx->partial + 2 * stride, stride, sweep ? sw.tiles : tiles,
-                         m_sweep - 2, ctx->mu_dev + 2, st);
+  launch_reduce_partials(ctx->partial + 2 * stride, stride, sw.tiles, m_deep0 - 2,
+                         ctx->mu_dev + 2, st);
+  launch_reduce_partials(ctx->partial + (size_t)m_deep0 * stride, stride, sn.tiles,
+                         m_deep - m_deep0, ctx->mu_dev + m_deep0, st);
+  launch_reduce_partials(ctx->partial + (size_t)m_deep * stride, stride, sw.tiles,
+                         m_sweep - m_deep, ctx->mu_dev + m_deep, st);
   launch_reduce_partials(ctx->partial + (size_t)m_sweep * stride, stride, tiles, M - m_sweep,
                          ctx->mu_dev + m_sweep, st);
   LSQT_HIP(hipGetLastError());

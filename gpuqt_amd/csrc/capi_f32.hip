@@ -103,10 +103,14 @@ int lsqt::moments_f32_enqueue(lsqt_ham* h, const double2* left, const double2* r
   hipStream_t st = ctx->stream;
   const int n = h->d.n;
   const int tiles = hpsi32_tiles(n);
-  // two steps per launch where the lattice allows
+  // several steps per launch where the lattice allows (the schedule of
+  // moments_enqueue)
   const SweepMap& sw = h->sweep32;
+  const SweepMap& sn = h->sweepn32;
   const bool sweep = sw.S > 0 && M >= 4;
-  const int stride_tiles = sweep ? std::max(tiles, sw.tiles) : tiles;
+  const bool deep = sweep && sn.S > 0 && M >= 4 + sn.ns;
+  const int stride_tiles =
+    std::max(tiles, std::max(sweep ? sw.tiles : 0, deep ? sn.tiles : 0));
   LSQT_TRY(ctx_reserve_moments(ctx, M, stride_tiles));
   LSQT_TRY(f32_prepare(h, 5));
   float2* A = h->f32->v[0];
@@ -117,18 +121,32 @@ int lsqt::moments_f32_enqueue(lsqt_ham* h, const double2* left, const double2* r
   launch_to_f32(right, A, n, st);
   launch_dot32(A, L, ctx->partial, n, st);
   launch_hpsi32(h->d, A, nullptr, B, L, ctx->partial + stride, false, true, st);
-  int m = 2;
+  int m = 2, m_deep0 = 2, m_deep = 2;
   if (sweep) {
-    // psi_m, psi_{m+1} from psi_{m-2} (A), psi_{m-1} (B) into the buffers of
-    // the [X, U] recursion (free during a moment set)
+    // the outputs go to the buffers of the [X, U] recursion (free during a
+    // moment set)
     float2* C = h->f32->v[2];
     float2* D = h->f32->v[3];
-    for (; m + 1 < M; m += 2) {
+    auto two = [&]() {
       launch_sweep2_32(h->d, sw, A, B, C, D, L, ctx->partial + (size_t)m * stride,
                        ctx->partial + (size_t)(m + 1) * stride, st);
       std::swap(A, C);
       std::swap(B, D);
+      m += 2;
+    };
+    if (deep) {
+      two();
+      m_deep0 = m;
+      for (; m + sn.ns - 1 < M; m += sn.ns) {
+        launch_sweepn_32(h->d, sn, A, B, C, D, L, ctx->partial + (size_t)m * stride, stride,
+                         st);
+        std::swap(A, C);
+        std::swap(B, D);
+      }
     }
+    m_deep = m;
+    while (m + 1 < M)
+      two();
   }
   const int m_sweep = m;
   for (; m < M; ++m) {
@@ -136,8 +154,12 @@ int lsqt::moments_f32_enqueue(lsqt_ham* h, const double2* left, const double2* r
     std::swap(A, B);
   }
   launch_reduce_partials(ctx->partial, stride, tiles, 2, ctx->mu_dev, st);
-  launch_reduce_partials(ctx->partial + 2 * stride, stride, sweep ? sw.tiles : tiles,
-                         m_sweep - 2, ctx->mu_dev + 2, st);
+  launch_reduce_partials(ctx->partial + 2 * stride, stride, sw.tiles, m_deep0 - 2,
+                         ctx->mu_dev + 2, st);
+  launch_reduce_partials(ctx->partial + (size_t)m_deep0 * stride, stride, sn.tiles,
+                         m_deep - m_deep0, ctx->mu_dev + m_deep0, st);
+  launch_reduce_partials(ctx->partial + (size_t)m_deep * stride, stride, sw.tiles,
+                         m_sweep - m_deep, ctx->mu_dev + m_deep, st);
   launch_reduce_partials(ctx->partial + (size_t)m_sweep * stride, stride, tiles, M - m_sweep,
                          ctx->mu_dev + m_sweep, st);
   LSQT_HIP(hipGetLastError());

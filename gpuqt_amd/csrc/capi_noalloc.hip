@@ -5,8 +5,8 @@
 // lsqt_moments keeps its recursion vectors on the Hamiltonian and its moment
 // partials on the context, both allocated on first use.  Here every buffer
 // beyond the caller's inputs and outputs lives in `scratch`, one fixed array
-// per GPU: the recursion vectors (four where lsqt_moments advances two steps
-// per launch), a ring of moment partials reduced every kRing moments, the
+// per GPU: the recursion vectors (four where lsqt_moments advances several
+// steps per launch), a ring of moment partials reduced every kRing moments, the
 // moments, and their split real / imaginary parts for the download.  The
 // kernels and their order are those of lsqt_moments, so the moments are
 // bit-identical to it.  Nothing in the scratch must start at zero: every
@@ -116,8 +116,11 @@ extern "C" int lsqt_moments_noalloc(lsqt_ham* h, const lsqt_vec* left, const lsq
   const int n = h->d.n;
   const int tiles = hpsi_tiles(n, h->d.K);
   const SweepMap& sw = h->sweep64;
+  const SweepMap& sn = h->sweepn64;
   const bool sweep = sw.S > 0 && M >= 4;  // as lsqt_moments
-  const int stride_tiles = sweep ? std::max(tiles, sw.tiles) : tiles;
+  const bool deep = sweep && sn.S > 0 && M >= 4 + sn.ns;
+  const int stride_tiles =
+    std::max(tiles, std::max(sweep ? sw.tiles : 0, deep ? sn.tiles : 0));
   if ((size_t)stride_tiles > kMaxTiles)
     return set_error(LSQT_ERR_INVALID, "%d moment partials per step exceed the ring",
                      stride_tiles);
@@ -129,35 +132,58 @@ extern "C" int lsqt_moments_noalloc(lsqt_ham* h, const lsqt_vec* left, const lsq
   double2* mu = base + kMuAt;
   double* split = reinterpret_cast<double*>(base + kSplitAt);
   const size_t stride = (size_t)stride_tiles;
-  // moments 2 .. m_sweep - 1 come from two-step launches (sw.tiles partials)
-  const int m_sweep = sweep ? 2 + (M - 2) / 2 * 2 : 2;
-  // the moment set of lsqt_moments, with moment m's partials in ring slot
-  // m % kRing and each full ring (and the last, partial one) reduced in turn,
-  // in pieces of one partial count
+  // the launch schedule of lsqt_moments: moments 2, 3 by a two-step launch,
+  // [4, m_deep) by sn.ns-step launches, [m_deep, m_sweep) by two-step ones,
+  // the rest one step per launch (a part is empty where its shape is absent)
+  const int m_deep0 = deep ? 4 : 2;
+  const int m_deep = deep ? 4 + (M - 4) / sn.ns * sn.ns : 2;
+  const int m_sweep = sweep ? m_deep + (M - m_deep) / 2 * 2 : 2;
+  auto tiles_of = [&](int m) {
+    return m < 2 ? tiles
+                 : (m < m_deep0 ? sw.tiles : (m < m_deep ? sn.tiles : (m < m_sweep ? sw.tiles : tiles)));
+  };
+  // moment m's partials in ring slot m % kRing; each full ring (and the last,
+  // partial one) is reduced in turn, in pieces of one partial count.  No
+  // launch straddles a ring boundary: multi-step launches start at multiples
+  // of their step count (two, or four from moment 4) and kRing is a multiple
+  // of both.
   auto partial = [&](int m) { return ring + (size_t)(m % kRing) * stride; };
   auto reduce = [&](int m_end) {
     const int m0 = (m_end - 1) / kRing * kRing;
-    const int cuts[4] = {m0, std::min(std::max(2, m0), m_end),
-                         std::min(std::max(m_sweep, m0), m_end), m_end};
-    for (int p = 0; p < 3; ++p) {
+    int cuts[6] = {m0, 2, m_deep0, m_deep, m_sweep, m_end};
+    for (int c = 1; c < 5; ++c)
+      cuts[c] = std::min(std::max(cuts[c], m0), m_end);
+    for (int p = 0; p < 5; ++p) {
       const int a = cuts[p], b = cuts[p + 1];
-      const int count = p == 1 ? (sweep ? sw.tiles : tiles) : tiles;
-      launch_reduce_partials(ring + (size_t)(a - m0) * stride, stride, count, b - a, mu + a,
-                             st);
+      if (b > a)
+        launch_reduce_partials(ring + (size_t)(a - m0) * stride, stride, tiles_of(a), b - a,
+                               mu + a, st);
     }
   };
   launch_copy(A, right->d, n, st);
   launch_dot(A, left->d, partial(0), n, st, hpsi_spt(h->d.K));
   launch_hpsi(h->d, A, nullptr, B, left->d, partial(1), false, true, st);
   int m = 2;
-  for (; m < m_sweep; m += 2) {
-    // a pair never straddles a ring boundary: m is even and kRing is even
+  auto two = [&]() {
     if (m % kRing == 0)
       reduce(m);
     launch_sweep2(h->d, sw, A, B, C, D, left->d, partial(m), partial(m + 1), st);
     std::swap(A, C);
     std::swap(B, D);
+    m += 2;
+  };
+  if (deep) {
+    two();
+    for (; m < m_deep; m += sn.ns) {
+      if (m % kRing == 0)
+        reduce(m);
+      launch_sweepn(h->d, sn, A, B, C, D, left->d, partial(m), stride, st);
+      std::swap(A, C);
+      std::swap(B, D);
+    }
   }
+  while (m < m_sweep)
+    two();
   for (; m < M; ++m) {
     if (m % kRing == 0)
       reduce(m);

@@ -17,6 +17,7 @@
 // once for all R vectors; VPW lanes share a site, each lane carries R/VPW
 // adjacent vectors.
 #include "kernels_sweep.h"
+#include "kernels_sweepn.h"
 #include "lsqt_device.h"
 #include "lsqt_launch.h"
 
@@ -617,6 +618,16 @@ __global__ __launch_bounds__(kBlock) void k_renormalise(double2* __restrict__ s,
 static inline int blocks_1d32(int n)
 {
   return (n + kBlock - 1) / kBlock;
+}
+
+void launch_sweepn_32(const HamDev& h, const SweepMap& sm, const float2* psi2,
+                      const float2* psi1, float2* out0, float2* out1, const float2* L,
+                      double2* part, size_t part_stride, hipStream_t st)
+{
+  // the shape sweep_shape_n makes: four steps, blocks of 2 slabs
+  const float2* left = L ? L : psi1;
+  if (sm.ns == 4 && sm.nb == 2)
+    sweepn_launch<float2, 4, 2>(h, sm, psi2, psi1, out0, out1, left, part, part_stride, st);
 }
 
 void launch_sweep2_32(const HamDev& h, const SweepMap& sm, const float2* psi2,

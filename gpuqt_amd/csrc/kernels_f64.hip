@@ -18,6 +18,7 @@
 // summed per moment in tile order at the end.  Nothing uses atomics, so
 // every result is run-to-run deterministic.
 #include "kernels_sweep.h"
+#include "kernels_sweepn.h"
 #include "lsqt_device.h"
 #include "lsqt_launch.h"
 
@@ -780,6 +781,22 @@ int sweep_check_blocks(int n)
 void launch_sweep_check(const HamDev& h, const SweepMap& sm, int* out, hipStream_t st)
 {
   k_sweep_check<<<blocks_1d(h.n), kBlock, 0, st>>>(h, sm, out);
+}
+
+void launch_sweepn(const HamDev& h, const SweepMap& sm, const double2* psi2,
+                   const double2* psi1, double2* out0, double2* out1, const double2* L,
+                   double2* part, size_t part_stride, hipStream_t st)
+{
+  // the shape sweep_shape_n makes: four steps, blocks of 1 slab
+  const double2* left = L ? L : psi1;
+  if (sm.ns == 4 && sm.nb == 1)
+    sweepn_launch<double2, 4, 1>(h, sm, psi2, psi1, out0, out1, left, part, part_stride, st);
+}
+
+size_t sweepn_lds(bool f32, int ns, int nb)
+{
+  const size_t v = f32 ? sizeof(float2) : sizeof(double2);
+  return v * ns * (nb + 2) * kSweepThreads + sizeof(double) * (kSweepThreads / 64) * 2 * ns;
 }
 
 void launch_sweep2(const HamDev& h, const SweepMap& sm, const double2* psi2,

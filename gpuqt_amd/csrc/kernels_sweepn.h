@@ -1,0 +1,277 @@
+// kernels_sweepn.h -- NS Chebyshev steps per launch on periodic slab
+// lattices (included by kernels_f64.hip and kernels_f32.hip after
+// kernels_sweep.h, whose SweepIO it uses).
+//
+// The deeper form of k_sweep2: from psi_{m-2} and psi_{m-1} a launch makes
+// psi_m .. psi_{m+NS-1} ("levels" 1 .. NS; level 0 is psi_{m-1}) and writes
+// only the last two, the ones the recursion continues from, with the moment
+// products <L|psi> of every level fused.  Per site and launch: psi_{m-2},
+// psi_{m-1}, L and the potential read, two states written -- the bytes of one
+// two-step launch for NS site-updates.
+//
+// A workgroup owns a chunk of positions of a run of slabs (as k_sweep2),
+// one position per thread: level l is computed on the chunk and
+// (NS - l) h positions either side (h the largest |position offset|; the
+// halos are computed again by the neighbouring chunks' tiles), and it lags
+// level l - 1 by one slab, since l at slab q needs l - 1 at q - 1 .. q + 1
+// (and l - 2 at q for the recursion).  Each level 0 .. NS - 1 keeps the last
+// B + 2 slabs it made in an LDS ring; the potential and left vector of the
+// last B + NS - 1 slabs are kept per thread in registers (a level reaches
+// slab q up to NS - 1 slabs after level 1 did), shifted by B slabs per
+// iteration.  A loop iteration loads B new slabs of level 0 (and psi_{m-2},
+// potential, left vector for level 1's B slabs), then computes each level's
+// B slabs in turn, one barrier between levels; every global load of an
+// iteration is used in it, before the first store.
+//
+// Per site the arithmetic is k_sweep2's (the one-step kernels', in the same
+// order): the FP64 instantiation gives the bits of k_hpsi and of the
+// reference at every level; only the moment partial sums group the sites by
+// sweep tile.
+#pragma once
+
+#include "kernels_sweep.h"
+
+namespace lsqt {
+
+// static LDS of k_sweepn<V, K, NS, B>: the level rings and the tile sums
+template <typename V, int NS, int B>
+constexpr size_t sweepn_lds_bytes()
+{
+  return sizeof(V) * NS * (B + 2) * kSweepThreads + sizeof(double) * (kSweepThreads / 64) * 2 * NS;
+}
+
+template <typename V, int K, int NS, int B, bool FAST>
+__global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
+                                                          const V* __restrict__ p2,
+                                                          const V* __restrict__ p1,
+                                                          V* __restrict__ o0, V* __restrict__ o1,
+                                                          const V* __restrict__ L,
+                                                          double2* __restrict__ part,
+                                                          size_t part_stride)
+{
+  static_assert(NS >= 3, "two steps per launch: k_sweep2");
+  using IO = SweepIO<V>;
+  constexpr int T = kSweepThreads;
+  constexpr int RR = B + 2;       // rows of a level ring
+  constexpr int RB = B + NS - 1;  // slabs of potential and left vector held
+  __shared__ V s_lv[NS][RR][T];   // levels 0 .. NS - 1
+  __shared__ double s_red[T / 64][2 * NS];
+  const int tile = xcd_tile(blockIdx.x, sm.blocks);
+  if (tile >= sm.tiles)
+    return;
+  const int t = threadIdx.x;
+  const int S = sm.S, nslab = sm.nslab, hh = sm.hh;
+  const int chunk = tile % sm.nch;
+  const int run = tile / sm.nch;
+  const int c0 = chunk * sm.cs;
+  const int clen = min(sm.cs, S - c0);
+  const int wlen = clen + 2 * NS * hh;  // positions held per slab (<= T)
+  int ws = c0 - NS * hh;                // first position held (periodic)
+  while (ws < 0)
+    ws += S;
+  const int x_lo = (int)((long long)run * nslab / sm.nruns);
+  const int x_hi = (int)((long long)(run + 1) * nslab / sm.nruns);
+  const int bx = x_hi - x_lo;
+  const int dir = (run & 1) ? -1 : 1;
+  // sweep index q of slab start + dir q; the run's slabs are q = NS - 1 ..
+  // NS - 2 + bx, level 0 is needed at q = -1 .. bx + 2 NS - 2
+  const int start = dir > 0 ? x_lo - (NS - 1) : x_hi + NS - 2;
+  const int q_last = bx + 2 * NS - 2;
+  auto slab_of = [&](int q) {
+    q = q < -1 ? -1 : (q > q_last ? q_last : q);
+    int x = start + dir * q;
+    while (x < 0)
+      x += nslab;
+    while (x >= nslab)
+      x -= nslab;
+    return x;
+  };
+  auto ring = [](int q) {  // q mod RR for q >= -8 RR
+    return (q + 8 * RR) % RR;
+  };
+  // this thread: window entry t (position pos), its neighbours' entries
+  const int te = t < wlen ? t : 0;
+  int pos = ws + te;
+  pos = pos >= S ? pos - S : pos;
+  const bool interior = t >= NS * hh && t < NS * hh + clen;
+  const int r = pos % sm.p;
+  int bq[K], dq[K];
+  double ha[K], hb[K];
+  bool okq[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const int4 e = sm.tab[q * sm.p + r];
+    okq[q] = e.w != 0;
+    dq[q] = e.x * dir;
+    int b = te + e.y;
+    b = b < 0 ? b + S : (b >= S ? b - S : b);  // the whole slab in one chunk
+    bq[q] = b < 0 ? 0 : (b < wlen ? b : wlen - 1);
+    const double2 hv = okq[q] ? h.tab[e.z] : make_double2(0.0, 0.0);
+    ha[q] = hv.x;
+    hb[q] = hv.y;
+  }
+  const double emax = h.emax;
+  const double inv_emax = 1.0 / h.emax;
+  auto gsite = [&](int q) { return (size_t)slab_of(q) * S + pos; };
+
+  // level 0 at q = -1, 0
+  {
+    const V a0 = p1[gsite(-1)], a1 = p1[gsite(0)];
+    s_lv[0][ring(-1)][t] = a0;
+    s_lv[0][ring(0)][t] = a1;
+  }
+  double acc_r[NS], acc_i[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l)
+    acc_r[l] = acc_i[l] = 0.0;
+  // potential and left vector of slabs k0 - NS + 1 + j, j < RB
+  double pv[RB];
+  V lv[RB];
+#pragma unroll
+  for (int j = 0; j < RB; ++j) {
+    pv[j] = 0.0;
+    lv[j] = IO::narrow(0.0, 0.0);
+  }
+
+  for (int k0 = 0; k0 <= bx + 2 * NS - 3; k0 += B) {
+    // loads: level 0 at k0 + 1 .. k0 + B; psi_{m-2}, potential and left
+    // vector at level 1's slabs k0 .. k0 + B - 1
+    V w[B], ip2[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      w[i] = p1[gsite(k0 + 1 + i)];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const size_t at = gsite(k0 + i);
+      ip2[i] = p2[at];
+      pv[NS - 1 + i] = IO::pot(h, (int)at);
+      lv[NS - 1 + i] = L[at];
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      s_lv[0][ring(k0 + 1 + i)][t] = w[i];
+    __syncthreads();
+#pragma unroll
+    for (int l = 1; l <= NS; ++l) {
+      V out[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const int q = k0 - (l - 1) + i;
+        const double2 self = IO::wide(s_lv[l - 1][ring(q)][te]);
+        const double v = pv[NS - l + i];  // slab q = k0 - NS + 1 + (NS - l + i)
+        double tr = v * self.x;
+        double ti = v * self.y;
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+          const double2 c = IO::wide(s_lv[l - 1][ring(q + dq[s])][bq[s]]);
+          if (okq[s]) {
+            if constexpr (FAST) {
+              tr += ha[s] * c.x;
+              ti += ha[s] * c.y;
+            } else {
+              tr += ha[s] * c.x - hb[s] * c.y;
+              ti += ha[s] * c.y + hb[s] * c.x;
+            }
+          }
+        }
+        if constexpr (FAST) {
+          tr *= inv_emax;
+          ti *= inv_emax;
+        } else {
+          tr /= emax;
+          ti /= emax;
+        }
+        const double2 prev = l == 1 ? IO::wide(ip2[i]) : IO::wide(s_lv[l - 2][ring(q)][te]);
+        tr = 2.0 * tr - prev.x;
+        ti = 2.0 * ti - prev.y;
+        const V o = IO::narrow(tr, ti);
+        out[i] = o;
+        if (l < NS)
+          s_lv[l][ring(q)][t] = o;
+        if (interior && q >= NS - 1 && q <= NS - 2 + bx) {
+          const double2 ow = IO::wide(o), lw = IO::wide(lv[NS - l + i]);
+          acc_r[l - 1] += ow.x * lw.x + ow.y * lw.y;
+          acc_i[l - 1] += ow.y * lw.x - ow.x * lw.y;
+        }
+      }
+      if (l >= NS - 1) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+          const int q = k0 - (l - 1) + i;
+          if (interior && q >= NS - 1 && q <= NS - 2 + bx)
+            (l == NS ? o1 : o0)[gsite(q)] = out[i];
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j + B < RB; ++j) {
+      pv[j] = pv[j + B];
+      lv[j] = lv[j + B];
+    }
+  }
+
+  // the tile's NS moment partials: wave64 shuffle tree, then the waves in order
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      acc_r[l] += __shfl_xor(acc_r[l], o, 64);
+      acc_i[l] += __shfl_xor(acc_i[l], o, 64);
+    }
+  }
+  if ((t & 63) == 0) {
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      s_red[t >> 6][2 * l] = acc_r[l];
+      s_red[t >> 6][2 * l + 1] = acc_i[l];
+    }
+  }
+  __syncthreads();
+  if (t < NS && part) {
+    double re = s_red[0][2 * t], im = s_red[0][2 * t + 1];
+#pragma unroll
+    for (int wv = 1; wv < T / 64; ++wv) {
+      re += s_red[wv][2 * t];
+      im += s_red[wv][2 * t + 1];
+    }
+    part[(size_t)t * part_stride + tile] = make_double2(re, im);
+  }
+}
+
+// Host launcher of k_sweepn: NS steps per launch in blocks of B slabs for
+// slot counts 2, 3, 4 and 6 (part: NS rows of partials part_stride apart)
+template <typename V, int NS, int B>
+void sweepn_launch(const HamDev& h, const SweepMap& sm, const V* p2, const V* p1, V* o0, V* o1,
+                   const V* L, double2* part, size_t part_stride, hipStream_t st)
+{
+  const bool fast = sizeof(V) == sizeof(float2) && sm.real;
+#define LSQT_SWN(KK)                                                                          \
+  do {                                                                                        \
+    if (fast)                                                                                 \
+      k_sweepn<V, KK, NS, B, sizeof(V) == sizeof(float2)>                                     \
+        <<<sm.blocks, kSweepThreads, 0, st>>>(h, sm, p2, p1, o0, o1, L, part, part_stride);   \
+    else                                                                                      \
+      k_sweepn<V, KK, NS, B, false>                                                           \
+        <<<sm.blocks, kSweepThreads, 0, st>>>(h, sm, p2, p1, o0, o1, L, part, part_stride);   \
+  } while (0)
+  switch (h.K) {
+    case 2:
+      LSQT_SWN(2);
+      break;
+    case 3:
+      LSQT_SWN(3);
+      break;
+    case 4:
+      LSQT_SWN(4);
+      break;
+    case 6:
+      LSQT_SWN(6);
+      break;
+    default:
+      break;
+  }
+#undef LSQT_SWN
+}
+
+}  // namespace lsqt

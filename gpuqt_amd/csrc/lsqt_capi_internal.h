@@ -50,6 +50,9 @@ struct lsqt_ham {
   // storage: the slab geometry and the launch shape of each (S = 0: none)
   lsqt::SweepMap sweep64{};
   lsqt::SweepMap sweep32{};
+  // the same geometry with more steps per launch (k_sweepn; S = 0: none)
+  lsqt::SweepMap sweepn64{};
+  lsqt::SweepMap sweepn32{};
   // two steps per launch on the block path (k_block_sweep2): the slab
   // geometry of every system (empty: none) and their slot tables on the
   // device (owned by `allocations`); launch shapes are made per block width

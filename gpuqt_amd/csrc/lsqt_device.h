@@ -287,6 +287,8 @@ struct SweepMap {
   int p;             // orbitals per pattern period
   const int4* tab;   // [slot][orbital]: slab offset, position offset, class, used
   int halo;          // largest |position offset| of the table
+  int ns;            // steps per launch of the shape (k_sweepn: 3 or 4)
+  int nb;            // slabs per loop iteration (k_sweepn)
 };
 
 // Two steps per launch for blocks of vectors (k_block_sweep2): the slab

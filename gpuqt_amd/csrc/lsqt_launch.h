@@ -53,6 +53,17 @@ void launch_sweep2(const HamDev& h, const SweepMap& sm, const double2* psi2,
 void launch_sweep2_32(const HamDev& h, const SweepMap& sm, const float2* psi2,
                       const float2* psi1, float2* out0, float2* out1, const float2* L,
                       double2* part0, double2* part1, hipStream_t st);
+// NS = sm.ns steps per launch (kernels_sweepn.h): from psi2 = psi_{m-2} and
+// psi1 = psi_{m-1}, out0 = psi_{m+NS-2} and out1 = psi_{m+NS-1} (four
+// distinct buffers), the tile partials of <L|psi_{m+l}> in part + l *
+// part_stride (sm.tiles each, l < NS); sweepn_lds: its static LDS bytes
+void launch_sweepn(const HamDev& h, const SweepMap& sm, const double2* psi2,
+                   const double2* psi1, double2* out0, double2* out1, const double2* L,
+                   double2* part, size_t part_stride, hipStream_t st);
+void launch_sweepn_32(const HamDev& h, const SweepMap& sm, const float2* psi2,
+                      const float2* psi1, float2* out0, float2* out1, const float2* L,
+                      double2* part, size_t part_stride, hipStream_t st);
+size_t sweepn_lds(bool f32, int ns, int nb);
 // the check of a sweep table against every packed word: one int per
 // workgroup of sweep_check_blocks(n), nonzero where a word differs
 int sweep_check_blocks(int n);

@@ -30,7 +30,9 @@ class HamInfo(C.Structure):
                 ("pattern_period", C.c_int), ("pattern_irregular", C.c_double),
                 ("sweep_slab", C.c_int), ("sweep_halo", C.c_int), ("sweep_tiles", C.c_int),
                 ("sweep_tiles_f32", C.c_int), ("bytes_per_site_sweep_f64", C.c_double),
-                ("bytes_per_site_sweep_f32", C.c_double), ("block_sweep_systems", C.c_int)]
+                ("bytes_per_site_sweep_f32", C.c_double), ("block_sweep_systems", C.c_int),
+                ("sweep_steps", C.c_int), ("sweep_steps_f32", C.c_int),
+                ("sweep_tiles_deep", C.c_int)]
 
 
 class Lattice(C.Structure):

@@ -95,6 +95,14 @@ typedef struct {
                               self and pair products take two steps per
                               launch (0: none; either every system of the
                               Hamiltonian or none) */
+  int sweep_steps;         /* Chebyshev steps per launch of the FP64 moment
+                              sets' deepest sweep: 4 (k_sweepn), 2 (k_sweep2
+                              only: LSQT_SWEEP_STEPS=2 in the environment at
+                              creation) or 0 (no sweep); a launch reads and
+                              writes bytes_per_site_sweep_f64 per site either
+                              way */
+  int sweep_steps_f32;     /* the same for FP32 states */
+  int sweep_tiles_deep;    /* workgroups per four-step launch, FP64 states */
 } lsqt_ham_info;
 
 const char* lsqt_strerror(int status);

@@ -1,8 +1,8 @@
-"""Two Chebyshev steps per launch on slab lattices (GPU).
+"""Several Chebyshev steps per launch on slab lattices (GPU).
 
 Where every neighbour of a site lies within one slab and a few positions of
-it (periodically), the moment sets advance two recursion steps per launch
-from LDS (k_sweep2).  Checked here:
+it (periodically), the moment sets advance two (k_sweep2) or four (k_sweepn)
+recursion steps per launch from LDS.  Checked here:
 
   - which Hamiltonians get the sweep: periodic honeycomb (whole-slab and
     chunked slabs), square and triangular lattices; none for a Peierls-phase
@@ -12,11 +12,13 @@ from LDS (k_sweep2).  Checked here:
     LSQT_SWEEP=0;
   - lsqt_kernel_polynomial_pair against two lsqt_kernel_polynomial calls,
     bit for bit (the per-site arithmetic is the same, in the same order);
-  - FP64 moment sets against the same Hamiltonian without the sweep (the
-    partial sums group the sites differently: the FP64 moment tolerance)
-    and against the oracle, for moment counts 2 .. 5 and odd / even large
-    ones; FP32-state moment sets against their one-step counterparts
-    within 1e-6 of mu_0.
+  - FP64 moment sets (four steps per launch from moment 4, two steps
+    before and for the remainder) against the same Hamiltonian without the
+    sweep and with two steps per launch only (the partial sums group the
+    sites differently: the FP64 moment tolerance) and against the oracle,
+    for moment counts 2 .. 5, 8 .. 13 (every remainder of the four-step
+    launches) and odd / even large ones; FP32-state moment sets against
+    their one-step counterparts within 1e-6 of mu_0.
 """
 import numpy as np
 import pytest
@@ -100,6 +102,7 @@ def test_sweep_geometry(gpu_ctx, tmp_path, monkeypatch):
         else:
             assert (info.sweep_slab, info.sweep_halo) == (slab, halo), name
             assert info.sweep_tiles > 0 and info.sweep_tiles_f32 > 0
+            assert info.sweep_steps == 4 and info.sweep_tiles_deep > 0
             assert 0 < info.bytes_per_site_sweep_f64 < 2 * info.bytes_per_site_f64
         H.close()
     # the cube: y-neighbours are a row (32 positions) away within a slab
@@ -182,17 +185,29 @@ def test_sweep_moments(gpu_ctx, tmp_path, monkeypatch):
         H0 = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, K, emax)
         monkeypatch.delenv("LSQT_SWEEP")
         assert H.info().sweep_slab > 0 and H0.info().sweep_slab == 0
+        monkeypatch.setenv("LSQT_SWEEP_STEPS", "2")
+        H2 = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, K, emax)
+        monkeypatch.delenv("LSQT_SWEEP_STEPS")
+        assert (H.info().sweep_steps, H.info().sweep_steps_f32) == (4, 4), name
+        assert (H2.info().sweep_steps, H2.info().sweep_steps_f32) == (2, 2), name
+        assert H0.info().sweep_steps == 0
         v, l = _random_vector(H, rng), _random_vector(H, rng)
-        for M in (2, 3, 4, 5, 64, 129):
-            assert_moments(H.moments(v, v, M), H0.moments(v, v, M), f"{name} M={M} self")
-            assert_moments(H.moments(l, v, M), H0.moments(l, v, M), f"{name} M={M} pair")
+        # 2 .. 5: one and two steps per launch only; 8 .. 13: one four-step
+        # launch, then every remainder; 64, 129: many
+        for M in (2, 3, 4, 5, 8, 9, 10, 11, 12, 13, 64, 129):
+            ref_self, ref_pair = H0.moments(v, v, M), H0.moments(l, v, M)
+            assert_moments(H.moments(v, v, M), ref_self, f"{name} M={M} self")
+            assert_moments(H.moments(l, v, M), ref_pair, f"{name} M={M} pair")
+            assert_moments(H2.moments(l, v, M), ref_pair, f"{name} M={M} pair, two steps")
             a, b = H.moments_f32(l, v, M), H0.moments_f32(l, v, M)
             assert np.abs(a - b).max() <= 1e-6 * n, f"{name} M={M} f32"
+            c = H2.moments_f32(l, v, M)
+            assert np.abs(c - b).max() <= 1e-6 * n, f"{name} M={M} f32, two steps"
         if om is not None:
             r = om.random_state()
             ov = H.vector(*r)
             ref_re, ref_im = om.moments(r, r, 64)
             assert_moments(H.moments(ov, ov, 64), ref_re + 1j * ref_im, f"{name} oracle")
             ov.close()
-        for o in (v, l, H, H0):
+        for o in (v, l, H, H0, H2):
             o.close()
