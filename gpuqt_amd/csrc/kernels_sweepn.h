@@ -113,6 +113,15 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
   const double emax = h.emax;
   const double inv_emax = 1.0 / h.emax;
   auto gsite = [&](int q) { return (size_t)slab_of(q) * S + pos; };
+  // psi_{m-2} and the potential are used on level 1's entries, the left
+  // vector on the interior: the other threads load their nearest such
+  // entry's (a cache line the wave reads anyway) instead of their own
+  auto wrap_pos = [&](int e) {
+    const int p = ws + e;
+    return p >= S ? p - S : p;
+  };
+  const int pos_p = wrap_pos(min(max(t, hh), max(wlen - hh - 1, 0)));
+  const int pos_l = wrap_pos(min(max(t, NS * hh), NS * hh + clen - 1));
 
   // level 0 at q = -1, 0
   {
@@ -142,10 +151,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
       w[i] = p1[gsite(k0 + 1 + i)];
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-      const size_t at = gsite(k0 + i);
-      ip2[i] = p2[at];
-      pv[NS - 1 + i] = IO::pot(h, (int)at);
-      lv[NS - 1 + i] = L[at];
+      const size_t x = (size_t)slab_of(k0 + i) * S;
+      ip2[i] = p2[x + pos_p];
+      pv[NS - 1 + i] = IO::pot(h, (int)(x + pos_p));
+      lv[NS - 1 + i] = L[x + pos_l];
     }
 #pragma unroll
     for (int i = 0; i < B; ++i)
