@@ -459,7 +459,8 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
         h->sweepn64 = sweep_shape_n(h->sweep64, false, 4, 1);
         h->sweepn32 = sweep_shape_n(h->sweep32, true, 4, 2);
       }
-      // the same geometry serves the block path's two-step sweep
+      // the same geometry serves the block path's sweeps
+      h->bsweep_steps = (steps && atoi(steps) == 2) ? 2 : 4;
       BlockSweepSeg b{};
       b.S = g.S;
       b.nslab = g.nslab;
@@ -560,6 +561,8 @@ void detect_block_sweep(lsqt_ham* h, int segments, const lsqt_lattice* lats,
   h->allocations.push_back(dev_tab);
   h->bsweep = segs;
   h->bsweep_tab = dev_tab;
+  const char* steps = getenv("LSQT_SWEEP_STEPS");
+  h->bsweep_steps = (steps && atoi(steps) == 2) ? 2 : 4;
 }
 
 // The slot pattern of a packed Hamiltonian.  On a periodic lattice whose
@@ -1176,6 +1179,7 @@ extern "C" int lsqt_ham_get_info(const lsqt_ham* h, lsqt_ham_info* info)
   info->sweep_steps = h->sweepn64.S ? h->sweepn64.ns : (h->sweep64.S ? 2 : 0);
   info->sweep_steps_f32 = h->sweepn32.S ? h->sweepn32.ns : (h->sweep32.S ? 2 : 0);
   info->sweep_tiles_deep = h->sweepn64.S ? h->sweepn64.tiles : 0;
+  info->block_sweep_steps = h->bsweep.empty() ? 0 : h->bsweep_steps;
   return LSQT_OK;
 }
 

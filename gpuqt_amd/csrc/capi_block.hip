@@ -87,13 +87,13 @@ struct SegTable {
 // a captured moment set: every argument its kernels were recorded with
 struct GraphKey {
   int self, M, R, W, rows, blocked, depth, tiles;
-  const void *left, *A, *B, *ring, *mu, *C, *D, *sweep;
+  const void *left, *A, *B, *ring, *mu, *C, *D, *sweep, *deep;
   bool operator<(const GraphKey& o) const
   {
     return std::tie(self, M, R, W, rows, blocked, depth, tiles, left, A, B, ring, mu, C, D,
-                    sweep) < std::tie(o.self, o.M, o.R, o.W, o.rows, o.blocked, o.depth,
-                                      o.tiles, o.left, o.A, o.B, o.ring, o.mu, o.C, o.D,
-                                      o.sweep);
+                    sweep, deep) < std::tie(o.self, o.M, o.R, o.W, o.rows, o.blocked,
+                                            o.depth, o.tiles, o.left, o.A, o.B, o.ring, o.mu,
+                                            o.C, o.D, o.sweep, o.deep);
   }
 };
 
@@ -112,7 +112,7 @@ struct lsqt_block_state {
   BlockScratch s;
   std::map<int, BlockShape> forced;  // R -> shape set by the caller
   std::map<int, SegTable> seg;       // rows -> tile table
-  std::map<int, BlockSweepShape> bsw;  // columns -> two-step sweep shape
+  std::map<int, BlockSweepShape> bsw;  // 8 columns + steps -> sweep shape
   // moment sets are captured once and replayed as hipGraphs (LSQT_GRAPHS=0
   // or lsqt_block_set_graphs(h, 0) launches them one by one instead)
   bool graphs = true;
@@ -482,16 +482,17 @@ int tile_map(lsqt_ham* h, int R, const BlockShape& c, TileMap* tm)
   return LSQT_OK;
 }
 
-// The two-step sweep's launch shape for blocks of `cols` columns (*out null
-// where the Hamiltonian has no sweep geometry for each of its systems or
-// the columns do not fit): W = 2 columns per lane up to 16 columns, else 4,
-// so cols / W lanes per site and kBlock W / cols positions per slab row; a
-// chunk holds them less 4 halo positions (the whole slab where it fits).
+// The launch shape of the block sweep with `steps` (2 or 4) steps per launch
+// for blocks of `cols` columns (*out null where the Hamiltonian has no sweep
+// geometry for each of its systems or the columns do not fit): W = 2
+// columns per lane up to 16 columns, else 4 (two steps only), so cols / W
+// lanes per site and kBlock W / cols positions per slab row; a chunk holds
+// them less 2 steps halo positions (the whole slab where it fits).
 // Runs of slabs per chunk: about twice as many tiles as resident
 // workgroups, and the dispatch order puts the longest runs first, in groups
 // of 8 neighbouring chunks dealt to the XCDs in turn, so that the systems of
 // an uneven batch finish together.
-int block_sweep_shape(lsqt_ham* h, int cols, BlockSweepShape** out)
+int block_sweep_shape(lsqt_ham* h, int cols, int steps, BlockSweepShape** out)
 {
   *out = nullptr;
   if (!h->blk)
@@ -501,12 +502,14 @@ int block_sweep_shape(lsqt_ham* h, int cols, BlockSweepShape** out)
     return LSQT_OK;
   if (!(h->d.K == 2 || h->d.K == 3 || h->d.K == 4 || h->d.K == 6))
     return LSQT_OK;
-  auto it = h->blk->bsw.find(cols);
+  if (steps == 4 && (cols > 16 || h->bsweep_steps < 4))
+    return LSQT_OK;  // four levels of wider entries leave too little LDS
+  auto it = h->blk->bsw.find(cols * 8 + steps);
   if (it != h->blk->bsw.end()) {
     *out = it->second.ok ? &it->second : nullptr;
     return LSQT_OK;
   }
-  BlockSweepShape& sh = h->blk->bsw[cols];
+  BlockSweepShape& sh = h->blk->bsw[cols * 8 + steps];
   const int W = cols <= 16 ? 2 : 4;
   const int P = kBlock * W / cols;
   std::vector<BlockSweepSeg> segs = h->bsweep;
@@ -521,16 +524,18 @@ int block_sweep_shape(lsqt_ham* h, int cols, BlockSweepShape** out)
       g.hh = 0;
       g.cs = g.S;
       g.nch = 1;
-    } else if (8 * g.halo <= P) {
+    } else if (4 * steps * g.halo <= P) {
+      // a chunk holds the row's positions less the halo of every step
       g.hh = g.halo;
-      g.cs = P - 4 * g.hh;
+      g.cs = P - 2 * steps * g.hh;
       g.nch = (g.S + g.cs - 1) / g.cs;
     } else {
       return LSQT_OK;
     }
     units += (long long)g.nch * g.nslab;
   }
-  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / block_sweep_lds(W)));
+  const size_t lds = steps == 4 ? block_sweepn_lds() : block_sweep_lds(W);
+  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / lds));
   const long long run_len = std::max(16ll, units / (2ll * kComputeUnits * per_cu));
   int tiles = 0;
   struct Entry {
@@ -653,19 +658,22 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, c
 // <T_m L|T_m R>.  The moments follow from T_{2m} = 2 T_m^2 - T_0 and
 // T_{2m+1} = 2 T_m T_{m+1} - T_1 with T_m Hermitian: mu_0 = S_0, mu_1 = S_1,
 // mu_k = 2 S_k - mu_{k mod 2}.  M/2 steps give moments 0..M-1.
-// With a two-step sweep shape (sw, outputs C and D), steps 2 .. are taken
-// two per launch (a last odd one and step 1 by the one-step kernels).
+// With sweep shapes (sw: two steps, deep: four; outputs C and D), steps
+// 2 .. are taken four per launch, then two, and the rest (and step 1) by the
+// one-step kernels.
 // Moments k >= 1 fill ring slots [0, used) from moment k_first, slots
 // stride tiles apart; a ring is reduced when the next launch's slots would
 // not fit or the launches change their tile map, and at the end.
 void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, float2*& A,
                         float2*& B, double2* ring, double2* mu, int M, hipStream_t st,
                         bool pairs = false, const BlockSweepShape* sw = nullptr,
-                        float2* C = nullptr, float2* D = nullptr)
+                        float2* C = nullptr, float2* D = nullptr,
+                        const BlockSweepShape* deep = nullptr)
 {
   const int cols = pairs ? 2 * R : R;
   const int mode = pairs ? kStepPair : kStepSelf;
-  const int stride = sw ? std::max(tm.tiles, sw->map.tiles) : tm.tiles;
+  const int stride = std::max(tm.tiles, std::max(sw ? sw->map.tiles : 0,
+                                                 deep ? deep->map.tiles : 0));
   const size_t slot = (size_t)stride * R;
   launch_block_dot(A, A, cols, tm, ring, st, pairs);
   launch_block_reduce(ring, tm, R, M, 0, 1, mu, st, stride);
@@ -686,6 +694,15 @@ void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& 
   };
   const int steps = M / 2;
   for (int m = 1; m <= steps;) {
+    if (deep && m >= 2 && m + 3 <= steps) {
+      // steps m .. m + 3: moments 2m - 1 .. 2m + 6
+      double2* p = take(8, &deep->red);
+      launch_block_sweepn(h->d, deep->map, cols, pairs, A, B, C, D, p, slot, st);
+      std::swap(A, C);
+      std::swap(B, D);
+      m += 4;
+      continue;
+    }
     if (sw && m >= 2 && m + 1 <= steps) {
       double2* p = take(4, &sw->red);
       double2* const part[4] = {p, p + slot, p + 2 * slot, p + 3 * slot};
@@ -804,11 +821,15 @@ int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Block
   // two steps per launch where the Hamiltonian's systems allow (product
   // identities only: their moments need no left vector)
   BlockSweepShape* sw = nullptr;
+  BlockSweepShape* deep = nullptr;
   if (mode != kDirect && M / 2 >= 3) {
     const int cols = mode == kPairProducts ? 2 * R : R;
-    LSQT_TRY(block_sweep_shape(h, cols, &sw));
+    LSQT_TRY(block_sweep_shape(h, cols, 2, &sw));
+    if (sw && M / 2 >= 5)
+      LSQT_TRY(block_sweep_shape(h, cols, 4, &deep));
     if (sw)
-      LSQT_TRY(block_sweep_scratch(h, cols, std::max(tm.tiles, sw->map.tiles), s));
+      LSQT_TRY(block_sweep_scratch(
+        h, cols, std::max(tm.tiles, std::max(sw->map.tiles, deep ? deep->map.tiles : 0)), s));
   }
   auto enqueue = [&]() {
     float2* a = A;
@@ -817,12 +838,12 @@ int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Block
       enqueue_steps(h, R, c, tm, left, a, b, s->ring, s->mu, M, 0, M, st);
     else
       enqueue_self_steps(h, R, c, tm, a, b, s->ring, s->mu, M, st, mode == kPairProducts, sw,
-                         s->C, s->D);
+                         s->C, s->D, deep);
   };
   if (h->blk->graphs) {
     const GraphKey key{mode, M, R, c.W, c.rows, c.blocked, c.depth, tm.tiles,
                        mode == kDirect ? left : nullptr, A, B, s->ring, s->mu,
-                       sw ? s->C : nullptr, sw ? s->D : nullptr, sw};
+                       sw ? s->C : nullptr, sw ? s->D : nullptr, sw, deep};
     auto it = h->blk->graph.find(key);
     if (it == h->blk->graph.end()) {
       hipGraph_t g = nullptr;

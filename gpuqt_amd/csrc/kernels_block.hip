@@ -1576,6 +1576,46 @@ void launch_block_sweep2(const HamDev& h, const BlockSweepMap& sm, int cols, int
 #undef LSQTB_SW
 }
 
+size_t block_sweepn_lds()
+{
+  return block_sweepn_lds_bytes<2, 4, 1>();
+}
+
+// four steps per launch, one slab per loop iteration, 2 columns per lane
+// (cols <= 16): the rings of four levels in 48 KB of LDS
+void launch_block_sweepn(const HamDev& h, const BlockSweepMap& sm, int cols, bool pairs,
+                         const float2* p2, const float2* p1, float2* o0, float2* o1,
+                         double2* part, size_t part_stride, hipStream_t st)
+{
+  const size_t lds = block_sweepn_lds();
+#define LSQTB_SWN(KK)                                                                       \
+  do {                                                                                      \
+    if (pairs)                                                                              \
+      k_block_sweepn<KK, 2, kPair, 4, 1><<<sm.blocks, kBlock, lds, st>>>(                   \
+        h, sm, cols, p2, p1, o0, o1, part, part_stride);                                    \
+    else                                                                                    \
+      k_block_sweepn<KK, 2, kSelf, 4, 1><<<sm.blocks, kBlock, lds, st>>>(                   \
+        h, sm, cols, p2, p1, o0, o1, part, part_stride);                                    \
+  } while (0)
+  switch (h.K) {
+    case 2:
+      LSQTB_SWN(2);
+      break;
+    case 3:
+      LSQTB_SWN(3);
+      break;
+    case 4:
+      LSQTB_SWN(4);
+      break;
+    case 6:
+      LSQTB_SWN(6);
+      break;
+    default:
+      break;
+  }
+#undef LSQTB_SWN
+}
+
 int block_sweep_check_blocks(const BlockSweepSeg& g)
 {
   return (int)(((long long)g.S * g.nslab + kBlock - 1) / kBlock);

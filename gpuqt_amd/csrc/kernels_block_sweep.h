@@ -251,6 +251,194 @@ __global__ __launch_bounds__(kBlock) void k_block_sweep2(
   store_products<IPM, W>(acc1, vpw, cols, r0, od.x, s_red, s2, s3);
 }
 
+// NS steps per launch (NS >= 3) for the same moment sets: levels 1 .. NS
+// (psi_m .. psi_{m+NS-1}; level 0 is psi_{m-1}), each one slab behind the
+// level before it and kept in an LDS ring of B + 2 slab rows (levels
+// 0 .. NS - 1), as k_sweepn does for single vectors; the potential of the
+// last B + NS - 1 slabs is kept per thread in registers.  Only the last two
+// levels are written.  Level l adds its two products to part[2 (l - 1)] and
+// part[2 (l - 1) + 1] (moments 2 (m + l - 1) - 1 and 2 (m + l - 1) of the
+// product identities).  A chunk holds the row's positions less 2 NS h.
+template <int W, int NS, int B>
+constexpr size_t block_sweepn_lds_bytes()
+{
+  return sizeof(float2) * NS * (B + 2) * kBlock * W;
+}
+
+template <int K, int W, int IPM, int NS, int B>
+__global__ __launch_bounds__(kBlock) void k_block_sweepn(
+  HamDev h, BlockSweepMap sm, int cols, const float2* __restrict__ p2,
+  const float2* __restrict__ p1, float2* __restrict__ o0, float2* __restrict__ o1,
+  double2* __restrict__ part, size_t part_stride)
+{
+  static_assert(W % 2 == 0 && NS >= 3, "W even, NS >= 3");
+  static_assert(block_sweepn_lds_bytes<W, NS, B>() >= sizeof(double) * 2 * (kBlock / 64) * 64 * 4,
+                "the tile sums reuse the rings");
+  constexpr int RR = B + 2;        // rows of a level ring
+  constexpr int RB = B + NS - 1;   // slabs of potential held per thread
+  constexpr int RW = kBlock * W;   // ring row length in float2
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  const int4 od = sm.order[xcd_tile(blockIdx.x, sm.blocks)];
+  if (od.x < 0)
+    return;
+  const BlockSweepSeg g = sm.seg[od.y];
+  float2* s_lv = reinterpret_cast<float2*>(s_dyn);  // [level][row][entry][W]
+  auto lv_at = [&](int l, int row) { return s_lv + ((size_t)l * RR + row) * RW; };
+
+  const int t = threadIdx.x;
+  const int vpw = cols / W;
+  const int tpos = t / vpw;             // held entry (window position) of this thread
+  const int r0 = (t - tpos * vpw) * W;  // its first column
+  const int S = g.S, nslab = g.nslab, hh = g.hh;
+  const int c0 = od.z * g.cs;
+  const int clen = min(g.cs, S - c0);
+  const int wlen = clen + 2 * NS * hh;
+  int ws = c0 - NS * hh;
+  while (ws < 0)
+    ws += S;
+  const int run = od.w;
+  const int x_lo = (int)((long long)run * nslab / g.nruns);
+  const int x_hi = (int)((long long)(run + 1) * nslab / g.nruns);
+  const int bx = x_hi - x_lo;
+  const int dir = (run & 1) ? -1 : 1;
+  // sweep index q of slab start + dir q: the run's slabs are q = NS - 1 ..
+  // NS - 2 + bx, level 0 is needed at q = -1 .. bx + 2 NS - 2
+  const int start = dir > 0 ? x_lo - (NS - 1) : x_hi + NS - 2;
+  const int q_last = bx + 2 * NS - 2;
+  auto slab_of = [&](int q) {
+    q = q < -1 ? -1 : (q > q_last ? q_last : q);
+    int x = start + dir * q;
+    while (x < 0)
+      x += nslab;
+    while (x >= nslab)
+      x -= nslab;
+    return x;
+  };
+  auto ring = [](int q) {  // q mod RR for q >= -8 RR
+    return (q + 8 * RR) % RR;
+  };
+  auto wrap_pos = [&](int e) {
+    const int p = ws + e;
+    return p >= S ? p - S : p;
+  };
+  const int te = tpos < wlen ? tpos : 0;
+  const int pos = wrap_pos(te);
+  // psi_{m-2} and the potential are used on level 1's entries only
+  const int pos_p = wrap_pos(min(max(tpos, hh), max(wlen - hh - 1, 0)));
+  const bool interior = tpos >= NS * hh && tpos < NS * hh + clen;
+  const int o_self = te * vpw * W + r0;
+  const int r = pos % g.p;
+  int eo[K], dq[K];
+  float ha[K], hb[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const int4 e = sm.tab[g.tab + q * g.p + r];
+    const bool ok = e.w != 0;
+    int b = te + (ok ? e.y : 0);
+    b = b < 0 ? b + S : (b >= S ? b - S : b);  // the whole slab in one chunk
+    b = b < 0 ? 0 : (b < wlen ? b : wlen - 1);
+    eo[q] = b * vpw * W + r0;
+    dq[q] = ok ? e.x * dir : 0;
+    const double2 hv = h.tab[ok ? e.z : 0];
+    ha[q] = ok ? (float)hv.x : 0.0f;
+    hb[q] = ok ? (float)hv.y : 0.0f;
+  }
+  const size_t base = (size_t)g.base;
+  auto gidx = [&](int slab, int p) {
+    return (base + (size_t)slab * S + p) * (size_t)cols + r0;
+  };
+  {
+    const Vec<W> a0 = ldv<W, false>(p1 + gidx(slab_of(-1), pos));
+    const Vec<W> a1 = ldv<W, false>(p1 + gidx(slab_of(0), pos));
+    lds_put<W>(lv_at(0, ring(-1)) + t * W, a0);
+    lds_put<W>(lv_at(0, ring(0)) + t * W, a1);
+  }
+  const float inv_scale = (float)(1.0 / h.emax);
+  Products<W> acc[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l)
+    clear_products(acc[l]);
+  float pv[RB];  // potential of slabs k0 - NS + 1 + j
+#pragma unroll
+  for (int j = 0; j < RB; ++j)
+    pv[j] = 0.0f;
+
+  for (int k0 = 0; k0 <= bx + 2 * NS - 3; k0 += B) {
+    Vec<W> w[B], ip2[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      w[i] = ldv<W, false>(p1 + gidx(slab_of(k0 + 1 + i), pos));
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const size_t at = base + (size_t)slab_of(k0 + i) * S + pos_p;
+      ip2[i] = ldv<W, true>(p2 + at * cols + r0);
+      pv[NS - 1 + i] = h.pot32[at];
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      lds_put<W>(lv_at(0, ring(k0 + 1 + i)) + t * W, w[i]);
+    __syncthreads();
+#pragma unroll
+    for (int l = 1; l <= NS; ++l) {
+      Vec<W> out[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const int q = k0 - (l - 1) + i;
+        const Vec<W> self = lds_vec<W>(lv_at(l - 1, ring(q)) + o_self);
+        const float v = pv[NS - l + i];
+        float tr[W], ti[W];
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+          tr[c] = v * self.v[c].x;
+          ti[c] = v * self.v[c].y;
+        }
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+          const Vec<W> nb = lds_vec<W>(lv_at(l - 1, ring(q + dq[s])) + eo[s]);
+#pragma unroll
+          for (int c = 0; c < W; ++c) {
+            const float cx = nb.v[c].x, cy = nb.v[c].y;
+            tr[c] = __builtin_fmaf(ha[s], cx, tr[c]);
+            tr[c] = __builtin_fmaf(-hb[s], cy, tr[c]);
+            ti[c] = __builtin_fmaf(ha[s], cy, ti[c]);
+            ti[c] = __builtin_fmaf(hb[s], cx, ti[c]);
+          }
+        }
+        const Vec<W> prev =
+          l == 1 ? ip2[i] : lds_vec<W>(lv_at(l - 2, ring(q)) + o_self);
+        Vec<W> o;
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+          o.v[c].x = __builtin_fmaf(2.0f, tr[c] * inv_scale, -prev.v[c].x);
+          o.v[c].y = __builtin_fmaf(2.0f, ti[c] * inv_scale, -prev.v[c].y);
+        }
+        out[i] = o;
+        if (l < NS)
+          lds_put<W>(lv_at(l, ring(q)) + t * W, o);
+        if (interior && q >= NS - 1 && q <= NS - 2 + bx)
+          add_products<IPM, W>(acc[l - 1], o, self, self);
+      }
+      if (l >= NS - 1) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+          const int q = k0 - (l - 1) + i;
+          if (interior && q >= NS - 1 && q <= NS - 2 + bx)
+            stv<W, true>((l == NS ? o1 : o0) + gidx(slab_of(q), pos), out[i]);
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j + B < RB; ++j)
+      pv[j] = pv[j + B];
+  }
+  auto s_red = reinterpret_cast<double(*)[kBlock / 64][64 * 4]>(s_dyn);
+#pragma unroll
+  for (int l = 0; l < NS; ++l)
+    store_products<IPM, W>(acc[l], vpw, cols, r0, od.x, s_red, part + 2 * l * part_stride,
+                           part + (2 * l + 1) * part_stride);
+}
+
 // Check of one system's slot table against its packed words: out[b] = 1
 // where some word of workgroup b's sites differs from the table's.
 __global__ __launch_bounds__(kBlock) void k_block_sweep_check(HamDev h, BlockSweepSeg g,

@@ -161,6 +161,14 @@ size_t block_sweep_lds(int W);
 void launch_block_sweep2(const HamDev& h, const BlockSweepMap& sm, int cols, int W, bool pairs,
                          const float2* p2, const float2* p1, float2* o0, float2* o1,
                          double2* const (&part)[4], hipStream_t st);
+// Four steps per launch on the block path (k_block_sweepn, cols <= 16):
+// from p2 = psi_{m-2} and p1 = psi_{m-1}, o0 = psi_{m+2} and o1 = psi_{m+3};
+// level l = 1 .. 4 (psi_{m+l-1}) adds its two products to part + (2 l - 2)
+// part_stride and part + (2 l - 1) part_stride (sm.tiles partials each)
+size_t block_sweepn_lds();
+void launch_block_sweepn(const HamDev& h, const BlockSweepMap& sm, int cols, bool pairs,
+                         const float2* p2, const float2* p1, float2* o0, float2* o1,
+                         double2* part, size_t part_stride, hipStream_t st);
 // the check of one system's sweep table against its packed words: one int
 // per workgroup of block_sweep_check_blocks(g), nonzero where a word differs
 int block_sweep_check_blocks(const BlockSweepSeg& g);

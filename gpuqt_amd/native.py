@@ -32,7 +32,7 @@ class HamInfo(C.Structure):
                 ("sweep_tiles_f32", C.c_int), ("bytes_per_site_sweep_f64", C.c_double),
                 ("bytes_per_site_sweep_f32", C.c_double), ("block_sweep_systems", C.c_int),
                 ("sweep_steps", C.c_int), ("sweep_steps_f32", C.c_int),
-                ("sweep_tiles_deep", C.c_int)]
+                ("sweep_tiles_deep", C.c_int), ("block_sweep_steps", C.c_int)]
 
 
 class Lattice(C.Structure):

@@ -103,6 +103,9 @@ typedef struct {
                               way */
   int sweep_steps_f32;     /* the same for FP32 states */
   int sweep_tiles_deep;    /* workgroups per four-step launch, FP64 states */
+  int block_sweep_steps;   /* most steps per launch of the block path's
+                              sweeps (4 up to 16 columns, else 2; 2 with
+                              LSQT_SWEEP_STEPS=2; 0: none) */
 } lsqt_ham_info;
 
 const char* lsqt_strerror(int status);

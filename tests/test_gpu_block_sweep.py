@@ -1,17 +1,20 @@
-"""Two Chebyshev steps per launch on the block path (GPU).
+"""Several Chebyshev steps per launch on the block path (GPU).
 
 Periodic lattices built on the device -- one, or several of uneven size side
 by side as a segmented Hamiltonian -- take the product-identity moment sets
-(self products of R vectors, pairs of a left and a right recursion) two
-steps per launch (k_block_sweep2), one launch for all systems.  The per-site
+(self products of R vectors, pairs of a left and a right recursion) four
+(k_block_sweepn, up to 16 columns) or two (k_block_sweep2) steps per launch,
+one launch for all systems.  The per-site
 arithmetic is that of the one-step block kernels, so the states agree bit
 for bit and the moments only by the grouping of their FP64 partial sums:
 every moment set is checked against the same Hamiltonian built without the
 sweep (LSQT_SWEEP=0) within 1e-12 of the set's scale -- a state that differed
 in one FP32 rounding would miss that by orders of magnitude.  Also checked:
-which Hamiltonians get the sweep, column counts 4 .. 32 (two and four
-columns per lane), moment counts from 6 (one two-step launch) up, odd and
-even, the VAC / MSD drivers, and hipGraph replays against eager launches.
+which Hamiltonians get the sweep, column counts 2 .. 32 (two and four
+columns per lane), moment counts from 6 (one two-step launch) and 10 (one
+four-step launch) up, odd and even, two steps per launch only
+(LSQT_SWEEP_STEPS=2) against four, the VAC / MSD drivers, and hipGraph
+replays against eager launches.
 """
 import os
 
@@ -44,13 +47,18 @@ UNEVEN = [("honeycomb", 40, 30), ("honeycomb", 64, 90), ("honeycomb", 25, 200),
           ("honeycomb", 100, 17), ("honeycomb", 7, 333)]
 
 
-def _pair_of_hams(ctx, tmp_path, monkeypatch, systems, emax=9.0, seed=0, tag="u"):
-    """(with the sweep, without it) on the same lattices and potential"""
+def _pair_of_hams(ctx, tmp_path, monkeypatch, systems, emax=9.0, seed=0, tag="u", env=None):
+    """(with the sweep, without it) on the same lattices and potential; env:
+    settings for the first"""
     from gpuqt_amd import native
     lats = [_lattice(tmp_path, f"{tag}{k}", *s) for k, s in enumerate(systems)]
     n = sum(_sites(l) for l in lats)
     pot = np.random.default_rng(seed).uniform(-1.0, 1.0, n)
+    for k, v in (env or {}).items():
+        monkeypatch.setenv(k, v)
     Hs = native.Hamiltonian.from_lattices(ctx, lats, None, pot, emax)
+    for k in (env or {}):
+        monkeypatch.delenv(k)
     monkeypatch.setenv("LSQT_SWEEP", "0")
     H1 = native.Hamiltonian.from_lattices(ctx, lats, None, pot, emax)
     monkeypatch.delenv("LSQT_SWEEP")
@@ -76,7 +84,8 @@ def test_block_sweep_geometry(gpu_ctx, tmp_path, monkeypatch):
     print("[gpu] block sweep geometry", flush=True)
     Hs, H1, n = _pair_of_hams(gpu_ctx, tmp_path, monkeypatch, UNEVEN)
     assert Hs.info().block_sweep_systems == len(UNEVEN)
-    assert H1.info().block_sweep_systems == 0
+    assert Hs.info().block_sweep_steps == 4
+    assert H1.info().block_sweep_systems == 0 and H1.info().block_sweep_steps == 0
     for o in (Hs, H1):
         o.close()
     # honeycombs and square lattices (3 and 4 slots) side by side
@@ -109,15 +118,18 @@ def test_block_sweep_geometry(gpu_ctx, tmp_path, monkeypatch):
     H.close()
 
 
-@pytest.mark.parametrize("R", [2, 4, 8, 16, 32])
-def test_block_sweep_self_products(gpu_ctx, tmp_path, monkeypatch, R):
+@pytest.mark.parametrize("R,steps", [(2, "4"), (4, "4"), (8, "4"), (8, "2"), (16, "4"),
+                                     (32, "4")])
+def test_block_sweep_self_products(gpu_ctx, tmp_path, monkeypatch, R, steps):
     from gpuqt_amd import native
-    print(f"[gpu] block sweep self products R={R}", flush=True)
-    Hs, H1, n = _pair_of_hams(gpu_ctx, tmp_path, monkeypatch, UNEVEN, seed=R)
+    print(f"[gpu] block sweep self products R={R}, {steps} steps", flush=True)
+    Hs, H1, n = _pair_of_hams(gpu_ctx, tmp_path, monkeypatch, UNEVEN, seed=R,
+                              env={"LSQT_SWEEP_STEPS": steps})
+    assert Hs.info().block_sweep_steps == int(steps)
     re, im = _phases(n, R, R)
     b = native.Block(gpu_ctx, n, R)
     b.upload(re, im)
-    for M in ([6, 7, 8, 9, 64, 131] if R == 8 else [10, 129]):
+    for M in ([6, 7, 8, 9, 10, 11, 12, 14, 16, 18, 64, 131] if R == 8 else [10, 129]):
         got = Hs.moments_block_self(b, M)
         ref = H1.moments_block_self(b, M)
         _check_sets(got, ref, f"self R={R} M={M}")
@@ -128,17 +140,18 @@ def test_block_sweep_self_products(gpu_ctx, tmp_path, monkeypatch, R):
         o.close()
 
 
-@pytest.mark.parametrize("R", [1, 2, 8, 16])
-def test_block_sweep_pair_products(gpu_ctx, tmp_path, monkeypatch, R):
+@pytest.mark.parametrize("R,steps", [(1, "4"), (2, "4"), (8, "4"), (8, "2"), (16, "4")])
+def test_block_sweep_pair_products(gpu_ctx, tmp_path, monkeypatch, R, steps):
     from gpuqt_amd import native
-    print(f"[gpu] block sweep pair products R={R}", flush=True)
-    Hs, H1, n = _pair_of_hams(gpu_ctx, tmp_path, monkeypatch, UNEVEN, seed=10 + R)
+    print(f"[gpu] block sweep pair products R={R}, {steps} steps", flush=True)
+    Hs, H1, n = _pair_of_hams(gpu_ctx, tmp_path, monkeypatch, UNEVEN, seed=10 + R,
+                              env={"LSQT_SWEEP_STEPS": steps})
     re, im = _phases(n, 2 * R, 20 + R)
     L = native.Block(gpu_ctx, n, R)
     Rb = native.Block(gpu_ctx, n, R)
     L.upload(re[:R], im[:R])
     Rb.upload(re[R:], im[R:])
-    for M in ([6, 7, 100, 131] if R == 8 else [12, 65]):
+    for M in ([6, 7, 10, 11, 13, 100, 131] if R == 8 else [12, 65]):
         got = Hs.moments_block_pair(L, Rb, M)
         ref = H1.moments_block_pair(L, Rb, M)
         _check_sets(got, ref, f"pair R={R} M={M}")
