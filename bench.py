@@ -478,9 +478,10 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.B
         "seconds": {"vac": t_vac, "msd": t_msd, "host_models": build_s,
                     "hamiltonian_on_device": ham_s},
         "launches": "one per recursion step for all realisations (segmented Hamiltonian), "
-                    "two steps per launch in the moment sets (k_block_sweep2: every "
+                    "four steps per launch in the moment sets (k_block_sweepn: every "
                     "realisation is a periodic honeycomb); moment sets replayed from captured "
                     "hipGraphs",
+        "block_sweep_steps": H.info().block_sweep_steps,
         "block_sweep_systems": H.info().block_sweep_systems,
     }
     # counted HBM traffic of the batch's kernels over their compulsory bytes
@@ -488,7 +489,9 @@ def bench_batch(ctx, realisations=64, R=8, M=1500, n_steps=16, dt=2.0, emax=wl.B
     n8 = sum(b.n for b in members[:8]) if len(members) >= 8 else None
     if n8 == 8973112 and R == 8:
         ratios = {}
-        for kname, per_site in (("k_block_sweep2<3,2,3,4>", 516.0),
+        for kname, per_site in (("k_block_sweepn<3,2,3,4,1>", 516.0),
+                                ("k_block_sweepn<3,2,2,4,1>", 260.0),
+                                ("k_block_sweep2<3,2,3,4>", 516.0),
                                 ("k_block_sweep2<3,2,2,4>", 260.0),
                                 ("k_block_step<1,3,2,true,3,false,false>", 400.0),
                                 ("k_block_step<1,3,2,true,2,false,false>", 208.0),
