@@ -826,12 +826,12 @@ def main():
 
     def launches_per_set(steps):
         # the moment set's launches of its dominant kernel: M - 1 one-step
-        # launches, (M - 2) // 2 two-step launches, or (M - 4) // 4 four-step
-        # launches after the two-step one at moment 2 (the set's other
-        # launches -- mu_0, mu_1, the remainder, the reduction -- are counted
-        # into their time, which overstates it by under 0.3 %)
-        if steps == 4 and M >= 8:
-            return (M - 4) // 4
+        # launches, (M - 2) // 2 two-step launches, or (M - 4) // steps
+        # launches of more steps after the two-step one at moment 2 (the set's
+        # other launches -- mu_0, mu_1, the remainder, the reduction -- are
+        # counted into their time, which overstates it by under 0.5 %)
+        if steps >= 3 and M >= 4 + steps:
+            return (M - 4) // steps
         if steps >= 2 and M >= 4:
             return (M - 2) // 2
         return M - 1
@@ -869,7 +869,7 @@ def main():
         and info.max_neighbor == 3
     kname = {4: "k_sweepn<double2,3,4,1,false>", 2: "k_sweep2<double2,3,4,false>"}.get(
         steps64, "k_hpsi<1,3,2,true,true,true>")
-    kname32 = {4: "k_sweepn<float2,3,4,2,true>", 2: "k_sweep2<float2,3,4,true>"}.get(
+    kname32 = {6: "k_sweepn<float2,3,6,4,true>", 2: "k_sweep2<float2,3,4,true>"}.get(
         steps32, "k_hpsi32")
     traffic = profiled_traffic(kname) if default_workload else None
     # the same launch time priced on other byte counts: the one-step kernel's

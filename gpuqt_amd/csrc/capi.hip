@@ -449,15 +449,16 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
       h->allocations.push_back(dev_tab);  // kept with the Hamiltonian
       h->sweep64 = sweep_shape(g, false);
       h->sweep32 = sweep_shape(g, true);
-      // four steps per launch on the same geometry (LSQT_SWEEP_STEPS=2 in
-      // the environment keeps two): FP64 one slab per loop iteration, FP32
-      // blocks of two (measured on the 8.0M-site honeycomb: 46.7 and 27.3 us
-      // per step against 74.4 and 38.3 with two steps per launch; three, five,
-      // six and eight steps and the other block sizes were slower)
+      // more steps per launch on the same geometry (LSQT_SWEEP_STEPS=2 in
+      // the environment keeps two): FP64 four steps, one slab per loop
+      // iteration; FP32 states six steps, blocks of four slabs (measured on
+      // the 8.0M-site honeycomb: 46.7 and 26.2 us per step against 74.4 and
+      // 38.3 with two steps per launch; three, five, six and eight FP64 steps,
+      // four FP32 steps and the other block sizes were slower)
       const char* steps = getenv("LSQT_SWEEP_STEPS");
       if (!(steps && atoi(steps) == 2)) {
         h->sweepn64 = sweep_shape_n(h->sweep64, false, 4, 1);
-        h->sweepn32 = sweep_shape_n(h->sweep32, true, 4, 2);
+        h->sweepn32 = sweep_shape_n(h->sweep32, true, 6, 4);
       }
       // the same geometry serves the block path's sweeps
       h->bsweep_steps = (steps && atoi(steps) == 2) ? 2 : 4;

@@ -101,7 +101,7 @@ typedef struct {
                               creation) or 0 (no sweep); a launch reads and
                               writes bytes_per_site_sweep_f64 per site either
                               way */
-  int sweep_steps_f32;     /* the same for FP32 states */
+  int sweep_steps_f32;     /* the same for FP32 states: 6, 2 or 0 */
   int sweep_tiles_deep;    /* workgroups per four-step launch, FP64 states */
   int block_sweep_steps;   /* most steps per launch of the block path's
                               sweeps (4 up to 16 columns, else 2; 2 with

@@ -26,12 +26,12 @@ def test_traffic_profile_entries():
     # 8.0M (slot pattern: words of the 0.1 % irregular sites, 1 mask bit)
     assert fp64 is not None and abs(fp64 / (72.137 * 8_000_000) - 1.0) < 0.05
     assert bench.profiled_traffic("k_hpsi32<1,3,true,true,true>") is not None
-    # the two- and four-step sweeps: 88 B (FP64) and 44 B (FP32) x 8.0M sites
-    # per launch
+    # the two-, four- (FP64) and six-step (FP32) sweeps: 88 B (FP64) and 44 B
+    # (FP32) x 8.0M sites per launch
     for name, per_site in (("k_sweep2<double2,3,4,false>", 88.0),
                            ("k_sweep2<float2,3,4,true>", 44.0),
                            ("k_sweepn<double2,3,4,1,false>", 88.0),
-                           ("k_sweepn<float2,3,4,2,true>", 44.0)):
+                           ("k_sweepn<float2,3,6,4,true>", 44.0)):
         t = bench.profiled_traffic(name)
         assert t is not None and abs(t / (per_site * 8_000_000) - 1.0) < 0.07, name
     for mode in (1, 2):  # <L|psi> and product-identity block steps, sliding window

@@ -2,7 +2,7 @@
 
 Where every neighbour of a site lies within one slab and a few positions of
 it (periodically), the moment sets advance two (k_sweep2) or four (k_sweepn)
-recursion steps per launch from LDS.  Checked here:
+recursion steps per launch from LDS (six for FP32 states).  Checked here:
 
   - which Hamiltonians get the sweep: periodic honeycomb (whole-slab and
     chunked slabs), square and triangular lattices; none for a Peierls-phase
@@ -12,8 +12,8 @@ recursion steps per launch from LDS.  Checked here:
     LSQT_SWEEP=0;
   - lsqt_kernel_polynomial_pair against two lsqt_kernel_polynomial calls,
     bit for bit (the per-site arithmetic is the same, in the same order);
-  - FP64 moment sets (four steps per launch from moment 4, two steps
-    before and for the remainder) against the same Hamiltonian without the
+  - FP64 moment sets (four steps per launch from moment 4 -- six with FP32
+    states --, two steps before and for the remainder) against the same Hamiltonian without the
     sweep and with two steps per launch only (the partial sums group the
     sites differently: the FP64 moment tolerance) and against the oracle,
     for moment counts 2 .. 5, 8 .. 13 (every remainder of the four-step
@@ -188,13 +188,14 @@ def test_sweep_moments(gpu_ctx, tmp_path, monkeypatch):
         monkeypatch.setenv("LSQT_SWEEP_STEPS", "2")
         H2 = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, K, emax)
         monkeypatch.delenv("LSQT_SWEEP_STEPS")
-        assert (H.info().sweep_steps, H.info().sweep_steps_f32) == (4, 4), name
+        assert (H.info().sweep_steps, H.info().sweep_steps_f32) == (4, 6), name
         assert (H2.info().sweep_steps, H2.info().sweep_steps_f32) == (2, 2), name
         assert H0.info().sweep_steps == 0
         v, l = _random_vector(H, rng), _random_vector(H, rng)
         # 2 .. 5: one and two steps per launch only; 8 .. 13: one four-step
-        # launch, then every remainder; 64, 129: many
-        for M in (2, 3, 4, 5, 8, 9, 10, 11, 12, 13, 64, 129):
+        # launch (FP64), then every remainder; 10 .. 17: one six-step launch
+        # (FP32 states); 64, 129: many
+        for M in (2, 3, 4, 5, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 64, 129):
             ref_self, ref_pair = H0.moments(v, v, M), H0.moments(l, v, M)
             assert_moments(H.moments(v, v, M), ref_self, f"{name} M={M} self")
             assert_moments(H.moments(l, v, M), ref_pair, f"{name} M={M} pair")
