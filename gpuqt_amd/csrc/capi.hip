@@ -1640,14 +1640,32 @@ int lsqt::evolve_enqueue(lsqt_ham* h, int direction, double t, double2* state)
 {
   hipStream_t st = h->ctx->stream;
   const int n = h->d.n;
-  LSQT_TRY(ham_scratch(h, 2));
+  const SweepMap& sn = h->sweepn64;
+  LSQT_TRY(ham_scratch(h, sn.S ? 4 : 2));
   double2* A = h->scratch[0];  // psi_{m-2}
   double2* B = h->scratch[1];  // psi_{m-1}
   BesselTerms bt = evolve_terms(direction, t);
   launch_copy(A, state, n, st);
   launch_hpsi(h->d, state, nullptr, B, nullptr, nullptr, false, false, st);
   launch_cheb01(A, B, state, bt.b0, bt.b1, direction, n, st);
-  for (size_t k = 0; k < bt.bm.size(); ++k) {
+  size_t k = 0;
+  if (sn.S > 0) {
+    // sn.ns terms per launch on a periodic lattice (the same additions in
+    // the same order), the rest below, from an even term
+    double2* C = h->scratch[2];
+    double2* D = h->scratch[3];
+    for (; k + sn.ns <= bt.bm.size(); k += sn.ns) {
+      SweepTerms terms{};
+      for (int l = 0; l < sn.ns; ++l) {
+        terms.bm[l] = bt.bm[k + l];
+        terms.label[l] = bt.label[k + l];
+      }
+      launch_sweepn_evo(h->d, sn, A, B, C, D, state, terms, st);
+      std::swap(A, C);
+      std::swap(B, D);
+    }
+  }
+  for (; k < bt.bm.size(); ++k) {
     const DeferredTerm d = deferred_term(bt, k);
     launch_cheb2(h->d, A, B, A, state, d.bm, d.label, st, d.bm_prev, d.label_prev);
     double2* tmp = A;

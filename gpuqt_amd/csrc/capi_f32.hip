@@ -205,7 +205,25 @@ int lsqt::evolve_f32_enqueue(lsqt_ham* h, int direction, double t, double2* stat
   launch_to_f32(state, A, n, st);
   launch_hpsi32(h->d, A, nullptr, B, nullptr, nullptr, false, false, st);
   launch_cheb01_32(B, state, bt.b0, bt.b1, direction, n, st);
-  for (size_t k = 0; k < bt.bm.size(); ++k) {
+  size_t k = 0;
+  const SweepMap& sn = h->sweepn32;
+  if (sn.S > 0) {
+    // sn.ns terms per launch on a periodic lattice, into the buffers of the
+    // [X, U] recursion (free here); the rest below, from an even term
+    float2* C = h->f32->v[2];
+    float2* D = h->f32->v[3];
+    for (; k + sn.ns <= bt.bm.size(); k += sn.ns) {
+      SweepTerms terms{};
+      for (int l = 0; l < sn.ns; ++l) {
+        terms.bm[l] = bt.bm[k + l];
+        terms.label[l] = bt.label[k + l];
+      }
+      launch_sweepn_evo_32(h->d, sn, A, B, C, D, state, terms, st);
+      std::swap(A, C);
+      std::swap(B, D);
+    }
+  }
+  for (; k < bt.bm.size(); ++k) {
     const DeferredTerm d = deferred_term(bt, k);
     launch_cheb2_32(h->d, A, B, A, state, d.bm, d.label, st, d.bm_prev, d.label_prev);
     float2* tmp = A;

@@ -27,11 +27,41 @@
 // order): the FP64 instantiation gives the bits of k_hpsi and of the
 // reference at every level; only the moment partial sums group the sites by
 // sweep tile.
+//
+// EVO: NS terms of the Chebyshev series of U(t) instead of moments: level l
+// adds its Bessel-weighted value to the FP64 state S (terms.bm / label of
+// level l: the phase cases of the one-term kernels; label 0 adds nothing),
+// the state of a site held in registers from its slab's level 1 to its
+// level NS and written once.  The additions are those of one term per
+// launch, in the same order, so the state's bits are k_cheb2's.
 #pragma once
 
 #include "kernels_sweep.h"
 
 namespace lsqt {
+
+__device__ __forceinline__ void sweep_add_term(double2& s, double bm, double tr, double ti,
+                                               int label)
+{
+  switch (label) {
+    case 1:
+      s.x += bm * tr;
+      s.y += bm * ti;
+      break;
+    case 2:
+      s.x -= bm * tr;
+      s.y -= bm * ti;
+      break;
+    case 3:
+      s.x += bm * ti;
+      s.y -= bm * tr;
+      break;
+    case 4:
+      s.x -= bm * ti;
+      s.y += bm * tr;
+      break;
+  }
+}
 
 // static LDS of k_sweepn<V, K, NS, B>: the level rings and the tile sums
 template <typename V, int NS, int B>
@@ -40,16 +70,18 @@ constexpr size_t sweepn_lds_bytes()
   return sizeof(V) * NS * (B + 2) * kSweepThreads + sizeof(double) * (kSweepThreads / 64) * 2 * NS;
 }
 
-template <typename V, int K, int NS, int B, bool FAST>
+template <typename V, int K, int NS, int B, bool FAST, bool EVO = false>
 __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
                                                           const V* __restrict__ p2,
                                                           const V* __restrict__ p1,
                                                           V* __restrict__ o0, V* __restrict__ o1,
                                                           const V* __restrict__ L,
                                                           double2* __restrict__ part,
-                                                          size_t part_stride)
+                                                          size_t part_stride,
+                                                          double2* __restrict__ state = nullptr,
+                                                          SweepTerms terms = SweepTerms{})
 {
-  static_assert(NS >= 3, "two steps per launch: k_sweep2");
+  static_assert(NS >= 3 && NS <= kSweepMaxSteps, "two steps per launch: k_sweep2");
   using IO = SweepIO<V>;
   constexpr int T = kSweepThreads;
   constexpr int RR = B + 2;       // rows of a level ring
@@ -133,13 +165,18 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
 #pragma unroll
   for (int l = 0; l < NS; ++l)
     acc_r[l] = acc_i[l] = 0.0;
-  // potential and left vector of slabs k0 - NS + 1 + j, j < RB
+  // potential and left vector (EVO: the state) of slabs k0 - NS + 1 + j,
+  // j < RB
   double pv[RB];
-  V lv[RB];
+  V lv[EVO ? 1 : RB];
+  double2 sv[EVO ? RB : 1];
 #pragma unroll
   for (int j = 0; j < RB; ++j) {
     pv[j] = 0.0;
-    lv[j] = IO::narrow(0.0, 0.0);
+    if constexpr (EVO)
+      sv[j] = make_double2(0.0, 0.0);
+    else
+      lv[j] = IO::narrow(0.0, 0.0);
   }
 
   for (int k0 = 0; k0 <= bx + 2 * NS - 3; k0 += B) {
@@ -154,7 +191,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
       const size_t x = (size_t)slab_of(k0 + i) * S;
       ip2[i] = p2[x + pos_p];
       pv[NS - 1 + i] = IO::pot(h, (int)(x + pos_p));
-      lv[NS - 1 + i] = L[x + pos_l];
+      if constexpr (EVO)
+        sv[NS - 1 + i] = state[x + pos_l];
+      else
+        lv[NS - 1 + i] = L[x + pos_l];
     }
 #pragma unroll
     for (int i = 0; i < B; ++i)
@@ -197,7 +237,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
         out[i] = o;
         if (l < NS)
           s_lv[l][ring(q)][t] = o;
-        if (interior && q >= NS - 1 && q <= NS - 2 + bx) {
+        if constexpr (EVO) {
+          // the term of this level in the FP64 state, from the value stored
+          const double2 ow = IO::wide(o);
+          sweep_add_term(sv[NS - l + i], terms.bm[l - 1], ow.x, ow.y, terms.label[l - 1]);
+        } else if (interior && q >= NS - 1 && q <= NS - 2 + bx) {
           const double2 ow = IO::wide(o), lw = IO::wide(lv[NS - l + i]);
           acc_r[l - 1] += ow.x * lw.x + ow.y * lw.y;
           acc_i[l - 1] += ow.y * lw.x - ow.x * lw.y;
@@ -207,8 +251,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
 #pragma unroll
         for (int i = 0; i < B; ++i) {
           const int q = k0 - (l - 1) + i;
-          if (interior && q >= NS - 1 && q <= NS - 2 + bx)
+          if (interior && q >= NS - 1 && q <= NS - 2 + bx) {
             (l == NS ? o1 : o0)[gsite(q)] = out[i];
+            if constexpr (EVO) {
+              if (l == NS)
+                state[gsite(q)] = sv[i];  // every term of the launch added
+            }
+          }
         }
       }
       __syncthreads();
@@ -216,9 +265,14 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
 #pragma unroll
     for (int j = 0; j + B < RB; ++j) {
       pv[j] = pv[j + B];
-      lv[j] = lv[j + B];
+      if constexpr (EVO)
+        sv[j] = sv[j + B];
+      else
+        lv[j] = lv[j + B];
     }
   }
+  if constexpr (EVO)
+    return;
 
   // the tile's NS moment partials: wave64 shuffle tree, then the waves in order
 #pragma unroll
@@ -246,6 +300,42 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
     }
     part[(size_t)t * part_stride + tile] = make_double2(re, im);
   }
+}
+
+// NS terms of U(t) per launch (EVO): from p2 = phi_{k-2}, p1 = phi_{k-1},
+// o0 = phi_{k+NS-2}, o1 = phi_{k+NS-1}, level l's term added into S with
+// terms.bm[l - 1] / label[l - 1]
+template <typename V, int NS, int B>
+void sweepn_evo_launch(const HamDev& h, const SweepMap& sm, const V* p2, const V* p1, V* o0,
+                       V* o1, double2* S, const SweepTerms& terms, hipStream_t st)
+{
+  const bool fast = sizeof(V) == sizeof(float2) && sm.real;
+#define LSQT_SWNE(KK)                                                                         \
+  do {                                                                                        \
+    if (fast)                                                                                 \
+      k_sweepn<V, KK, NS, B, sizeof(V) == sizeof(float2), true>                               \
+        <<<sm.blocks, kSweepThreads, 0, st>>>(h, sm, p2, p1, o0, o1, p1, nullptr, 0, S, terms); \
+    else                                                                                      \
+      k_sweepn<V, KK, NS, B, false, true>                                                     \
+        <<<sm.blocks, kSweepThreads, 0, st>>>(h, sm, p2, p1, o0, o1, p1, nullptr, 0, S, terms); \
+  } while (0)
+  switch (h.K) {
+    case 2:
+      LSQT_SWNE(2);
+      break;
+    case 3:
+      LSQT_SWNE(3);
+      break;
+    case 4:
+      LSQT_SWNE(4);
+      break;
+    case 6:
+      LSQT_SWNE(6);
+      break;
+    default:
+      break;
+  }
+#undef LSQT_SWNE
 }
 
 // Host launcher of k_sweepn: NS steps per launch in blocks of B slabs for

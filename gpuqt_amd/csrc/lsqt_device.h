@@ -291,6 +291,14 @@ struct SweepMap {
   int nb;            // slabs per loop iteration (k_sweepn)
 };
 
+// the series coefficients of the levels of a multi-step launch of the
+// evolution (k_sweepn): 2 J_m(t) and the phase case of each term
+constexpr int kSweepMaxSteps = 8;
+struct SweepTerms {
+  double bm[kSweepMaxSteps];
+  int label[kSweepMaxSteps];
+};
+
 // Two steps per launch for blocks of vectors (k_block_sweep2): the slab
 // geometry of one system of a (possibly segmented) Hamiltonian, and its
 // chunks and runs for one launch shape.

@@ -64,6 +64,15 @@ void launch_sweepn_32(const HamDev& h, const SweepMap& sm, const float2* psi2,
                       const float2* psi1, float2* out0, float2* out1, const float2* L,
                       double2* part, size_t part_stride, hipStream_t st);
 size_t sweepn_lds(bool f32, int ns, int nb);
+// sm.ns terms of U(t) per launch: from psi2 = phi_{k-2}, psi1 = phi_{k-1},
+// out0 = phi_{k+ns-2}, out1 = phi_{k+ns-1}, term l added into the FP64 state
+// S with terms.bm[l] / label[l], in order (the additions of one-term launches)
+void launch_sweepn_evo(const HamDev& h, const SweepMap& sm, const double2* psi2,
+                       const double2* psi1, double2* out0, double2* out1, double2* S,
+                       const SweepTerms& terms, hipStream_t st);
+void launch_sweepn_evo_32(const HamDev& h, const SweepMap& sm, const float2* psi2,
+                          const float2* psi1, float2* out0, float2* out1, double2* S,
+                          const SweepTerms& terms, hipStream_t st);
 // the check of a sweep table against every packed word: one int per
 // workgroup of sweep_check_blocks(n), nonzero where a word differs
 int sweep_check_blocks(int n);
