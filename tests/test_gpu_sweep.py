@@ -18,7 +18,9 @@ recursion steps per launch from LDS (six for FP32 states).  Checked here:
     sites differently: the FP64 moment tolerance) and against the oracle,
     for moment counts 2 .. 5, 8 .. 13 (every remainder of the four-step
     launches) and odd / even large ones; FP32-state moment sets against
-    their one-step counterparts within 1e-6 of mu_0.
+    their one-step counterparts within 1e-6 of mu_0;
+  - U(t) with four Bessel terms per launch (six with FP32 states) against one
+    term per launch, bit for bit in FP64.
 """
 import numpy as np
 import pytest
@@ -211,4 +213,43 @@ def test_sweep_moments(gpu_ctx, tmp_path, monkeypatch):
             assert_moments(H.moments(ov, ov, 64), ref_re + 1j * ref_im, f"{name} oracle")
             ov.close()
         for o in (v, l, H, H0, H2):
+            o.close()
+
+
+def test_sweep_evolution(gpu_ctx, tmp_path, monkeypatch):
+    """U(t) with four Bessel terms per launch (six with FP32 states) against
+    one term per launch (LSQT_SWEEP_STEPS=2 leaves the evolution one term
+    per launch): FP64 states bit for bit, FP32-state evolution within FP32
+    rounding; series lengths with every remainder."""
+    from gpuqt_amd import native
+    rng = np.random.default_rng(21)
+    for name, arrays, n, K, emax, slab, _, _ in _systems(tmp_path):
+        if not slab:
+            continue
+        print(f"[gpu] sweep evolution: {name}", flush=True)
+        H = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, K, emax)
+        monkeypatch.setenv("LSQT_SWEEP_STEPS", "2")
+        H2 = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, K, emax)
+        monkeypatch.delenv("LSQT_SWEEP_STEPS")
+        ph = rng.uniform(0, 2 * np.pi, n)
+        re, im = np.cos(ph), np.sin(ph)
+        for t in (0.4, 2.5, 7.0, 20.0):
+            for direction in (1, -1):
+                a, b = H.vector(re, im), H2.vector(re, im)
+                H.evolve(direction, t, a)
+                H2.evolve(direction, t, b)
+                ga, gb = a.download(), b.download()
+                assert_bits(ga[0], gb[0], f"{name} t={t} dir={direction} re")
+                assert_bits(ga[1], gb[1], f"{name} t={t} dir={direction} im")
+                a.close()
+                b.close()
+            a, b = H.vector(re, im), H2.vector(re, im)
+            H.evolve_f32(1, t, a)
+            H2.evolve_f32(1, t, b)
+            ga, gb = a.download(), b.download()
+            err = max(np.abs(ga[0] - gb[0]).max(), np.abs(ga[1] - gb[1]).max())
+            assert err <= 2e-6, f"{name} t={t} f32: {err}"
+            a.close()
+            b.close()
+        for o in (H, H2):
             o.close()
