@@ -1056,6 +1056,13 @@ void block_evolve(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Bl
 {
   hipStream_t st = h->ctx->stream;
   const int nR = (int)((size_t)h->d.n * R);
+  // four terms per launch where the lattice allows (its launch shape and
+  // the two more recursion buffers are made on first use; without them the
+  // terms take one launch each)
+  BlockSweepShape* deep = nullptr;
+  if (block_sweep_shape(h, R, 4, &deep) != LSQT_OK ||
+      (deep && block_sweep_scratch(h, R, tm.tiles, s) != LSQT_OK))
+    deep = nullptr;
   float2* A = s->A;
   float2* B = s->B;
   if (renormalise)
@@ -1065,7 +1072,24 @@ void block_evolve(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Bl
   launch_block_evo(h->d, R, c.W, tm, 0, nullptr, A, B, nullptr, nullptr, nullptr, nullptr, 0.0,
                    0, st);
   launch_cheb01_32(B, S, bt.b0, bt.b1, direction, nR, st);
-  for (size_t k = 0; k < bt.bm.size(); ++k) {
+  size_t k = 0;
+  float2* C = s->C;
+  float2* D = s->D;
+  if (deep) {
+    // four terms per launch on periodic lattices (the same additions in the
+    // same order), the rest below, from an even term
+    for (; k + 4 <= bt.bm.size(); k += 4) {
+      SweepTerms terms{};
+      for (int l = 0; l < 4; ++l) {
+        terms.bm[l] = bt.bm[k + l];
+        terms.label[l] = bt.label[k + l];
+      }
+      launch_block_sweepn_evo(h->d, deep->map, R, A, B, C, D, S, terms, st);
+      std::swap(A, C);
+      std::swap(B, D);
+    }
+  }
+  for (; k < bt.bm.size(); ++k) {
     const DeferredTerm d = deferred_term(bt, k);
     launch_block_evo(h->d, R, c.W, tm, 1, A, B, A, nullptr, nullptr, nullptr, S, d.bm, d.label,
                      st, d.bm_prev, d.label_prev);

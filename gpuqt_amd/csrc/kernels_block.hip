@@ -725,12 +725,6 @@ __global__ __launch_bounds__(kBlock) void k_block_win(
   store_products<IPM, W>(acc, vpw, R, r0, tile, s_red, partial, partial2);
 }
 
-}  // namespace lsqt
-
-#include "kernels_block_sweep.h"
-
-namespace lsqt {
-
 // mu[(k*R + r)*M + m0 + i] = sum over the tiles t of segment k of
 // partial[i][t][r] (slots `stride` tiles apart), in tile order, i <
 // gridDim.x, k = blockIdx.z (one segment of all tiles without a segment
@@ -850,6 +844,12 @@ __device__ __forceinline__ void accumulate_phase(double2& s, double bm, double t
       break;
   }
 }
+
+}  // namespace lsqt
+
+#include "kernels_block_sweep.h"
+
+namespace lsqt {
 
 // Expansion terms into S (kCheb2, kCheb2x): label / bm for this launch's
 // term (the new s2, or s2x), label_prev / bm_prev for the term of the
@@ -1614,6 +1614,34 @@ void launch_block_sweepn(const HamDev& h, const BlockSweepMap& sm, int cols, boo
       break;
   }
 #undef LSQTB_SWN
+}
+
+// four terms of U(t) per launch on blocks of cols <= 16 columns
+void launch_block_sweepn_evo(const HamDev& h, const BlockSweepMap& sm, int cols,
+                             const float2* p2, const float2* p1, float2* o0, float2* o1,
+                             double2* state, const SweepTerms& terms, hipStream_t st)
+{
+  const size_t lds = block_sweepn_lds();
+#define LSQTB_SWNE(KK)                                                                     \
+  k_block_sweepn<KK, 2, kSelf, 4, 1, true><<<sm.blocks, kBlock, lds, st>>>(                \
+    h, sm, cols, p2, p1, o0, o1, nullptr, 0, state, terms)
+  switch (h.K) {
+    case 2:
+      LSQTB_SWNE(2);
+      break;
+    case 3:
+      LSQTB_SWNE(3);
+      break;
+    case 4:
+      LSQTB_SWNE(4);
+      break;
+    case 6:
+      LSQTB_SWNE(6);
+      break;
+    default:
+      break;
+  }
+#undef LSQTB_SWNE
 }
 
 int block_sweep_check_blocks(const BlockSweepSeg& g)

@@ -265,11 +265,18 @@ constexpr size_t block_sweepn_lds_bytes()
   return sizeof(float2) * NS * (B + 2) * kBlock * W;
 }
 
-template <int K, int W, int IPM, int NS, int B>
+// EVO: NS terms of the Chebyshev series of U(t) instead of moment products,
+// with the per-site arithmetic of the one-term block kernel (k_block_evo):
+// level l adds terms.bm / label[l - 1] times its value into the FP64 state
+// blocks (cols double2 per site), each site's state held in registers from
+// its slab's level 1 to level NS and written once; the additions of one
+// term per launch, in the same order.
+template <int K, int W, int IPM, int NS, int B, bool EVO = false>
 __global__ __launch_bounds__(kBlock) void k_block_sweepn(
   HamDev h, BlockSweepMap sm, int cols, const float2* __restrict__ p2,
   const float2* __restrict__ p1, float2* __restrict__ o0, float2* __restrict__ o1,
-  double2* __restrict__ part, size_t part_stride)
+  double2* __restrict__ part, size_t part_stride, double2* __restrict__ state = nullptr,
+  SweepTerms terms = SweepTerms{})
 {
   static_assert(W % 2 == 0 && NS >= 3, "W even, NS >= 3");
   static_assert(block_sweepn_lds_bytes<W, NS, B>() >= sizeof(double) * 2 * (kBlock / 64) * 64 * 4,
@@ -323,8 +330,10 @@ __global__ __launch_bounds__(kBlock) void k_block_sweepn(
   };
   const int te = tpos < wlen ? tpos : 0;
   const int pos = wrap_pos(te);
-  // psi_{m-2} and the potential are used on level 1's entries only
+  // psi_{m-2} and the potential are used on level 1's entries only, the
+  // state (EVO) on the interior
   const int pos_p = wrap_pos(min(max(tpos, hh), max(wlen - hh - 1, 0)));
+  const int pos_l = wrap_pos(min(max(tpos, NS * hh), NS * hh + clen - 1));
   const bool interior = tpos >= NS * hh && tpos < NS * hh + clen;
   const int o_self = te * vpw * W + r0;
   const int r = pos % g.p;
@@ -358,10 +367,17 @@ __global__ __launch_bounds__(kBlock) void k_block_sweepn(
 #pragma unroll
   for (int l = 0; l < NS; ++l)
     clear_products(acc[l]);
-  float pv[RB];  // potential of slabs k0 - NS + 1 + j
+  float pv[RB];  // potential (EVO: and state) of slabs k0 - NS + 1 + j
+  double2 sv[EVO ? RB : 1][W];
 #pragma unroll
-  for (int j = 0; j < RB; ++j)
+  for (int j = 0; j < RB; ++j) {
     pv[j] = 0.0f;
+    if constexpr (EVO) {
+#pragma unroll
+      for (int c = 0; c < W; ++c)
+        sv[j][c] = make_double2(0.0, 0.0);
+    }
+  }
 
   for (int k0 = 0; k0 <= bx + 2 * NS - 3; k0 += B) {
     Vec<W> w[B], ip2[B];
@@ -373,6 +389,12 @@ __global__ __launch_bounds__(kBlock) void k_block_sweepn(
       const size_t at = base + (size_t)slab_of(k0 + i) * S + pos_p;
       ip2[i] = ldv<W, true>(p2 + at * cols + r0);
       pv[NS - 1 + i] = h.pot32[at];
+      if constexpr (EVO) {
+        const size_t as = (base + (size_t)slab_of(k0 + i) * S + pos_l) * cols + r0;
+#pragma unroll
+        for (int c = 0; c < W; ++c)
+          sv[NS - 1 + i][c] = state[as + c];
+      }
     }
 #pragma unroll
     for (int i = 0; i < B; ++i)
@@ -398,10 +420,15 @@ __global__ __launch_bounds__(kBlock) void k_block_sweepn(
 #pragma unroll
           for (int c = 0; c < W; ++c) {
             const float cx = nb.v[c].x, cy = nb.v[c].y;
-            tr[c] = __builtin_fmaf(ha[s], cx, tr[c]);
-            tr[c] = __builtin_fmaf(-hb[s], cy, tr[c]);
-            ti[c] = __builtin_fmaf(ha[s], cy, ti[c]);
-            ti[c] = __builtin_fmaf(hb[s], cx, ti[c]);
+            if constexpr (EVO) {  // k_block_evo's order
+              tr[c] += __builtin_fmaf(ha[s], cx, -hb[s] * cy);
+              ti[c] += __builtin_fmaf(ha[s], cy, hb[s] * cx);
+            } else {
+              tr[c] = __builtin_fmaf(ha[s], cx, tr[c]);
+              tr[c] = __builtin_fmaf(-hb[s], cy, tr[c]);
+              ti[c] = __builtin_fmaf(ha[s], cy, ti[c]);
+              ti[c] = __builtin_fmaf(hb[s], cx, ti[c]);
+            }
           }
         }
         const Vec<W> prev =
@@ -415,23 +442,46 @@ __global__ __launch_bounds__(kBlock) void k_block_sweepn(
         out[i] = o;
         if (l < NS)
           lds_put<W>(lv_at(l, ring(q)) + t * W, o);
-        if (interior && q >= NS - 1 && q <= NS - 2 + bx)
+        if constexpr (EVO) {
+#pragma unroll
+          for (int c = 0; c < W; ++c)
+            accumulate_phase(sv[NS - l + i][c], terms.bm[l - 1], (double)o.v[c].x,
+                             (double)o.v[c].y, terms.label[l - 1]);
+        } else if (interior && q >= NS - 1 && q <= NS - 2 + bx) {
           add_products<IPM, W>(acc[l - 1], o, self, self);
+        }
       }
       if (l >= NS - 1) {
 #pragma unroll
         for (int i = 0; i < B; ++i) {
           const int q = k0 - (l - 1) + i;
-          if (interior && q >= NS - 1 && q <= NS - 2 + bx)
+          if (interior && q >= NS - 1 && q <= NS - 2 + bx) {
             stv<W, true>((l == NS ? o1 : o0) + gidx(slab_of(q), pos), out[i]);
+            if constexpr (EVO) {
+              if (l == NS) {  // every term of the launch added
+                const size_t as = gidx(slab_of(q), pos);
+#pragma unroll
+                for (int c = 0; c < W; ++c)
+                  state[as + c] = sv[i][c];
+              }
+            }
+          }
         }
       }
       __syncthreads();
     }
 #pragma unroll
-    for (int j = 0; j + B < RB; ++j)
+    for (int j = 0; j + B < RB; ++j) {
       pv[j] = pv[j + B];
+      if constexpr (EVO) {
+#pragma unroll
+        for (int c = 0; c < W; ++c)
+          sv[j][c] = sv[j + B][c];
+      }
+    }
   }
+  if constexpr (EVO)
+    return;
   auto s_red = reinterpret_cast<double(*)[kBlock / 64][64 * 4]>(s_dyn);
 #pragma unroll
   for (int l = 0; l < NS; ++l)

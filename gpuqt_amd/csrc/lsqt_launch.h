@@ -178,6 +178,12 @@ size_t block_sweepn_lds();
 void launch_block_sweepn(const HamDev& h, const BlockSweepMap& sm, int cols, bool pairs,
                          const float2* p2, const float2* p1, float2* o0, float2* o1,
                          double2* part, size_t part_stride, hipStream_t st);
+// four terms of U(t) per launch on the block path (cols <= 16): from p2 =
+// phi_{k-2}, p1 = phi_{k-1}, o0 = phi_{k+2}, o1 = phi_{k+3}, term l added
+// into the FP64 state blocks with terms.bm[l] / label[l], in order
+void launch_block_sweepn_evo(const HamDev& h, const BlockSweepMap& sm, int cols,
+                             const float2* p2, const float2* p1, float2* o0, float2* o1,
+                             double2* state, const SweepTerms& terms, hipStream_t st);
 // the check of one system's sweep table against its packed words: one int
 // per workgroup of block_sweep_check_blocks(g), nonzero where a word differs
 int block_sweep_check_blocks(const BlockSweepSeg& g);
