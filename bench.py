@@ -868,9 +868,9 @@ def main():
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     default_workload = (args.nx, args.ny) == (2000, 1000) and info.layout == 2 \
         and info.max_neighbor == 3
-    kname = {4: "k_sweepn<double2,3,4,1,false,false>", 2: "k_sweep2<double2,3,4,false>"}.get(
+    kname = {4: "k_sweepn<double2,3,4,1,false,false,true>", 2: "k_sweep2<double2,3,4,false>"}.get(
         steps64, "k_hpsi<1,3,2,true,true,true>")
-    kname32 = {6: "k_sweepn<float2,3,6,4,true,false>", 2: "k_sweep2<float2,3,4,true>"}.get(
+    kname32 = {6: "k_sweepn<float2,3,6,4,true,false,false>", 2: "k_sweep2<float2,3,4,true>"}.get(
         steps32, "k_hpsi32")
     traffic = profiled_traffic(kname) if default_workload else None
     # the same launch time priced on other byte counts: the one-step kernel's

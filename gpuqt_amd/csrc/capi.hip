@@ -314,6 +314,11 @@ SweepMap sweep_shape(SweepMap g, bool f32)
 // (none where that leaves fewer than half), runs as sweep_shape.
 SweepMap sweep_shape_n(SweepMap g, bool f32, int ns, int nb)
 {
+  // the written levels go through buffer resources over one slab
+  if ((long long)g.S * sizeof(double2) >= (1ll << 31)) {
+    g.S = 0;
+    return g;
+  }
   const bool whole = g.hh == 0 && g.nch == 1 && g.cs == g.S;
   if (!whole) {
     g.cs = kSweepThreads - 2 * ns * g.hh;
@@ -433,9 +438,13 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
       }
       g.wmax = g.cs + 4 * g.hh;
       g.real = 1;
-      for (size_t e = 0; e < tab.size(); ++e)
+      g.full = 1;
+      for (size_t e = 0; e < tab.size(); ++e) {
         if (tab[e].w && (tab[e].z >= (int)classes.size() || classes[tab[e].z].y != 0.0))
           g.real = 0;
+        if (!tab[e].w)
+          g.full = 0;
+      }
       if (hipMemcpy(dev_tab, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice) !=
           hipSuccess)
         break;

@@ -32,6 +32,8 @@
 // puts the longest runs first on every XCD.
 #pragma once
 
+#include <type_traits>
+
 namespace lsqt {
 
 template <int W>
@@ -259,6 +261,10 @@ __global__ __launch_bounds__(kBlock) void k_block_sweep2(
 // levels are written.  Level l adds its two products to part[2 (l - 1)] and
 // part[2 (l - 1) + 1] (moments 2 (m + l - 1) - 1 and 2 (m + l - 1) of the
 // product identities).  A chunk holds the row's positions less 2 NS h.
+// As in k_sweepn: each iteration's global loads are issued one iteration
+// ahead and waited for at the end of the iteration before, ahead of its
+// stores; the loop is unrolled by the period of the ring rows, so a
+// neighbour's LDS offset is a per-thread constant of its slot and row.
 template <int W, int NS, int B>
 constexpr size_t block_sweepn_lds_bytes()
 {
@@ -284,6 +290,9 @@ __global__ __launch_bounds__(kBlock) void k_block_sweepn(
   constexpr int RR = B + 2;        // rows of a level ring
   constexpr int RB = B + NS - 1;   // slabs of potential held per thread
   constexpr int RW = kBlock * W;   // ring row length in float2
+  constexpr int RL = RR * RW;      // a level's ring in float2
+  constexpr int U = RR / sweep_gcd(B, RR);  // iterations until the rows repeat
+  static_assert(U >= 1 && U <= 3, "unrolled by one, two or three iterations");
   extern __shared__ __align__(16) unsigned char s_dyn[];
   const int4 od = sm.order[xcd_tile(blockIdx.x, sm.blocks)];
   if (od.x < 0)
@@ -379,105 +388,175 @@ __global__ __launch_bounds__(kBlock) void k_block_sweepn(
     }
   }
 
-  for (int k0 = 0; k0 <= bx + 2 * NS - 3; k0 += B) {
-    Vec<W> w[B], ip2[B];
+  // neighbour s's entry (float2 offset in a level's ring) when the site's
+  // slab sits in row c
+  int nbo[K][RR];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+#pragma unroll
+    for (int c = 0; c < RR; ++c)
+      nbo[q][c] = (((c + dq[q]) % RR + RR) % RR) * RW + eo[q];
+  }
+  // the slabs of iteration k: level 0 at k + 1 .. k + B; psi_{m-2},
+  // potential (EVO: state) at level 1's slabs k .. k + B - 1 (past the last
+  // iteration: slabs already read)
+  const int k_last = bx + 2 * NS - 3;
+  Vec<W> nw[B], np2[B];
+  float npv[B];
+  double2 nsv[EVO ? B : 1][W];
+  auto load_slabs = [&](int k) {
 #pragma unroll
     for (int i = 0; i < B; ++i)
-      w[i] = ldv<W, false>(p1 + gidx(slab_of(k0 + 1 + i), pos));
+      nw[i] = ldv<W, false>(p1 + gidx(slab_of(k + 1 + i), pos));
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-      const size_t at = base + (size_t)slab_of(k0 + i) * S + pos_p;
-      ip2[i] = ldv<W, true>(p2 + at * cols + r0);
-      pv[NS - 1 + i] = h.pot32[at];
+      const int sl = slab_of(min(k + i, k_last));
+      const size_t at = base + (size_t)sl * S + pos_p;
+      np2[i] = ldv<W, true>(p2 + at * cols + r0);
+      npv[i] = h.pot32[at];
       if constexpr (EVO) {
-        const size_t as = (base + (size_t)slab_of(k0 + i) * S + pos_l) * cols + r0;
+        const size_t as = (base + (size_t)sl * S + pos_l) * cols + r0;
 #pragma unroll
         for (int c = 0; c < W; ++c)
-          sv[NS - 1 + i][c] = state[as + c];
+          nsv[i][c] = state[as + c];
       }
     }
+  };
+  auto hold_slabs = [&]() {  // the loads of load_slabs landed
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        sweep_hold(nw[i].v[c]);
+        sweep_hold(np2[i].v[c]);
+        if constexpr (EVO)
+          sweep_hold(nsv[i][c]);
+      }
+      sweep_hold(npv[i]);
+    }
+  };
+  load_slabs(0);
+  hold_slabs();
+
+  // one iteration, k0 = c B modulo RR: the ring row of slab k0 + d is
+  // row(d), a constant once the loops over levels and slabs are unrolled
+  auto iterate = [&](auto phase, int k0) {
+    constexpr int c = decltype(phase)::value;
+    auto row = [](int d) { return ((c * B + d) % RR + RR) % RR; };
+    Vec<W> w[B], ip2[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      w[i] = nw[i];
+      ip2[i] = np2[i];
+      pv[NS - 1 + i] = npv[i];
+      if constexpr (EVO) {
+#pragma unroll
+        for (int cc = 0; cc < W; ++cc)
+          sv[NS - 1 + i][cc] = nsv[i][cc];
+      }
+    }
+    load_slabs(k0 + B);
 #pragma unroll
     for (int i = 0; i < B; ++i)
-      lds_put<W>(lv_at(0, ring(k0 + 1 + i)) + t * W, w[i]);
+      lds_put<W>(s_lv + row(1 + i) * RW + t * W, w[i]);
     __syncthreads();
+    Vec<W> out[2][B];  // levels NS - 1 and NS, stored at the iteration's end
 #pragma unroll
     for (int l = 1; l <= NS; ++l) {
-      Vec<W> out[B];
 #pragma unroll
       for (int i = 0; i < B; ++i) {
         const int q = k0 - (l - 1) + i;
-        const Vec<W> self = lds_vec<W>(lv_at(l - 1, ring(q)) + o_self);
+        const int rq = row(i - (l - 1));
+        const float2* lvl = s_lv + (l - 1) * RL;
+        const Vec<W> self = lds_vec<W>(lvl + rq * RW + o_self);
         const float v = pv[NS - l + i];
         float tr[W], ti[W];
 #pragma unroll
-        for (int c = 0; c < W; ++c) {
-          tr[c] = v * self.v[c].x;
-          ti[c] = v * self.v[c].y;
+        for (int cc = 0; cc < W; ++cc) {
+          tr[cc] = v * self.v[cc].x;
+          ti[cc] = v * self.v[cc].y;
         }
 #pragma unroll
         for (int s = 0; s < K; ++s) {
-          const Vec<W> nb = lds_vec<W>(lv_at(l - 1, ring(q + dq[s])) + eo[s]);
+          const Vec<W> nb = lds_vec<W>(lvl + nbo[s][rq]);
 #pragma unroll
-          for (int c = 0; c < W; ++c) {
-            const float cx = nb.v[c].x, cy = nb.v[c].y;
+          for (int cc = 0; cc < W; ++cc) {
+            const float cx = nb.v[cc].x, cy = nb.v[cc].y;
             if constexpr (EVO) {  // k_block_evo's order
-              tr[c] += __builtin_fmaf(ha[s], cx, -hb[s] * cy);
-              ti[c] += __builtin_fmaf(ha[s], cy, hb[s] * cx);
+              tr[cc] += __builtin_fmaf(ha[s], cx, -hb[s] * cy);
+              ti[cc] += __builtin_fmaf(ha[s], cy, hb[s] * cx);
             } else {
-              tr[c] = __builtin_fmaf(ha[s], cx, tr[c]);
-              tr[c] = __builtin_fmaf(-hb[s], cy, tr[c]);
-              ti[c] = __builtin_fmaf(ha[s], cy, ti[c]);
-              ti[c] = __builtin_fmaf(hb[s], cx, ti[c]);
+              tr[cc] = __builtin_fmaf(ha[s], cx, tr[cc]);
+              tr[cc] = __builtin_fmaf(-hb[s], cy, tr[cc]);
+              ti[cc] = __builtin_fmaf(ha[s], cy, ti[cc]);
+              ti[cc] = __builtin_fmaf(hb[s], cx, ti[cc]);
             }
           }
         }
         const Vec<W> prev =
-          l == 1 ? ip2[i] : lds_vec<W>(lv_at(l - 2, ring(q)) + o_self);
+          l == 1 ? ip2[i] : lds_vec<W>(s_lv + (l - 2) * RL + rq * RW + o_self);
         Vec<W> o;
 #pragma unroll
-        for (int c = 0; c < W; ++c) {
-          o.v[c].x = __builtin_fmaf(2.0f, tr[c] * inv_scale, -prev.v[c].x);
-          o.v[c].y = __builtin_fmaf(2.0f, ti[c] * inv_scale, -prev.v[c].y);
+        for (int cc = 0; cc < W; ++cc) {
+          o.v[cc].x = __builtin_fmaf(2.0f, tr[cc] * inv_scale, -prev.v[cc].x);
+          o.v[cc].y = __builtin_fmaf(2.0f, ti[cc] * inv_scale, -prev.v[cc].y);
         }
-        out[i] = o;
+        if (l >= NS - 1)
+          out[l - (NS - 1)][i] = o;
         if (l < NS)
-          lds_put<W>(lv_at(l, ring(q)) + t * W, o);
+          lds_put<W>(s_lv + l * RL + rq * RW + t * W, o);
         if constexpr (EVO) {
 #pragma unroll
-          for (int c = 0; c < W; ++c)
-            accumulate_phase(sv[NS - l + i][c], terms.bm[l - 1], (double)o.v[c].x,
-                             (double)o.v[c].y, terms.label[l - 1]);
+          for (int cc = 0; cc < W; ++cc)
+            accumulate_phase(sv[NS - l + i][cc], terms.bm[l - 1], (double)o.v[cc].x,
+                             (double)o.v[cc].y, terms.label[l - 1]);
         } else if (interior && q >= NS - 1 && q <= NS - 2 + bx) {
           add_products<IPM, W>(acc[l - 1], o, self, self);
         }
       }
-      if (l >= NS - 1) {
+      __syncthreads();
+    }
+    hold_slabs();  // the next iteration's loads, before this iteration's stores
 #pragma unroll
-        for (int i = 0; i < B; ++i) {
-          const int q = k0 - (l - 1) + i;
-          if (interior && q >= NS - 1 && q <= NS - 2 + bx) {
-            stv<W, true>((l == NS ? o1 : o0) + gidx(slab_of(q), pos), out[i]);
-            if constexpr (EVO) {
-              if (l == NS) {  // every term of the launch added
-                const size_t as = gidx(slab_of(q), pos);
+    for (int j = 0; j < 2; ++j) {
 #pragma unroll
-                for (int c = 0; c < W; ++c)
-                  state[as + c] = sv[i][c];
-              }
+      for (int i = 0; i < B; ++i) {
+        const int q = k0 - (NS - 2 + j) + i;  // level NS - 1 + j
+        if (interior && q >= NS - 1 && q <= NS - 2 + bx) {
+          stv<W, true>((j ? o1 : o0) + gidx(slab_of(q), pos), out[j][i]);
+          if constexpr (EVO) {
+            if (j) {  // every term of the launch added
+              const size_t as = gidx(slab_of(q), pos);
+#pragma unroll
+              for (int cc = 0; cc < W; ++cc)
+                state[as + cc] = sv[i][cc];
             }
           }
         }
       }
-      __syncthreads();
     }
 #pragma unroll
     for (int j = 0; j + B < RB; ++j) {
       pv[j] = pv[j + B];
       if constexpr (EVO) {
 #pragma unroll
-        for (int c = 0; c < W; ++c)
-          sv[j][c] = sv[j + B][c];
+        for (int cc = 0; cc < W; ++cc)
+          sv[j][cc] = sv[j + B][cc];
       }
+    }
+  };
+
+  for (int k0 = 0; k0 <= k_last; k0 += U * B) {
+    iterate(std::integral_constant<int, 0>{}, k0);
+    if constexpr (U > 1) {
+      if (k0 + B > k_last)
+        break;
+      iterate(std::integral_constant<int, 1>{}, k0 + B);
+    }
+    if constexpr (U > 2) {
+      if (k0 + 2 * B > k_last)
+        break;
+      iterate(std::integral_constant<int, 2>{}, k0 + 2 * B);
     }
   }
   if constexpr (EVO)

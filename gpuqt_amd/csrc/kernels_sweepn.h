@@ -18,15 +18,29 @@
 // B + 2 slabs it made in an LDS ring; the potential and left vector of the
 // last B + NS - 1 slabs are kept per thread in registers (a level reaches
 // slab q up to NS - 1 slabs after level 1 did), shifted by B slabs per
-// iteration.  A loop iteration loads B new slabs of level 0 (and psi_{m-2},
-// potential, left vector for level 1's B slabs), then computes each level's
-// B slabs in turn, one barrier between levels; every global load of an
-// iteration is used in it, before the first store.
+// iteration.  A loop iteration takes B new slabs of level 0 (and psi_{m-2},
+// potential, left vector for level 1's B slabs) from registers loaded one
+// iteration earlier, issues the loads of the next iteration's slabs, then
+// computes each level's B slabs in turn, one barrier between levels: the
+// loads are in flight while an iteration computes.  At the iteration's end
+// the loads are waited for, then the two written levels go out through
+// buffer stores that every lane issues (a lane with nothing to write gets an
+// offset the hardware drops).  Loads and stores share one counter: a wait
+// for a load issued before a store waits for the store too, so no load is
+// waited for after the stores that follow it -- the stores have the next
+// iteration to complete.
+//
+// The loop is unrolled by the period of the ring rows (RR / gcd(B, RR)
+// iterations), so every level's ring row is a constant and a neighbour's LDS
+// address is a per-thread offset of its slot and row, computed once.
 //
 // Per site the arithmetic is k_sweep2's (the one-step kernels', in the same
 // order): the FP64 instantiation gives the bits of k_hpsi and of the
 // reference at every level; only the moment partial sums group the sites by
-// sweep tile.
+// sweep tile.  A slot that an orbital does not use adds nothing: FULL (every
+// orbital uses every slot) adds all of them, FAST (FP32, real hoppings) adds
+// the unused ones' zero hoppings, the exact FP64 form keeps the sum of the
+// used ones by a select.
 //
 // EVO: NS terms of the Chebyshev series of U(t) instead of moments: level l
 // adds its Bessel-weighted value to the FP64 state S (terms.bm / label of
@@ -35,6 +49,8 @@
 // level NS and written once.  The additions are those of one term per
 // launch, in the same order, so the state's bits are k_cheb2's.
 #pragma once
+
+#include <type_traits>
 
 #include "kernels_sweep.h"
 
@@ -64,13 +80,30 @@ __device__ __forceinline__ void sweep_add_term(double2& s, double bm, double tr,
 }
 
 // static LDS of k_sweepn<V, K, NS, B>: the level rings and the tile sums
+// store of val at base[idx] through a buffer resource over elems elements
+// (elems * sizeof(V) < 2^31): with ok false the offset lies past the range
+// and the hardware drops the store
+template <typename V>
+__device__ __forceinline__ void store_or_drop(V* base, int elems, int idx, bool ok, V val)
+{
+  using u32x4 = unsigned __attribute__((ext_vector_type(4)));
+  using u32x2 = unsigned __attribute__((ext_vector_type(2)));
+  const __amdgpu_buffer_rsrc_t r =
+    __builtin_amdgcn_make_buffer_rsrc(base, 0, elems * (int)sizeof(V), 0x00020000);
+  const int off = ok ? idx * (int)sizeof(V) : (int)0x80000000u;
+  if constexpr (sizeof(V) == 16)
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, val), r, off, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, val), r, off, 0, 0);
+}
+
 template <typename V, int NS, int B>
 constexpr size_t sweepn_lds_bytes()
 {
   return sizeof(V) * NS * (B + 2) * kSweepThreads + sizeof(double) * (kSweepThreads / 64) * 2 * NS;
 }
 
-template <typename V, int K, int NS, int B, bool FAST, bool EVO = false>
+template <typename V, int K, int NS, int B, bool FAST, bool EVO = false, bool FULL = false>
 __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
                                                           const V* __restrict__ p2,
                                                           const V* __restrict__ p1,
@@ -86,7 +119,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
   constexpr int T = kSweepThreads;
   constexpr int RR = B + 2;       // rows of a level ring
   constexpr int RB = B + NS - 1;  // slabs of potential and left vector held
-  __shared__ V s_lv[NS][RR][T];   // levels 0 .. NS - 1
+  constexpr int U = RR / sweep_gcd(B, RR);  // iterations until the rows repeat
+  static_assert(U >= 1 && U <= 3, "unrolled by one, two or three iterations");
+  constexpr int kRow = T * (int)sizeof(V);  // bytes of a ring row
+  constexpr int kLevel = RR * kRow;         // bytes of a level's ring
+  __shared__ V s_lv[NS][RR][T];             // levels 0 .. NS - 1
   __shared__ double s_red[T / 64][2 * NS];
   const int tile = xcd_tile(blockIdx.x, sm.blocks);
   if (tile >= sm.tiles)
@@ -142,6 +179,18 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
     ha[q] = hv.x;
     hb[q] = hv.y;
   }
+  // byte offsets in a level's ring: entry te (the site), entry t (the one
+  // written), slot s's neighbour when the site's slab sits in row c
+  const int so = te * (int)sizeof(V), to = t * (int)sizeof(V);
+  int nbo[K][RR];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+#pragma unroll
+    for (int c = 0; c < RR; ++c)
+      nbo[q][c] = (((c + dq[q]) % RR + RR) % RR) * kRow + bq[q] * (int)sizeof(V);
+  }
+  char* const lds = reinterpret_cast<char*>(&s_lv[0][0][0]);
+  auto at = [&](int off) -> V& { return *reinterpret_cast<V*>(lds + off); };
   const double emax = h.emax;
   const double inv_emax = 1.0 / h.emax;
   auto gsite = [&](int q) { return (size_t)slab_of(q) * S + pos; };
@@ -179,48 +228,94 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
       lv[j] = IO::narrow(0.0, 0.0);
   }
 
-  for (int k0 = 0; k0 <= bx + 2 * NS - 3; k0 += B) {
-    // loads: level 0 at k0 + 1 .. k0 + B; psi_{m-2}, potential and left
-    // vector at level 1's slabs k0 .. k0 + B - 1
-    V w[B], ip2[B];
+  // the slabs of iteration k: level 0 at k + 1 .. k + B; psi_{m-2},
+  // potential and left vector (EVO: state) at level 1's slabs k .. k + B - 1
+  // (past the last iteration: slabs already read, so cache hits)
+  const int k_last = bx + 2 * NS - 3;
+  V nw[B], np2[B], nl[EVO ? 1 : B];
+  double npv[B];
+  double2 ns[EVO ? B : 1];
+  auto load_slabs = [&](int k) {
 #pragma unroll
     for (int i = 0; i < B; ++i)
-      w[i] = p1[gsite(k0 + 1 + i)];
+      nw[i] = p1[gsite(k + 1 + i)];
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-      const size_t x = (size_t)slab_of(k0 + i) * S;
-      ip2[i] = p2[x + pos_p];
-      pv[NS - 1 + i] = IO::pot(h, (int)(x + pos_p));
+      const size_t x = (size_t)slab_of(min(k + i, k_last)) * S;
+      np2[i] = p2[x + pos_p];
+      npv[i] = IO::pot(h, (int)(x + pos_p));
       if constexpr (EVO)
-        sv[NS - 1 + i] = state[x + pos_l];
+        ns[i] = state[x + pos_l];
       else
-        lv[NS - 1 + i] = L[x + pos_l];
+        nl[i] = L[x + pos_l];
     }
+  };
+  load_slabs(0);
+  auto hold_slabs = [&]() {  // the loads of load_slabs landed
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      sweep_hold(nw[i]);
+      sweep_hold(np2[i]);
+      sweep_hold(npv[i]);
+      if constexpr (EVO)
+        sweep_hold(ns[i]);
+      else
+        sweep_hold(nl[i]);
+    }
+  };
+  hold_slabs();
+
+  // one iteration, k0 = c B modulo RR: the ring row of slab k0 + d is
+  // row(d), a constant once the loops over levels and slabs are unrolled
+  auto iterate = [&](auto phase, int k0) {
+    constexpr int c = decltype(phase)::value;
+    auto row = [](int d) { return ((c * B + d) % RR + RR) % RR; };
+    V w[B], ip2[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      w[i] = nw[i];
+      ip2[i] = np2[i];
+      pv[NS - 1 + i] = npv[i];
+      if constexpr (EVO)
+        sv[NS - 1 + i] = ns[i];
+      else
+        lv[NS - 1 + i] = nl[i];
+    }
+    load_slabs(k0 + B);
 #pragma unroll
     for (int i = 0; i < B; ++i)
-      s_lv[0][ring(k0 + 1 + i)][t] = w[i];
+      at(row(1 + i) * kRow + to) = w[i];
     __syncthreads();
+    V out[2][B];  // levels NS - 1 and NS, stored at the iteration's end
 #pragma unroll
     for (int l = 1; l <= NS; ++l) {
-      V out[B];
 #pragma unroll
       for (int i = 0; i < B; ++i) {
         const int q = k0 - (l - 1) + i;
-        const double2 self = IO::wide(s_lv[l - 1][ring(q)][te]);
+        const int rq = row(i - (l - 1));
+        const double2 self = IO::wide(at((l - 1) * kLevel + rq * kRow + so));
         const double v = pv[NS - l + i];  // slab q = k0 - NS + 1 + (NS - l + i)
         double tr = v * self.x;
         double ti = v * self.y;
+        // every neighbour read first (a branch per slot would wait for each
+        // read in turn)
+        double2 cn[K];
+#pragma unroll
+        for (int s = 0; s < K; ++s)
+          cn[s] = IO::wide(at((l - 1) * kLevel + nbo[s][rq]));
 #pragma unroll
         for (int s = 0; s < K; ++s) {
-          const double2 c = IO::wide(s_lv[l - 1][ring(q + dq[s])][bq[s]]);
-          if (okq[s]) {
-            if constexpr (FAST) {
-              tr += ha[s] * c.x;
-              ti += ha[s] * c.y;
-            } else {
-              tr += ha[s] * c.x - hb[s] * c.y;
-              ti += ha[s] * c.y + hb[s] * c.x;
-            }
+          if constexpr (FAST) {  // unused slots: zero hoppings
+            tr += ha[s] * cn[s].x;
+            ti += ha[s] * cn[s].y;
+          } else if constexpr (FULL) {
+            tr += ha[s] * cn[s].x - hb[s] * cn[s].y;
+            ti += ha[s] * cn[s].y + hb[s] * cn[s].x;
+          } else {  // the sum of the used slots, in order
+            const double nr = tr + (ha[s] * cn[s].x - hb[s] * cn[s].y);
+            const double ni = ti + (ha[s] * cn[s].y + hb[s] * cn[s].x);
+            tr = okq[s] ? nr : tr;
+            ti = okq[s] ? ni : ti;
           }
         }
         if constexpr (FAST) {
@@ -230,13 +325,15 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
           tr /= emax;
           ti /= emax;
         }
-        const double2 prev = l == 1 ? IO::wide(ip2[i]) : IO::wide(s_lv[l - 2][ring(q)][te]);
+        const double2 prev =
+          l == 1 ? IO::wide(ip2[i]) : IO::wide(at((l - 2) * kLevel + rq * kRow + so));
         tr = 2.0 * tr - prev.x;
         ti = 2.0 * ti - prev.y;
         const V o = IO::narrow(tr, ti);
-        out[i] = o;
+        if (l >= NS - 1)
+          out[l - (NS - 1)][i] = o;
         if (l < NS)
-          s_lv[l][ring(q)][t] = o;
+          at(l * kLevel + rq * kRow + to) = o;
         if constexpr (EVO) {
           // the term of this level in the FP64 state, from the value stored
           const double2 ow = IO::wide(o);
@@ -247,20 +344,22 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
           acc_i[l - 1] += ow.y * lw.x - ow.x * lw.y;
         }
       }
-      if (l >= NS - 1) {
+      __syncthreads();
+    }
+    hold_slabs();  // the next iteration's loads, before this iteration's stores
 #pragma unroll
-        for (int i = 0; i < B; ++i) {
-          const int q = k0 - (l - 1) + i;
-          if (interior && q >= NS - 1 && q <= NS - 2 + bx) {
-            (l == NS ? o1 : o0)[gsite(q)] = out[i];
-            if constexpr (EVO) {
-              if (l == NS)
-                state[gsite(q)] = sv[i];  // every term of the launch added
-            }
-          }
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const int q = k0 - (NS - 2 + j) + i;  // level NS - 1 + j
+        const bool ok = interior && q >= NS - 1 && q <= NS - 2 + bx;
+        const size_t x = (size_t)slab_of(q) * S;
+        store_or_drop(j ? o1 + x : o0 + x, S, pos, ok, out[j][i]);
+        if constexpr (EVO) {
+          if (j)  // every term of the launch added
+            store_or_drop(state + x, S, pos, ok, sv[i]);
         }
       }
-      __syncthreads();
     }
 #pragma unroll
     for (int j = 0; j + B < RB; ++j) {
@@ -269,6 +368,20 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
         sv[j] = sv[j + B];
       else
         lv[j] = lv[j + B];
+    }
+  };
+
+  for (int k0 = 0; k0 <= k_last; k0 += U * B) {
+    iterate(std::integral_constant<int, 0>{}, k0);
+    if constexpr (U > 1) {
+      if (k0 + B > k_last)
+        break;
+      iterate(std::integral_constant<int, 1>{}, k0 + B);
+    }
+    if constexpr (U > 2) {
+      if (k0 + 2 * B > k_last)
+        break;
+      iterate(std::integral_constant<int, 2>{}, k0 + 2 * B);
     }
   }
   if constexpr (EVO)
@@ -315,6 +428,9 @@ void sweepn_evo_launch(const HamDev& h, const SweepMap& sm, const V* p2, const V
     if (fast)                                                                                 \
       k_sweepn<V, KK, NS, B, sizeof(V) == sizeof(float2), true>                               \
         <<<sm.blocks, kSweepThreads, 0, st>>>(h, sm, p2, p1, o0, o1, p1, nullptr, 0, S, terms); \
+    else if (sm.full)                                                                         \
+      k_sweepn<V, KK, NS, B, false, true, true>                                               \
+        <<<sm.blocks, kSweepThreads, 0, st>>>(h, sm, p2, p1, o0, o1, p1, nullptr, 0, S, terms); \
     else                                                                                      \
       k_sweepn<V, KK, NS, B, false, true>                                                     \
         <<<sm.blocks, kSweepThreads, 0, st>>>(h, sm, p2, p1, o0, o1, p1, nullptr, 0, S, terms); \
@@ -349,6 +465,9 @@ void sweepn_launch(const HamDev& h, const SweepMap& sm, const V* p2, const V* p1
   do {                                                                                        \
     if (fast)                                                                                 \
       k_sweepn<V, KK, NS, B, sizeof(V) == sizeof(float2)>                                     \
+        <<<sm.blocks, kSweepThreads, 0, st>>>(h, sm, p2, p1, o0, o1, L, part, part_stride);   \
+    else if (sm.full)                                                                         \
+      k_sweepn<V, KK, NS, B, false, false, true>                                              \
         <<<sm.blocks, kSweepThreads, 0, st>>>(h, sm, p2, p1, o0, o1, L, part, part_stride);   \
     else                                                                                      \
       k_sweepn<V, KK, NS, B, false>                                                           \

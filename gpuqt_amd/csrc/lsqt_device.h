@@ -289,6 +289,7 @@ struct SweepMap {
   int halo;          // largest |position offset| of the table
   int ns;            // steps per launch of the shape (k_sweepn: 3 or 4)
   int nb;            // slabs per loop iteration (k_sweepn)
+  int full;          // every orbital uses every slot
 };
 
 // the series coefficients of the levels of a multi-step launch of the
@@ -368,6 +369,31 @@ inline Grid make_grid(int n, int spt)
   int tiles = (n + per_tile - 1) / per_tile;
   int blocks = (tiles + kXcds - 1) / kXcds * kXcds;
   return Grid{tiles, blocks};
+}
+
+// the multi-step sweeps unroll their loop by the period of their ring rows
+constexpr int sweep_gcd(int a, int b)
+{
+  return b ? sweep_gcd(b, a % b) : a;
+}
+
+// an empty asm reading the registers of v: the compiler places the wait for
+// the load that wrote them here
+__device__ __forceinline__ void sweep_hold(float v)
+{
+  asm volatile("" : : "v"(v));
+}
+__device__ __forceinline__ void sweep_hold(double v)
+{
+  asm volatile("" : : "v"(v));
+}
+__device__ __forceinline__ void sweep_hold(float2 v)
+{
+  asm volatile("" : : "v"(v.x), "v"(v.y));
+}
+__device__ __forceinline__ void sweep_hold(double2 v)
+{
+  asm volatile("" : : "v"(v.x), "v"(v.y));
 }
 
 // workgroup b runs on XCD b % 8; map it to tile (b % 8) * (blocks/8) + b / 8

@@ -30,8 +30,8 @@ def test_traffic_profile_entries():
     # (FP32) x 8.0M sites per launch
     for name, per_site in (("k_sweep2<double2,3,4,false>", 88.0),
                            ("k_sweep2<float2,3,4,true>", 44.0),
-                           ("k_sweepn<double2,3,4,1,false,false>", 88.0),
-                           ("k_sweepn<float2,3,6,4,true,false>", 44.0)):
+                           ("k_sweepn<double2,3,4,1,false,false,true>", 88.0),
+                           ("k_sweepn<float2,3,6,4,true,false,false>", 44.0)):
         t = bench.profiled_traffic(name)
         assert t is not None and abs(t / (per_site * 8_000_000) - 1.0) < 0.07, name
     for mode in (1, 2):  # <L|psi> and product-identity block steps, sliding window

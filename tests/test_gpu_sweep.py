@@ -5,7 +5,9 @@ it (periodically), the moment sets advance two (k_sweep2) or four (k_sweepn)
 recursion steps per launch from LDS (six for FP32 states).  Checked here:
 
   - which Hamiltonians get the sweep: periodic honeycomb (whole-slab and
-    chunked slabs), square and triangular lattices; none for a Peierls-phase
+    chunked slabs), square and triangular lattices, a ladder whose orbitals
+    leave a slot unused in a periodic pattern (the exact sum over the used
+    slots; the other lattices use every slot); none for a Peierls-phase
     square lattice (its hoppings change from slab to slab), the 3D cube (its
     in-slab neighbours are a row of 32 positions away), vacancies
     (renumbered sites), open boundaries (empty slots at the edges) or
@@ -54,6 +56,33 @@ def _lattice(nx, ny, offsets, rng, phase=None, disorder=1.0):
     return arrays, n, K
 
 
+def _ladder(nx, ny, rng):
+    """Rows y = 0, 1 and 2, 3 (mod 4) bonded in pairs, row 3 to the next
+    row 0, every row to its x neighbours: rows 0 and 3 have four bonds, rows 1
+    and 2 three, so the fourth slot is unused on half the orbitals (a
+    periodic pattern of period 4 with empty slots)."""
+    assert ny % 4 == 0
+    n = nx * ny
+    idx = np.arange(n)
+    x, y = idx // ny, idx % ny
+    site = lambda xx, yy: (xx % nx) * ny + yy % ny
+    up = {0: 1, 1: -1, 2: 1, 3: -1}      # the pair bond, slot 2
+    nl = np.empty((n, 4), np.int32)
+    nl[:, 0] = site(x - 1, y)
+    nl[:, 1] = site(x + 1, y)
+    nl[:, 2] = site(x, y + np.vectorize(up.get)(y % 4))
+    link = np.where(y % 4 == 0, -1, np.where(y % 4 == 3, 1, 0))  # rows 0 and 3, slot 3
+    nl[:, 3] = np.where(link != 0, site(x, y + link), idx)
+    nn = np.where(link != 0, 4, 3).astype(np.int32)
+    hr = np.full((n, 4), -1.0)
+    xx = np.zeros((n, 4))
+    xx[:, 0], xx[:, 1] = -1, 1
+    arrays = dict(neighbor_number=nn, neighbor_list=nl.reshape(-1),
+                  potential=rng.uniform(-1, 1, n), hopping_real=hr.reshape(-1),
+                  hopping_imag=np.zeros(n * 4), xx=xx.reshape(-1))
+    return arrays, n, 4
+
+
 CHAINS = [(-1, 0), (1, 0)]
 SQUARE = [(-1, 0), (1, 0), (0, -1), (0, 1)]
 TRIANGULAR = [(-1, 0), (1, 0), (0, -1), (0, 1), (1, -1), (-1, 1)]
@@ -84,6 +113,8 @@ def _systems(tmp_path):
     out.append(("triangular 70x777", a, n, K, 7.0, 777, 1, None))
     a, n, K = _lattice(160, 256, CHAINS, rng)                 # 2 slots, no in-slab bonds
     out.append(("chains 160x256", a, n, K, 4.0, 256, 0, None))
+    a, n, K = _ladder(120, 400, rng)                          # empty slots in the pattern
+    out.append(("ladder 120x400, three and four bonds", a, n, K, 5.0, 400, 1, None))
     a, n, K = _lattice(90, 400, SQUARE, rng,                  # complex, the same in every slab
                        phase=lambda x: 0.7 + 0 * x)
     out.append(("square 90x400 with a uniform complex phase", a, n, K, 5.0, 400, 1, None))
