@@ -860,17 +860,17 @@ def main():
     total_updates, elapsed, avg_launch_ms, rows = timed("f64", args.steps, args.warmup)
     rows = np.array(rows)
     value = total_updates / elapsed
-    # dominant kernel: the four-step sweep (four site-updates per site per
-    # launch) where the lattice has one, else the fused one-step kernel
+    # dominant kernel: the multi-step sweep (steps64 site-updates per site
+    # per launch) where the lattice has one, else the fused one-step kernel
     updates_per_site = steps64
     bytes_per_site = info.bytes_per_site_sweep_f64 if sweep else info.bytes_per_site_f64
     bytes_per_launch = bytes_per_site * model.n
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     default_workload = (args.nx, args.ny) == (2000, 1000) and info.layout == 2 \
         and info.max_neighbor == 3
-    kname = {4: "k_sweepn<double2,3,4,1,false,false,true>", 2: "k_sweep2<double2,3,4,false>"}.get(
+    kname = {6: "k_sweepn<double2,3,6,1,false,false,true>", 2: "k_sweep2<double2,3,4,false>"}.get(
         steps64, "k_hpsi<1,3,2,true,true,true>")
-    kname32 = {6: "k_sweepn<float2,3,6,4,true,false,false>", 2: "k_sweep2<float2,3,4,true>"}.get(
+    kname32 = {6: "k_sweepn<float2,3,6,1,true,false,false>", 2: "k_sweep2<float2,3,4,true>"}.get(
         steps32, "k_hpsi32")
     traffic = profiled_traffic(kname) if default_workload else None
     # the same launch time priced on other byte counts: the one-step kernel's
@@ -965,10 +965,11 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": (kname + " (four Chebyshev steps per launch: psi_m .. psi_{m+3} from "
-                       "psi_{m-2}, psi_{m-1}, each level one slab behind the one before in LDS "
-                       "rings, the last two written, fused with <L|psi> of all four)"
-                       if steps64 == 4 else
+            "kernel": (kname + f" ({steps64} Chebyshev steps per launch: psi_m .. "
+                       f"psi_{{m+{steps64 - 1}}} from psi_{{m-2}}, psi_{{m-1}}, each level one "
+                       "slab behind the one before in LDS rings, the last two written, fused "
+                       "with <L|psi> of every level)"
+                       if steps64 > 2 else
                        kname + " (two Chebyshev steps per launch: psi_m and psi_{m+1} from "
                        "psi_{m-2}, psi_{m-1}, four slabs at a time from LDS, fused with "
                        "<L|psi_m> and <L|psi_{m+1}>)" if sweep else

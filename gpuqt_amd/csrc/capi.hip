@@ -459,15 +459,16 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
       h->sweep64 = sweep_shape(g, false);
       h->sweep32 = sweep_shape(g, true);
       // more steps per launch on the same geometry (LSQT_SWEEP_STEPS=2 in
-      // the environment keeps two): FP64 four steps, one slab per loop
-      // iteration; FP32 states six steps, blocks of four slabs (measured on
-      // the 8.0M-site honeycomb: 46.7 and 26.2 us per step against 74.4 and
-      // 38.3 with two steps per launch; three, five, six and eight FP64 steps,
-      // four FP32 steps and the other block sizes were slower)
+      // the environment keeps two): six steps, one slab per loop iteration,
+      // FP64 and FP32 states (measured on the 8.0M-site honeycomb: 32.7 and
+      // 18.3 us per step against 74.4 and 38.3 with two steps per launch;
+      // FP64 four, five, seven and eight steps 39.8, 34.3, 40.3, 40.2; FP32
+      // six steps in blocks of two and four slabs 18.4 and 19.9, seven and
+      // eight steps 18.9)
       const char* steps = getenv("LSQT_SWEEP_STEPS");
       if (!(steps && atoi(steps) == 2)) {
-        h->sweepn64 = sweep_shape_n(h->sweep64, false, 4, 1);
-        h->sweepn32 = sweep_shape_n(h->sweep32, true, 6, 4);
+        h->sweepn64 = sweep_shape_n(h->sweep64, false, 6, 1);
+        h->sweepn32 = sweep_shape_n(h->sweep32, true, 6, 1);
       }
       // the same geometry serves the block path's sweeps
       h->bsweep_steps = (steps && atoi(steps) == 2) ? 2 : 4;

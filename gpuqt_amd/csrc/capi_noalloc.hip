@@ -6,8 +6,9 @@
 // partials on the context, both allocated on first use.  Here every buffer
 // beyond the caller's inputs and outputs lives in `scratch`, one fixed array
 // per GPU: the recursion vectors (four where lsqt_moments advances several
-// steps per launch), a ring of moment partials reduced every kRing moments, the
-// moments, and their split real / imaginary parts for the download.  The
+// steps per launch), a ring of kRing moments' partials (reduced whenever the
+// next launch's moments would not fit), the moments, and their split real /
+// imaginary parts for the download.  The
 // kernels and their order are those of lsqt_moments, so the moments are
 // bit-identical to it.  Nothing in the scratch must start at zero: every
 // partial a reduction reads is written by the step before it.
@@ -24,6 +25,7 @@ namespace {
 constexpr int kNoAllocMaxSites = LSQT_NOALLOC_MAX_SITES;
 constexpr int kNoAllocMaxMoments = LSQT_NOALLOC_MAX_MOMENTS;
 constexpr int kRing = 64;                                     // moments per partial ring
+static_assert(kRing >= kSweepMaxSteps, "a multi-step launch's moments fit the ring");
 constexpr size_t kMaxTiles = (kNoAllocMaxSites + kBlock - 1) / kBlock;  // at 1 site / thread
 // offsets in double2 units; every region starts on a 256-byte boundary
 constexpr size_t kVecA = 0;
@@ -142,31 +144,34 @@ extern "C" int lsqt_moments_noalloc(lsqt_ham* h, const lsqt_vec* left, const lsq
     return m < 2 ? tiles
                  : (m < m_deep0 ? sw.tiles : (m < m_deep ? sn.tiles : (m < m_sweep ? sw.tiles : tiles)));
   };
-  // moment m's partials in ring slot m % kRing; each full ring (and the last,
-  // partial one) is reduced in turn, in pieces of one partial count.  No
-  // launch straddles a ring boundary: multi-step launches start at multiples
-  // of their step count (two, or four from moment 4) and kRing is a multiple
-  // of both.
-  auto partial = [&](int m) { return ring + (size_t)(m % kRing) * stride; };
+  // the ring holds the partials of moments rb .. rb + kRing - 1 (moment m in
+  // slot m - rb); before a launch whose moments would not fit, the ring's
+  // moments are reduced, in pieces of one partial count, and it starts over
+  // at the launch's first moment
+  int rb = 0;
+  auto partial = [&](int m) { return ring + (size_t)(m - rb) * stride; };
   auto reduce = [&](int m_end) {
-    const int m0 = (m_end - 1) / kRing * kRing;
-    int cuts[6] = {m0, 2, m_deep0, m_deep, m_sweep, m_end};
+    int cuts[6] = {rb, 2, m_deep0, m_deep, m_sweep, m_end};
     for (int c = 1; c < 5; ++c)
-      cuts[c] = std::min(std::max(cuts[c], m0), m_end);
+      cuts[c] = std::min(std::max(cuts[c], rb), m_end);
     for (int p = 0; p < 5; ++p) {
       const int a = cuts[p], b = cuts[p + 1];
       if (b > a)
-        launch_reduce_partials(ring + (size_t)(a - m0) * stride, stride, tiles_of(a), b - a,
+        launch_reduce_partials(ring + (size_t)(a - rb) * stride, stride, tiles_of(a), b - a,
                                mu + a, st);
     }
+    rb = m_end;
   };
   launch_copy(A, right->d, n, st);
   launch_dot(A, left->d, partial(0), n, st, hpsi_spt(h->d.K));
   launch_hpsi(h->d, A, nullptr, B, left->d, partial(1), false, true, st);
   int m = 2;
-  auto two = [&]() {
-    if (m % kRing == 0)
+  auto take = [&](int count) {  // room in the ring for moments m .. m + count - 1
+    if (m + count - rb > kRing)
       reduce(m);
+  };
+  auto two = [&]() {
+    take(2);
     launch_sweep2(h->d, sw, A, B, C, D, left->d, partial(m), partial(m + 1), st);
     std::swap(A, C);
     std::swap(B, D);
@@ -175,8 +180,7 @@ extern "C" int lsqt_moments_noalloc(lsqt_ham* h, const lsqt_vec* left, const lsq
   if (deep) {
     two();
     for (; m < m_deep; m += sn.ns) {
-      if (m % kRing == 0)
-        reduce(m);
+      take(sn.ns);
       launch_sweepn(h->d, sn, A, B, C, D, left->d, partial(m), stride, st);
       std::swap(A, C);
       std::swap(B, D);
@@ -185,8 +189,7 @@ extern "C" int lsqt_moments_noalloc(lsqt_ham* h, const lsqt_vec* left, const lsq
   while (m < m_sweep)
     two();
   for (; m < M; ++m) {
-    if (m % kRing == 0)
-      reduce(m);
+    take(1);
     launch_hpsi(h->d, B, A, A, left->d, partial(m), true, true, st);
     std::swap(A, B);
   }

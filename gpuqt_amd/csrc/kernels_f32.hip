@@ -624,18 +624,18 @@ void launch_sweepn_32(const HamDev& h, const SweepMap& sm, const float2* psi2,
                       const float2* psi1, float2* out0, float2* out1, const float2* L,
                       double2* part, size_t part_stride, hipStream_t st)
 {
-  // the shape sweep_shape_n makes for FP32 states: six steps, blocks of 4 slabs
+  // the shape sweep_shape_n makes for FP32 states: six steps, blocks of 1 slab
   const float2* left = L ? L : psi1;
-  if (sm.ns == 6 && sm.nb == 4)
-    sweepn_launch<float2, 6, 4>(h, sm, psi2, psi1, out0, out1, left, part, part_stride, st);
+  if (sm.ns == 6 && sm.nb == 1)
+    sweepn_launch<float2, 6, 1>(h, sm, psi2, psi1, out0, out1, left, part, part_stride, st);
 }
 
 void launch_sweepn_evo_32(const HamDev& h, const SweepMap& sm, const float2* psi2,
                           const float2* psi1, float2* out0, float2* out1, double2* S,
                           const SweepTerms& terms, hipStream_t st)
 {
-  if (sm.ns == 6 && sm.nb == 4)
-    sweepn_evo_launch<float2, 6, 4>(h, sm, psi2, psi1, out0, out1, S, terms, st);
+  if (sm.ns == 6 && sm.nb == 1)
+    sweepn_evo_launch<float2, 6, 1>(h, sm, psi2, psi1, out0, out1, S, terms, st);
 }
 
 void launch_sweep2_32(const HamDev& h, const SweepMap& sm, const float2* psi2,

@@ -787,18 +787,18 @@ void launch_sweepn(const HamDev& h, const SweepMap& sm, const double2* psi2,
                    const double2* psi1, double2* out0, double2* out1, const double2* L,
                    double2* part, size_t part_stride, hipStream_t st)
 {
-  // the shape sweep_shape_n makes: four steps, blocks of 1 slab
+  // the shape sweep_shape_n makes: six steps, blocks of 1 slab
   const double2* left = L ? L : psi1;
-  if (sm.ns == 4 && sm.nb == 1)
-    sweepn_launch<double2, 4, 1>(h, sm, psi2, psi1, out0, out1, left, part, part_stride, st);
+  if (sm.ns == 6 && sm.nb == 1)
+    sweepn_launch<double2, 6, 1>(h, sm, psi2, psi1, out0, out1, left, part, part_stride, st);
 }
 
 void launch_sweepn_evo(const HamDev& h, const SweepMap& sm, const double2* psi2,
                        const double2* psi1, double2* out0, double2* out1, double2* S,
                        const SweepTerms& terms, hipStream_t st)
 {
-  if (sm.ns == 4 && sm.nb == 1)
-    sweepn_evo_launch<double2, 4, 1>(h, sm, psi2, psi1, out0, out1, S, terms, st);
+  if (sm.ns == 6 && sm.nb == 1)
+    sweepn_evo_launch<double2, 6, 1>(h, sm, psi2, psi1, out0, out1, S, terms, st);
 }
 
 size_t sweepn_lds(bool f32, int ns, int nb)

@@ -26,27 +26,30 @@ def test_traffic_profile_entries():
     # 8.0M (slot pattern: words of the 0.1 % irregular sites, 1 mask bit)
     assert fp64 is not None and abs(fp64 / (72.137 * 8_000_000) - 1.0) < 0.05
     assert bench.profiled_traffic("k_hpsi32<1,3,true,true,true>") is not None
-    # the two-, four- (FP64) and six-step (FP32) sweeps: 88 B (FP64) and 44 B
-    # (FP32) x 8.0M sites per launch
+    # the two- and six-step sweeps: 88 B (FP64) and 44 B (FP32) x 8.0M sites
+    # per launch
     for name, per_site in (("k_sweep2<double2,3,4,false>", 88.0),
                            ("k_sweep2<float2,3,4,true>", 44.0),
-                           ("k_sweepn<double2,3,4,1,false,false,true>", 88.0),
-                           ("k_sweepn<float2,3,6,4,true,false,false>", 44.0)):
+                           ("k_sweepn<double2,3,6,1,false,false,true>", 88.0),
+                           ("k_sweepn<float2,3,6,1,true,false,false>", 44.0)):
         t = bench.profiled_traffic(name)
         assert t is not None and abs(t / (per_site * 8_000_000) - 1.0) < 0.07, name
     for mode in (1, 2):  # <L|psi> and product-identity block steps, sliding window
         t = bench.profiled_traffic(f"k_block_win<1,6,2,true,{mode}>")
         assert t is not None and t > 0
     # the uneven batch's pair and self-product steps (8 realisations, 8.97M
-    # sites, 400 and 208 B per site) and their two- and four-step sweeps (516
-    # and 260 B per site and launch) at their compulsory traffic within 7 %
-    for name, per_site in (("k_block_step<1,3,2,true,3,false,false>", 400.0),
-                           ("k_block_step<1,3,2,true,2,false,false>", 208.0),
-                           ("k_block_sweep2<3,2,3,4>", 516.0),
-                           ("k_block_sweep2<3,2,2,4>", 260.0),
-                           ("k_block_sweepn<3,2,3,4,1,false>", 516.0),
-                           ("k_block_sweepn<3,2,2,4,1,false>", 260.0),
-                           ("k_block_sweepn<3,2,2,4,1,true>", 516.0)):
+    # sites, 400 and 208 B per site) at their compulsory traffic within 7 %,
+    # their two- and four-step sweeps (516 and 260 B per site and launch)
+    # within 10 %: a chunk holds 24 of a row's 32 positions, and the 8 halo
+    # positions both neighbouring tiles read come from L2 only where the two
+    # tiles run close together in time
+    for name, per_site, tol in (("k_block_step<1,3,2,true,3,false,false>", 400.0, 0.07),
+                                ("k_block_step<1,3,2,true,2,false,false>", 208.0, 0.07),
+                                ("k_block_sweep2<3,2,3,4>", 516.0, 0.10),
+                                ("k_block_sweep2<3,2,2,4>", 260.0, 0.10),
+                                ("k_block_sweepn<3,2,3,4,1,false>", 516.0, 0.10),
+                                ("k_block_sweepn<3,2,2,4,1,false>", 260.0, 0.10),
+                                ("k_block_sweepn<3,2,2,4,1,true>", 516.0, 0.10)):
         t = bench.profiled_traffic(name)
-        assert t is not None and abs(t / (per_site * 8_973_112) - 1.0) < 0.07, name
+        assert t is not None and abs(t / (per_site * 8_973_112) - 1.0) < tol, name
     assert bench.profiled_traffic("no_such_kernel<1>") is None

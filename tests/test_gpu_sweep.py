@@ -14,15 +14,15 @@ recursion steps per launch from LDS (six for FP32 states).  Checked here:
     LSQT_SWEEP=0;
   - lsqt_kernel_polynomial_pair against two lsqt_kernel_polynomial calls,
     bit for bit (the per-site arithmetic is the same, in the same order);
-  - FP64 moment sets (four steps per launch from moment 4 -- six with FP32
-    states --, two steps before and for the remainder) against the same Hamiltonian without the
+  - FP64 and FP32-state moment sets (six steps per launch from moment 4,
+    two steps before and for the remainder) against the same Hamiltonian without the
     sweep and with two steps per launch only (the partial sums group the
     sites differently: the FP64 moment tolerance) and against the oracle,
-    for moment counts 2 .. 5, 8 .. 13 (every remainder of the four-step
+    for moment counts 2 .. 5, 10 .. 15 (every remainder of the six-step
     launches) and odd / even large ones; FP32-state moment sets against
     their one-step counterparts within 1e-6 of mu_0;
-  - U(t) with four Bessel terms per launch (six with FP32 states) against one
-    term per launch, bit for bit in FP64.
+  - U(t) with six Bessel terms per launch against one term per launch, bit
+    for bit in FP64.
 """
 import numpy as np
 import pytest
@@ -135,7 +135,7 @@ def test_sweep_geometry(gpu_ctx, tmp_path, monkeypatch):
         else:
             assert (info.sweep_slab, info.sweep_halo) == (slab, halo), name
             assert info.sweep_tiles > 0 and info.sweep_tiles_f32 > 0
-            assert info.sweep_steps == 4 and info.sweep_tiles_deep > 0
+            assert info.sweep_steps == 6 and info.sweep_tiles_deep > 0
             assert 0 < info.bytes_per_site_sweep_f64 < 2 * info.bytes_per_site_f64
         H.close()
     # the cube: y-neighbours are a row (32 positions) away within a slab
@@ -221,13 +221,12 @@ def test_sweep_moments(gpu_ctx, tmp_path, monkeypatch):
         monkeypatch.setenv("LSQT_SWEEP_STEPS", "2")
         H2 = native.Hamiltonian.from_arrays(gpu_ctx, arrays, n, K, emax)
         monkeypatch.delenv("LSQT_SWEEP_STEPS")
-        assert (H.info().sweep_steps, H.info().sweep_steps_f32) == (4, 6), name
+        assert (H.info().sweep_steps, H.info().sweep_steps_f32) == (6, 6), name
         assert (H2.info().sweep_steps, H2.info().sweep_steps_f32) == (2, 2), name
         assert H0.info().sweep_steps == 0
         v, l = _random_vector(H, rng), _random_vector(H, rng)
-        # 2 .. 5: one and two steps per launch only; 8 .. 13: one four-step
-        # launch (FP64), then every remainder; 10 .. 17: one six-step launch
-        # (FP32 states); 64, 129: many
+        # 2 .. 5: one and two steps per launch only; 10 .. 15: one six-step
+        # launch, then every remainder; 16, 17: two; 64, 129: many
         for M in (2, 3, 4, 5, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 64, 129):
             ref_self, ref_pair = H0.moments(v, v, M), H0.moments(l, v, M)
             assert_moments(H.moments(v, v, M), ref_self, f"{name} M={M} self")
@@ -248,7 +247,7 @@ def test_sweep_moments(gpu_ctx, tmp_path, monkeypatch):
 
 
 def test_sweep_evolution(gpu_ctx, tmp_path, monkeypatch):
-    """U(t) with four Bessel terms per launch (six with FP32 states) against
+    """U(t) with six Bessel terms per launch (FP64 and FP32 states) against
     one term per launch (LSQT_SWEEP_STEPS=2 leaves the evolution one term
     per launch): FP64 states bit for bit, FP32-state evolution within FP32
     rounding; series lengths with every remainder."""

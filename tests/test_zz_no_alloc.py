@@ -37,7 +37,10 @@ def _free_device_memory():
 
 def test_noalloc_equals_moments(gpu_ctx, tmp_path):
     from gpuqt_amd import native
+    # multi-step launches whose moments cross the 64-moment partial ring
+    # (200, 131) and a set that fits it (64)
     cases = [("honeycomb", lambda gi, d: gi.write_honeycomb(d, 100, 60), 200),
+             ("honeycomb131", lambda gi, d: gi.write_honeycomb(d, 100, 60), 131),
              ("square", lambda gi, d: gi.write_square(d, 67, 63), 64)]
     for name, write, M in cases:
         m = _model(tmp_path, name, write)
