@@ -471,7 +471,7 @@ void detect_sweep(lsqt_ham* h, const std::vector<long long>& candidates)
         h->sweepn32 = sweep_shape_n(h->sweep32, true, 6, 1);
       }
       // the same geometry serves the block path's sweeps
-      h->bsweep_steps = (steps && atoi(steps) == 2) ? 2 : 4;
+      h->bsweep_steps = (steps && atoi(steps) == 2) ? 2 : kBlockSweepSteps;
       BlockSweepSeg b{};
       b.S = g.S;
       b.nslab = g.nslab;
@@ -573,7 +573,7 @@ void detect_block_sweep(lsqt_ham* h, int segments, const lsqt_lattice* lats,
   h->bsweep = segs;
   h->bsweep_tab = dev_tab;
   const char* steps = getenv("LSQT_SWEEP_STEPS");
-  h->bsweep_steps = (steps && atoi(steps) == 2) ? 2 : 4;
+  h->bsweep_steps = (steps && atoi(steps) == 2) ? 2 : kBlockSweepSteps;
 }
 
 // The slot pattern of a packed Hamiltonian.  On a periodic lattice whose

@@ -482,7 +482,8 @@ int tile_map(lsqt_ham* h, int R, const BlockShape& c, TileMap* tm)
   return LSQT_OK;
 }
 
-// The launch shape of the block sweep with `steps` (2 or 4) steps per launch
+// The launch shape of the block sweep with `steps` (2 or kBlockSweepSteps)
+// steps per launch
 // for blocks of `cols` columns (*out null where the Hamiltonian has no sweep
 // geometry for each of its systems or the columns do not fit): W = 2
 // columns per lane up to 16 columns, else 4 (two steps only), so cols / W
@@ -502,8 +503,8 @@ int block_sweep_shape(lsqt_ham* h, int cols, int steps, BlockSweepShape** out)
     return LSQT_OK;
   if (!(h->d.K == 2 || h->d.K == 3 || h->d.K == 4 || h->d.K == 6))
     return LSQT_OK;
-  if (steps == 4 && (cols > 16 || h->bsweep_steps < 4))
-    return LSQT_OK;  // four levels of wider entries leave too little LDS
+  if (steps == kBlockSweepSteps && (cols > 16 || h->bsweep_steps < kBlockSweepSteps))
+    return LSQT_OK;  // the deep sweep's levels of wider entries leave too little LDS
   auto it = h->blk->bsw.find(cols * 8 + steps);
   if (it != h->blk->bsw.end()) {
     *out = it->second.ok ? &it->second : nullptr;
@@ -534,7 +535,7 @@ int block_sweep_shape(lsqt_ham* h, int cols, int steps, BlockSweepShape** out)
     }
     units += (long long)g.nch * g.nslab;
   }
-  const size_t lds = steps == 4 ? block_sweepn_lds() : block_sweep_lds(W);
+  const size_t lds = steps == kBlockSweepSteps ? block_sweepn_lds() : block_sweep_lds(W);
   const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / lds));
   const long long run_len = std::max(16ll, units / (2ll * kComputeUnits * per_cu));
   int tiles = 0;
@@ -658,9 +659,9 @@ void enqueue_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, c
 // <T_m L|T_m R>.  The moments follow from T_{2m} = 2 T_m^2 - T_0 and
 // T_{2m+1} = 2 T_m T_{m+1} - T_1 with T_m Hermitian: mu_0 = S_0, mu_1 = S_1,
 // mu_k = 2 S_k - mu_{k mod 2}.  M/2 steps give moments 0..M-1.
-// With sweep shapes (sw: two steps, deep: four; outputs C and D), steps
-// 2 .. are taken four per launch, then two, and the rest (and step 1) by the
-// one-step kernels.
+// With sweep shapes (sw: two steps, deep: kBlockSweepSteps; outputs C and
+// D), steps 2 .. are taken kBlockSweepSteps per launch, then two, and the
+// rest (and step 1) by the one-step kernels.
 // Moments k >= 1 fill ring slots [0, used) from moment k_first, slots
 // stride tiles apart; a ring is reduced when the next launch's slots would
 // not fit or the launches change their tile map, and at the end.
@@ -694,13 +695,13 @@ void enqueue_self_steps(lsqt_ham* h, int R, const BlockShape& c, const TileMap& 
   };
   const int steps = M / 2;
   for (int m = 1; m <= steps;) {
-    if (deep && m >= 2 && m + 3 <= steps) {
-      // steps m .. m + 3: moments 2m - 1 .. 2m + 6
-      double2* p = take(8, &deep->red);
+    if (deep && m >= 2 && m + kBlockSweepSteps - 1 <= steps) {
+      // steps m .. m + kBlockSweepSteps - 1: two moments each from 2m - 1
+      double2* p = take(2 * kBlockSweepSteps, &deep->red);
       launch_block_sweepn(h->d, deep->map, cols, pairs, A, B, C, D, p, slot, st);
       std::swap(A, C);
       std::swap(B, D);
-      m += 4;
+      m += kBlockSweepSteps;
       continue;
     }
     if (sw && m >= 2 && m + 1 <= steps) {
@@ -825,8 +826,8 @@ int moment_set(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Block
   if (mode != kDirect && M / 2 >= 3) {
     const int cols = mode == kPairProducts ? 2 * R : R;
     LSQT_TRY(block_sweep_shape(h, cols, 2, &sw));
-    if (sw && M / 2 >= 5)
-      LSQT_TRY(block_sweep_shape(h, cols, 4, &deep));
+    if (sw && M / 2 >= kBlockSweepSteps + 1)
+      LSQT_TRY(block_sweep_shape(h, cols, kBlockSweepSteps, &deep));
     if (sw)
       LSQT_TRY(block_sweep_scratch(
         h, cols, std::max(tm.tiles, std::max(sw->map.tiles, deep ? deep->map.tiles : 0)), s));
@@ -1056,11 +1057,11 @@ void block_evolve(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Bl
 {
   hipStream_t st = h->ctx->stream;
   const int nR = (int)((size_t)h->d.n * R);
-  // four terms per launch where the lattice allows (its launch shape and
-  // the two more recursion buffers are made on first use; without them the
-  // terms take one launch each)
+  // kBlockSweepSteps terms per launch where the lattice allows (its launch
+  // shape and the two more recursion buffers are made on first use; without
+  // them the terms take one launch each)
   BlockSweepShape* deep = nullptr;
-  if (block_sweep_shape(h, R, 4, &deep) != LSQT_OK ||
+  if (block_sweep_shape(h, R, kBlockSweepSteps, &deep) != LSQT_OK ||
       (deep && block_sweep_scratch(h, R, tm.tiles, s) != LSQT_OK))
     deep = nullptr;
   float2* A = s->A;
@@ -1076,11 +1077,11 @@ void block_evolve(lsqt_ham* h, int R, const BlockShape& c, const TileMap& tm, Bl
   float2* C = s->C;
   float2* D = s->D;
   if (deep) {
-    // four terms per launch on periodic lattices (the same additions in the
-    // same order), the rest below, from an even term
-    for (; k + 4 <= bt.bm.size(); k += 4) {
+    // kBlockSweepSteps terms per launch on periodic lattices (the same
+    // additions in the same order), the rest below, from an even term
+    for (; k + kBlockSweepSteps <= bt.bm.size(); k += kBlockSweepSteps) {
       SweepTerms terms{};
-      for (int l = 0; l < 4; ++l) {
+      for (int l = 0; l < kBlockSweepSteps; ++l) {
         terms.bm[l] = bt.bm[k + l];
         terms.label[l] = bt.label[k + l];
       }

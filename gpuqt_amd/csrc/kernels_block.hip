@@ -1578,7 +1578,7 @@ void launch_block_sweep2(const HamDev& h, const BlockSweepMap& sm, int cols, int
 
 size_t block_sweepn_lds()
 {
-  return block_sweepn_lds_bytes<2, 4, 1>();
+  return block_sweepn_lds_bytes<2, kBlockSweepSteps, 1>();
 }
 
 // four steps per launch, one slab per loop iteration, 2 columns per lane
@@ -1591,10 +1591,10 @@ void launch_block_sweepn(const HamDev& h, const BlockSweepMap& sm, int cols, boo
 #define LSQTB_SWN(KK)                                                                       \
   do {                                                                                      \
     if (pairs)                                                                              \
-      k_block_sweepn<KK, 2, kPair, 4, 1><<<sm.blocks, kBlock, lds, st>>>(                   \
+      k_block_sweepn<KK, 2, kPair, kBlockSweepSteps, 1><<<sm.blocks, kBlock, lds, st>>>(                   \
         h, sm, cols, p2, p1, o0, o1, part, part_stride);                                    \
     else                                                                                    \
-      k_block_sweepn<KK, 2, kSelf, 4, 1><<<sm.blocks, kBlock, lds, st>>>(                   \
+      k_block_sweepn<KK, 2, kSelf, kBlockSweepSteps, 1><<<sm.blocks, kBlock, lds, st>>>(                   \
         h, sm, cols, p2, p1, o0, o1, part, part_stride);                                    \
   } while (0)
   switch (h.K) {
@@ -1623,7 +1623,7 @@ void launch_block_sweepn_evo(const HamDev& h, const BlockSweepMap& sm, int cols,
 {
   const size_t lds = block_sweepn_lds();
 #define LSQTB_SWNE(KK)                                                                     \
-  k_block_sweepn<KK, 2, kSelf, 4, 1, true><<<sm.blocks, kBlock, lds, st>>>(                \
+  k_block_sweepn<KK, 2, kSelf, kBlockSweepSteps, 1, true><<<sm.blocks, kBlock, lds, st>>>(                \
     h, sm, cols, p2, p1, o0, o1, nullptr, 0, state, terms)
   switch (h.K) {
     case 2:

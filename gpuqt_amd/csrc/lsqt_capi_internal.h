@@ -58,7 +58,7 @@ struct lsqt_ham {
   // device (owned by `allocations`); launch shapes are made per block width
   std::vector<lsqt::BlockSweepSeg> bsweep;
   const int4* bsweep_tab = nullptr;
-  int bsweep_steps = 4;  // most steps per launch (LSQT_SWEEP_STEPS=2 at creation: 2)
+  int bsweep_steps = lsqt::kBlockSweepSteps;  // most steps per launch (LSQT_SWEEP_STEPS=2: 2)
   // segmented Hamiltonian: independent systems on sites [seg_starts[k],
   // seg_starts[k+1]); empty for one system
   std::vector<int> seg_starts;

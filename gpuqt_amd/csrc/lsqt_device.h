@@ -371,6 +371,11 @@ inline Grid make_grid(int n, int spt)
   return Grid{tiles, blocks};
 }
 
+// steps (U(t) terms) per launch of the block path's deep sweep, k_block_sweepn
+// (uneven batch, 16 and 8 columns: 4.54e11 site-updates/s with four, 4.37e11
+// with six, whose 72 KB of rings leave two workgroups per CU)
+constexpr int kBlockSweepSteps = 4;
+
 // the multi-step sweeps unroll their loop by the period of their ring rows
 constexpr int sweep_gcd(int a, int b)
 {

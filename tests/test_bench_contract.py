@@ -27,13 +27,15 @@ def test_traffic_profile_entries():
     assert fp64 is not None and abs(fp64 / (72.137 * 8_000_000) - 1.0) < 0.05
     assert bench.profiled_traffic("k_hpsi32<1,3,true,true,true>") is not None
     # the two- and six-step sweeps: 88 B (FP64) and 44 B (FP32) x 8.0M sites
-    # per launch
-    for name, per_site in (("k_sweep2<double2,3,4,false>", 88.0),
-                           ("k_sweep2<float2,3,4,true>", 44.0),
-                           ("k_sweepn<double2,3,6,1,false,false,true>", 88.0),
-                           ("k_sweepn<float2,3,6,1,true,false,false>", 44.0)):
+    # per launch, within 7 %; the FP32 six-step sweep within 15 %: its four
+    # workgroups per CU make runs of about 33 slabs, each of which re-reads
+    # 2 x 5 slabs at its ends (mostly, not always, from L2)
+    for name, per_site, tol in (("k_sweep2<double2,3,4,false>", 88.0, 0.07),
+                                ("k_sweep2<float2,3,4,true>", 44.0, 0.07),
+                                ("k_sweepn<double2,3,6,1,false,false,true>", 88.0, 0.07),
+                                ("k_sweepn<float2,3,6,1,true,false,false>", 44.0, 0.15)):
         t = bench.profiled_traffic(name)
-        assert t is not None and abs(t / (per_site * 8_000_000) - 1.0) < 0.07, name
+        assert t is not None and abs(t / (per_site * 8_000_000) - 1.0) < tol, name
     for mode in (1, 2):  # <L|psi> and product-identity block steps, sliding window
         t = bench.profiled_traffic(f"k_block_win<1,6,2,true,{mode}>")
         assert t is not None and t > 0

@@ -129,7 +129,7 @@ def test_block_sweep_self_products(gpu_ctx, tmp_path, monkeypatch, R, steps):
     re, im = _phases(n, R, R)
     b = native.Block(gpu_ctx, n, R)
     b.upload(re, im)
-    for M in ([6, 7, 8, 9, 10, 11, 12, 14, 16, 18, 64, 131] if R == 8 else [10, 129]):
+    for M in ([6, 7, 8, 9, 10, 11, 12, 14, 15, 16, 18, 26, 64, 131] if R == 8 else [10, 129]):
         got = Hs.moments_block_self(b, M)
         ref = H1.moments_block_self(b, M)
         _check_sets(got, ref, f"self R={R} M={M}")
@@ -151,7 +151,7 @@ def test_block_sweep_pair_products(gpu_ctx, tmp_path, monkeypatch, R, steps):
     Rb = native.Block(gpu_ctx, n, R)
     L.upload(re[:R], im[:R])
     Rb.upload(re[R:], im[R:])
-    for M in ([6, 7, 10, 11, 13, 100, 131] if R == 8 else [12, 65]):
+    for M in ([6, 7, 10, 11, 13, 14, 15, 100, 131] if R == 8 else [12, 65]):
         got = Hs.moments_block_pair(L, Rb, M)
         ref = H1.moments_block_pair(L, Rb, M)
         _check_sets(got, ref, f"pair R={R} M={M}")
