@@ -204,11 +204,21 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
   const int pos_p = wrap_pos(min(max(t, hh), max(wlen - hh - 1, 0)));
   const int pos_l = wrap_pos(min(max(t, NS * hh), NS * hh + clen - 1));
 
+  // FP64: this thread's own entries of the level rings also in registers
+  // (same rows), read there as the site's value and psi_{m-2}; a thread past
+  // the window (te = 0) holds its own values, which nothing reads
+  constexpr bool kOwn = sizeof(V) == sizeof(double2);
+  V own[kOwn ? NS : 1][kOwn ? RR : 1];
   // level 0 at q = -1, 0
   {
     const V a0 = p1[gsite(-1)], a1 = p1[gsite(0)];
     s_lv[0][ring(-1)][t] = a0;
     s_lv[0][ring(0)][t] = a1;
+    if constexpr (kOwn) {
+#pragma unroll
+      for (int c = 0; c < RR; ++c)
+        own[0][c] = c == ring(0) ? a1 : a0;
+    }
   }
   double acc_r[NS], acc_i[NS];
 #pragma unroll
@@ -283,8 +293,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
     }
     load_slabs(k0 + B);
 #pragma unroll
-    for (int i = 0; i < B; ++i)
+    for (int i = 0; i < B; ++i) {
       at(row(1 + i) * kRow + to) = w[i];
+      if constexpr (kOwn)
+        own[0][row(1 + i)] = w[i];
+    }
     __syncthreads();
     V out[2][B];  // levels NS - 1 and NS, stored at the iteration's end
 #pragma unroll
@@ -293,7 +306,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
       for (int i = 0; i < B; ++i) {
         const int q = k0 - (l - 1) + i;
         const int rq = row(i - (l - 1));
-        const double2 self = IO::wide(at((l - 1) * kLevel + rq * kRow + so));
+        const double2 self =
+          IO::wide(kOwn ? own[kOwn ? l - 1 : 0][kOwn ? rq : 0] : at((l - 1) * kLevel + rq * kRow + so));
         const double v = pv[NS - l + i];  // slab q = k0 - NS + 1 + (NS - l + i)
         double tr = v * self.x;
         double ti = v * self.y;
@@ -326,14 +340,19 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweepn(HamDev h, SweepMap sm,
           ti /= emax;
         }
         const double2 prev =
-          l == 1 ? IO::wide(ip2[i]) : IO::wide(at((l - 2) * kLevel + rq * kRow + so));
+          l == 1 ? IO::wide(ip2[i])
+                 : IO::wide(kOwn ? own[kOwn ? l - 2 : 0][kOwn ? rq : 0]
+                                 : at((l - 2) * kLevel + rq * kRow + so));
         tr = 2.0 * tr - prev.x;
         ti = 2.0 * ti - prev.y;
         const V o = IO::narrow(tr, ti);
         if (l >= NS - 1)
           out[l - (NS - 1)][i] = o;
-        if (l < NS)
+        if (l < NS) {
           at(l * kLevel + rq * kRow + to) = o;
+          if constexpr (kOwn)
+            own[l][rq] = o;
+        }
         if constexpr (EVO) {
           // the term of this level in the FP64 state, from the value stored
           const double2 ow = IO::wide(o);
